@@ -1,0 +1,188 @@
+/* novaphy_hip.h -- C ABI of the MI355X PBF fluid path (libnpx_hip.so).
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry point returns
+ * 0 on success or a negative NPX_E* code; npx_last_error() gives the message
+ * for the context (or the last failed create when ctx is NULL).
+ *
+ * Each entry point replaces one function of the reference's fluid path
+ * (FantasyVR/NovaPhy); the reference interface it stands in for is cited on
+ * the declaration.  Vectors are AoS float triples in the caller's (original)
+ * particle order; the device keeps its own cell-sorted SoA copy.
+ */
+#ifndef NOVAPHY_HIP_H
+#define NOVAPHY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPX_OK 0
+#define NPX_EINVAL (-1)     /* bad argument (std::invalid_argument in the reference) */
+#define NPX_EDEVICE (-2)    /* HIP runtime error / no device                          */
+#define NPX_ENOMEM (-3)     /* capacity or device memory exhausted                    */
+#define NPX_EDOMAIN (-4)    /* a particle left the cell window (see npx_last_error)  */
+#define NPX_ESTATE (-5)     /* call out of order (e.g. no particles set)              */
+
+/* Creation flags. */
+#define NPX_FLAG_FAST_MATH 1u   /* FMA / rsqrt arithmetic instead of the reference's
+                                   operation order (results within tolerance, not
+                                   bitwise).  Default: reference operation order.  */
+#define NPX_FLAG_NO_GRAPH 2u    /* launch kernels directly, no hipGraph replay      */
+
+typedef struct npx_ctx npx_ctx;
+
+/* Mirror of PBFSettings, include/novaphy/fluid/pbf_solver.h:13-38 (same field
+ * order and meaning; corr_delta_q is a fraction of kernel_radius). */
+typedef struct npx_pbf_params {
+  float rest_density;
+  float kernel_radius;
+  float particle_radius;
+  int32_t solver_iterations;
+  float epsilon;
+  float xsph_viscosity;
+  float vorticity_epsilon;
+  float corr_delta_q;
+  float corr_k;
+  int32_t corr_n;
+  int32_t use_domain_bounds;
+  float domain_lower[3];
+  float domain_upper[3];
+} npx_pbf_params;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a device context for up to `capacity` particles of one scene
+ * (PBFSolver(const PBFSettings&), src/fluid/pbf_solver.cpp:14-16, plus the
+ * device buffers).  Uses the current HIP device. */
+int npx_create(const npx_pbf_params* params, int32_t capacity, uint32_t flags, npx_ctx** out);
+
+/* Destroys a context and frees its device memory. */
+int npx_destroy(npx_ctx* ctx);
+
+/* Message for the last error on ctx (or of the last failed npx_create when
+ * ctx is NULL).  Never NULL. */
+const char* npx_last_error(const npx_ctx* ctx);
+
+/* Number of visible HIP devices (0 when there is none). */
+int32_t npx_device_count(void);
+
+/* Replaces the solver settings (PBFSolver::settings(), pbf_solver.h:79). */
+int npx_set_params(npx_ctx* ctx, const npx_pbf_params* params);
+
+/* ParticleState::init(positions, v0) with per-particle velocities
+ * (src/fluid/particle_state.cpp:11-20): uploads n particles, zeroes density
+ * and lambda.  vel3 may be NULL (zero velocity). */
+int npx_set_particles(npx_ctx* ctx, const float* pos3, const float* vel3, int32_t n);
+
+/* Reads the particle state back in original order; any pointer may be NULL.
+ * (ParticleState fields, include/novaphy/fluid/particle_state.h:17-22.) */
+int npx_get_particles(npx_ctx* ctx, float* pos3, float* vel3, float* rho, float* lambda);
+
+/* Number of particles currently held. */
+int32_t npx_num_particles(const npx_ctx* ctx);
+
+/* One PBF step: PBFSolver::step(state, dt, gravity, particle_mass),
+ * src/fluid/pbf_solver.cpp:18-127 (predict, grid, I x (density, lambda,
+ * delta-p with s_corr, apply), velocity update with speed clamp, XSPH,
+ * vorticity confinement when vorticity_epsilon != 0, commit with clamp). */
+int npx_step(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass);
+
+/* nsteps steps with the same dt / gravity / mass, without host round trips. */
+int npx_step_n(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass, int32_t nsteps);
+
+/* Blocks until all work queued on the context's stream has finished. */
+int npx_synchronize(npx_ctx* ctx);
+
+/* Grid diagnostics of the last step's neighbour grid (built on the predicted
+ * positions, src/fluid/pbf_solver.cpp:37-51), in original order:
+ *   keys      Teschner key of the particle's cell (neighbor_search.cpp:89-97)
+ *   cand_cnt  length of its candidate list (query_neighbors, duplicates kept)
+ *   nbr_cnt   candidates j != i with squaredNorm(x*_i - x*_j) < h^2
+ * Any pointer may be NULL. */
+int npx_debug_grid(npx_ctx* ctx, uint64_t* keys, int32_t* cand_cnt, int32_t* nbr_cnt);
+
+/* Flattened candidate lists (original indices) of the last step's grid, in
+ * the reference's visiting order; returns the total length in *total (lists
+ * are written only when cap >= total). */
+int npx_debug_candidates(npx_ctx* ctx, int32_t* lists, int64_t cap, int64_t* total);
+
+/* ------------------------------------------------------ standalone grid */
+
+/* SpatialHashGrid::build(positions) + query_neighbors(point, radius) for a
+ * batch of query points (neighbor_search.cpp:14-55): writes per-query
+ * candidate counts and, when cap suffices, the concatenated lists. */
+int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* query3,
+                   int32_t nq, int32_t* counts, int32_t* lists, int64_t cap, int64_t* total);
+
+/* ------------------------------------------------------ Akinci boundary */
+
+/* Uploads boundary particles (BoundaryParticle, boundary.h:21-25): body-local
+ * positions, owning body (-1 = world) and psi volumes (may be NULL = 0). */
+int npx_set_boundary(npx_ctx* ctx, const float* local3, const int32_t* body, const float* psi,
+                     int32_t nb, int32_t nbodies);
+
+/* boundary_world_positions + compute_boundary_volumes (boundary.cpp:152-192):
+ * psi_b = 1 / sum_b' W(x_b - x_b') at the given body poses (7 floats each:
+ * position xyz, quaternion wxyz).  Stores psi on the device; psi_out may be
+ * NULL. */
+int npx_compute_boundary_volumes(npx_ctx* ctx, const float* pose7, int32_t nbodies,
+                                 float* psi_out);
+
+/* boundary_world_positions (boundary.cpp:178-192) for this step's poses;
+ * out3 may be NULL. */
+int npx_boundary_world_positions(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out3);
+
+/* FluidWorld::apply_boundary_density (fluid_world.cpp:101-123):
+ * rho_i += sum_b rho0 psi_b W(x_i - x_b) on committed positions. */
+int npx_apply_boundary_density(npx_ctx* ctx);
+
+/* FluidWorld::apply_coupling_forces (fluid_world.cpp:125-174): per-body
+ * force and torque (about the pose position) summed in boundary-particle
+ * order, skipping world-owned and static bodies and |F_b|^2 <= 1e-20.
+ * Writes nbodies x 3 floats each. */
+int npx_coupling_forces(npx_ctx* ctx, const float* pose7, const uint8_t* is_static,
+                        int32_t nbodies, float* force3, float* torque3);
+
+/* ------------------------------------------------------ batched scenes */
+
+/* Creates a context that holds many independent scenes in one set of device
+ * arrays; every step advances all of them with one launch per pass (no
+ * reference counterpart: the reference steps one FluidWorld at a time). */
+int npx_batch_create(int32_t slot_capacity, int32_t cell_capacity, int32_t max_scenes,
+                     uint32_t flags, npx_ctx** out);
+
+/* Adds a scene (its own PBFSettings, dt, gravity, particle mass and initial
+ * state); scene ids are stable until removed.  Bounded domains only. */
+int npx_scene_add(npx_ctx* ctx, const npx_pbf_params* params, float dt, const float gravity[3],
+                  float particle_mass, const float* pos3, const float* vel3, int32_t n,
+                  int32_t* scene_id);
+
+/* Removes a scene and frees its slots and cells. */
+int npx_scene_remove(npx_ctx* ctx, int32_t scene_id);
+
+/* Reads one scene's state back in its original particle order. */
+int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, float* rho,
+                  float* lambda);
+
+/* Advances every scene by nsteps (each with its own dt / gravity / mass). */
+int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
+
+/* Device bytes needed by a scene of n particles and params (admission). */
+int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n);
+
+/* ------------------------------------------------------ measurement */
+
+/* Enables per-kernel HIP-event timing on the context's stream; while on,
+ * kernels launch directly (no graph).  npx_kernel_times returns, per kernel
+ * name, the accumulated milliseconds and launch count since enabling. */
+int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable);
+int npx_kernel_times(npx_ctx* ctx, char* names, int32_t names_cap, float* ms, int32_t* launches,
+                     int32_t max_kernels, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOVAPHY_HIP_H */

@@ -1,0 +1,643 @@
+// npx.hip -- libnpx_hip: the MI355X PBF fluid path behind a C ABI.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off
+//        -fhip-fp32-correctly-rounded-divide-sqrt -shared -fPIC
+// (novaphy_amd/build.py).  The float operations of the default kernels are
+// the reference's, one IEEE operation at a time, so results are bitwise equal
+// to the FP32 CPU restatement of src/fluid/*.cpp on the same input.
+#include "npx_engine.hpp"
+
+using npx::Ctx;
+using npx::SceneH;
+using npx::cdiv;
+
+namespace {
+
+thread_local std::string g_create_error = "no error";
+
+int check_params(Ctx* c, const npx_pbf_params* p) {
+  if (!p) return c->fail(NPX_EINVAL, "params is NULL");
+  if (!(p->kernel_radius > 0.0f) || !std::isfinite(p->kernel_radius))
+    return c->fail(NPX_EINVAL, "kernel_radius must be positive and finite");
+  if (!(p->rest_density > 0.0f)) return c->fail(NPX_EINVAL, "rest_density must be positive");
+  if (p->solver_iterations < 0) return c->fail(NPX_EINVAL, "solver_iterations must be >= 0");
+  return NPX_OK;
+}
+
+// ---------------------------------------------------------------- boundary
+int set_boundary(Ctx* c, const float* local3, const int32_t* body, const float* psi, int nb, int nbodies) {
+  if (nb < 0 || nbodies < 0) return c->fail(NPX_EINVAL, "negative boundary size");
+  if (nb > 0 && (!local3 || !body)) return c->fail(NPX_EINVAL, "boundary arrays are NULL");
+  c->nb = nb;
+  c->nbodies = nbodies;
+  const size_t n = std::max(nb, 1);
+  NPX_HIP_C(c, c->BL.ensure(n));
+  NPX_HIP_C(c, c->BP.ensure(n));
+  NPX_HIP_C(c, c->BPS.ensure(n));
+  NPX_HIP_C(c, c->FB.ensure(n));
+  NPX_HIP_C(c, c->BODY.ensure(n));
+  NPX_HIP_C(c, c->PSI.ensure(n));
+  NPX_HIP_C(c, c->BODY_PTR.ensure((size_t)nbodies + 1));
+  NPX_HIP_C(c, c->BODY_IDX.ensure(n));
+  NPX_HIP_C(c, c->POSE.ensure((size_t)std::max(nbodies, 1) * 7));
+  NPX_HIP_C(c, c->STATIC.ensure((size_t)std::max(nbodies, 1)));
+  NPX_HIP_C(c, c->OUT6.ensure((size_t)std::max(nbodies, 1) * 6));
+  std::vector<float4> bl(n);
+  std::vector<float> ps(n, 0.0f);
+  std::vector<int> bd(n, -1);
+  std::vector<int> cnt(nbodies + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    bl[b] = make_float4(local3[3 * b], local3[3 * b + 1], local3[3 * b + 2], 0.0f);
+    bd[b] = body[b];
+    if (psi) ps[b] = psi[b];
+    if (body[b] >= 0 && body[b] < nbodies) cnt[body[b] + 1]++;
+  }
+  for (int k = 0; k < nbodies; ++k) cnt[k + 1] += cnt[k];
+  std::vector<int> idx(n, 0), fillp(cnt.begin(), cnt.end());
+  for (int b = 0; b < nb; ++b)
+    if (body[b] >= 0 && body[b] < nbodies) idx[fillp[body[b]]++] = b;  // ascending b per body
+  NPX_HIP_C(c, hipMemcpy(c->BL.p, bl.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  NPX_HIP_C(c, hipMemcpy(c->BODY.p, bd.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  NPX_HIP_C(c, hipMemcpy(c->PSI.p, ps.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  NPX_HIP_C(c, hipMemcpy(c->BODY_PTR.p, cnt.data(), sizeof(int) * (nbodies + 1), hipMemcpyHostToDevice));
+  NPX_HIP_C(c, hipMemcpy(c->BODY_IDX.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  return NPX_OK;
+}
+
+int upload_poses(Ctx* c, const float* pose7, int nbodies) {
+  if (nbodies != c->nbodies) return c->fail(NPX_EINVAL, "expected %d body poses, got %d", c->nbodies, nbodies);
+  if (nbodies > 0) {
+    if (!pose7) return c->fail(NPX_EINVAL, "pose7 is NULL");
+    NPX_HIP_C(c, hipMemcpyAsync(c->POSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice, c->stream));
+  }
+  return NPX_OK;
+}
+
+int bpos(Ctx* c, const float* pose7, int nbodies) {
+  NPX_TRY(upload_poses(c, pose7, nbodies));
+  NPX_TRY(c->launch("bpos", k_bpos, cdiv(c->nb, 256), 256, (const float4*)c->BL.p, (const int*)c->BODY.p,
+                    (const float*)c->PSI.p, c->nb, (const float*)c->POSE.p, nbodies, c->BP.p));
+  return NPX_OK;
+}
+
+int compute_volumes(Ctx* c, const float* pose7, int nbodies, float* psi_out) {
+  if (c->nb == 0) return NPX_OK;
+  NPX_TRY(bpos(c, pose7, nbodies));
+  // window = bounding box of the boundary particles (+2 cells)
+  std::vector<float4> bp(c->nb);
+  NPX_HIP_C(c, hipMemcpyAsync(bp.data(), c->BP.p, sizeof(float4) * c->nb, hipMemcpyDeviceToHost, c->stream));
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  const float h = c->scenes[0].prm.kernel_radius;
+  const float inv_h = 1.0f / h;
+  int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+  for (int b = 0; b < c->nb; ++b) {
+    const float p[3] = {bp[b].x, bp[b].y, bp[b].z};
+    for (int d = 0; d < 3; ++d) {
+      const float f = std::floor(p[d] * inv_h);
+      if (!(std::fabs(f) < 1.0e8f)) return c->fail(NPX_EDOMAIN, "boundary particle position is not finite");
+      lo[d] = std::min(lo[d], (int)f);
+      hi[d] = std::max(hi[d], (int)f);
+    }
+  }
+  int wlo[3], wdim[3];
+  long long ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    wlo[d] = lo[d] - 2;
+    wdim[d] = hi[d] + 2 - wlo[d] + 1;
+    ncell *= wdim[d];
+  }
+  if (ncell > (1ll << 30)) return c->fail(NPX_EDOMAIN, "boundary extent too large for a cell window");
+  NPX_HIP_C(c, c->aux_clsmap.ensure((size_t)ncell));
+  NPX_TRY(c->build_class_table(wlo, wdim, 0, c->aux_clsmap.p));
+  GridWin W;
+  W.inv_h = inv_h;
+  W.wlo0 = wlo[0];
+  W.wlo1 = wlo[1];
+  W.wlo2 = wlo[2];
+  W.wx = wdim[0];
+  W.wy = wdim[1];
+  W.wz = wdim[2];
+  W.ncell = (int)ncell;
+  NPX_TRY(c->aux_sort(W, c->aux_clsmap.p, 0, c->BP.p, c->nb, nullptr, c->BPS.p));
+  const float h2 = h * h;
+  const float h9 = h2 * h2 * h2 * h2 * h;
+  const float c6 = 315.0f / ((64.0f * npx::kPI) * h9);
+  NPX_TRY(c->launch("psi", k_psi, cdiv(c->nb, 256), 256, W, h2, c6, (const float4*)c->BP.p, c->nb,
+                    (const int*)c->aux_clsmap.p, (const int*)c->Acstart.p, (const float4*)c->BPS.p, c->PSI.p));
+  if (psi_out)
+    NPX_HIP_C(c, hipMemcpyAsync(psi_out, c->PSI.p, sizeof(float) * c->nb, hipMemcpyDeviceToHost, c->stream));
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  return NPX_OK;
+}
+
+int boundary_density(Ctx* c) {
+  SceneH& H = c->scenes[0];
+  if (c->nb == 0 || !H.used || H.n == 0) return NPX_OK;
+  if (!H.window_valid) return c->fail(NPX_ESTATE, "no step has run yet");
+  const GridWin W = c->scene_win(H);
+  const int* cm = c->clsmap.p + H.cls_base;
+  NPX_TRY(c->aux_sort(W, cm, H.cls_base, c->BP.p, c->nb, nullptr, c->BPS.p));
+  NPX_TRY(c->launch("boundary_density", k_boundary_density, c->hw_blocks, NPX_BLOCK, (const SceneC*)c->d_scenes.p,
+                    (const int*)c->d_blk_scene.p, W, (const float4*)c->X.p, cm,
+                    (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->BPS.p, c->RHO.p, c->d_flags.p));
+  return NPX_OK;
+}
+
+int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, float* force3, float* torque3) {
+  SceneH& H = c->scenes[0];
+  if (nbodies != c->nbodies) return c->fail(NPX_EINVAL, "expected %d bodies, got %d", c->nbodies, nbodies);
+  if (nbodies == 0) return NPX_OK;
+  std::vector<float> out(6 * nbodies, 0.0f);
+  if (c->nb > 0 && H.used && H.n > 0) {
+    if (!H.window_valid) return c->fail(NPX_ESTATE, "no step has run yet");
+    NPX_TRY(upload_poses(c, pose7, nbodies));
+    if (!is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
+    NPX_HIP_C(c, hipMemcpyAsync(c->STATIC.p, is_static, nbodies, hipMemcpyHostToDevice, c->stream));
+    NPX_HIP_C(c, c->FP.ensure(c->slot_cap));
+    NPX_HIP_C(c, c->FS.ensure(c->slot_cap));
+    NPX_TRY(c->launch("fluid_payload", k_fluid_payload, c->hw_blocks, NPX_BLOCK, (const SceneC*)c->d_scenes.p,
+                      (const int*)c->d_blk_scene.p, (const float4*)c->X.p, (const float*)c->RHO.p, c->FP.p));
+    const GridWin W = c->scene_win(H);
+    const int* cm = c->clsmap.p + H.cls_base;
+    NPX_TRY(c->aux_sort(W, cm, H.cls_base, c->FP.p + H.seg_off, H.n, c->ID.p + H.seg_off, c->FS.p));
+    const float h = H.prm.kernel_radius;
+    const float h6 = h * h * h * h * h * h;
+    const float cs = 45.0f / (npx::kPI * h6);
+    NPX_TRY(c->launch("coupling", k_coupling_particles, cdiv(c->nb, 256), 256, W, h, h * h, cs, H.prm.rest_density,
+                      H.mass, (const float4*)c->BP.p, (const int*)c->BODY.p, c->nb, (const uint8_t*)c->STATIC.p,
+                      nbodies, cm, (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->FS.p, c->FB.p));
+    NPX_TRY(c->launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)c->BODY_PTR.p,
+                      (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
+                      (const float*)c->POSE.p, c->OUT6.p));
+    NPX_HIP_C(c, hipMemcpyAsync(out.data(), c->OUT6.p, sizeof(float) * 6 * nbodies, hipMemcpyDeviceToHost,
+                                c->stream));
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    NPX_TRY(c->check_flags());
+  }
+  for (int k = 0; k < nbodies; ++k) {
+    if (force3) {
+      force3[3 * k] = out[6 * k];
+      force3[3 * k + 1] = out[6 * k + 1];
+      force3[3 * k + 2] = out[6 * k + 2];
+    }
+    if (torque3) {
+      torque3[3 * k] = out[6 * k + 3];
+      torque3[3 * k + 1] = out[6 * k + 4];
+      torque3[3 * k + 2] = out[6 * k + 5];
+    }
+  }
+  return NPX_OK;
+}
+
+// ------------------------------------------------------- standalone grid
+int grid_query(Ctx* c, const float* pos3, int n, float cell, const float* q3, int nq, int32_t* counts,
+               int32_t* lists, int64_t cap, int64_t* total) {
+  *total = 0;
+  if (nq == 0) return NPX_OK;
+  if (!(cell > 0.0f)) return c->fail(NPX_EINVAL, "cell_size must be positive");
+  const float inv = 1.0f / cell;
+  int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+  auto acc = [&](const float* p) -> bool {
+    for (int d = 0; d < 3; ++d) {
+      const float f = std::floor(p[d] * inv);
+      if (!(std::fabs(f) < 1.0e8f)) return false;
+      lo[d] = std::min(lo[d], (int)f);
+      hi[d] = std::max(hi[d], (int)f);
+    }
+    return true;
+  };
+  for (int i = 0; i < n; ++i)
+    if (!acc(pos3 + 3 * i)) return c->fail(NPX_EDOMAIN, "point %d is not finite", i);
+  for (int i = 0; i < nq; ++i)
+    if (!acc(q3 + 3 * i)) return c->fail(NPX_EDOMAIN, "query %d is not finite", i);
+  int wlo[3], wdim[3];
+  long long ncell = 1;
+  for (int d = 0; d < 3; ++d) {
+    wlo[d] = lo[d] - 2;
+    wdim[d] = hi[d] + 2 - wlo[d] + 1;
+    ncell *= wdim[d];
+  }
+  if (ncell > (1ll << 28)) return c->fail(NPX_EDOMAIN, "points span too many cells for one window (%lld)", ncell);
+  NPX_HIP_C(c, c->aux_clsmap.ensure((size_t)ncell));
+  NPX_TRY(c->build_class_table(wlo, wdim, 0, c->aux_clsmap.p));
+  GridWin W;
+  W.inv_h = inv;
+  W.wlo0 = wlo[0];
+  W.wlo1 = wlo[1];
+  W.wlo2 = wlo[2];
+  W.wx = wdim[0];
+  W.wy = wdim[1];
+  W.wz = wdim[2];
+  W.ncell = (int)ncell;
+  npx::DBuf<float4> P, PS, Q;
+  npx::DBuf<int> cnt;
+  npx::DBuf<long long> off;
+  npx::DBuf<int> out;
+  NPX_HIP_C(c, P.ensure(std::max(n, 1)));
+  NPX_HIP_C(c, PS.ensure(std::max(n, 1)));
+  NPX_HIP_C(c, Q.ensure(nq));
+  NPX_HIP_C(c, cnt.ensure(nq));
+  NPX_HIP_C(c, off.ensure(nq));
+  std::vector<float4> hp(std::max(n, 1)), hq(nq);
+  for (int i = 0; i < n; ++i) hp[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], 0.0f);
+  for (int i = 0; i < nq; ++i) hq[i] = make_float4(q3[3 * i], q3[3 * i + 1], q3[3 * i + 2], 0.0f);
+  if (n > 0) NPX_HIP_C(c, hipMemcpy(P.p, hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  NPX_HIP_C(c, hipMemcpy(Q.p, hq.data(), sizeof(float4) * nq, hipMemcpyHostToDevice));
+  NPX_TRY(c->aux_sort(W, c->aux_clsmap.p, 0, P.p, n, nullptr, PS.p));
+  NPX_TRY(c->launch("query_count", k_query_count, cdiv(nq, 256), 256, W, (const float4*)Q.p, nq,
+                    (const int*)c->aux_clsmap.p, (const int*)c->Acstart.p, cnt.p));
+  std::vector<int> hc(nq);
+  NPX_HIP_C(c, hipMemcpy(hc.data(), cnt.p, sizeof(int) * nq, hipMemcpyDeviceToHost));
+  std::vector<long long> ho(nq);
+  long long t = 0;
+  for (int i = 0; i < nq; ++i) {
+    ho[i] = t;
+    t += hc[i];
+    if (counts) counts[i] = hc[i];
+  }
+  *total = t;
+  if (lists && cap >= t && t > 0) {
+    NPX_HIP_C(c, out.ensure((size_t)t));
+    NPX_HIP_C(c, hipMemcpy(off.p, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice));
+    NPX_TRY(c->launch("query_lists", k_query_lists, cdiv(nq, 256), 256, W, (const float4*)Q.p, nq,
+                      (const int*)c->aux_clsmap.p, (const int*)c->Acstart.p, (const int*)c->APERM.p,
+                      (const long long*)off.p, out.p));
+    NPX_HIP_C(c, hipMemcpy(lists, out.p, sizeof(int) * t, hipMemcpyDeviceToHost));
+  }
+  P.release();
+  PS.release();
+  Q.release();
+  cnt.release();
+  off.release();
+  out.release();
+  return NPX_OK;
+}
+
+void destroy(Ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& t : c->timers) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  npx::DBuf<float4>* f4[] = {&c->X, &c->X2, &c->V, &c->V2, &c->XS0, &c->XS1, &c->VT, &c->OM,
+                             &c->BL, &c->BP, &c->BPS, &c->FP, &c->FS, &c->FB};
+  for (auto* b : f4) b->release();
+  npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->PSI, &c->POSE, &c->OUT6};
+  for (auto* b : f1) b->release();
+  npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
+                          &c->PERM, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
+                          &c->ATKEY, &c->APERM, &c->aux_clsmap};
+  for (auto* b : i1) b->release();
+  c->htk.release();
+  c->d_scenes.release();
+  c->STATIC.release();
+  if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int32_t npx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+const char* npx_last_error(const npx_ctx* ctx) {
+  if (!ctx) return g_create_error.c_str();
+  return reinterpret_cast<const Ctx*>(ctx)->err.c_str();
+}
+
+int npx_create(const npx_pbf_params* params, int32_t capacity, uint32_t flags, npx_ctx** out) {
+  if (!out) return NPX_EINVAL;
+  *out = nullptr;
+  Ctx* c = new Ctx();
+  int r = check_params(c, params);
+  if (r == NPX_OK) r = c->init(std::max(capacity, 1), 0, 1, flags, false);
+  if (r != NPX_OK) {
+    g_create_error = c->err;
+    destroy(c);
+    return r;
+  }
+  c->scenes[0].prm = *params;
+  *out = reinterpret_cast<npx_ctx*>(c);
+  return NPX_OK;
+}
+
+int npx_destroy(npx_ctx* ctx) {
+  destroy(reinterpret_cast<Ctx*>(ctx));
+  return NPX_OK;
+}
+
+int npx_set_params(npx_ctx* ctx, const npx_pbf_params* params) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  NPX_TRY(check_params(c, params));
+  SceneH& H = c->scenes[0];
+  const bool window_change = H.prm.kernel_radius != params->kernel_radius ||
+                             H.prm.use_domain_bounds != params->use_domain_bounds ||
+                             std::memcmp(H.prm.domain_lower, params->domain_lower, sizeof(float) * 3) != 0 ||
+                             std::memcmp(H.prm.domain_upper, params->domain_upper, sizeof(float) * 3) != 0;
+  H.prm = *params;
+  c->table_dirty = true;
+  if (window_change && H.used) {
+    H.window_valid = false;
+    NPX_TRY(c->scene_window(H));
+  }
+  return NPX_OK;
+}
+
+int npx_set_particles(npx_ctx* ctx, const float* pos3, const float* vel3, int32_t n) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (c->batch) return c->fail(NPX_ESTATE, "use npx_scene_add on a batch context");
+  if (n < 0 || (n > 0 && !pos3)) return c->fail(NPX_EINVAL, "bad particle arrays");
+  NPX_TRY(c->reserve_slots(n));
+  SceneH& H = c->scenes[0];
+  H.used = true;
+  H.n = n;
+  H.seg_off = 0;
+  H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
+  std::fill(c->h_blk_scene.begin(), c->h_blk_scene.end(), -1);
+  for (int b = 0; b < cdiv(n, NPX_BLOCK); ++b) c->h_blk_scene[b] = 0;
+  c->table_dirty = true;
+  H.window_valid = false;
+  NPX_TRY(c->scene_window(H));
+  NPX_TRY(c->upload_particles(H, pos3, vel3));
+  NPX_TRY(c->upload_tables());
+  return NPX_OK;
+}
+
+int npx_get_particles(npx_ctx* ctx, float* pos3, float* vel3, float* rho, float* lambda) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  return c->download(c->scenes[0], pos3, vel3, rho, lambda);
+}
+
+int32_t npx_num_particles(const npx_ctx* ctx) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
+  if (!c) return 0;
+  int t = 0;
+  for (const auto& H : c->scenes)
+    if (H.used) t += H.n;
+  return t;
+}
+
+int npx_step_n(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass, int32_t nsteps) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (c->batch) return c->fail(NPX_ESTATE, "use npx_batch_step on a batch context");
+  if (!gravity) return c->fail(NPX_EINVAL, "gravity is NULL");
+  SceneH& H = c->scenes[0];
+  if (H.dt != dt || H.mass != particle_mass || std::memcmp(H.g, gravity, sizeof(float) * 3) != 0) {
+    H.dt = dt;
+    H.mass = particle_mass;
+    std::memcpy(H.g, gravity, sizeof(float) * 3);
+    c->table_dirty = true;
+  }
+  if (!H.used || H.n == 0) return NPX_OK;
+  return c->step_n(nsteps);
+}
+
+int npx_step(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass) {
+  return npx_step_n(ctx, dt, gravity, particle_mass, 1);
+}
+
+int npx_synchronize(npx_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  return c->check_flags();
+}
+
+int npx_debug_grid(npx_ctx* ctx, uint64_t* keys, int32_t* cand_cnt, int32_t* nbr_cnt) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
+  return c->debug_grid(keys, cand_cnt, nbr_cnt);
+}
+
+int npx_debug_candidates(npx_ctx* ctx, int32_t* lists, int64_t cap, int64_t* total) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !total) return NPX_EINVAL;
+  if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
+  return c->debug_candidates(lists, cap, total);
+}
+
+int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* query3, int32_t nq, int32_t* counts,
+                   int32_t* lists, int64_t cap, int64_t* total) {
+  if (!total || n < 0 || nq < 0 || (n > 0 && !pos3) || (nq > 0 && !query3)) {
+    g_create_error = "npx_grid_query: bad arguments";
+    return NPX_EINVAL;
+  }
+  Ctx* c = new Ctx();
+  npx_pbf_params p{};
+  p.kernel_radius = cell_size;
+  p.rest_density = 1.0f;
+  int r = c->init(1, 0, 1, 0, false);
+  if (r == NPX_OK) {
+    c->scenes[0].prm = p;
+    r = grid_query(c, pos3, n, cell_size, query3, nq, counts, lists, cap, total);
+  }
+  if (r != NPX_OK) g_create_error = c->err;
+  destroy(c);
+  return r;
+}
+
+int npx_set_boundary(npx_ctx* ctx, const float* local3, const int32_t* body, const float* psi, int32_t nb,
+                     int32_t nbodies) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (c->batch) return c->fail(NPX_ESTATE, "boundary coupling is for single-scene contexts");
+  return set_boundary(c, local3, body, psi, nb, nbodies);
+}
+
+int npx_compute_boundary_volumes(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* psi_out) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  return compute_volumes(c, pose7, nbodies, psi_out);
+}
+
+int npx_boundary_world_positions(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out3) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (c->nb == 0) return NPX_OK;
+  NPX_TRY(bpos(c, pose7, nbodies));
+  if (out3) {
+    std::vector<float4> bp(c->nb);
+    NPX_HIP_C(c, hipMemcpyAsync(bp.data(), c->BP.p, sizeof(float4) * c->nb, hipMemcpyDeviceToHost, c->stream));
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < c->nb; ++b) {
+      out3[3 * b] = bp[b].x;
+      out3[3 * b + 1] = bp[b].y;
+      out3[3 * b + 2] = bp[b].z;
+    }
+  }
+  return NPX_OK;
+}
+
+int npx_apply_boundary_density(npx_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  return boundary_density(c);
+}
+
+int npx_coupling_forces(npx_ctx* ctx, const float* pose7, const uint8_t* is_static, int32_t nbodies, float* force3,
+                        float* torque3) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  return coupling(c, pose7, is_static, nbodies, force3, torque3);
+}
+
+// ------------------------------------------------------------- batch
+int npx_batch_create(int32_t slot_capacity, int32_t cell_capacity, int32_t max_scenes, uint32_t flags,
+                     npx_ctx** out) {
+  if (!out || slot_capacity <= 0 || cell_capacity <= 0 || max_scenes <= 0) {
+    g_create_error = "npx_batch_create: capacities must be positive";
+    return NPX_EINVAL;
+  }
+  *out = nullptr;
+  Ctx* c = new Ctx();
+  int r = c->init(slot_capacity, cell_capacity, max_scenes, flags, true);
+  if (r != NPX_OK) {
+    g_create_error = c->err;
+    destroy(c);
+    return r;
+  }
+  *out = reinterpret_cast<npx_ctx*>(c);
+  return NPX_OK;
+}
+
+int npx_scene_add(npx_ctx* ctx, const npx_pbf_params* params, float dt, const float gravity[3], float particle_mass,
+                  const float* pos3, const float* vel3, int32_t n, int32_t* scene_id) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_add needs a batch context");
+  NPX_TRY(check_params(c, params));
+  if (!params->use_domain_bounds) return c->fail(NPX_EINVAL, "batched scenes need use_domain_bounds");
+  if (n < 0 || (n > 0 && !pos3) || !gravity) return c->fail(NPX_EINVAL, "bad scene arrays");
+  int sid = -1;
+  for (int s = 0; s < c->max_scenes; ++s)
+    if (!c->scenes[s].used) {
+      sid = s;
+      break;
+    }
+  if (sid < 0) return c->fail(NPX_ENOMEM, "all %d scene slots are in use", c->max_scenes);
+  SceneH H;
+  H.prm = *params;
+  H.dt = dt;
+  H.mass = particle_mass;
+  std::memcpy(H.g, gravity, sizeof(float) * 3);
+  H.n = n;
+  H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
+  const int b0 = c->slot_alloc.alloc(H.seg_blocks);
+  if (b0 < 0) return c->fail(NPX_ENOMEM, "slot capacity exhausted (%d particles requested)", n);
+  H.seg_off = b0 * NPX_BLOCK;
+  int r = c->scene_window(H);
+  if (r != NPX_OK) {
+    c->slot_alloc.release(b0, H.seg_blocks);
+    return r;
+  }
+  H.used = true;
+  c->scenes[sid] = H;
+  for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = (b < cdiv(n, NPX_BLOCK)) ? sid : -1;
+  c->table_dirty = true;
+  NPX_TRY(c->upload_particles(c->scenes[sid], pos3, vel3));
+  NPX_TRY(c->upload_tables());
+  if (scene_id) *scene_id = sid;
+  return NPX_OK;
+}
+
+int npx_scene_remove(npx_ctx* ctx, int32_t scene_id) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+  SceneH& H = c->scenes[scene_id];
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  const int b0 = H.seg_off / NPX_BLOCK;
+  for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = -1;
+  c->slot_alloc.release(b0, H.seg_blocks);
+  if (H.window_valid) c->cls_alloc.release(H.cls_base, H.ncell);
+  H = SceneH();
+  c->table_dirty = true;
+  return c->upload_tables();
+}
+
+int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, float* rho, float* lambda) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+  return c->download(c->scenes[scene_id], pos3, vel3, rho, lambda);
+}
+
+int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  return c->step_n(nsteps);
+}
+
+int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
+  if (!params || !(params->kernel_radius > 0.0f)) return -1;
+  int wlo[3], wdim[3];
+  Ctx::window_from_bounds(1.0f / params->kernel_radius, params->domain_lower, params->domain_upper,
+                          NPX_WINDOW_MARGIN, wlo, wdim);
+  const long long cells = (long long)wdim[0] * wdim[1] * wdim[2];
+  const long long slots = (long long)cdiv(std::max(n, 1), NPX_BLOCK) * NPX_BLOCK;
+  // 8 float4 + 4 float + 9 int arrays per slot; class map + start per cell
+  return slots * (8 * 16 + 4 * 4 + 9 * 4) + cells * 8;
+}
+
+int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  for (auto& t : c->timers) {
+    c->ev_pool.push_back(t.a);
+    c->ev_pool.push_back(t.b);
+  }
+  c->timers.clear();
+  c->timing = enable != 0;
+  return NPX_OK;
+}
+
+int npx_kernel_times(npx_ctx* ctx, char* names, int32_t names_cap, float* ms, int32_t* launches,
+                     int32_t max_kernels, int32_t* count) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !count) return NPX_EINVAL;
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<double, int>> acc;
+  for (auto& t : c->timers) {
+    float e = 0.0f;
+    NPX_HIP_C(c, hipEventElapsedTime(&e, t.a, t.b));
+    if (!acc.count(t.name)) order.push_back(t.name);
+    acc[t.name].first += e;
+    acc[t.name].second += 1;
+  }
+  *count = (int32_t)order.size();
+  std::string joined;
+  for (size_t k = 0; k < order.size(); ++k) {
+    if ((int)k < max_kernels) {
+      if (ms) ms[k] = (float)acc[order[k]].first;
+      if (launches) launches[k] = acc[order[k]].second;
+    }
+    joined += order[k];
+    joined += '\n';
+  }
+  if (names && names_cap > 0) {
+    std::strncpy(names, joined.c_str(), names_cap - 1);
+    names[names_cap - 1] = '\0';
+  }
+  return NPX_OK;
+}
+
+}  // extern "C"

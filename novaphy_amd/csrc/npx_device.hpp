@@ -1,0 +1,115 @@
+// npx_device.hpp -- per-scene constants and the SPH arithmetic of the PBF path.
+//
+// Every function here reproduces the reference's float operation order
+// (include/novaphy/fluid/sph_kernel.h, src/fluid/pbf_solver.cpp) when built
+// with -ffp-contract=off and correctly rounded sqrt / divide, so the GPU step
+// is bitwise equal to the FP32 CPU restatement on the same input.  The FAST
+// variants (NPX_FLAG_FAST_MATH) trade that for FMA / rsqrt arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NPX_BLOCK 256
+#define NPX_WINDOW_MARGIN 4
+
+// Constants of one scene, computed on the host with the reference's float
+// operations (each value is what the reference computes per call).
+struct __align__(16) SceneC {
+  float h, h2, inv_h, c6;          // c6 = 315 / (64*PI*h^9)        sph_kernel.h:29
+  float cs, rho0, inv_rho0, mass;  // cs = 45 / (PI*h^6)            sph_kernel.h:44
+  float m_inv_rho0, eps, xsph_c, vort_eps;
+  float corr_k, wdq, dt, inv_dt;   // wdq = poly6((dq*h)^2)         pbf_solver.cpp:188-189
+  float max_speed, max_speed_sq, gdt0, gdt1;
+  float gdt2, clo0, clo1, clo2;    // clamp box: domain_lower + r .. domain_upper - r
+  float chi0, chi1, chi2, cut2;     // cut2: beyond it poly6 and spiky_grad both vanish
+  int corr_n, iters, bounds, xsph_on;
+  int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
+  int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
+  int seg_off, n, seg_blocks, vort_on;
+};
+
+struct __align__(16) GridWin {  // window of an auxiliary grid (boundary / committed fluid)
+  float inv_h;
+  int wlo0, wlo1, wlo2;
+  int wx, wy, wz, ncell;
+};
+
+// ------------------------------------------------------------ arithmetic
+__device__ __forceinline__ float sqn3(float x, float y, float z) {
+  return x * x + (y * y + z * z);  // Eigen 3-vector squaredNorm order
+}
+
+// SPHKernels::poly6 (sph_kernel.h:24-30)
+__device__ __forceinline__ float poly6(float r_sq, float h2, float c6) {
+  if (r_sq >= h2) return 0.0f;
+  float d = h2 - r_sq;
+  return c6 * d * d * d;
+}
+
+// SPHKernels::spiky_grad (sph_kernel.h:39-46): returns the scalar that
+// multiplies r, or flags out-of-support with *in = false (zero vector).
+__device__ __forceinline__ float spiky_coeff(float r_sq, float h, float cs, bool* in) {
+  float r_len = sqrtf(r_sq);
+  if (r_len <= 1e-7f || r_len >= h) {
+    *in = false;
+    return 0.0f;
+  }
+  *in = true;
+  float diff = h - r_len;
+  return (-cs) * diff * diff / r_len;
+}
+
+// Fast variant: one rsqrt, FMA-friendly; support tested on r_sq
+// (|r| <= 1e-7  <=>  r_sq <= 1e-14, |r| >= h  <=>  r_sq >= h^2).
+__device__ __forceinline__ float spiky_coeff_fast(float r_sq, float h, float h2, float cs, bool* in) {
+  if (r_sq <= 1e-14f || r_sq >= h2) {
+    *in = false;
+    return 0.0f;
+  }
+  *in = true;
+  float rinv = __frsqrt_rn(r_sq);
+  float diff = h - r_sq * rinv;
+  return (-cs) * diff * diff * rinv;
+}
+
+__device__ __forceinline__ uint64_t teschner_key(int cx, int cy, int cz) {
+  return ((uint64_t)(int64_t)cx * 73856093ull) ^ ((uint64_t)(int64_t)cy * 19349663ull) ^
+         ((uint64_t)(int64_t)cz * 83492791ull);
+}
+
+__device__ __forceinline__ int cell_coord(float p, float inv_h) {
+  return (int)floorf(p * inv_h);  // SpatialHashGrid::world_to_cell, neighbor_search.cpp:99-105
+}
+
+// Visits the candidate list of a particle whose cell has local window index
+// l, in the reference's order: dz outer, dy, dx inner, each offset's bucket in
+// ascending particle index, a bucket visited again when two offsets share a
+// Teschner key (query_neighbors, neighbor_search.cpp:28-49).  clsmap maps a
+// window cell to its class (cells with equal keys share one class); cstart is
+// the exclusive prefix of class populations, minus `shift` giving the slot.
+template <class F>
+__device__ __forceinline__ void for_each_candidate(int l, int wx, int wxy, const int* __restrict__ clsmap,
+                                                   const int* __restrict__ cstart, int shift, F&& f) {
+#pragma unroll 1
+  for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int base = l + dy * wx + dz * wxy;
+      const int c0 = clsmap[base - 1];
+      const int c1 = clsmap[base];
+      const int c2 = clsmap[base + 1];
+      if (c1 == c0 + 1 && c2 == c0 + 2) {
+        const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
+        for (int j = b; j < e; ++j) f(j);
+      } else {
+        const int cc[3] = {c0, c1, c2};
+#pragma unroll 1
+        for (int k = 0; k < 3; ++k) {
+          const int b = cstart[cc[k]] - shift, e = cstart[cc[k] + 1] - shift;
+          for (int j = b; j < e; ++j) f(j);
+        }
+      }
+    }
+  }
+}

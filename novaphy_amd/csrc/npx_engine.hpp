@@ -1,0 +1,740 @@
+// npx_engine.hpp -- host side of libnpx_hip: device buffers, scene table,
+// launch sequence of one PBF step, Akinci coupling and the C ABI.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/novaphy_hip.h"
+#include "npx_kernels.hpp"
+
+namespace npx {
+
+static const float kPI = 3.14159265358979323846f;  // math_utils.h:53
+
+#define NPX_HIP(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(NPX_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                                   \
+  } while (0)
+
+#define NPX_HIP_C(c, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return (c)->fail(NPX_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define NPX_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != NPX_OK) return r_; \
+  } while (0)
+
+template <class T>
+struct DBuf {  // plain device buffer
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t need) {
+    if (need <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc((void**)&p, std::max<size_t>(need, 1) * sizeof(T));
+    if (e == hipSuccess) n = need;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// First-fit allocator over [0, cap) in abstract units.
+struct RangeAlloc {
+  int cap = 0;
+  std::map<int, int> free_;  // start -> length
+  void init(int c) {
+    cap = c;
+    free_.clear();
+    if (c > 0) free_[0] = c;
+  }
+  int alloc(int len) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= len) {
+        int s = it->first, l = it->second;
+        free_.erase(it);
+        if (l > len) free_[s + len] = l - len;
+        return s;
+      }
+    }
+    return -1;
+  }
+  void release(int s, int len) {
+    auto it = free_.emplace(s, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+struct SceneH {
+  bool used = false;
+  npx_pbf_params prm{};
+  float dt = 0.0f, g[3] = {0, 0, 0}, mass = 0.0f;
+  int n = 0;
+  int seg_off = 0, seg_blocks = 0;
+  int cls_base = 0, ncell = 0;
+  int wlo[3] = {0, 0, 0}, wdim[3] = {0, 0, 0};
+  bool window_valid = false;
+};
+
+// Smallest float x with sqrtf(x) >= h (so for r^2 >= max(h^2, x) both poly6
+// and spiky_grad of the reference return zero).
+static float spiky_cut(float h) {
+  float x = h * h;
+  // walk down while sqrt still reaches h, then up until it does
+  while (x > 0.0f && std::sqrt(std::nextafter(x, 0.0f)) >= h) x = std::nextafter(x, 0.0f);
+  while (std::sqrt(x) < h) x = std::nextafter(x, 1e30f);
+  return x;
+}
+
+static SceneC make_scene_consts(const SceneH& H) {
+  const npx_pbf_params& P = H.prm;
+  SceneC S;
+  std::memset(&S, 0, sizeof(S));
+  const float h = P.kernel_radius;
+  S.h = h;
+  S.h2 = h * h;
+  S.inv_h = 1.0f / h;
+  const float h2 = h * h;
+  const float h9 = h2 * h2 * h2 * h2 * h;
+  const float k64pi = 64.0f * kPI;
+  S.c6 = 315.0f / (k64pi * h9);
+  const float h6 = h * h * h * h * h * h;
+  S.cs = 45.0f / (kPI * h6);
+  S.rho0 = P.rest_density;
+  S.inv_rho0 = 1.0f / P.rest_density;
+  S.mass = H.mass;
+  S.m_inv_rho0 = H.mass * S.inv_rho0;
+  S.eps = P.epsilon;
+  S.xsph_c = P.xsph_viscosity;
+  S.xsph_on = P.xsph_viscosity > 0.0f ? 1 : 0;
+  S.vort_eps = P.vorticity_epsilon;
+  S.vort_on = P.vorticity_epsilon != 0.0f ? 1 : 0;
+  S.corr_k = P.corr_k;
+  S.corr_n = P.corr_n;
+  {
+    const float dq = P.corr_delta_q * h;
+    const float r_sq = dq * dq;
+    float w = 0.0f;
+    if (!(r_sq >= h * h)) {
+      const float diff = h2 - r_sq;
+      w = S.c6 * diff * diff * diff;
+    }
+    S.wdq = w;
+  }
+  S.dt = H.dt;
+  S.inv_dt = 1.0f / H.dt;
+  S.max_speed = h / H.dt;
+  S.max_speed_sq = S.max_speed * S.max_speed;
+  S.gdt0 = H.g[0] * H.dt;
+  S.gdt1 = H.g[1] * H.dt;
+  S.gdt2 = H.g[2] * H.dt;
+  const float pr = P.particle_radius;
+  S.clo0 = P.domain_lower[0] + pr;
+  S.clo1 = P.domain_lower[1] + pr;
+  S.clo2 = P.domain_lower[2] + pr;
+  S.chi0 = P.domain_upper[0] - pr;
+  S.chi1 = P.domain_upper[1] - pr;
+  S.chi2 = P.domain_upper[2] - pr;
+  S.cut2 = std::max(S.h2, spiky_cut(h));
+  S.iters = P.solver_iterations;
+  S.bounds = P.use_domain_bounds ? 1 : 0;
+  S.wlo0 = H.wlo[0];
+  S.wlo1 = H.wlo[1];
+  S.wlo2 = H.wlo[2];
+  S.wx = H.wdim[0];
+  S.wy = H.wdim[1];
+  S.wz = H.wdim[2];
+  S.cls_base = H.cls_base;
+  S.ncell = H.ncell;
+  S.seg_off = H.seg_off;
+  S.n = H.n;
+  S.seg_blocks = H.seg_blocks;
+  return S;
+}
+
+static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+struct KTimer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t flags = 0;
+  bool fast = false;
+  bool batch = false;
+  std::string err;
+
+  int slot_cap = 0;   // slots (multiple of NPX_BLOCK)
+  int cls_cap = 0;    // classes
+  int max_scenes = 1;
+  RangeAlloc slot_alloc;  // in blocks
+  RangeAlloc cls_alloc;   // in cells
+  std::vector<SceneH> scenes;
+  int hw_blocks = 0;
+  bool table_dirty = true;
+  std::vector<int> h_blk_scene;
+
+  // particle arrays (slot order)
+  DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM;
+  DBuf<float> RHO, RHO2, LAM, LAM2;
+  DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
+  // class tables
+  DBuf<int> clsmap, cstart, bsum;
+  DBuf<unsigned long long> htk;
+  DBuf<int> htv;
+  DBuf<SceneC> d_scenes;
+  DBuf<int> d_blk_scene;
+  DBuf<int> d_flags;
+  int* h_flags = nullptr;  // pinned
+  int xs_final = 0;
+
+  // boundary (single-scene contexts)
+  int nb = 0, nbodies = 0;
+  DBuf<float4> BL, BP, BPS, FP, FS, FB;
+  DBuf<int> BODY, BODY_PTR, BODY_IDX;
+  DBuf<float> PSI;
+  DBuf<float> POSE;
+  DBuf<uint8_t> STATIC;
+  DBuf<float> OUT6;
+  DBuf<int> Acstart, ACLS, AOFF, ATPERM, ATKEY, APERM;
+  DBuf<int> aux_clsmap;
+
+  // timing
+  bool timing = false;
+  std::vector<KTimer> timers;
+  std::vector<hipEvent_t> ev_pool;
+
+  int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    err = buf;
+    return code;
+  }
+
+  hipEvent_t get_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    return e;
+  }
+
+  template <class K, class... A>
+  int launch(const char* name, K kern, int blocks, int threads, A... args) {
+    if (blocks <= 0) return NPX_OK;
+    KTimer t;
+    if (timing) {
+      t.name = name;
+      t.a = get_event();
+      t.b = get_event();
+      NPX_HIP(hipEventRecord(t.a, stream));
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, stream, args...);
+    NPX_HIP(hipGetLastError());
+    if (timing) {
+      NPX_HIP(hipEventRecord(t.b, stream));
+      timers.push_back(t);
+    }
+    return NPX_OK;
+  }
+
+  // ---------------------------------------------------------------- setup
+  int init(int slots, int cells, int nscenes, uint32_t fl, bool is_batch) {
+    flags = fl;
+    fast = (fl & NPX_FLAG_FAST_MATH) != 0;
+    batch = is_batch;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NPX_EDEVICE, "no HIP device available");
+    NPX_HIP(hipGetDevice(&device));
+    NPX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    max_scenes = std::max(1, nscenes);
+    scenes.assign(max_scenes, SceneH());
+    NPX_HIP(d_scenes.ensure(max_scenes));
+    NPX_HIP(d_flags.ensure(1));
+    NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int), hipHostMallocDefault));
+    NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int), stream));
+    NPX_TRY(reserve_slots(slots));
+    cls_cap = 0;
+    if (cells > 0) NPX_TRY(reserve_cells(cells));
+    slot_alloc.init(slot_cap / NPX_BLOCK);
+    cls_alloc.init(cls_cap);
+    return NPX_OK;
+  }
+
+  int reserve_slots(int slots) {
+    int cap = cdiv(std::max(slots, 1), NPX_BLOCK) * NPX_BLOCK;
+    if (cap <= slot_cap) return NPX_OK;
+    if (slot_cap > 0 && batch) return fail(NPX_ENOMEM, "batch slot capacity is fixed at creation");
+    size_t n = (size_t)cap;
+    NPX_HIP(X.ensure(n));
+    NPX_HIP(X2.ensure(n));
+    NPX_HIP(V.ensure(n));
+    NPX_HIP(V2.ensure(n));
+    NPX_HIP(XS0.ensure(n));
+    NPX_HIP(XS1.ensure(n));
+    NPX_HIP(VT.ensure(n));
+    NPX_HIP(OM.ensure(n));
+    NPX_HIP(RHO.ensure(n));
+    NPX_HIP(RHO2.ensure(n));
+    NPX_HIP(LAM.ensure(n));
+    NPX_HIP(LAM2.ensure(n));
+    NPX_HIP(ID.ensure(n));
+    NPX_HIP(ID2.ensure(n));
+    NPX_HIP(CELL.ensure(n));
+    NPX_HIP(CELL2.ensure(n));
+    NPX_HIP(CLS.ensure(n));
+    NPX_HIP(OFF.ensure(n));
+    NPX_HIP(TPERM.ensure(n));
+    NPX_HIP(TKEY.ensure(n));
+    NPX_HIP(PERM.ensure(n));
+    NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK));
+    slot_cap = cap;
+    h_blk_scene.assign(slot_cap / NPX_BLOCK, -1);
+    return NPX_OK;
+  }
+
+  int reserve_cells(int cells) {
+    if (cells <= cls_cap) return NPX_OK;
+    NPX_HIP(clsmap.ensure((size_t)cells));
+    NPX_HIP(cstart.ensure((size_t)cells + 1));
+    NPX_HIP(bsum.ensure((size_t)cdiv(cells + 1, 1024) + 1));
+    cls_cap = cells;
+    return NPX_OK;
+  }
+
+  int scan(int* data, int n) {
+    const int nb = cdiv(n, 1024);
+    NPX_HIP(bsum.ensure((size_t)nb + 1));
+    NPX_TRY(launch("scan_reduce", k_scan_reduce, nb, 256, (const int*)data, n, bsum.p));
+    NPX_TRY(launch("scan_bsum", k_scan_bsum, 1, 1024, bsum.p, nb));
+    NPX_TRY(launch("scan_down", k_scan_down, nb, 256, data, n, (const int*)bsum.p));
+    return NPX_OK;
+  }
+
+  // Builds the class map of a window into out[0..ncell) with values cls_base + class.
+  int build_class_table(const int wlo[3], const int wdim[3], int cls_base, int* out) {
+    const long long ncell = (long long)wdim[0] * wdim[1] * wdim[2];
+    unsigned cap = 1;
+    while ((long long)cap < 2 * ncell) cap <<= 1;
+    NPX_HIP(htk.ensure(cap));
+    NPX_HIP(htv.ensure(cap));
+    const int nc = (int)ncell;
+    NPX_TRY(launch("ht_clear", k_ht_clear, cdiv(cap, 256), 256, htk.p, htv.p, (int)cap));
+    NPX_TRY(launch("ht_insert", k_ht_insert, cdiv(nc, 256), 256, htk.p, htv.p, cap - 1, wlo[0], wlo[1], wlo[2],
+                   wdim[0], wdim[1], nc));
+    NPX_TRY(launch("ht_lookup", k_ht_lookup, cdiv(nc, 256), 256, (const unsigned long long*)htk.p,
+                   (const int*)htv.p, cap - 1, wlo[0], wlo[1], wlo[2], wdim[0], wdim[1], nc, cls_base, out));
+    return NPX_OK;
+  }
+
+  static void window_from_bounds(float inv_h, const float lo[3], const float hi[3], int margin, int wlo[3],
+                                 int wdim[3]) {
+    for (int d = 0; d < 3; ++d) {
+      int a = (int)std::floor(lo[d] * inv_h) - margin;
+      int b = (int)std::floor(hi[d] * inv_h) + margin;
+      wlo[d] = a;
+      wdim[d] = b - a + 1;
+    }
+  }
+
+  int upload_tables() {
+    if (!table_dirty) return NPX_OK;
+    std::vector<SceneC> sc(max_scenes);
+    for (int s = 0; s < max_scenes; ++s) {
+      if (scenes[s].used)
+        sc[s] = make_scene_consts(scenes[s]);
+      else
+        std::memset(&sc[s], 0, sizeof(SceneC));
+    }
+    NPX_HIP(hipMemcpyAsync(d_scenes.p, sc.data(), sizeof(SceneC) * max_scenes, hipMemcpyHostToDevice, stream));
+    NPX_HIP(hipMemcpyAsync(d_blk_scene.p, h_blk_scene.data(), sizeof(int) * h_blk_scene.size(),
+                           hipMemcpyHostToDevice, stream));
+    // the host vectors must stay alive until the copies are done
+    NPX_HIP(hipStreamSynchronize(stream));
+    hw_blocks = 0;
+    for (int b = 0; b < (int)h_blk_scene.size(); ++b)
+      if (h_blk_scene[b] >= 0) hw_blocks = b + 1;
+    table_dirty = false;
+    return NPX_OK;
+  }
+
+  // --------------------------------------------------------- scenes
+  int scene_window(SceneH& H) {
+    if (!H.prm.use_domain_bounds) return NPX_OK;  // set per step from the particle bounds
+    int wlo[3], wdim[3];
+    window_from_bounds(1.0f / H.prm.kernel_radius, H.prm.domain_lower, H.prm.domain_upper, NPX_WINDOW_MARGIN,
+                       wlo, wdim);
+    return set_window(H, wlo, wdim);
+  }
+
+  int set_window(SceneH& H, const int wlo[3], const int wdim[3]) {
+    const long long nc = (long long)wdim[0] * wdim[1] * wdim[2];
+    if (wdim[0] < 3 || wdim[1] < 3 || wdim[2] < 3 || nc > (1ll << 30))
+      return fail(NPX_EDOMAIN, "cell window %dx%dx%d is invalid or too large", wdim[0], wdim[1], wdim[2]);
+    if (H.window_valid && H.ncell == nc && std::equal(wlo, wlo + 3, H.wlo) && std::equal(wdim, wdim + 3, H.wdim))
+      return NPX_OK;
+    if (!batch) {
+      // single scene: the class range is the whole table
+      NPX_TRY(reserve_cells((int)nc));
+      H.cls_base = 0;
+    } else {
+      if (H.window_valid) cls_alloc.release(H.cls_base, H.ncell);
+      int b = cls_alloc.alloc((int)nc);
+      if (b < 0) return fail(NPX_ENOMEM, "cell capacity exhausted (%lld cells requested)", nc);
+      H.cls_base = b;
+    }
+    H.ncell = (int)nc;
+    std::copy(wlo, wlo + 3, H.wlo);
+    std::copy(wdim, wdim + 3, H.wdim);
+    H.window_valid = true;
+    table_dirty = true;
+    NPX_TRY(build_class_table(H.wlo, H.wdim, H.cls_base, clsmap.p + H.cls_base));
+    return NPX_OK;
+  }
+
+  int upload_particles(SceneH& H, const float* pos3, const float* vel3) {
+    const int n = H.n;
+    std::vector<float4> x(n), v(n);
+    std::vector<int> id(n);
+    for (int i = 0; i < n; ++i) {
+      x[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], 0.0f);
+      v[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.0f) : make_float4(0, 0, 0, 0);
+      id[i] = i;
+    }
+    const size_t o = H.seg_off;
+    if (n > 0) {
+      NPX_HIP(hipMemcpyAsync(X.p + o, x.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+      NPX_HIP(hipMemcpyAsync(V.p + o, v.data(), sizeof(float4) * n, hipMemcpyHostToDevice, stream));
+      NPX_HIP(hipMemcpyAsync(ID.p + o, id.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
+      NPX_HIP(hipMemsetAsync(RHO.p + o, 0, sizeof(float) * n, stream));
+      NPX_HIP(hipMemsetAsync(LAM.p + o, 0, sizeof(float) * n, stream));
+    }
+    NPX_HIP(hipStreamSynchronize(stream));
+    return NPX_OK;
+  }
+
+  int download(const SceneH& H, float* pos3, float* vel3, float* rho, float* lam) {
+    NPX_TRY(check_flags());
+    const int n = H.n;
+    if (n == 0) return NPX_OK;
+    std::vector<float4> x(n), v(n);
+    std::vector<float> r(n), l(n);
+    std::vector<int> id(n);
+    const size_t o = H.seg_off;
+    NPX_HIP(hipMemcpyAsync(x.data(), X.p + o, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(v.data(), V.p + o, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(r.data(), RHO.p + o, sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(l.data(), LAM.p + o, sizeof(float) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(id.data(), ID.p + o, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    for (int q = 0; q < n; ++q) {
+      const int i = id[q];
+      if (i < 0 || i >= n) return fail(NPX_ESTATE, "corrupt particle id table");
+      if (pos3) {
+        pos3[3 * i] = x[q].x;
+        pos3[3 * i + 1] = x[q].y;
+        pos3[3 * i + 2] = x[q].z;
+      }
+      if (vel3) {
+        vel3[3 * i] = v[q].x;
+        vel3[3 * i + 1] = v[q].y;
+        vel3[3 * i + 2] = v[q].z;
+      }
+      if (rho) rho[i] = r[q];
+      if (lam) lam[i] = l[q];
+    }
+    return NPX_OK;
+  }
+
+  int check_flags() {
+    NPX_HIP(hipMemcpyAsync(h_flags, d_flags.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    if (*h_flags & NPX_FLAG_WINDOW) {
+      *h_flags = 0;
+      NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int), stream));
+      return fail(NPX_EDOMAIN,
+                  "a particle left the cell window of its scene (more than %d cells outside the domain); "
+                  "results of the affected steps are invalid",
+                  NPX_WINDOW_MARGIN - 1);
+    }
+    return NPX_OK;
+  }
+
+  // Unbounded scene: window from the bounding box of x* (host round trip).
+  int fit_window_unbounded(SceneH& H) {
+    const int n = H.n;
+    std::vector<float4> x(n), v(n);
+    const size_t o = H.seg_off;
+    NPX_HIP(hipMemcpyAsync(x.data(), X.p + o, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(v.data(), V.p + o, sizeof(float4) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    const float inv_h = 1.0f / H.prm.kernel_radius;
+    const float gdt[3] = {H.g[0] * H.dt, H.g[1] * H.dt, H.g[2] * H.dt};
+    int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+    for (int i = 0; i < n; ++i) {
+      const float xs[3] = {x[i].x + (v[i].x + gdt[0]) * H.dt, x[i].y + (v[i].y + gdt[1]) * H.dt,
+                           x[i].z + (v[i].z + gdt[2]) * H.dt};
+      for (int d = 0; d < 3; ++d) {
+        const float c = std::floor(xs[d] * inv_h);
+        if (!(std::fabs(c) < 1.0e8f))
+          return fail(NPX_EDOMAIN, "predicted position is not finite or beyond the cell range");
+        lo[d] = std::min(lo[d], (int)c);
+        hi[d] = std::max(hi[d], (int)c);
+      }
+    }
+    bool ok = H.window_valid;
+    for (int d = 0; d < 3 && ok; ++d) ok = lo[d] - 1 >= H.wlo[d] && hi[d] + 1 <= H.wlo[d] + H.wdim[d] - 1;
+    if (ok) return NPX_OK;
+    int wlo[3], wdim[3];
+    for (int d = 0; d < 3; ++d) {
+      wlo[d] = lo[d] - NPX_WINDOW_MARGIN;
+      wdim[d] = (hi[d] + NPX_WINDOW_MARGIN) - wlo[d] + 1;
+    }
+    return set_window(H, wlo, wdim);
+  }
+
+  // --------------------------------------------------------- the step
+  int total_cls() const { return batch ? cls_cap : (scenes[0].used ? scenes[0].ncell : 0); }
+
+  // predict + grid: writes the cell-sorted state into the *2 buffers and x*
+  // into XS0; the caller swaps the buffers to commit the reorder.
+  int grid_pass(bool carry_rho) {
+    const int ncls = total_cls();
+    const int blocks = hw_blocks;
+    NPX_TRY(launch("fill_counts", k_fill_i32, cdiv(ncls + 1, 256), 256, cstart.p, 0, ncls + 1));
+    NPX_TRY(launch("predict_bin", k_predict_bin, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const float4*)X.p, (const float4*)V.p, (const int*)clsmap.p, cstart.p,
+                   CLS.p, OFF.p, CELL.p, d_flags.p));
+    NPX_TRY(scan(cstart.p, ncls + 1));
+    NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
+                   (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p));
+    NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
+                   (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p));
+    NPX_TRY(launch("gather", k_gather, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
+                   (const int*)PERM.p, (const float4*)X.p, (const float4*)V.p, (const int*)ID.p, (const int*)CELL.p,
+                   X2.p, V2.p, XS0.p, ID2.p, CELL2.p));
+    if (carry_rho) {
+      NPX_TRY(launch("gather_rho", k_gather_scalar2, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                     (const int*)d_blk_scene.p, (const int*)PERM.p, (const float*)RHO.p, (const float*)LAM.p, RHO2.p,
+                     LAM2.p));
+    }
+    return NPX_OK;
+  }
+
+  int prepare_step() {
+    for (int s = 0; s < max_scenes; ++s) {
+      SceneH& H = scenes[s];
+      if (!H.used || H.n == 0) continue;
+      if (!H.prm.use_domain_bounds) NPX_TRY(fit_window_unbounded(H));
+      if (!H.window_valid) return fail(NPX_ESTATE, "scene %d has no cell window", s);
+    }
+    NPX_TRY(upload_tables());
+    return NPX_OK;
+  }
+
+  int any_particles() const {
+    for (const auto& H : scenes)
+      if (H.used && H.n > 0) return 1;
+    return 0;
+  }
+
+  int step_once() {
+    if (!any_particles()) return NPX_OK;  // n == 0: PBFSolver::step returns (pbf_solver.cpp:21)
+    NPX_TRY(prepare_step());
+    int imax = 0;
+    bool any_vort = false, carry_rho = false;
+    for (const auto& H : scenes) {
+      if (!H.used || H.n == 0) continue;
+      imax = std::max(imax, H.prm.solver_iterations);
+      any_vort = any_vort || H.prm.vorticity_epsilon != 0.0f;
+      carry_rho = carry_rho || H.prm.solver_iterations <= 0;
+    }
+    NPX_TRY(grid_pass(carry_rho));
+    std::swap(X, X2);
+    std::swap(V, V2);
+    std::swap(ID, ID2);
+    std::swap(CELL, CELL2);
+    if (carry_rho) {
+      std::swap(RHO, RHO2);
+      std::swap(LAM, LAM2);
+    }
+    const int blocks = hw_blocks;
+    const SceneC* sc = d_scenes.p;
+    const int* bs = d_blk_scene.p;
+    DBuf<float4>* xs[2] = {&XS0, &XS1};
+    int cur = 0;
+    const int npass = std::max(imax, 1);
+    for (int it = 0; it < npass; ++it) {
+      if (fast)
+        NPX_TRY(launch("density_lambda", k_density_lambda<true>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it));
+      else
+        NPX_TRY(launch("density_lambda", k_density_lambda<false>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it));
+      if (fast)
+        NPX_TRY(launch("delta", k_delta<true>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
+                       (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
+                       (const float*)RHO.p, VT.p));
+      else
+        NPX_TRY(launch("delta", k_delta<false>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+                       (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it,
+                       (const float4*)X.p, (const float*)RHO.p, VT.p));
+      cur = 1 - cur;
+    }
+    if (any_vort)
+      NPX_TRY(launch("omega", k_omega, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
+                     (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p));
+    NPX_TRY(launch("xsph_commit", k_xsph_commit, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+                   (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p,
+                   (const float4*)OM.p, X.p, V.p));
+    xs_final = cur;
+    return NPX_OK;
+  }
+
+  int step_n(int nsteps) {
+    for (int k = 0; k < nsteps; ++k) NPX_TRY(step_once());
+    return NPX_OK;
+  }
+
+  // --------------------------------------------------------- diagnostics
+  int debug_grid(uint64_t* keys, int32_t* cand, int32_t* nbr) {
+    SceneH& H = scenes[0];
+    if (!H.used || H.n == 0) return NPX_OK;
+    NPX_TRY(prepare_step());
+    NPX_TRY(grid_pass(false));
+    const int n = H.n;
+    DBuf<int> dc, dn;
+    DBuf<unsigned long long> dk;
+    NPX_HIP(dc.ensure(slot_cap));
+    NPX_HIP(dn.ensure(slot_cap));
+    NPX_HIP(dk.ensure(slot_cap));
+    NPX_TRY(launch("debug_counts", k_debug_counts, hw_blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const int*)clsmap.p, (const int*)cstart.p, (const int*)CELL2.p,
+                   (const float4*)XS0.p, dc.p, dn.p, dk.p));
+    std::vector<int> hc(n), hn(n), hid(n);
+    std::vector<unsigned long long> hk(n);
+    NPX_HIP(hipMemcpyAsync(hc.data(), dc.p, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(hn.data(), dn.p, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(hk.data(), dk.p, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemcpyAsync(hid.data(), ID2.p, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    dc.release();
+    dn.release();
+    dk.release();
+    NPX_TRY(check_flags());
+    for (int q = 0; q < n; ++q) {
+      const int i = hid[q];
+      if (keys) keys[i] = hk[q];
+      if (cand) cand[i] = hc[q];
+      if (nbr) nbr[i] = hn[q];
+    }
+    return NPX_OK;
+  }
+
+  int debug_candidates(int32_t* lists, int64_t cap, int64_t* total) {
+    SceneH& H = scenes[0];
+    *total = 0;
+    if (!H.used || H.n == 0) return NPX_OK;
+    const int n = H.n;
+    std::vector<int32_t> cnt(n);
+    std::vector<uint64_t> keys(n);
+    NPX_TRY(debug_grid(keys.data(), cnt.data(), nullptr));  // grid state stays in the *2 buffers
+    std::vector<long long> off_id(n + 1, 0);
+    for (int i = 0; i < n; ++i) off_id[i + 1] = off_id[i] + cnt[i];
+    *total = off_id[n];
+    if (!lists || cap < *total) return NPX_OK;
+    std::vector<int> hid(n);
+    NPX_HIP(hipMemcpy(hid.data(), ID2.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<long long> off_slot(n);
+    for (int q = 0; q < n; ++q) off_slot[q] = off_id[hid[q]];
+    DBuf<long long> doff;
+    DBuf<int> dout;
+    NPX_HIP(doff.ensure(n));
+    NPX_HIP(dout.ensure((size_t)std::max<long long>(*total, 1)));
+    NPX_HIP(hipMemcpy(doff.p, off_slot.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    NPX_TRY(launch("debug_lists", k_debug_lists, hw_blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const int*)clsmap.p, (const int*)cstart.p, (const int*)CELL2.p,
+                   (const int*)ID2.p, (const long long*)doff.p, dout.p));
+    NPX_HIP(hipMemcpyAsync(lists, dout.p, sizeof(int) * (*total), hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    doff.release();
+    dout.release();
+    return NPX_OK;
+  }
+
+  // --------------------------------------------------------- aux grids
+  // Sorts n float4 points (payload in w) into the classes of window W
+  // (clsmap values are class ids relative to cls_base), ordering each class
+  // by key (NULL = point index).  Results: Acstart (ncls+1), APERM, PS.
+  int aux_sort(const GridWin& W, const int* cm, int cls_base, const float4* P, int n, const int* key, float4* PS) {
+    NPX_HIP(Acstart.ensure((size_t)W.ncell + 1));
+    NPX_HIP(ACLS.ensure(std::max(n, 1)));
+    NPX_HIP(AOFF.ensure(std::max(n, 1)));
+    NPX_HIP(ATPERM.ensure(std::max(n, 1)));
+    NPX_HIP(ATKEY.ensure(std::max(n, 1)));
+    NPX_HIP(APERM.ensure(std::max(n, 1)));
+    int* cst = Acstart.p - cls_base;  // indexed by class id
+    NPX_TRY(launch("aux_fill", k_fill_i32, cdiv(W.ncell + 1, 256), 256, Acstart.p, 0, W.ncell + 1));
+    NPX_TRY(launch("aux_bin", k_aux_bin, cdiv(n, 256), 256, W, P, n, cm, cst, ACLS.p, AOFF.p));
+    NPX_TRY(scan(Acstart.p, W.ncell + 1));
+    NPX_TRY(launch("aux_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)ACLS.p,
+                   (const int*)AOFF.p, key, ATPERM.p, ATKEY.p));
+    NPX_TRY(launch("aux_rank", k_aux_rank, cdiv(n, 256), 256, n, W.ncell + cls_base, (const int*)cst,
+                   (const int*)ACLS.p, (const int*)ATPERM.p, (const int*)ATKEY.p, P, APERM.p, PS));
+    return NPX_OK;
+  }
+
+  GridWin scene_win(const SceneH& H) const {
+    GridWin W;
+    W.inv_h = 1.0f / H.prm.kernel_radius;
+    W.wlo0 = H.wlo[0];
+    W.wlo1 = H.wlo[1];
+    W.wlo2 = H.wlo[2];
+    W.wx = H.wdim[0];
+    W.wy = H.wdim[1];
+    W.wz = H.wdim[2];
+    W.ncell = H.ncell;
+    return W;
+  }
+};
+
+}  // namespace npx

@@ -1,0 +1,774 @@
+// npx_kernels.hpp -- HIP kernels of the PBF fluid step for gfx950.
+//
+// Layout in HBM (one set of arrays for every scene of a context):
+//   particle slots  cell-sorted; scene s owns slots [seg_off, seg_off + n),
+//                   segments aligned to NPX_BLOCK so a block never straddles
+//                   two scenes and its scene constants are wave-uniform.
+//   X, V            float4 committed position / velocity (w unused)
+//   XS[2]           float4 predicted position x* (w = lambda), ping-pong
+//   RHO             float density; VT float4 (v after the speed clamp, w = m/rho~)
+//   OM              float4 vorticity (xyz) and |omega| (w)
+//   ID / CELL       original particle index / local window cell of x*_0
+//   class tables    clsmap[window cell] -> class; cstart[class] exclusive prefix
+//                   of class populations (classes of scene s contiguous).
+#pragma once
+
+#include "npx_device.hpp"
+
+#define NPX_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+// Device error flags (bit set = condition seen).
+#define NPX_FLAG_WINDOW 1   // a particle's cell fell outside the window interior
+
+// ------------------------------------------------------------------ utils
+__global__ void k_fill_i32(int* __restrict__ p, int v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Exclusive scan of int32 in three phases (1024 elements per block).
+__global__ void __launch_bounds__(256) k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ bsum) {
+  __shared__ int red[256];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) s += in[base + k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_bsum(int* __restrict__ bsum, int nb) {
+  __shared__ int tmp[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = (i < nb) ? bsum[i] : 0;
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int t = ((int)threadIdx.x >= o) ? tmp[threadIdx.x - o] : 0;
+      __syncthreads();
+      tmp[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int incl = tmp[threadIdx.x];
+    if (i < nb) bsum[i] = carry + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += incl;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan_down(int* __restrict__ data, int n, const int* __restrict__ bsum) {
+  __shared__ int tmp[256];
+  const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+  int v[4];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = (base + k < n) ? data[base + k] : 0;
+    s += v[k];
+  }
+  tmp[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    int t = ((int)threadIdx.x >= o) ? tmp[threadIdx.x - o] : 0;
+    __syncthreads();
+    tmp[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = bsum[blockIdx.x] + tmp[threadIdx.x] - s;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) data[base + k] = run;
+    run += v[k];
+  }
+}
+
+// ------------------------------------------------------ class tables
+// Cells of a window whose Teschner keys are equal share one class (the
+// reference's hash bucket, neighbor_search.cpp:14-22); class = smallest local
+// cell index carrying that key.  Built once per window with an open-addressing
+// table; atomicMin makes the result independent of insertion order.
+__global__ void k_ht_clear(unsigned long long* __restrict__ tk, int* __restrict__ tv, int cap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cap) {
+    tk[i] = NPX_EMPTY_KEY;
+    tv[i] = 0x7FFFFFFF;
+  }
+}
+
+__device__ __forceinline__ unsigned ht_slot(uint64_t key, unsigned mask) {
+  return (unsigned)((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+}
+
+__global__ void k_ht_insert(unsigned long long* __restrict__ tk, int* __restrict__ tv, unsigned mask, int wlo0,
+                            int wlo1, int wlo2, int wx, int wy, int ncell) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= ncell) return;
+  int lx = l % wx, t = l / wx, ly = t % wy, lz = t / wy;
+  uint64_t key = teschner_key(wlo0 + lx, wlo1 + ly, wlo2 + lz);
+  if (key == NPX_EMPTY_KEY) return;  // cannot be stored; such a cell is its own class
+  unsigned s = ht_slot(key, mask);
+  for (unsigned probe = 0; probe <= mask; ++probe) {
+    unsigned long long prev = atomicCAS(&tk[s], NPX_EMPTY_KEY, (unsigned long long)key);
+    if (prev == NPX_EMPTY_KEY || prev == key) {
+      atomicMin(&tv[s], l);
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+__global__ void k_ht_lookup(const unsigned long long* __restrict__ tk, const int* __restrict__ tv, unsigned mask,
+                            int wlo0, int wlo1, int wlo2, int wx, int wy, int ncell, int cls_base,
+                            int* __restrict__ clsmap) {
+  int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= ncell) return;
+  int lx = l % wx, t = l / wx, ly = t % wy, lz = t / wy;
+  uint64_t key = teschner_key(wlo0 + lx, wlo1 + ly, wlo2 + lz);
+  int cls = l;
+  if (key != NPX_EMPTY_KEY) {
+    unsigned s = ht_slot(key, mask);
+    for (unsigned probe = 0; probe <= mask; ++probe) {
+      if (tk[s] == key) {
+        cls = tv[s];
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+  clsmap[l] = cls_base + cls;
+}
+
+// ------------------------------------------------------ fluid sort
+// Pass 1 (PBFSolver::step predict, pbf_solver.cpp:28-34, and grid build
+// :37-40): v' = v + g*dt, x* = x + v'*dt, cell of x*, class, and the slot
+// offset inside the class from an atomic counter.  Nothing is written back to
+// X / V: the gather pass recomputes v' and x* with the same operations.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
+              const float4* __restrict__ V, const int* __restrict__ clsmap, int* __restrict__ ccount,
+              int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of, int* __restrict__ flags) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const float4 x = X[q];
+  const float4 v = V[q];
+  const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
+  const float sx = x.x + vx * S.dt, sy = x.y + vy * S.dt, sz = x.z + vz * S.dt;
+  int lx = cell_coord(sx, S.inv_h) - S.wlo0;
+  int ly = cell_coord(sy, S.inv_h) - S.wlo1;
+  int lz = cell_coord(sz, S.inv_h) - S.wlo2;
+  // The 27-cell neighbourhood must lie inside the window; outside it the
+  // class table is undefined, so the step is flagged as failed.
+  if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
+    atomicOr(flags, NPX_FLAG_WINDOW);
+    lx = min(max(lx, 1), S.wx - 2);
+    ly = min(max(ly, 1), S.wy - 2);
+    lz = min(max(lz, 1), S.wz - 2);
+  }
+  const int l = lx + S.wx * (ly + S.wy * lz);
+  const int c = clsmap[S.cls_base + l];
+  cls_of[q] = c;
+  cell_of[q] = l;
+  off_of[q] = atomicAdd(&ccount[c], 1);
+}
+
+// Pass 2: provisional slot = class start + atomic offset.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
+          const int* __restrict__ cls_of, const int* __restrict__ off_of, const int* __restrict__ id,
+          int* __restrict__ tperm, int* __restrict__ tkey) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int t = cstart[cls_of[q]] - shift + off_of[q];
+  tperm[t] = q;
+  tkey[t] = id[q];
+}
+
+// Pass 3: final slot = class start + rank of the particle's original index
+// inside its class, so each bucket lists particles in ascending index like
+// the reference's push_back order (neighbor_search.cpp:18-21).  The work per
+// class is count^2 compares, a small fraction of one neighbour pass.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
+       const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
+       int* __restrict__ perm) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int t = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (t >= S.seg_off + S.n) return;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int p = tperm[t];
+  const int c = cls_of[p];
+  const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+  const int my = tkey[t];
+  int r = 0;
+  for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  perm[b + r] = p;
+}
+
+// Pass 4: reorder the state into the new slot order and form x*.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
+         const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ id,
+         const int* __restrict__ cell_of, float4* __restrict__ X2, float4* __restrict__ V2, float4* __restrict__ XS,
+         int* __restrict__ id2, int* __restrict__ cell2) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const int p = perm[q];
+  const float4 x = X[p];
+  const float4 v = V[p];
+  const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
+  X2[q] = x;
+  V2[q] = make_float4(vx, vy, vz, 0.0f);
+  XS[q] = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
+  id2[q] = id[p];
+  cell2[q] = cell_of[p];
+}
+
+// Carries density and lambda into the new slot order (only needed when a
+// scene runs zero solver iterations and keeps the previous values).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
+                 const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ A2,
+                 float* __restrict__ B2) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const int p = perm[q];
+  A2[q] = A[p];
+  B2[q] = B[p];
+}
+
+// ------------------------------------------------------ PBF passes
+#define NPX_SCENE_PROLOGUE()                                 \
+  const int s = blk_scene[blockIdx.x];                       \
+  if (s < 0) return;                                         \
+  const SceneC& S = scenes[s];                               \
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;        \
+  if (q >= S.seg_off + S.n) return;                          \
+  const int shift = cstart[S.cls_base] - S.seg_off;          \
+  const int wxy = S.wx * S.wy;                               \
+  const int* __restrict__ cm = clsmap + S.cls_base;          \
+  const int l = cell[q];
+
+// compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
+// (:144-179): one sweep accumulates rho_i over all candidates (self and
+// duplicates included) and, for j != i, sum |grad_j|^2 and grad_i; lambda is
+// then 0 when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).
+template <bool FAST>
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                 const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter) {
+  NPX_SCENE_PROLOGUE();
+  if (iter >= S.iters) return;
+  const float4 xi = XS[q];
+  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
+  const float cut = FAST ? h2 : S.cut2;
+  float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    const float4 xj = XS[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    float r_sq;
+    if (FAST)
+      r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+    else
+      r_sq = sqn3(rx, ry, rz);
+    if (r_sq >= cut) return;  // poly6 and spiky_grad both vanish: adding 0 changes nothing
+    rho += m * poly6(r_sq, h2, c6);
+    if (j == q) return;
+    bool in;
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
+    if (!in) return;
+    const float ax = mr * (co * rx), ay = mr * (co * ry), az = mr * (co * rz);
+    sgs += FAST ? __builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az)) : sqn3(ax, ay, az);
+    gx += ax;
+    gy += ay;
+    gz += az;
+  });
+  RHO[q] = rho;
+  const float Ci = rho * S.inv_rho0 - 1.0f;
+  float lam = 0.0f;
+  if (!(Ci < 0.0f)) {
+    sgs += sqn3(gx, gy, gz);
+    lam = -Ci / (sgs + S.eps);
+  }
+  XS[q].w = lam;
+  LAM[q] = lam;
+}
+
+// Velocity update with the h/dt speed clamp (pbf_solver.cpp:81-93); VT.w
+// holds m / rho~ (rho~ = rho if > 1e-6 else rho0) for the XSPH sum.
+__device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px, float py, float pz,
+                                                const float4* __restrict__ X, const float* __restrict__ RHO,
+                                                float4* __restrict__ VT) {
+  const float4 x = X[q];
+  float vx = (px - x.x) * S.inv_dt, vy = (py - x.y) * S.inv_dt, vz = (pz - x.z) * S.inv_dt;
+  const float sp = sqn3(vx, vy, vz);
+  if (sp > S.max_speed_sq) {
+    const float f = S.max_speed / sqrtf(sp);
+    vx = vx * f;
+    vy = vy * f;
+    vz = vz * f;
+  }
+  const float r = RHO[q];
+  const float rt = (r > 1e-6f) ? r : S.rho0;
+  VT[q] = make_float4(vx, vy, vz, S.mass / rt);
+}
+
+// compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
+// Jacobi apply (:71-77) into the other x* buffer.  On the last iteration it
+// also performs the velocity update with the h/dt speed clamp (:81-93) and
+// stores v in VT with w = m / rho~ (rho~ = rho if > 1e-6 else rho0) for XSPH.
+template <bool FAST>
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+        const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+        float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
+        float4* __restrict__ VT) {
+  NPX_SCENE_PROLOGUE();
+  const float4 xi = XS[q];
+  const bool last = (iter == max(S.iters, 1) - 1);
+  if (iter >= S.iters) {
+    // scenes with fewer iterations than the batch maximum pass x* through so
+    // every scene ends in the same ping-pong buffer
+    XSn[q] = xi;
+    if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
+    return;
+  }
+  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
+  const float wdq = S.wdq, negk = -S.corr_k;
+  const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
+  const float cut = FAST ? h2 : S.cut2;
+  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    if (j == q) return;
+    const float4 xj = XS[j];
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    float r_sq;
+    if (FAST)
+      r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+    else
+      r_sq = sqn3(rx, ry, rz);
+    if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
+    bool in;
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
+    if (!in) return;
+    const float ls = xi.w + xj.w;
+    float sc = 0.0f;
+    if (ls < -1e-10f && scorr_on) {
+      float ratio = poly6(r_sq, h2, c6) / wdq;
+      ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
+      float rp = ratio * ratio;
+      if (n4) rp = rp * rp;
+      sc = negk * rp;
+    }
+    const float coef = (ls + sc) * m;
+    if (FAST) {
+      const float cc = coef * co;
+      dx = __builtin_fmaf(cc, rx, dx);
+      dy = __builtin_fmaf(cc, ry, dy);
+      dz = __builtin_fmaf(cc, rz, dz);
+    } else {
+      dx += coef * (co * rx);
+      dy += coef * (co * ry);
+      dz += coef * (co * rz);
+    }
+  });
+  const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
+  XSn[q] = make_float4(px, py, pz, 0.0f);
+  if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+}
+
+// Vorticity (this project's definition; the reference declares
+// vorticity_epsilon but never uses it): omega_i = sum_{j!=i} (m/rho~_j)
+// (v_j - v_i) x gradW(x*_j - x*_i), stored with |omega_i| in w.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_omega(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+        const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+        const float4* __restrict__ VT, float4* __restrict__ OM) {
+  NPX_SCENE_PROLOGUE();
+  const float4 xi = XS[q];
+  const float4 vi = VT[q];
+  const float h = S.h, cs = S.cs;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    if (j == q) return;
+    const float4 xj = XS[j];
+    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+    const float r_sq = sqn3(rx, ry, rz);
+    bool in;
+    const float co = spiky_coeff(r_sq, h, cs, &in);
+    if (!in) return;
+    const float4 vj = VT[j];
+    const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
+    const float gx = co * rx, gy = co * ry, gz = co * rz;
+    const float cx = ay * gz - az * gy, cy = az * gx - ax * gz, cz = ax * gy - ay * gx;
+    wx += vj.w * cx;
+    wy += vj.w * cy;
+    wz += vj.w * cz;
+  });
+  OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
+}
+
+// apply_xsph_viscosity (pbf_solver.cpp:225-250) + vorticity force + commit
+// with the domain clamp (:102-126), writing the new committed X / V.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_xsph_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+              const float4* __restrict__ VT, const float4* __restrict__ OM, float4* __restrict__ X,
+              float4* __restrict__ V) {
+  NPX_SCENE_PROLOGUE();
+  const float4 xi = XS[q];
+  const float4 vi = VT[q];
+  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs;
+  float vx = vi.x, vy = vi.y, vz = vi.z;
+  if (S.xsph_on) {
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+      if (j == q) return;
+      const float4 xj = XS[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r_sq = sqn3(rx, ry, rz);
+      if (r_sq >= h2) return;
+      const float w = poly6(r_sq, h2, c6);
+      const float4 vj = VT[j];
+      cx += (vj.w * (vj.x - vi.x)) * w;
+      cy += (vj.w * (vj.y - vi.y)) * w;
+      cz += (vj.w * (vj.z - vi.z)) * w;
+    });
+    vx = vx + S.xsph_c * cx;
+    vy = vy + S.xsph_c * cy;
+    vz = vz + S.xsph_c * cz;
+  }
+  if (S.vort_on) {
+    const float4 oi = OM[q];
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+      if (j == q) return;
+      const float4 xj = XS[j];
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r_sq = sqn3(rx, ry, rz);
+      bool in;
+      const float co = spiky_coeff(r_sq, h, cs, &in);
+      if (!in) return;
+      const float4 vj = VT[j];
+      const float4 oj = OM[j];
+      const float sc = vj.w * (oj.w - oi.w);
+      ex += sc * (co * rx);
+      ey += sc * (co * ry);
+      ez += sc * (co * rz);
+    });
+    const float el = sqrtf(sqn3(ex, ey, ez));
+    if (el > 1e-9f) {
+      const float inv = 1.0f / el;
+      const float nx = ex * inv, ny = ey * inv, nz = ez * inv;
+      const float fx = S.vort_eps * (ny * oi.z - nz * oi.y);
+      const float fy = S.vort_eps * (nz * oi.x - nx * oi.z);
+      const float fz = S.vort_eps * (nx * oi.y - ny * oi.x);
+      vx = vx + S.dt * fx;
+      vy = vy + S.dt * fy;
+      vz = vz + S.dt * fz;
+    }
+  }
+  float px = xi.x, py = xi.y, pz = xi.z;
+  if (S.bounds) {
+    if (px < S.clo0) { px = S.clo0; vx = 0.0f; } else if (px > S.chi0) { px = S.chi0; vx = 0.0f; }
+    if (py < S.clo1) { py = S.clo1; vy = 0.0f; } else if (py > S.chi1) { py = S.chi1; vy = 0.0f; }
+    if (pz < S.clo2) { pz = S.clo2; vz = 0.0f; } else if (pz > S.chi2) { pz = S.chi2; vz = 0.0f; }
+  }
+  X[q] = make_float4(px, py, pz, 0.0f);
+  V[q] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// Grid diagnostics on x*_0: candidate count (duplicates kept) and in-range
+// count (j != i, r^2 < h^2), plus the Teschner key of the particle's cell.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_debug_counts(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+               int* __restrict__ cand, int* __restrict__ nbr, unsigned long long* __restrict__ keys) {
+  NPX_SCENE_PROLOGUE();
+  const float4 xi = XS[q];
+  int nc = 0, nn = 0;
+  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    ++nc;
+    if (j == q) return;
+    const float4 xj = XS[j];
+    if (sqn3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) < S.h2) ++nn;
+  });
+  cand[q] = nc;
+  nbr[q] = nn;
+  const int lx = l % S.wx, t = l / S.wx, ly = t % S.wy, lz = t / S.wy;
+  keys[q] = teschner_key(S.wlo0 + lx, S.wlo1 + ly, S.wlo2 + lz);
+}
+
+// Writes each particle's candidate list (as original indices) at out + off[q].
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_debug_lists(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+              const int* __restrict__ cstart, const int* __restrict__ cell, const int* __restrict__ id,
+              const long long* __restrict__ off, int* __restrict__ out) {
+  NPX_SCENE_PROLOGUE();
+  long long o = off[q];
+  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { out[o++] = id[j]; });
+}
+
+// ------------------------------------------------------ auxiliary grids
+// Boundary particles and committed fluid positions get their own
+// SpatialHashGrid in the reference (fluid_world.cpp:106-107, 134-135,
+// boundary.cpp:162-163).  Points outside the window interior are left out
+// (cls = -1): no point inside the window can have them within h.
+__global__ void k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const int* __restrict__ clsmap,
+                          int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = P[i];
+  int lx = cell_coord(p.x, W.inv_h) - W.wlo0;
+  int ly = cell_coord(p.y, W.inv_h) - W.wlo1;
+  int lz = cell_coord(p.z, W.inv_h) - W.wlo2;
+  if (lx < 1 || ly < 1 || lz < 1 || lx > W.wx - 2 || ly > W.wy - 2 || lz > W.wz - 2) {
+    cls_of[i] = -1;
+    return;
+  }
+  const int c = clsmap[lx + W.wx * (ly + W.wy * lz)];
+  cls_of[i] = c;
+  off_of[i] = atomicAdd(&ccount[c], 1);
+}
+
+__global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* __restrict__ cls_of,
+                              const int* __restrict__ off_of, const int* __restrict__ key, int* __restrict__ tperm,
+                              int* __restrict__ tkey) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cls_of[i];
+  if (c < 0) return;
+  const int t = cstart[c] + off_of[i];
+  tperm[t] = i;
+  tkey[t] = key ? key[i] : i;
+}
+
+// Rank inside the class by key and write the sorted payload copy.
+__global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
+                           const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
+                           int* __restrict__ perm, float4* __restrict__ PS) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cstart[ncls]) return;
+  const int i = tperm[t];
+  const int c = cls_of[i];
+  const int b = cstart[c], e = cstart[c + 1];
+  const int my = tkey[t];
+  int r = 0;
+  for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  perm[b + r] = i;
+  PS[b + r] = P[i];
+}
+
+// Cell (local window index) of an arbitrary point, or -1 if its 27-cell
+// neighbourhood leaves the window.
+__device__ __forceinline__ int aux_cell(const GridWin& W, float x, float y, float z) {
+  const int lx = cell_coord(x, W.inv_h) - W.wlo0;
+  const int ly = cell_coord(y, W.inv_h) - W.wlo1;
+  const int lz = cell_coord(z, W.inv_h) - W.wlo2;
+  if (lx < 1 || ly < 1 || lz < 1 || lx > W.wx - 2 || ly > W.wy - 2 || lz > W.wz - 2) return -1;
+  return lx + W.wx * (ly + W.wy * lz);
+}
+
+// compute_boundary_volumes (boundary.cpp:152-176): psi_b = 1 / sum W, over
+// the boundary candidates of b (self and every body included).
+__global__ void k_psi(GridWin W, float h2, float c6, const float4* __restrict__ BP, int nb,
+                      const int* __restrict__ clsmap, const int* __restrict__ cstart, const float4* __restrict__ PS,
+                      float* __restrict__ psi) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const float4 xb = BP[b];
+  const int l = aux_cell(W, xb.x, xb.y, xb.z);
+  float sum = 0.0f;
+  if (l >= 0) {
+    for_each_candidate(l, W.wx, W.wx * W.wy, clsmap, cstart, 0, [&](int j) {
+      const float4 xj = PS[j];
+      sum += poly6(sqn3(xb.x - xj.x, xb.y - xj.y, xb.z - xj.z), h2, c6);
+    });
+  }
+  psi[b] = (sum > 1e-10f) ? 1.0f / sum : 0.0f;
+}
+
+// boundary_world_positions (boundary.cpp:178-192): body pose applied with
+// Eigen's quaternion*vector form; world-owned particles keep their position.
+// Output w = psi.
+__global__ void k_bpos(const float4* __restrict__ BL, const int* __restrict__ body, const float* __restrict__ psi,
+                       int nb, const float* __restrict__ pose7, int nbodies, float4* __restrict__ BP) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const float4 p = BL[b];
+  const int bi = body[b];
+  float ox = p.x, oy = p.y, oz = p.z;
+  if (bi >= 0 && bi < nbodies) {
+    const float* P = pose7 + 7 * bi;
+    const float qw = P[3], qx = P[4], qy = P[5], qz = P[6];
+    float ux = qy * p.z - qz * p.y, uy = qz * p.x - qx * p.z, uz = qx * p.y - qy * p.x;
+    ux = ux + ux;
+    uy = uy + uy;
+    uz = uz + uz;
+    const float cx = qy * uz - qz * uy, cy = qz * ux - qx * uz, cz = qx * uy - qy * ux;
+    const float rx = (p.x + qw * ux) + cx, ry = (p.y + qw * uy) + cy, rz = (p.z + qw * uz) + cz;
+    ox = P[0] + rx;
+    oy = P[1] + ry;
+    oz = P[2] + rz;
+  }
+  BP[b] = make_float4(ox, oy, oz, psi[b]);
+}
+
+// FluidWorld::apply_boundary_density (fluid_world.cpp:101-123) on the
+// committed positions: rho_i += sum_b (rho0 psi_b) W(x_i - x_b).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_boundary_density(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, GridWin W,
+                   const float4* __restrict__ X, const int* __restrict__ bclsmap, const int* __restrict__ bcstart,
+                   const float4* __restrict__ BPS, float* __restrict__ RHO, int* __restrict__ flags) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const float4 xi = X[q];
+  const int l = aux_cell(W, xi.x, xi.y, xi.z);
+  if (l < 0) {
+    atomicOr(flags, NPX_FLAG_WINDOW);
+    return;
+  }
+  float bd = 0.0f;
+  const float h2 = S.h2, c6 = S.c6, rho0 = S.rho0;
+  for_each_candidate(l, W.wx, W.wx * W.wy, bclsmap, bcstart, 0, [&](int j) {
+    const float4 xb = BPS[j];
+    bd += rho0 * xb.w * poly6(sqn3(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z), h2, c6);
+  });
+  RHO[q] = RHO[q] + bd;
+}
+
+// Payload for the committed-fluid grid: (x, rho) with key = original index.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_fluid_payload(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
+                const float* __restrict__ RHO, float4* __restrict__ P) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const float4 x = X[q];
+  P[q] = make_float4(x.x, x.y, x.z, RHO[q]);
+}
+
+// FluidWorld::apply_coupling_forces, per boundary particle
+// (fluid_world.cpp:140-160): F_b = sum_f ((m p_f / rho_f^2) rho0 psi_b) gradW.
+__global__ void k_coupling_particles(GridWin W, float h, float h2, float cs, float rho0, float mass,
+                                     const float4* __restrict__ BP, const int* __restrict__ body, int nb,
+                                     const uint8_t* __restrict__ is_static, int nbodies,
+                                     const int* __restrict__ fclsmap, const int* __restrict__ fcstart,
+                                     const float4* __restrict__ FS, float4* __restrict__ FB) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int bi = body[b];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (bi >= 0 && bi < nbodies && !is_static[bi]) {
+    const float4 xb = BP[b];
+    const float psi_b = xb.w;
+    const int l = aux_cell(W, xb.x, xb.y, xb.z);
+    if (l >= 0) {
+      for_each_candidate(l, W.wx, W.wx * W.wy, fclsmap, fcstart, 0, [&](int j) {
+        const float4 xf = FS[j];
+        const float rx = xb.x - xf.x, ry = xb.y - xf.y, rz = xb.z - xf.z;
+        const float r_sq = sqn3(rx, ry, rz);
+        if (r_sq >= h2) return;
+        const float rho_f = (xf.w < 1.0f) ? 1.0f : xf.w;
+        const float pr = rho_f - rho0;
+        const float pressure = (0.0f < pr) ? pr : 0.0f;
+        const float pc = mass * pressure / (rho_f * rho_f);
+        bool in;
+        const float co = spiky_coeff(r_sq, h, cs, &in);
+        if (!in) return;
+        const float k = pc * rho0 * psi_b;
+        fx += k * (co * rx);
+        fy += k * (co * ry);
+        fz += k * (co * rz);
+      });
+    }
+  }
+  FB[b] = make_float4(fx, fy, fz, 0.0f);
+}
+
+// Per-body sums in boundary-particle order (World::apply_force / apply_torque,
+// world.cpp:189-216): one thread per body walks its particles ascending, so
+// the float accumulation order is the reference's and the result is
+// deterministic.
+__global__ void k_body_reduce(const int* __restrict__ body_ptr, const int* __restrict__ body_idx, int nbodies,
+                              const float4* __restrict__ FB, const float4* __restrict__ BP,
+                              const float* __restrict__ pose7, float* __restrict__ out6) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbodies) return;
+  const float cxm = pose7[7 * k], cym = pose7[7 * k + 1], czm = pose7[7 * k + 2];
+  float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
+  for (int u = body_ptr[k]; u < body_ptr[k + 1]; ++u) {
+    const int b = body_idx[u];
+    const float4 f = FB[b];
+    if (sqn3(f.x, f.y, f.z) > 1e-20f) {
+      Fx += f.x;
+      Fy += f.y;
+      Fz += f.z;
+      const float4 xb = BP[b];
+      const float ax = xb.x - cxm, ay = xb.y - cym, az = xb.z - czm;
+      Tx += ay * f.z - az * f.y;
+      Ty += az * f.x - ax * f.z;
+      Tz += ax * f.y - ay * f.x;
+    }
+  }
+  out6[6 * k + 0] = Fx;
+  out6[6 * k + 1] = Fy;
+  out6[6 * k + 2] = Fz;
+  out6[6 * k + 3] = Tx;
+  out6[6 * k + 4] = Ty;
+  out6[6 * k + 5] = Tz;
+}
+
+// query_neighbors for arbitrary points (standalone SpatialHashGrid API).
+__global__ void k_query_count(GridWin W, const float4* __restrict__ Q, int nq, const int* __restrict__ clsmap,
+                              const int* __restrict__ cstart, int* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const float4 p = Q[i];
+  const int l = aux_cell(W, p.x, p.y, p.z);
+  int c = 0;
+  if (l >= 0) for_each_candidate(l, W.wx, W.wx * W.wy, clsmap, cstart, 0, [&](int) { ++c; });
+  counts[i] = c;
+}
+
+__global__ void k_query_lists(GridWin W, const float4* __restrict__ Q, int nq, const int* __restrict__ clsmap,
+                              const int* __restrict__ cstart, const int* __restrict__ perm,
+                              const long long* __restrict__ off, int* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const float4 p = Q[i];
+  const int l = aux_cell(W, p.x, p.y, p.z);
+  long long o = off[i];
+  if (l >= 0) for_each_candidate(l, W.wx, W.wx * W.wy, clsmap, cstart, 0, [&](int j) { out[o++] = perm[j]; });
+}

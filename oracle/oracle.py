@@ -1,0 +1,377 @@
+"""ctypes wrapper around liboracle (oracle/pbf_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbf_oracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("rest_density", C.c_float), ("kernel_radius", C.c_float),
+        ("particle_radius", C.c_float), ("solver_iterations", C.c_int32),
+        ("epsilon", C.c_float), ("xsph_viscosity", C.c_float),
+        ("vorticity_epsilon", C.c_float), ("corr_delta_q", C.c_float),
+        ("corr_k", C.c_float), ("corr_n", C.c_int32),
+        ("use_domain_bounds", C.c_int32),
+        ("domain_lower", C.c_float * 3), ("domain_upper", C.c_float * 3),
+    ]
+
+
+class OrcShape(C.Structure):
+    _fields_ = [("type", C.c_int32), ("body", C.c_int32), ("a", C.c_float * 4),
+                ("local_pos", C.c_float * 3), ("local_rot", C.c_float * 4)]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH) or (
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "pbf_oracle.cpp"))):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+        u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        vp = C.c_void_p
+        L.orc_poly6.restype = C.c_float
+        L.orc_poly6.argtypes = [C.c_float, C.c_float]
+        L.orc_poly6_f64.restype = C.c_double
+        L.orc_poly6_f64.argtypes = [C.c_double, C.c_double]
+        L.orc_spiky_grad.argtypes = [f32p, C.c_float, f32p]
+        L.orc_spiky.restype = C.c_float
+        L.orc_spiky.argtypes = [C.c_float, C.c_float]
+        L.orc_cubic_spline.restype = C.c_float
+        L.orc_cubic_spline.argtypes = [C.c_float, C.c_float]
+        L.orc_hash_cell.restype = C.c_uint64
+        L.orc_hash_cell.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_generate_fluid_block.restype = C.c_int32
+        L.orc_generate_fluid_block.argtypes = [f32p, f32p, C.c_float, vp]
+        L.orc_grid_debug.restype = C.c_int64
+        L.orc_grid_debug.argtypes = [C.c_int32, f32p, C.c_float, C.c_float, vp, vp, vp, vp,
+                                     C.c_int64]
+        L.orc_grid_query.restype = C.c_int64
+        L.orc_grid_query.argtypes = [C.c_int32, f32p, C.c_float, C.c_int32, f32p, vp, vp,
+                                     C.c_int64]
+        L.orc_grid_all_pairs.restype = C.c_int64
+        L.orc_grid_all_pairs.argtypes = [C.c_int32, f32p, C.c_float, C.c_float, vp, C.c_int64]
+        L.orc_pbf_run_f32.restype = C.c_int32
+        L.orc_pbf_run_f32.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p, f32p, f32p,
+                                      C.c_float, f32p, C.c_float, C.c_int32]
+        L.orc_pbf_run_f64.restype = C.c_int32
+        L.orc_pbf_run_f64.argtypes = [C.POINTER(OrcParams), C.c_int32, f64p, f64p, f64p, f64p,
+                                      C.c_double, f64p, C.c_double, C.c_int32]
+        L.orc_sim_create.restype = vp
+        L.orc_sim_create.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p]
+        L.orc_sim_step.restype = C.c_int32
+        L.orc_sim_step.argtypes = [vp, C.c_float, f32p, C.c_float, C.c_int32]
+        L.orc_sim_get.argtypes = [vp, vp, vp, vp, vp]
+        L.orc_sim_destroy.argtypes = [vp]
+        L.orc_sample_boundaries.restype = C.c_int32
+        L.orc_sample_boundaries.argtypes = [C.c_int32, C.POINTER(OrcShape), C.c_float,
+                                            C.c_float, vp, vp]
+        L.orc_boundary_world_positions.restype = C.c_int32
+        L.orc_boundary_world_positions.argtypes = [C.c_int32, f32p, i32p, C.c_int32, f32p, f32p]
+        L.orc_boundary_volumes.restype = C.c_int32
+        L.orc_boundary_volumes.argtypes = [C.c_int32, f32p, C.c_float, f32p]
+        L.orc_boundary_volumes_f64.restype = C.c_int32
+        L.orc_boundary_volumes_f64.argtypes = [C.c_int32, f64p, C.c_double, f64p]
+        L.orc_boundary_density.restype = C.c_int32
+        L.orc_boundary_density.argtypes = [C.c_int32, f32p, f32p, C.c_int32, f32p, f32p,
+                                           C.c_float, C.c_float]
+        L.orc_boundary_density_f64.restype = C.c_int32
+        L.orc_boundary_density_f64.argtypes = [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p,
+                                               C.c_double, C.c_double]
+        L.orc_coupling.restype = C.c_int32
+        L.orc_coupling.argtypes = [C.c_int32, f32p, f32p, C.c_int32, f32p, f32p, i32p, C.c_int32,
+                                   f32p, u8p, C.c_float, C.c_float, C.c_float, f32p, f32p, vp]
+        L.orc_coupling_f64.restype = C.c_int32
+        L.orc_coupling_f64.argtypes = [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p, i32p,
+                                       C.c_int32, f64p, u8p, C.c_double, C.c_double, C.c_double,
+                                       f64p, f64p]
+        L.orc_vorticity_omega.restype = C.c_int32
+        L.orc_vorticity_omega.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p, f32p,
+                                          C.c_float, f32p]
+        _ = u64p
+        _lib = L
+    return _lib
+
+
+def params(p) -> OrcParams:
+    """OrcParams from any object with PBFSettings field names."""
+    o = OrcParams()
+    o.rest_density = p.rest_density
+    o.kernel_radius = p.kernel_radius
+    o.particle_radius = p.particle_radius
+    o.solver_iterations = int(p.solver_iterations)
+    o.epsilon = p.epsilon
+    o.xsph_viscosity = p.xsph_viscosity
+    o.vorticity_epsilon = getattr(p, "vorticity_epsilon", 0.0)
+    o.corr_delta_q = getattr(p, "corr_delta_q", 0.3)
+    o.corr_k = getattr(p, "corr_k", 0.001)
+    o.corr_n = int(getattr(p, "corr_n", 4))
+    o.use_domain_bounds = 1 if p.use_domain_bounds else 0
+    for d in range(3):
+        o.domain_lower[d] = p.domain_lower[d]
+        o.domain_upper[d] = p.domain_upper[d]
+    return o
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ----------------------------------------------------------------- kernels
+def poly6(r_sq, h):
+    return lib().orc_poly6(r_sq, h)
+
+
+def poly6_f64(r_sq, h):
+    return lib().orc_poly6_f64(r_sq, h)
+
+
+def spiky_grad(r, h):
+    out = np.zeros(3, np.float32)
+    lib().orc_spiky_grad(np.ascontiguousarray(r, np.float32), h, out)
+    return out
+
+
+def spiky(r_len, h):
+    return lib().orc_spiky(r_len, h)
+
+
+def cubic_spline(r_len, h):
+    return lib().orc_cubic_spline(r_len, h)
+
+
+def hash_cell(cx, cy, cz) -> int:
+    return int(lib().orc_hash_cell(cx, cy, cz))
+
+
+def generate_fluid_block(lower, upper, spacing):
+    lo = np.ascontiguousarray(lower, np.float32)
+    hi = np.ascontiguousarray(upper, np.float32)
+    n = lib().orc_generate_fluid_block(lo, hi, spacing, None)
+    out = np.zeros((n, 3), np.float32)
+    lib().orc_generate_fluid_block(lo, hi, spacing, _ptr(out))
+    return out
+
+
+# ----------------------------------------------------------------- grid
+def grid_debug(pos, cell, radius, with_lists=False):
+    """keys, cand_cnt, nbr_cnt (and flattened candidate lists) of one build."""
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    keys = np.zeros(n, np.uint64)
+    cc = np.zeros(n, np.int32)
+    nc = np.zeros(n, np.int32)
+    total = lib().orc_grid_debug(n, pos, cell, radius, _ptr(keys), _ptr(cc), _ptr(nc), None, 0)
+    lists = None
+    if with_lists:
+        lists = np.zeros(total, np.int32)
+        lib().orc_grid_debug(n, pos, cell, radius, None, None, None, _ptr(lists), total)
+    return keys, cc, nc, lists
+
+
+def grid_query(pos, cell, queries):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 3)
+    counts = np.zeros(q.shape[0], np.int32)
+    total = lib().orc_grid_query(pos.shape[0], pos, cell, q.shape[0], q, _ptr(counts), None, 0)
+    lst = np.zeros(total, np.int32)
+    lib().orc_grid_query(pos.shape[0], pos, cell, q.shape[0], q, None, _ptr(lst), total)
+    return counts, lst
+
+
+def grid_all_pairs(pos, cell, radius):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    total = lib().orc_grid_all_pairs(pos.shape[0], pos, cell, radius, None, 0)
+    pairs = np.zeros((total, 2), np.int32)
+    lib().orc_grid_all_pairs(pos.shape[0], pos, cell, radius, _ptr(pairs), total)
+    return pairs
+
+
+# ----------------------------------------------------------------- PBF
+def pbf_run(prm, pos, vel, dt, gravity, mass, nsteps=1):
+    """nsteps of the FP32 restatement; returns (pos, vel, rho, lambda)."""
+    pos = np.array(pos, np.float32, copy=True).reshape(-1, 3)
+    vel = np.array(vel, np.float32, copy=True).reshape(-1, 3)
+    n = pos.shape[0]
+    rho = np.zeros(n, np.float32)
+    lam = np.zeros(n, np.float32)
+    op = params(prm)
+    lib().orc_pbf_run_f32(C.byref(op), n, pos, vel, rho, lam, np.float32(dt),
+                          np.ascontiguousarray(gravity, np.float32), np.float32(mass), nsteps)
+    return pos, vel, rho, lam
+
+
+def pbf_run_f64(prm, pos, vel, dt, gravity, mass, nsteps=1):
+    """The same steps in FP64 (inputs widened exactly from float32)."""
+    pos = np.array(pos, np.float64, copy=True).reshape(-1, 3)
+    vel = np.array(vel, np.float64, copy=True).reshape(-1, 3)
+    n = pos.shape[0]
+    rho = np.zeros(n, np.float64)
+    lam = np.zeros(n, np.float64)
+    op = params(prm)
+    lib().orc_pbf_run_f64(C.byref(op), n, pos, vel, rho, lam, float(np.float32(dt)),
+                          np.ascontiguousarray(np.float32(gravity), np.float64),
+                          float(np.float32(mass)), nsteps)
+    return pos, vel, rho, lam
+
+
+class Sim:
+    """Persistent FP32 restatement state (for timing)."""
+
+    def __init__(self, prm, pos, vel):
+        self._op = params(prm)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+        self.n = pos.shape[0]
+        self._h = lib().orc_sim_create(C.byref(self._op), self.n, pos, vel)
+
+    def step(self, dt, gravity, mass, nsteps=1):
+        lib().orc_sim_step(self._h, np.float32(dt), np.ascontiguousarray(gravity, np.float32),
+                           np.float32(mass), nsteps)
+
+    def get(self):
+        pos = np.zeros((self.n, 3), np.float32)
+        vel = np.zeros((self.n, 3), np.float32)
+        rho = np.zeros(self.n, np.float32)
+        lam = np.zeros(self.n, np.float32)
+        lib().orc_sim_get(self._h, _ptr(pos), _ptr(vel), _ptr(rho), _ptr(lam))
+        return pos, vel, rho, lam
+
+    def close(self):
+        if self._h:
+            lib().orc_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------- boundary
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_PLANE = 0, 1, 2
+
+
+def make_shape(kind, body, a, local_pos=(0, 0, 0), local_rot=(1, 0, 0, 0)):
+    s = OrcShape()
+    s.type = kind
+    s.body = body
+    for i, v in enumerate(a):
+        s.a[i] = v
+    for i in range(3):
+        s.local_pos[i] = local_pos[i]
+    for i in range(4):
+        s.local_rot[i] = local_rot[i]
+    return s
+
+
+def sample_boundaries(shapes, spacing, plane_extent=1.0):
+    arr = (OrcShape * len(shapes))(*shapes)
+    n = lib().orc_sample_boundaries(len(shapes), arr, spacing, plane_extent, None, None)
+    local = np.zeros((n, 3), np.float32)
+    body = np.zeros(n, np.int32)
+    lib().orc_sample_boundaries(len(shapes), arr, spacing, plane_extent, _ptr(local), _ptr(body))
+    return local, body
+
+
+def boundary_world_positions(local, body, poses7):
+    local = np.ascontiguousarray(local, np.float32).reshape(-1, 3)
+    body = np.ascontiguousarray(body, np.int32)
+    poses7 = np.ascontiguousarray(poses7, np.float32).reshape(-1, 7)
+    out = np.zeros_like(local)
+    lib().orc_boundary_world_positions(local.shape[0], local, body, poses7.shape[0],
+                                       poses7.reshape(-1) if poses7.size else np.zeros(7, np.float32),
+                                       out)
+    return out
+
+
+def boundary_volumes(wpos, h):
+    wpos = np.ascontiguousarray(wpos, np.float32).reshape(-1, 3)
+    psi = np.zeros(wpos.shape[0], np.float32)
+    lib().orc_boundary_volumes(wpos.shape[0], wpos, h, psi)
+    return psi
+
+
+def boundary_volumes_f64(wpos, h):
+    wpos = np.ascontiguousarray(wpos, np.float64).reshape(-1, 3)
+    psi = np.zeros(wpos.shape[0], np.float64)
+    lib().orc_boundary_volumes_f64(wpos.shape[0], wpos, float(np.float32(h)), psi)
+    return psi
+
+
+def boundary_density(pos, rho, bpos, psi, h, rho0):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    rho = np.array(rho, np.float32, copy=True)
+    bpos = np.ascontiguousarray(bpos, np.float32).reshape(-1, 3)
+    psi = np.ascontiguousarray(psi, np.float32)
+    lib().orc_boundary_density(pos.shape[0], pos, rho, bpos.shape[0], bpos, psi, h, rho0)
+    return rho
+
+
+def coupling(pos, rho, bpos, psi, body, poses7, is_static, h, rho0, mass, per_particle=False):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float32)
+    bpos = np.ascontiguousarray(bpos, np.float32).reshape(-1, 3)
+    psi = np.ascontiguousarray(psi, np.float32)
+    body = np.ascontiguousarray(body, np.int32)
+    poses7 = np.ascontiguousarray(poses7, np.float32).reshape(-1, 7)
+    nb = poses7.shape[0]
+    st = np.ascontiguousarray(is_static, np.uint8)
+    f = np.zeros((nb, 3), np.float32)
+    t = np.zeros((nb, 3), np.float32)
+    pp = np.zeros((bpos.shape[0], 3), np.float32) if per_particle else None
+    lib().orc_coupling(pos.shape[0], pos, rho, bpos.shape[0], bpos, psi, body, nb,
+                       poses7.reshape(-1), st, h, rho0, mass, f.reshape(-1), t.reshape(-1), _ptr(pp))
+    return (f, t, pp) if per_particle else (f, t)
+
+
+def coupling_f64(pos, rho, bpos, psi, body, poses7, is_static, h, rho0, mass):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float64)
+    bpos = np.ascontiguousarray(bpos, np.float64).reshape(-1, 3)
+    psi = np.ascontiguousarray(psi, np.float64)
+    body = np.ascontiguousarray(body, np.int32)
+    poses7 = np.ascontiguousarray(poses7, np.float64).reshape(-1, 7)
+    nb = poses7.shape[0]
+    st = np.ascontiguousarray(is_static, np.uint8)
+    f = np.zeros((nb, 3), np.float64)
+    t = np.zeros((nb, 3), np.float64)
+    lib().orc_coupling_f64(pos.shape[0], pos, rho, bpos.shape[0], bpos, psi, body, nb,
+                           poses7.reshape(-1), st, float(np.float32(h)), float(np.float32(rho0)),
+                           float(np.float32(mass)), f.reshape(-1), t.reshape(-1))
+    return f, t
+
+
+def vorticity_omega(prm, pos, vel, rho, mass):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float32)
+    om = np.zeros_like(pos)
+    op = params(prm)
+    lib().orc_vorticity_omega(C.byref(op), pos.shape[0], pos, vel, rho, np.float32(mass), om)
+    return om
