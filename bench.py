@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X PBF step (one JSON line on stdout).
+
+Default workload: the single large dam break -- 100x100x100 = 1M FP32
+particles, r = 0.025 m, h = 0.1 m, rest density 1000, 10x5x5 m box (the
+100-particle lattice spans 4.95 m, so the box keeps the 2:1:1 aspect at
+10x5x5 m), seeded +-1 mm jitter, dt = 1/120 s, 4 iterations, XSPH c = 0.01,
+vorticity confinement eps = 1e-4, s_corr k = 0.1, n = 4, dq = 0.2h.
+
+A step is one PBFSolver::step over all particles (predict, grid, 4 x
+(density+lambda, delta-p), velocity update, vorticity, XSPH, commit).  Inputs
+are resident on the device before the timed region.  The timed region is
+bracketed by device synchronisation on both sides; per-kernel durations come
+from HIP events recorded on the context's stream during the same region.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PBF particle-steps/s at 1M + % HBM roofline; batched p50/p99 step latency"
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic (compulsory) FP32/int32 bytes per particle for each pass; a
+# step is 161 + 48*I bytes (+48 with vorticity): predict+key 40, sort 13,
+# reorder 56, density+lambda 20 per iteration, delta-p+apply 28 per
+# iteration, velocity+XSPH+clamp+commit 52, vorticity omega pass 36 (+12 read
+# in the final pass).
+PASS_BYTES = {"density_lambda": 20, "delta": 28, "xsph_commit": 52, "omega": 36, "predict_bin": 40,
+              "gather": 56}
+
+
+def step_bytes(iters: int, vort: bool) -> int:
+    return 161 + 48 * iters + (48 if vort else 0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_single(sc, steps, warmup, flags, cpu_seconds):
+    from novaphy_amd import capi
+    n = sc.n
+    ctx = capi.Context(sc.params, capacity=n, flags=flags)
+    ctx.set_particles(sc.positions, sc.velocities)
+    log(f"[bench] {sc.name}: n={n}, warmup {warmup} steps")
+    ctx.step(sc.dt, sc.gravity, sc.mass, warmup)
+    ctx.synchronize()
+    start_state = ctx.get_particles()
+    ctx.set_kernel_timing(True)
+    ctx.synchronize()
+    log(f"[bench] timed region: {steps} steps")
+    t0 = time.perf_counter()
+    ctx.step(sc.dt, sc.gravity, sc.mass, steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    kt = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    wall = t1 - t0
+    value = n * steps / wall
+    # dominant kernel by total time
+    main = max((k for k in kt if k in PASS_BYTES), key=lambda k: kt[k][0])
+    ms_tot, launches = kt[main]
+    avg_s = ms_tot / 1e3 / launches
+    achieved = PASS_BYTES[main] * n / avg_s / 1e9
+    iters = sc.params.solver_iterations
+    vort = sc.params.vorticity_epsilon != 0
+    B = step_bytes(iters, vort)
+    out = {
+        "value": value,
+        "ms_per_step": wall * 1e3 / steps,
+        "roofline": {"bound": "hbm", "kernel": main, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                     "bytes_per_particle": PASS_BYTES[main], "particles_per_launch": n,
+                     "avg_launch_us": round(avg_s * 1e6, 2)},
+        "step_roofline": {"bytes_per_particle_step": B, "achieved": round(B * value / 1e9, 2),
+                          "unit": "GB/s", "frac": round(B * value / 1e9 / HBM_PEAK_GBPS, 5)},
+        "kernels": {k: {"total_ms": round(v[0], 3), "launches": v[1],
+                        "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2)} for k, v in kt.items()},
+    }
+    ctx.close()
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_baseline(sc, start_state, cpu_seconds)
+    return out
+
+
+def cpu_baseline(sc, state, budget_s):
+    """The oracle (serial FP32 restatement of the reference's CPU solver, same
+    unordered_map grid and candidate vectors) on this host's cores, from the
+    GPU timed region's starting state."""
+    from oracle import oracle as O
+    O.build()
+    pos, vel = state[0], state[1]
+    sim = O.Sim(sc.params, pos, vel)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        sim.step(sc.dt, sc.gravity, sc.mass, 1)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s * 0.5 or steps >= 20:
+            break
+    wall = time.perf_counter() - t0
+    sim.close()
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = os.cpu_count()
+    return {"value": sc.n * steps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} serial step(s) of the same {sc.n}-particle scene from the GPU timed "
+                      f"region's starting state ({wall:.1f} s; host has {aff} cores available)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="1m", choices=["1m", "8k"])
+    ap.add_argument("--fast-math", action="store_true",
+                    help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="budget of the host CPU baseline sample (0 disables)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent scenes shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from novaphy_amd import capi, scenes
+    if capi.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    sc = scenes.dam_break_1m() if args.workload == "1m" else scenes.dam_break_8k()
+    flags = capi.FLAG_FAST_MATH if args.fast_math else 0
+    r = run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds)
+    p = sc.params
+    line = {
+        "metric": METRIC,
+        "value": r["value"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{sc.name}: {sc.n} FP32 particles, box {p.domain_upper}, h={p.kernel_radius}, "
+                        f"{p.solver_iterations} iterations, vorticity {p.vorticity_epsilon}, "
+                        f"s_corr k={p.corr_k} n={p.corr_n} dq={p.corr_delta_q}h, dt=1/120, "
+                        f"{'fast-math' if args.fast_math else 'reference operation order'}",
+            "particles": sc.n,
+        },
+        "roofline": r["roofline"],
+        "step_roofline": r["step_roofline"],
+        "kernels": r["kernels"],
+    }
+    if "cpu_baseline" in r:
+        line["cpu_baseline"] = r["cpu_baseline"]
+        line["speedup_vs_cpu"] = round(r["value"] / r["cpu_baseline"]["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,11 +56,11 @@ int set_boundary(Ctx* c, const float* local3, const int32_t* body, const float* 
   std::vector<int> idx(n, 0), fillp(cnt.begin(), cnt.end());
   for (int b = 0; b < nb; ++b)
     if (body[b] >= 0 && body[b] < nbodies) idx[fillp[body[b]]++] = b;  // ascending b per body
-  NPX_HIP_C(c, hipMemcpy(c->BL.p, bl.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
-  NPX_HIP_C(c, hipMemcpy(c->BODY.p, bd.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  NPX_HIP_C(c, hipMemcpy(c->PSI.p, ps.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-  NPX_HIP_C(c, hipMemcpy(c->BODY_PTR.p, cnt.data(), sizeof(int) * (nbodies + 1), hipMemcpyHostToDevice));
-  NPX_HIP_C(c, hipMemcpy(c->BODY_IDX.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(c->BL.p, bl.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(c->BODY.p, bd.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(c->PSI.p, ps.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(c->BODY_PTR.p, cnt.data(), sizeof(int) * (nbodies + 1), hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(c->BODY_IDX.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   return NPX_OK;
 }
 
@@ -68,7 +68,7 @@ int upload_poses(Ctx* c, const float* pose7, int nbodies) {
   if (nbodies != c->nbodies) return c->fail(NPX_EINVAL, "expected %d body poses, got %d", c->nbodies, nbodies);
   if (nbodies > 0) {
     if (!pose7) return c->fail(NPX_EINVAL, "pose7 is NULL");
-    NPX_HIP_C(c, hipMemcpyAsync(c->POSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice, c->stream));
+    NPX_TRY(c->copy_sync(c->POSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice));
   }
   return NPX_OK;
 }
@@ -152,7 +152,7 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
     if (!H.window_valid) return c->fail(NPX_ESTATE, "no step has run yet");
     NPX_TRY(upload_poses(c, pose7, nbodies));
     if (!is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
-    NPX_HIP_C(c, hipMemcpyAsync(c->STATIC.p, is_static, nbodies, hipMemcpyHostToDevice, c->stream));
+    NPX_TRY(c->copy_sync(c->STATIC.p, is_static, nbodies, hipMemcpyHostToDevice));
     NPX_HIP_C(c, c->FP.ensure(c->slot_cap));
     NPX_HIP_C(c, c->FS.ensure(c->slot_cap));
     NPX_TRY(c->launch("fluid_payload", k_fluid_payload, c->hw_blocks, NPX_BLOCK, (const SceneC*)c->d_scenes.p,
@@ -241,13 +241,13 @@ int grid_query(Ctx* c, const float* pos3, int n, float cell, const float* q3, in
   std::vector<float4> hp(std::max(n, 1)), hq(nq);
   for (int i = 0; i < n; ++i) hp[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], 0.0f);
   for (int i = 0; i < nq; ++i) hq[i] = make_float4(q3[3 * i], q3[3 * i + 1], q3[3 * i + 2], 0.0f);
-  if (n > 0) NPX_HIP_C(c, hipMemcpy(P.p, hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
-  NPX_HIP_C(c, hipMemcpy(Q.p, hq.data(), sizeof(float4) * nq, hipMemcpyHostToDevice));
+  if (n > 0) NPX_TRY(c->copy_sync(P.p, hp.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+  NPX_TRY(c->copy_sync(Q.p, hq.data(), sizeof(float4) * nq, hipMemcpyHostToDevice));
   NPX_TRY(c->aux_sort(W, c->aux_clsmap.p, 0, P.p, n, nullptr, PS.p));
   NPX_TRY(c->launch("query_count", k_query_count, cdiv(nq, 256), 256, W, (const float4*)Q.p, nq,
                     (const int*)c->aux_clsmap.p, (const int*)c->Acstart.p, cnt.p));
   std::vector<int> hc(nq);
-  NPX_HIP_C(c, hipMemcpy(hc.data(), cnt.p, sizeof(int) * nq, hipMemcpyDeviceToHost));
+  NPX_TRY(c->copy_sync(hc.data(), cnt.p, sizeof(int) * nq, hipMemcpyDeviceToHost));
   std::vector<long long> ho(nq);
   long long t = 0;
   for (int i = 0; i < nq; ++i) {
@@ -258,11 +258,11 @@ int grid_query(Ctx* c, const float* pos3, int n, float cell, const float* q3, in
   *total = t;
   if (lists && cap >= t && t > 0) {
     NPX_HIP_C(c, out.ensure((size_t)t));
-    NPX_HIP_C(c, hipMemcpy(off.p, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice));
+    NPX_TRY(c->copy_sync(off.p, ho.data(), sizeof(long long) * nq, hipMemcpyHostToDevice));
     NPX_TRY(c->launch("query_lists", k_query_lists, cdiv(nq, 256), 256, W, (const float4*)Q.p, nq,
                       (const int*)c->aux_clsmap.p, (const int*)c->Acstart.p, (const int*)c->APERM.p,
                       (const long long*)off.p, out.p));
-    NPX_HIP_C(c, hipMemcpy(lists, out.p, sizeof(int) * t, hipMemcpyDeviceToHost));
+    NPX_TRY(c->copy_sync(lists, out.p, sizeof(int) * t, hipMemcpyDeviceToHost));
   }
   P.release();
   PS.release();

@@ -277,6 +277,15 @@ struct Ctx {
     return NPX_OK;
   }
 
+  // Stream-ordered blocking copy: every kernel runs on `stream` (non-blocking
+  // with respect to the null stream), so copies must be ordered on it too.
+  int copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (bytes == 0) return NPX_OK;
+    NPX_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    return NPX_OK;
+  }
+
   // ---------------------------------------------------------------- setup
   int init(int slots, int cells, int nscenes, uint32_t fl, bool is_batch) {
     flags = fl;
@@ -683,14 +692,14 @@ struct Ctx {
     *total = off_id[n];
     if (!lists || cap < *total) return NPX_OK;
     std::vector<int> hid(n);
-    NPX_HIP(hipMemcpy(hid.data(), ID2.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    NPX_TRY(copy_sync(hid.data(), ID2.p, sizeof(int) * n, hipMemcpyDeviceToHost));
     std::vector<long long> off_slot(n);
     for (int q = 0; q < n; ++q) off_slot[q] = off_id[hid[q]];
     DBuf<long long> doff;
     DBuf<int> dout;
     NPX_HIP(doff.ensure(n));
     NPX_HIP(dout.ensure((size_t)std::max<long long>(*total, 1)));
-    NPX_HIP(hipMemcpy(doff.p, off_slot.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    NPX_TRY(copy_sync(doff.p, off_slot.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
     NPX_TRY(launch("debug_lists", k_debug_lists, hw_blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                    (const int*)d_blk_scene.p, (const int*)clsmap.p, (const int*)cstart.p, (const int*)CELL2.p,
                    (const int*)ID2.p, (const long long*)doff.p, dout.p));

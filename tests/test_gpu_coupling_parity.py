@@ -1,0 +1,123 @@
+"""Akinci boundary path on the GPU (through the C ABI) against the oracle:
+psi volumes, boundary world positions, boundary density and per-body coupling
+force / torque after a PBF step, all bitwise."""
+import numpy as np
+import pytest
+
+from novaphy_amd import capi, scenes
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _scene_with_bodies():
+    """A small dam break with a ground plane, two spheres and a rotated box
+    sitting in / on the fluid (coupling forces non-zero)."""
+    sc = scenes.dam_break(16, 10, 16, (2.0, 1.0, 1.0), seed=11, steps=1)
+    shapes = [
+        O.make_shape(O.SHAPE_PLANE, -1, [0, 1, 0, 0.0]),
+        O.make_shape(O.SHAPE_SPHERE, 0, [0.15, 0, 0, 0]),
+        O.make_shape(O.SHAPE_SPHERE, 1, [0.1, 0, 0, 0]),
+        O.make_shape(O.SHAPE_BOX, 2, [0.12, 0.08, 0.1, 0]),
+    ]
+    q = np.array([0.9238795, 0.0, 0.3826834, 0.0], np.float32)  # 45 deg about y
+    poses = np.array([[0.35, 0.45, 0.40, 1, 0, 0, 0],
+                      [0.60, 0.30, 0.55, 1, 0, 0, 0],
+                      [0.45, 0.52, 0.70, *q]], np.float32)
+    static = np.array([0, 0, 0], np.uint8)
+    local, body = O.sample_boundaries(shapes, 0.05, 1.0)
+    return sc, local, body, poses, static
+
+
+def test_psi_volumes_bitexact(gpu):
+    sc, local, body, poses, static = _scene_with_bodies()
+    _say("case: psi volumes nb=", local.shape[0])
+    c = capi.Context(sc.params, capacity=sc.n)
+    c.set_particles(sc.positions, sc.velocities)
+    c.set_boundary(local, body, None, nbodies=poses.shape[0])
+    psi = c.compute_boundary_volumes(poses)
+    wpos = O.boundary_world_positions(local, body, poses)
+    opsi = O.boundary_volumes(wpos, sc.params.kernel_radius)
+    np.testing.assert_array_equal(psi, opsi)
+    gw = c.boundary_world_positions(poses, read_back=True)
+    np.testing.assert_array_equal(gw, wpos)
+    c.close()
+
+
+def test_boundary_density_and_coupling_bitexact(gpu):
+    sc, local, body, poses, static = _scene_with_bodies()
+    h = sc.params.kernel_radius
+    _say("case: boundary density + coupling n=", sc.n, "nb=", local.shape[0])
+    c = capi.Context(sc.params, capacity=sc.n)
+    c.set_particles(sc.positions, sc.velocities)
+    c.set_boundary(local, body, None, nbodies=poses.shape[0])
+    psi = c.compute_boundary_volumes(poses)
+    # FluidWorld::step order (fluid_world.cpp:46-99)
+    c.boundary_world_positions(poses)
+    c.step(sc.dt, sc.gravity, sc.mass, 1)
+    c.apply_boundary_density()
+    f, t = c.coupling_forces(poses, static)
+    gp, gv, gr, gl = c.get_particles()
+
+    wpos = O.boundary_world_positions(local, body, poses)
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 1)
+    orho_b = O.boundary_density(op, orho, wpos, psi, h, sc.params.rest_density)
+    of, ot = O.coupling(op, orho_b, wpos, psi, body, poses, static, h, sc.params.rest_density, sc.mass)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gr, orho_b)
+    assert np.abs(of).max() > 0, "the scene must produce coupling forces"
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(t, ot)
+    # FP64 recompute of the coupling for the record
+    f64, t64 = O.coupling_f64(op, orho_b, wpos, psi, body, poses, static, h, sc.params.rest_density, sc.mass)
+    _say("  coupling FP32 vs FP64: force max-rel %.2e" % (np.abs(f - f64).max() / np.abs(f64).max()))
+    # static bodies receive nothing
+    f2, t2 = c.coupling_forces(poses, np.array([1, 0, 1], np.uint8))
+    assert np.all(f2[0] == 0) and np.all(f2[2] == 0)
+    np.testing.assert_array_equal(f2[1], of[1])
+    c.close()
+
+
+def test_batched_scenes_equal_alone(gpu):
+    """Each scene advanced inside a batch matches the same scene alone,
+    bitwise (per-scene arithmetic is independent of the batch)."""
+    dims = [(6, 9, 5), (12, 7, 10), (9, 9, 9), (20, 6, 8)]
+    scs = [scenes.batched_scene(nx, ny, nz, 0.4, seed=100 + k, steps=10) for k, (nx, ny, nz) in enumerate(dims)]
+    _say("case: batch of", len(scs), "scenes, sizes", [s.n for s in scs])
+    slots = sum(((s.n + 255) // 256) * 256 for s in scs)
+    cells = sum(capi.scene_bytes(s.params, 1) for s in scs)  # generous upper bound on cells
+    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=8)
+    ids = [b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities) for s in scs]
+    b.step(10)
+    for s, sid in zip(scs, ids):
+        bp, bv, br, bl = b.get(sid)
+        c = capi.Context(s.params, capacity=s.n)
+        c.set_particles(s.positions, s.velocities)
+        c.step(s.dt, s.gravity, s.mass, 10)
+        ap, av, ar, al = c.get_particles()
+        c.close()
+        np.testing.assert_array_equal(bp, ap)
+        np.testing.assert_array_equal(bv, av)
+        np.testing.assert_array_equal(br, ar)
+    # remove one scene, add another, step again: survivors continue unchanged
+    b.remove(ids[1])
+    extra = scenes.batched_scene(7, 7, 7, 0.3, seed=999, steps=5)
+    eid = b.add(extra.params, extra.dt, extra.gravity, extra.mass, extra.positions, extra.velocities)
+    b.step(5)
+    ep, _, _, _ = b.get(eid)
+    c = capi.Context(extra.params, capacity=extra.n)
+    c.set_particles(extra.positions, extra.velocities)
+    c.step(extra.dt, extra.gravity, extra.mass, 5)
+    np.testing.assert_array_equal(ep, c.get_particles()[0])
+    c.close()
+    s0 = scs[0]
+    c = capi.Context(s0.params, capacity=s0.n)
+    c.set_particles(s0.positions, s0.velocities)
+    c.step(s0.dt, s0.gravity, s0.mass, 15)
+    np.testing.assert_array_equal(b.get(ids[0])[0], c.get_particles()[0])
+    c.close()
+    b.close()

@@ -1,0 +1,169 @@
+"""The reference's Python-level tests (tests/python/test_pbf.py,
+test_coupling.py, test_fluid.py, test_performance_monitor.py) run against the
+C++ host (novaphy_amd._core), whose fluid work executes on the GPU; plus a
+host-level parity check of FluidWorld against the oracle."""
+import json
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import novaphy_amd as novaphy
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _make_pbf_state(spacing=0.025, block_size=0.3):  # test_pbf.py:30-49
+    block = novaphy.FluidBlockDef()
+    block.lower = np.array([0, 0, 0], dtype=np.float32)
+    block.upper = np.array([block_size] * 3, dtype=np.float32)
+    block.particle_spacing = spacing
+    positions = novaphy.generate_fluid_block(block)
+    settings = novaphy.PBFSettings()
+    settings.kernel_radius = spacing * 4.0
+    settings.rest_density = 1000.0
+    settings.solver_iterations = 4
+    settings.epsilon = 100.0
+    state = novaphy.ParticleState()
+    state.init(positions)
+    return state, settings, settings.particle_mass(spacing)
+
+
+def test_pbf_solver_falls_and_matches_oracle(gpu):
+    _say("case: PBFSolver standalone")
+    state, settings, mass = _make_pbf_state()
+    solver = novaphy.PBFSolver(settings)
+    g = np.array([0, -9.81, 0], dtype=np.float32)
+    y0 = np.mean([p[1] for p in state.positions])
+    x0 = state.positions_numpy()
+    for _ in range(10):
+        solver.step(state, 1.0 / 60.0, g, mass)
+    assert np.mean([p[1] for p in state.positions]) < y0
+    assert max(np.linalg.norm(v) for v in state.velocities) > 0
+    op, ov, orho, ol = O.pbf_run(settings, x0, np.zeros_like(x0), np.float32(1.0 / 60.0), g, mass, 10)
+    np.testing.assert_array_equal(state.positions_numpy(), op)
+    np.testing.assert_array_equal(state.densities_numpy(), orho)
+
+
+def test_pbf_zero_gravity_com(gpu):  # test_pbf.py:80-97
+    state, settings, mass = _make_pbf_state()
+    solver = novaphy.PBFSolver(settings)
+    c0 = state.positions_numpy().mean(0)
+    for _ in range(5):
+        solver.step(state, 1.0 / 60.0, np.zeros(3, np.float32), mass)
+    assert np.linalg.norm(state.positions_numpy().mean(0) - c0) < 0.05
+
+
+def test_spatial_hash_grid_reference_tests(gpu):  # test_fluid.py:188-260
+    grid = novaphy.SpatialHashGrid(0.2)
+    grid.build([np.array([0.5, 0.5, 0.5], np.float32)])
+    assert 0 in grid.query_neighbors(np.array([0.5, 0.5, 0.5], np.float32), 0.2)
+    grid = novaphy.SpatialHashGrid(0.1)
+    grid.build([np.array([0, 0, 0], np.float32), np.array([5, 5, 5], np.float32)])
+    nb = grid.query_neighbors(np.array([0, 0, 0], np.float32), 0.1)
+    assert 0 in nb and 1 not in nb
+    np.random.seed(42)
+    pts = [np.array(np.random.uniform(0, 1, 3), dtype=np.float32) for _ in range(500)]
+    grid = novaphy.SpatialHashGrid(0.1)
+    grid.build(pts)
+    q = np.array([0.5, 0.5, 0.5], np.float32)
+    got = grid.query_neighbors(q, 0.1)
+    brute = {i for i, p in enumerate(pts) if np.linalg.norm(p - q) < 0.1}
+    assert brute <= set(got)
+    _, ol = O.grid_query(np.array(pts), 0.1, q[None])
+    assert got == ol.tolist()
+    # tests/cpp/test_core_span_apis.cpp:42-50
+    g2 = novaphy.SpatialHashGrid(0.25)
+    p3 = [np.array(v, np.float32) for v in ([0, 0, 0], [0.1, 0, 0], [1, 0, 0])]
+    g2.build(p3)
+    assert g2.query_all_pairs(p3, 0.2) == [(0, 1)]
+
+
+def _coupled_world(extent=0.5, iters=2):  # test_coupling.py:105-138
+    builder = novaphy.ModelBuilder()
+    builder.add_ground_plane(y=0.0)
+    body = novaphy.RigidBody.from_sphere(0.5, 0.2)
+    t = novaphy.Transform.from_translation(np.array([0.1, 0.8, 0.1], dtype=np.float32))
+    idx = builder.add_body(body, t)
+    builder.add_shape(novaphy.CollisionShape.make_sphere(0.2, idx))
+    model = builder.build()
+    block = novaphy.FluidBlockDef()
+    block.lower = np.array([0, 0.05, 0], dtype=np.float32)
+    block.upper = np.array([0.3, 0.4, 0.3], dtype=np.float32)
+    block.particle_spacing = 0.05
+    pbf = novaphy.PBFSettings()
+    pbf.kernel_radius = 0.2
+    pbf.solver_iterations = iters
+    return novaphy.FluidWorld(model, [block], novaphy.SolverSettings(), pbf, boundary_extent=extent)
+
+
+def test_fluid_world_reference_behaviour(gpu):
+    _say("case: FluidWorld coupled")
+    world = _coupled_world()
+    assert world.num_boundary_particles > 0 and world.num_particles > 0
+    for _ in range(5):
+        world.step(1.0 / 60.0)
+    assert world.state.transforms[0].position[1] < 0.8
+    vols = [b.volume for b in world.boundary_particles]
+    assert min(vols) > 0
+    # fluid world without rigid shapes (test_coupling.py:184-204)
+    model = novaphy.ModelBuilder().build()
+    block = novaphy.FluidBlockDef()
+    block.lower = np.array([0, 1, 0], np.float32)
+    block.upper = np.array([0.2, 1.2, 0.2], np.float32)
+    block.particle_spacing = 0.05
+    pbf = novaphy.PBFSettings()
+    pbf.kernel_radius = 0.2
+    pbf.solver_iterations = 3
+    w = novaphy.FluidWorld(model, [block], novaphy.SolverSettings(), pbf)
+    y0 = np.mean([p[1] for p in w.fluid_state.positions])
+    for _ in range(10):
+        w.step(1.0 / 60.0)
+    assert np.mean([p[1] for p in w.fluid_state.positions]) < y0
+
+
+def test_fluid_world_first_step_matches_oracle(gpu):
+    """One FluidWorld step = PBF + boundary density + coupling on the GPU; the
+    fluid state and the forces handed to the rigid step match the oracle."""
+    world = _coupled_world(extent=0.5, iters=2)
+    x0 = world.fluid_state.positions_numpy()
+    bp = world.boundary_particles
+    local = np.array([b.local_position for b in bp], np.float32)
+    body = np.array([b.body_index for b in bp], np.int32)
+    psi = np.array([b.volume for b in bp], np.float32)
+    t0 = world.state.transforms[0]
+    pose = np.array([[*t0.position, t0.rotation[3], *t0.rotation[:3]]], np.float32)
+    wpos = O.boundary_world_positions(local, body, pose)
+    np.testing.assert_array_equal(psi, O.boundary_volumes(wpos, 0.2))
+    dt = np.float32(1.0 / 60.0)
+    world.step(dt)
+    prm = world.pbf_settings
+    op, ov, orho, _ = O.pbf_run(prm, x0, np.zeros_like(x0), dt, (0, -9.81, 0), world.particle_mass, 1)
+    orho_b = O.boundary_density(op, orho, wpos, psi, 0.2, 1000.0)
+    np.testing.assert_array_equal(world.fluid_state.positions_numpy(), op)
+    np.testing.assert_array_equal(world.fluid_state.densities_numpy(), orho_b)
+
+
+def test_performance_monitor_fluid_phases(gpu, tmp_path):  # test_performance_monitor.py:119-156
+    world = _coupled_world(extent=0.5, iters=2)
+    mon = world.performance_monitor
+    mon.enabled = True
+    mon.trace_enabled = True
+    world.step(1.0 / 60.0)
+    phases = {p.name: p for p in mon.phase_stats()}
+    for name in ("fluid.total", "fluid.pbf.total", "fluid.rigid_step", "world.total", "world.broadphase.sap",
+                 "fluid.pbf.iteration.compute_density", "fluid.pbf.iteration.compute_delta_position"):
+        assert name in phases, name
+    metrics = {m.name: m.value for m in mon.last_frame_metrics()}
+    assert metrics["fluid_particles"] > 0 and metrics["boundary_particles"] >= 0
+    assert metrics["pbf_solver_iterations"] == 2.0
+    assert abs(metrics["kernel_radius"] - 0.2) < 1e-6
+    out = tmp_path / "trace.json"
+    mon.write_trace_json(str(out))
+    ev = json.loads(out.read_text())["traceEvents"]
+    assert any(e["ph"] == "X" for e in ev) and any(e["ph"] == "C" for e in ev)
