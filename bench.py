@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--workload", default="1m", choices=["1m", "8k"])
     ap.add_argument("--fast-math", action="store_true",
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
+    ap.add_argument("--corr-k", type=float, default=None,
+                    help="override the s_corr k of the workload (the configured value is 0.1)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the host CPU baseline sample (0 disables)")
     args = ap.parse_args()
@@ -139,6 +141,9 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     sc = scenes.dam_break_1m() if args.workload == "1m" else scenes.dam_break_8k()
+    if args.corr_k is not None:
+        import dataclasses
+        sc = dataclasses.replace(sc, params=dataclasses.replace(sc.params, corr_k=args.corr_k))
     flags = capi.FLAG_FAST_MATH if args.fast_math else 0
     r = run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds)
     p = sc.params

@@ -169,6 +169,17 @@ int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, floa
 /* Advances every scene by nsteps (each with its own dt / gravity / mass). */
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
 
+/* npx_batch_step with a device timestamp after every step: step_ms[k] is the
+ * duration of step k between consecutive HIP events on the context's stream
+ * (may be NULL).  Blocks until the steps are done. */
+int npx_batch_step_timed(npx_ctx* ctx, int32_t nsteps, float* step_ms);
+
+/* Per-scene status since the last call, as NPX_STATUS_* bits, and clears it:
+ * lets a caller retire one failing scene without failing the batch. */
+#define NPX_STATUS_WINDOW 1     /* a particle left the scene's cell window  */
+#define NPX_STATUS_NONFINITE 2  /* NaN / inf in positions or velocities      */
+int npx_scene_status(npx_ctx* ctx, int32_t* status, int32_t max_scenes);
+
 /* Device bytes needed by a scene of n particles and params (admission). */
 int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n);
 

@@ -25,6 +25,9 @@ NPX_ESTATE = -5
 FLAG_FAST_MATH = 1
 FLAG_NO_GRAPH = 2
 
+STATUS_WINDOW = 1
+STATUS_NONFINITE = 2
+
 # Every symbol of include/novaphy_hip.h.
 SYMBOLS = (
     "npx_create", "npx_destroy", "npx_last_error", "npx_device_count", "npx_set_params",
@@ -33,7 +36,7 @@ SYMBOLS = (
     "npx_set_boundary", "npx_compute_boundary_volumes", "npx_boundary_world_positions",
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
-    "npx_set_kernel_timing", "npx_kernel_times",
+    "npx_set_kernel_timing", "npx_kernel_times", "npx_batch_step_timed", "npx_scene_status",
 )
 
 
@@ -120,6 +123,8 @@ def load() -> C.CDLL:
         "npx_scene_bytes": (i64, [P(NpxParams), i32]),
         "npx_set_kernel_timing": (C.c_int, [vp, i32]),
         "npx_kernel_times": (C.c_int, [vp, C.c_char_p, i32, vp, vp, i32, P(i32)]),
+        "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
+        "npx_scene_status": (C.c_int, [vp, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -350,6 +355,18 @@ class Batch:
 
     def step(self, nsteps: int = 1):
         self._c(self._L.npx_batch_step(self._h, int(nsteps)))
+
+    def step_timed(self, nsteps: int = 1) -> np.ndarray:
+        """Steps with device timestamps; returns per-step milliseconds."""
+        out = np.zeros(max(int(nsteps), 1), np.float32)
+        self._c(self._L.npx_batch_step_timed(self._h, int(nsteps), _p(out)))
+        return out[:int(nsteps)]
+
+    def status(self, max_scenes: int) -> np.ndarray:
+        """Per-scene NPX_STATUS_* bits since the last call (cleared on read)."""
+        out = np.zeros(int(max_scenes), np.int32)
+        self._c(self._L.npx_scene_status(self._h, _p(out), int(max_scenes)))
+        return out
 
     def synchronize(self):
         self._c(self._L.npx_synchronize(self._h))

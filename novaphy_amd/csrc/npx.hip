@@ -276,6 +276,7 @@ int grid_query(Ctx* c, const float* pos3, int n, float cell, const float* q3, in
 void destroy(Ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->drop_graphs();
   for (auto& t : c->timers) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -583,6 +584,18 @@ int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return NPX_EINVAL;
   return c->step_n(nsteps);
+}
+
+int npx_batch_step_timed(npx_ctx* ctx, int32_t nsteps, float* step_ms) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || nsteps < 0) return NPX_EINVAL;
+  return c->step_timed(nsteps, step_ms);
+}
+
+int npx_scene_status(npx_ctx* ctx, int32_t* status, int32_t max_scenes) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !status) return NPX_EINVAL;
+  return c->scene_status(status, max_scenes);
 }
 
 int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {

@@ -298,9 +298,9 @@ struct Ctx {
     max_scenes = std::max(1, nscenes);
     scenes.assign(max_scenes, SceneH());
     NPX_HIP(d_scenes.ensure(max_scenes));
-    NPX_HIP(d_flags.ensure(1));
-    NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int), hipHostMallocDefault));
-    NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int), stream));
+    NPX_HIP(d_flags.ensure(1 + max_scenes));
+    NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
+    NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));
     cls_cap = 0;
     if (cells > 0) NPX_TRY(reserve_cells(cells));
@@ -499,13 +499,51 @@ struct Ctx {
     NPX_HIP(hipStreamSynchronize(stream));
     if (*h_flags & NPX_FLAG_WINDOW) {
       *h_flags = 0;
-      NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int), stream));
+      NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
       return fail(NPX_EDOMAIN,
                   "a particle left the cell window of its scene (more than %d cells outside the domain); "
                   "results of the affected steps are invalid",
                   NPX_WINDOW_MARGIN - 1);
     }
     return NPX_OK;
+  }
+
+  // Per-scene status bits (NPX_FLAG_*), cleared after reading.
+  int scene_status(int32_t* out, int cap) {
+    NPX_TRY(upload_tables());
+    NPX_TRY(launch("scene_health", k_scene_health, hw_blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const float4*)X.p, (const float4*)V.p, d_flags.p));
+    NPX_HIP(hipMemcpyAsync(h_flags, d_flags.p, sizeof(int) * (1 + max_scenes), hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    for (int s = 0; s < std::min(cap, max_scenes); ++s) out[s] = h_flags[1 + s];
+    return NPX_OK;
+  }
+
+  // Steps with a device timestamp after each one; step_ms[k] = duration of
+  // step k measured between consecutive events on the context's stream.
+  int step_timed(int nsteps, float* step_ms) {
+    std::vector<hipEvent_t> ev(nsteps + 1);
+    for (auto& e : ev) e = get_event();
+    NPX_HIP(hipEventRecord(ev[0], stream));
+    int r = NPX_OK;
+    for (int k = 0; k < nsteps && r == NPX_OK; ++k) {
+      r = step_n(1);
+      if (r == NPX_OK) {
+        hipError_t e = hipEventRecord(ev[k + 1], stream);
+        if (e != hipSuccess) r = fail(NPX_EDEVICE, "hipEventRecord: %s", hipGetErrorString(e));
+      }
+    }
+    if (r == NPX_OK) {
+      NPX_HIP(hipStreamSynchronize(stream));
+      for (int k = 0; k < nsteps; ++k) {
+        float ms = 0.0f;
+        NPX_HIP(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+        if (step_ms) step_ms[k] = ms;
+      }
+    }
+    for (auto e : ev) ev_pool.push_back(e);
+    return r;
   }
 
   // Unbounded scene: window from the bounding box of x* (host round trip).
@@ -586,9 +624,8 @@ struct Ctx {
     return 0;
   }
 
-  int step_once() {
-    if (!any_particles()) return NPX_OK;  // n == 0: PBFSolver::step returns (pbf_solver.cpp:21)
-    NPX_TRY(prepare_step());
+  // Enqueues one step (no host synchronisation, so it can be captured).
+  int enqueue_step() {
     int imax = 0;
     bool any_vort = false, carry_rho = false;
     for (const auto& H : scenes) {
@@ -598,14 +635,7 @@ struct Ctx {
       carry_rho = carry_rho || H.prm.solver_iterations <= 0;
     }
     NPX_TRY(grid_pass(carry_rho));
-    std::swap(X, X2);
-    std::swap(V, V2);
-    std::swap(ID, ID2);
-    std::swap(CELL, CELL2);
-    if (carry_rho) {
-      std::swap(RHO, RHO2);
-      std::swap(LAM, LAM2);
-    }
+    swap_committed(carry_rho);
     const int blocks = hw_blocks;
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
@@ -639,8 +669,105 @@ struct Ctx {
     return NPX_OK;
   }
 
+  // ---------------------------------------------------------- hipGraph
+  // A step is captured once per parity of the committed-state buffers (the
+  // step reads X and writes X2, then the roles swap), so two graphs cover
+  // every step; they are re-captured when the launch shape or any buffer
+  // changes.  Scene constants live in device memory, so dt / gravity / mass
+  // updates need no re-capture.
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    const void* in = nullptr;
+  };
+  GraphSlot graphs[2];
+  std::vector<const void*> graph_key;
+  int graph_captures = 0;
+
+  std::vector<const void*> current_graph_key() const {
+    int imax = 0, vort = 0, carry = 0;
+    for (const auto& H : scenes) {
+      if (!H.used || H.n == 0) continue;
+      imax = std::max(imax, H.prm.solver_iterations);
+      vort |= H.prm.vorticity_epsilon != 0.0f;
+      carry |= H.prm.solver_iterations <= 0;
+    }
+    // pointer pairs are order-insensitive (they swap every step)
+    auto lo = [](const void* a, const void* b) { return a < b ? a : b; };
+    auto hi = [](const void* a, const void* b) { return a < b ? b : a; };
+    return {(const void*)(intptr_t)hw_blocks, (const void*)(intptr_t)total_cls(), (const void*)(intptr_t)imax,
+            (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
+            hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
+            lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, clsmap.p, cstart.p, bsum.p,
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p};
+  }
+
+  void drop_graphs() {
+    for (auto& g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = GraphSlot();
+    }
+    graph_key.clear();
+  }
+
+  void swap_committed(bool carry_rho) {
+    std::swap(X, X2);
+    std::swap(V, V2);
+    std::swap(ID, ID2);
+    std::swap(CELL, CELL2);
+    if (carry_rho) {
+      std::swap(RHO, RHO2);
+      std::swap(LAM, LAM2);
+    }
+  }
+
+  int step_graph() {
+    auto key = current_graph_key();
+    if (key != graph_key) {
+      drop_graphs();
+      graph_key = key;
+    }
+    const int slot = (graphs[0].in == X.p) ? 0 : (graphs[1].in == X.p) ? 1 : -1;
+    if (slot >= 0) {
+      NPX_HIP(hipGraphLaunch(graphs[slot].exec, stream));
+      swap_committed(((intptr_t)key[3] & 1) != 0);
+      xs_final = graph_xs_final;
+      return NPX_OK;
+    }
+    const int free_slot = graphs[0].exec ? 1 : 0;
+    const void* in = X.p;
+    hipGraph_t g = nullptr;
+    NPX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int r = enqueue_step();
+    hipError_t e = hipStreamEndCapture(stream, &g);
+    if (r != NPX_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return fail(NPX_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(NPX_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    graphs[free_slot].exec = exec;
+    graphs[free_slot].in = in;
+    graph_xs_final = xs_final;
+    ++graph_captures;
+    // the capture swapped the host pointers as the step would; now run it
+    NPX_HIP(hipGraphLaunch(exec, stream));
+    return NPX_OK;
+  }
+  int graph_xs_final = 0;
+
   int step_n(int nsteps) {
-    for (int k = 0; k < nsteps; ++k) NPX_TRY(step_once());
+    const bool use_graph = !(flags & NPX_FLAG_NO_GRAPH) && !timing;
+    for (int k = 0; k < nsteps; ++k) {
+      if (!any_particles()) return NPX_OK;  // n == 0: PBFSolver::step returns (pbf_solver.cpp:21)
+      NPX_TRY(prepare_step());
+      if (use_graph)
+        NPX_TRY(step_graph());
+      else
+        NPX_TRY(enqueue_step());
+    }
     return NPX_OK;
   }
 

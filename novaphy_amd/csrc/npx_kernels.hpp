@@ -18,7 +18,13 @@
 #define NPX_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
 
 // Device error flags (bit set = condition seen).
-#define NPX_FLAG_WINDOW 1   // a particle's cell fell outside the window interior
+#define NPX_FLAG_WINDOW 1     // a particle's cell fell outside the window interior
+#define NPX_FLAG_NONFINITE 2  // a position or velocity is NaN / inf
+// flags[0] collects every condition; flags[1 + s] those of scene s.
+__device__ __forceinline__ void raise_flag(int* flags, int s, int bit) {
+  atomicOr(flags, bit);
+  atomicOr(flags + 1 + s, bit);
+}
 
 // ------------------------------------------------------------------ utils
 __global__ void k_fill_i32(int* __restrict__ p, int v, int n) {
@@ -173,7 +179,7 @@ k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   // The 27-cell neighbourhood must lie inside the window; outside it the
   // class table is undefined, so the step is flagged as failed.
   if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
-    atomicOr(flags, NPX_FLAG_WINDOW);
+    raise_flag(flags, s, NPX_FLAG_WINDOW);
     lx = min(max(lx, 1), S.wx - 2);
     ly = min(max(ly, 1), S.wy - 2);
     lz = min(max(lz, 1), S.wz - 2);
@@ -535,6 +541,21 @@ k_debug_lists(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { out[o++] = id[j]; });
 }
 
+// Per-scene health check: non-finite positions or velocities.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_scene_health(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
+               const float4* __restrict__ V, int* __restrict__ flags) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  if (q >= S.seg_off + S.n) return;
+  const float4 x = X[q], v = V[q];
+  const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
+                  isfinite(v.z);
+  if (!ok) raise_flag(flags, s, NPX_FLAG_NONFINITE);
+}
+
 // ------------------------------------------------------ auxiliary grids
 // Boundary particles and committed fluid positions get their own
 // SpatialHashGrid in the reference (fluid_world.cpp:106-107, 134-135,
@@ -654,7 +675,7 @@ k_boundary_density(const SceneC* __restrict__ scenes, const int* __restrict__ bl
   const float4 xi = X[q];
   const int l = aux_cell(W, xi.x, xi.y, xi.z);
   if (l < 0) {
-    atomicOr(flags, NPX_FLAG_WINDOW);
+    raise_flag(flags, s, NPX_FLAG_WINDOW);
     return;
   }
   float bd = 0.0f;

@@ -1,0 +1,193 @@
+"""Continuous-batching server for fluid rollouts on one GPU.
+
+Requests (a dam-break scene + a horizon in steps) arrive over time.  Between
+two device steps the server
+  * retires requests that reached their horizon (state read back, slots freed),
+  * evicts requests whose scene went bad (NaN / inf, left its cell window)
+    without disturbing the others,
+  * admits waiting requests in arrival order while the HBM budget, the scene
+    slots and the batch capacities allow (a request larger than the whole
+    budget is rejected at once),
+then advances every admitted request by one step in the same launches
+(npx_batch_step_timed).  Particle and grid state stay on the device for the
+whole rollout.  Per-step latency samples are device timestamps between
+consecutive steps; a request's result is bitwise the same alone or in any mix
+(per-scene arithmetic does not depend on the batch).
+
+The backend is ``capi.Batch``; the scheduling logic takes any object with
+the same methods (add / remove / get / step_timed / status), which is how the
+CPU tests exercise admission, eviction and cancellation without a GPU.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+DONE, CANCELLED, REJECTED, FAILED, RUNNING, WAITING = (
+    "done", "cancelled", "rejected", "failed", "running", "waiting")
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    scene: object                 # scenes.Scene: params, positions, velocities, dt, gravity, mass
+    horizon: int                  # steps to run
+    arrival: float = 0.0          # arrival time (seconds on the server's clock)
+    status: str = WAITING
+    steps_done: int = 0
+    sid: int = -1
+    bytes: int = 0
+    step_ms: List[float] = dataclasses.field(default_factory=list)
+    result: Optional[tuple] = None
+    error: str = ""
+    admitted_at: float = 0.0
+    finished_at: float = 0.0
+
+
+class Server:
+    def __init__(self, backend, hbm_budget_bytes: int, max_scenes: int,
+                 bytes_fn: Callable[[object, int], int], clock: Callable[[], float] = time.perf_counter,
+                 health_every: int = 10):
+        self.b = backend
+        self.budget = int(hbm_budget_bytes)
+        self.max_scenes = int(max_scenes)
+        self.bytes_fn = bytes_fn
+        self.clock = clock
+        self.health_every = max(1, int(health_every))
+        self.pending: "collections.deque[Request]" = collections.deque()
+        self.active: Dict[int, Request] = {}
+        self.finished: Dict[int, Request] = {}
+        self.used = 0
+        self.steps = 0
+        self.step_ms: List[float] = []
+        self.particle_steps = 0
+        self.busy_s = 0.0
+
+    # ---------------------------------------------------------- intake
+    def submit(self, req: Request) -> None:
+        req.bytes = self.bytes_fn(req.scene.params, req.scene.n)
+        if req.bytes > self.budget:
+            req.status = REJECTED
+            req.error = f"needs {req.bytes} B, budget is {self.budget} B"
+            self.finished[req.rid] = req
+            return
+        req.status = WAITING
+        self.pending.append(req)
+
+    def cancel(self, rid: int) -> bool:
+        for r in list(self.pending):
+            if r.rid == rid:
+                self.pending.remove(r)
+                r.status = CANCELLED
+                self.finished[rid] = r
+                return True
+        r = self.active.get(rid)
+        if r is not None:
+            self._retire(r, CANCELLED, read=False)
+            return True
+        return False
+
+    # ---------------------------------------------------------- scheduling
+    def _retire(self, r: Request, status: str, read: bool = True) -> None:
+        if read:
+            r.result = self.b.get(r.sid)
+        self.b.remove(r.sid)
+        self.used -= r.bytes
+        r.status = status
+        r.finished_at = self.clock()
+        del self.active[r.rid]
+        self.finished[r.rid] = r
+
+    def _admit(self, now: float) -> None:
+        while self.pending and self.pending[0].arrival <= now:
+            r = self.pending[0]
+            if len(self.active) >= self.max_scenes or self.used + r.bytes > self.budget:
+                return                       # FIFO: later arrivals wait behind the head
+            sc = r.scene
+            try:
+                sid = self.b.add(sc.params, sc.dt, sc.gravity, sc.mass, sc.positions, sc.velocities)
+            except Exception as e:            # capacity exhausted on the device side
+                if "capacity" in str(e) or "slots" in str(e):
+                    return
+                self.pending.popleft()
+                r.status = FAILED
+                r.error = str(e)
+                self.finished[r.rid] = r
+                continue
+            self.pending.popleft()
+            r.sid = sid
+            r.status = RUNNING
+            r.admitted_at = now
+            self.used += r.bytes
+            self.active[r.rid] = r
+
+    def _check_health(self) -> None:
+        st = self.b.status(self.max_scenes)
+        for r in list(self.active.values()):
+            if st[r.sid]:
+                r.error = f"scene status {int(st[r.sid])} (1 = left its cell window, 2 = non-finite state)"
+                self._retire(r, FAILED, read=False)
+
+    def step(self) -> bool:
+        """One scheduling round + one device step.  Returns False when idle
+        (nothing active and nothing that has arrived)."""
+        now = self.clock()
+        self._admit(now)
+        if not self.active:
+            return False
+        t0 = time.perf_counter()
+        ms = float(self.b.step_timed(1)[0])
+        self.busy_s += time.perf_counter() - t0
+        self.steps += 1
+        self.step_ms.append(ms)
+        for r in list(self.active.values()):
+            r.steps_done += 1
+            r.step_ms.append(ms)
+            self.particle_steps += r.scene.n
+        if self.steps % self.health_every == 0:
+            self._check_health()
+        for r in list(self.active.values()):
+            if r.steps_done >= r.horizon:
+                self._retire(r, DONE)
+        return True
+
+    def run(self, until_empty: bool = True, max_rounds: int = 10 ** 9) -> None:
+        rounds = 0
+        while rounds < max_rounds:
+            rounds += 1
+            if not self.step():
+                if not self.pending:
+                    if until_empty:
+                        return
+                else:
+                    # idle until the next arrival on the server's clock
+                    nxt = self.pending[0].arrival
+                    wait = nxt - self.clock()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+
+    # ---------------------------------------------------------- report
+    def latency_percentiles(self):
+        samples = np.concatenate([np.asarray(r.step_ms, np.float64) for r in self.finished.values()
+                                  if r.step_ms] or [np.zeros(0)])
+        if samples.size == 0:
+            return {"p50_ms": None, "p99_ms": None, "samples": 0}
+        return {"p50_ms": float(np.percentile(samples, 50)), "p99_ms": float(np.percentile(samples, 99)),
+                "samples": int(samples.size)}
+
+
+def poisson_requests(n: int, rate_hz: float, seed: int, make_scene: Callable[[int, np.random.Generator], object],
+                     horizon_range=(50, 500)) -> List[Request]:
+    """n requests with exponential inter-arrival times at rate_hz (seeded)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        h = int(rng.integers(horizon_range[0], horizon_range[1] + 1))
+        out.append(Request(rid=k, scene=make_scene(k, rng), horizon=h, arrival=t))
+    return out
