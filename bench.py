@@ -93,6 +93,61 @@ def run_single(sc, steps, warmup, flags, cpu_seconds):
     return out
 
 
+def run_batch(count, steps, warmup, flags, loop_steps):
+    """Batched scenes (one launch per pass over every scene, graph-replayed)
+    against the plain per-scene loop of single-scene contexts."""
+    from novaphy_amd import capi, scenes
+    scs = scenes.batched_1024(count=count, steps=steps)
+    total = sum(s.n for s in scs)
+    slots = sum(((s.n + 255) // 256) * 256 for s in scs)
+    cells = sum(int(np.prod(scene_window_dims(s))) for s in scs)
+    log(f"[bench] batch: {count} scenes, {total} particles, {cells} cells")
+    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=count, flags=flags)
+    for s in scs:
+        b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities)
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    ms = b.step_timed(steps)
+    wall = time.perf_counter() - t0
+    b.close()
+    value = total * steps / wall
+    out = {"value": value, "ms_per_step": wall * 1e3 / steps, "particles": total, "scenes": count,
+           "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99))}
+    # plain per-scene loop: one single-scene context per scene, stepped in turn
+    log(f"[bench] per-scene loop: {count} contexts, {loop_steps} steps")
+    ctxs = []
+    for s in scs:
+        c = capi.Context(s.params, capacity=s.n, flags=flags)
+        c.set_particles(s.positions, s.velocities)
+        ctxs.append((c, s))
+    for c, s in ctxs:
+        c.step(s.dt, s.gravity, s.mass, warmup)
+    for c, _ in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for c, s in ctxs:
+            c.step(s.dt, s.gravity, s.mass, 1)
+    for c, _ in ctxs:
+        c.synchronize()
+    lwall = time.perf_counter() - t0
+    for c, _ in ctxs:
+        c.close()
+    out["loop_value"] = total * loop_steps / lwall
+    out["loop_ms_per_step"] = lwall * 1e3 / loop_steps
+    out["speedup_vs_loop"] = value / out["loop_value"]
+    return out
+
+
+def scene_window_dims(sc):
+    h = sc.params.kernel_radius
+    inv = np.float32(1.0) / np.float32(h)
+    lo = [int(np.floor(np.float32(v) * inv)) - 4 for v in sc.params.domain_lower]
+    hi = [int(np.floor(np.float32(v) * inv)) + 4 for v in sc.params.domain_upper]
+    return [b - a + 1 for a, b in zip(lo, hi)]
+
+
 def cpu_baseline(sc, state, budget_s):
     """The oracle (serial FP32 restatement of the reference's CPU solver, same
     unordered_map grid and candidate vectors) on this host's cores, from the
@@ -124,7 +179,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="1m", choices=["1m", "8k"])
+    ap.add_argument("--workload", default="1m", choices=["1m", "8k", "batch"])
+    ap.add_argument("--scenes", type=int, default=1024, help="scene count of the batch workload")
+    ap.add_argument("--loop-steps", type=int, default=5, help="timed steps of the per-scene loop baseline")
     ap.add_argument("--fast-math", action="store_true",
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
     ap.add_argument("--corr-k", type=float, default=None,
@@ -140,11 +197,27 @@ def main():
     if capi.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    flags = capi.FLAG_FAST_MATH if args.fast_math else 0
+    if args.workload == "batch":
+        r = run_batch(args.scenes, args.steps, args.warmup, flags, args.loop_steps)
+        line = {
+            "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"batched_{args.scenes}: {args.scenes} dam breaks, N log-uniform "
+                                   f"2k-64k ({r['particles']} particles), one launch per pass over all "
+                                   "scenes, hipGraph-replayed", "particles": r["particles"]},
+            "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
+            "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"]},
+            "speedup_vs_per_scene_loop": round(r["speedup_vs_loop"], 2),
+        }
+        print(json.dumps(line), flush=True)
+        return
     sc = scenes.dam_break_1m() if args.workload == "1m" else scenes.dam_break_8k()
     if args.corr_k is not None:
         import dataclasses
         sc = dataclasses.replace(sc, params=dataclasses.replace(sc.params, corr_k=args.corr_k))
-    flags = capi.FLAG_FAST_MATH if args.fast_math else 0
     r = run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds)
     p = sc.params
     line = {

@@ -1,0 +1,71 @@
+"""Autotuner for the launch tunables of a device context.
+
+The tunables are the block size of the neighbour passes (64 / 128 / 256; a
+block stages its particles' candidate rows in LDS, so the block size trades
+LDS occupancy against staging redundancy) and the neighbour-list skin (0 = off,
+else a fraction of h).  The cell size is not tunable: the reference's
+candidate order is defined on cells of size h, and the GPU reproduces it
+exactly.  The sort is a counting sort with a per-cell rank, so there is no
+radix to choose.
+
+Results are bitwise independent of every tunable, so the tuner times real
+steps of the workload it is tuning (typically the warm-up steps) and the
+simulation state afterwards is exactly what it would have been untuned.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, Iterable, Optional, Tuple
+
+import numpy as np
+
+DEFAULT_CANDIDATES: Tuple[Tuple[int, float], ...] = (
+    (256, 0.0), (128, 0.0), (64, 0.0), (256, 0.1), (128, 0.2))
+
+
+def _timed_steps(target, nsteps: int, step_args) -> float:
+    """Median milliseconds per step over nsteps steps."""
+    if hasattr(target, "step_timed"):
+        return float(np.median(target.step_timed(nsteps)))
+    dt, g, m = step_args
+    times = []
+    for _ in range(nsteps):
+        target.synchronize()
+        t0 = time.perf_counter()
+        target.step(dt, g, m, 1)
+        target.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(times))
+
+
+def autotune(target, candidates: Iterable[Tuple[int, float]] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
+             step_args=None, cache_path: Optional[str] = None, key: Optional[str] = None):
+    """Times each (block_size, skin) candidate on ``target`` (a capi.Batch, or
+    a capi.Context with ``step_args`` = (dt, gravity, mass)), applies the
+    fastest and returns (best, {candidate: ms_per_step}).  With ``cache_path``
+    and ``key`` the choice is stored / reused."""
+    if cache_path and key and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache = json.load(f)
+        if key in cache:
+            b, sk = cache[key]["best"]
+            target.set_tuning(int(b), float(sk))
+            return (int(b), float(sk)), {}
+    results: Dict[Tuple[int, float], float] = {}
+    for b, sk in candidates:
+        target.set_tuning(int(b), float(sk))
+        _timed_steps(target, 1, step_args)           # graph capture / warm caches
+        results[(int(b), float(sk))] = _timed_steps(target, steps_per_candidate, step_args)
+    best = min(results, key=results.get)
+    target.set_tuning(*best)
+    if cache_path and key:
+        cache = {}
+        if os.path.exists(cache_path):
+            with open(cache_path) as f:
+                cache = json.load(f)
+        cache[key] = {"best": list(best), "ms": {f"{b}/{sk}": v for (b, sk), v in results.items()}}
+        with open(cache_path, "w") as f:
+            json.dump(cache, f, indent=1)
+    return best, results

@@ -36,7 +36,7 @@ SYMBOLS = (
     "npx_set_boundary", "npx_compute_boundary_volumes", "npx_boundary_world_positions",
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
-    "npx_set_kernel_timing", "npx_kernel_times", "npx_batch_step_timed", "npx_scene_status",
+    "npx_set_kernel_timing", "npx_kernel_times", "npx_batch_step_timed", "npx_scene_status", "npx_set_tuning",
 )
 
 
@@ -125,6 +125,7 @@ def load() -> C.CDLL:
         "npx_kernel_times": (C.c_int, [vp, C.c_char_p, i32, vp, vp, i32, P(i32)]),
         "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "npx_scene_status": (C.c_int, [vp, vp, i32]),
+        "npx_set_tuning": (C.c_int, [vp, i32, f32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -266,12 +267,15 @@ class Context:
         self._c(self._L.npx_coupling_forces(self._h, _p(poses7), _p(st), self.nbodies, _p(f), _p(t)))
         return f, t
 
-    # -- measurement
+    # -- measurement / tuning
     def set_kernel_timing(self, on: bool):
         self._c(self._L.npx_set_kernel_timing(self._h, 1 if on else 0))
 
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
+
+    def set_tuning(self, block_size: int, skin_fraction: float):
+        self._c(self._L.npx_set_tuning(self._h, int(block_size), C.c_float(skin_fraction)))
 
 
 def _kernel_times(L, h):
@@ -376,6 +380,9 @@ class Batch:
 
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
+
+    def set_tuning(self, block_size: int, skin_fraction: float):
+        self._c(self._L.npx_set_tuning(self._h, int(block_size), C.c_float(skin_fraction)))
 
 
 def scene_bytes(params, n: int) -> int:

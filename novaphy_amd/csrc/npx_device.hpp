@@ -10,7 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define NPX_BLOCK 256
+#define NPX_BLOCK 256      // scene segment alignment and block size of the sort kernels
+#define NPX_MAX_BLOCK 256  // largest block size of the neighbour passes (tunable, <= NPX_BLOCK)
 #define NPX_WINDOW_MARGIN 4
 
 // Constants of one scene, computed on the host with the reference's float
@@ -27,7 +28,15 @@ struct __align__(16) SceneC {
   int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
   int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
   int seg_off, n, seg_blocks, vort_on;
+  float skin2, disp_limit, pad1, pad2;  // neighbour-list skin radius^2, allowed summed displacement
 };
+
+// Per-step neighbour lists: NPX_NBR_CAP entries per particle, column-major
+// (entry k of slot q at k * stride + q) so a wave reads one entry per lane
+// with consecutive addresses.  Skin = 0.1 h; a list is used while twice the
+// summed per-iteration maximum displacement stays below half the skin.
+#define NPX_NBR_CAP 64
+#define NPX_MAX_DISP_IT 32
 
 struct __align__(16) GridWin {  // window of an auxiliary grid (boundary / committed fluid)
   float inv_h;

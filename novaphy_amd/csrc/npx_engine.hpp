@@ -116,7 +116,7 @@ static float spiky_cut(float h) {
   return x;
 }
 
-static SceneC make_scene_consts(const SceneH& H) {
+static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   const npx_pbf_params& P = H.prm;
   SceneC S;
   std::memset(&S, 0, sizeof(S));
@@ -179,6 +179,11 @@ static SceneC make_scene_consts(const SceneH& H) {
   S.seg_off = H.seg_off;
   S.n = H.n;
   S.seg_blocks = H.seg_blocks;
+  if (skin_frac > 0.0f) {
+    const float hs = h + skin_frac * h;
+    S.skin2 = hs * hs;
+    S.disp_limit = 0.25f * (skin_frac * h);  // 2 * sum(max |dx|) <= skin / 2
+  }
   return S;
 }
 
@@ -205,12 +210,16 @@ struct Ctx {
   std::vector<SceneH> scenes;
   int hw_blocks = 0;
   bool table_dirty = true;
+  // tunables (results are bitwise independent of them)
+  int nbr_block = 256;     // block size of the neighbour passes (64..NPX_MAX_BLOCK)
+  float skin_frac = 0.0f;  // neighbour-list skin as a fraction of h (0 = lists off)
   std::vector<int> h_blk_scene;
 
   // particle arrays (slot order)
   DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM;
   DBuf<float> RHO, RHO2, LAM, LAM2;
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
+  DBuf<int> NBR, NCNT, DISP;  // per-step neighbour lists and displacement bounds
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -260,6 +269,11 @@ struct Ctx {
 
   template <class K, class... A>
   int launch(const char* name, K kern, int blocks, int threads, A... args) {
+    return launch_shm(name, kern, blocks, threads, 0, args...);
+  }
+
+  template <class K, class... A>
+  int launch_shm(const char* name, K kern, int blocks, int threads, size_t shmem, A... args) {
     if (blocks <= 0) return NPX_OK;
     KTimer t;
     if (timing) {
@@ -268,7 +282,7 @@ struct Ctx {
       t.b = get_event();
       NPX_HIP(hipEventRecord(t.a, stream));
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, stream, args...);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), shmem, stream, args...);
     NPX_HIP(hipGetLastError());
     if (timing) {
       NPX_HIP(hipEventRecord(t.b, stream));
@@ -299,6 +313,7 @@ struct Ctx {
     scenes.assign(max_scenes, SceneH());
     NPX_HIP(d_scenes.ensure(max_scenes));
     NPX_HIP(d_flags.ensure(1 + max_scenes));
+    NPX_HIP(DISP.ensure((size_t)max_scenes * NPX_MAX_DISP_IT));
     NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
     NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));
@@ -335,9 +350,13 @@ struct Ctx {
     NPX_HIP(TPERM.ensure(n));
     NPX_HIP(TKEY.ensure(n));
     NPX_HIP(PERM.ensure(n));
-    NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK));
+    NPX_HIP(NBR.ensure(n * NPX_NBR_CAP));
+    NPX_HIP(NCNT.ensure(n));
+    // +4 entries of -1: neighbour passes with blocks larger than a segment
+    // may run past the last segment
+    NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
     slot_cap = cap;
-    h_blk_scene.assign(slot_cap / NPX_BLOCK, -1);
+    h_blk_scene.assign(slot_cap / NPX_BLOCK + 4, -1);
     return NPX_OK;
   }
 
@@ -390,7 +409,7 @@ struct Ctx {
     std::vector<SceneC> sc(max_scenes);
     for (int s = 0; s < max_scenes; ++s) {
       if (scenes[s].used)
-        sc[s] = make_scene_consts(scenes[s]);
+        sc[s] = make_scene_consts(scenes[s], skin_frac);
       else
         std::memset(&sc[s], 0, sizeof(SceneC));
     }
@@ -400,7 +419,7 @@ struct Ctx {
     // the host vectors must stay alive until the copies are done
     NPX_HIP(hipStreamSynchronize(stream));
     hw_blocks = 0;
-    for (int b = 0; b < (int)h_blk_scene.size(); ++b)
+    for (int b = 0; b < slot_cap / NPX_BLOCK; ++b)
       if (h_blk_scene[b] >= 0) hw_blocks = b + 1;
     table_dirty = false;
     return NPX_OK;
@@ -636,35 +655,40 @@ struct Ctx {
     }
     NPX_TRY(grid_pass(carry_rho));
     swap_committed(carry_rho);
-    const int blocks = hw_blocks;
+    const int blocks = cdiv((long long)hw_blocks * NPX_BLOCK, nbr_block);
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
+    const NbrList L{NBR.p, NCNT.p, DISP.p, slot_cap};
+    const int scap = stage_cap(nbr_block);
+    const size_t shm = sizeof(float4) * (size_t)scap;
+    NPX_TRY(launch("clear_disp", k_clear_disp, cdiv(max_scenes * NPX_MAX_DISP_IT, 256), 256, DISP.p,
+                   max_scenes * NPX_MAX_DISP_IT));
     int cur = 0;
     const int npass = std::max(imax, 1);
     for (int it = 0; it < npass; ++it) {
       if (fast)
-        NPX_TRY(launch("density_lambda", k_density_lambda<true>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
-                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it));
+        NPX_TRY(launch_shm("density_lambda", k_density_lambda<true>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
+                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it, L, NBR.p, NCNT.p, scap));
       else
-        NPX_TRY(launch("density_lambda", k_density_lambda<false>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
-                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it));
+        NPX_TRY(launch_shm("density_lambda", k_density_lambda<false>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
+                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it, L, NBR.p, NCNT.p, scap));
       if (fast)
-        NPX_TRY(launch("delta", k_delta<true>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
+        NPX_TRY(launch_shm("delta", k_delta<true>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
                        (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
-                       (const float*)RHO.p, VT.p));
+                       (const float*)RHO.p, VT.p, L, DISP.p, scap));
       else
-        NPX_TRY(launch("delta", k_delta<false>, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+        NPX_TRY(launch_shm("delta", k_delta<false>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
                        (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it,
-                       (const float4*)X.p, (const float*)RHO.p, VT.p));
+                       (const float4*)X.p, (const float*)RHO.p, VT.p, L, DISP.p, scap));
       cur = 1 - cur;
     }
     if (any_vort)
-      NPX_TRY(launch("omega", k_omega, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
-                     (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p));
-    NPX_TRY(launch("xsph_commit", k_xsph_commit, blocks, NPX_BLOCK, sc, bs, (const int*)clsmap.p,
+      NPX_TRY(launch_shm("omega", k_omega, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
+                     (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, L, scap));
+    NPX_TRY(launch_shm("xsph_commit", k_xsph_commit, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
                    (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p,
-                   (const float4*)OM.p, X.p, V.p));
+                   (const float4*)OM.p, X.p, V.p, L, scap));
     xs_final = cur;
     return NPX_OK;
   }
@@ -694,11 +718,12 @@ struct Ctx {
     // pointer pairs are order-insensitive (they swap every step)
     auto lo = [](const void* a, const void* b) { return a < b ? a : b; };
     auto hi = [](const void* a, const void* b) { return a < b ? b : a; };
-    return {(const void*)(intptr_t)hw_blocks, (const void*)(intptr_t)total_cls(), (const void*)(intptr_t)imax,
+    return {(const void*)(intptr_t)hw_blocks, (const void*)(intptr_t)nbr_block, (const void*)(intptr_t)total_cls(),
+            (const void*)(intptr_t)imax,
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, NBR.p, NCNT.p, DISP.p};
   }
 
   void drop_graphs() {
@@ -729,7 +754,7 @@ struct Ctx {
     const int slot = (graphs[0].in == X.p) ? 0 : (graphs[1].in == X.p) ? 1 : -1;
     if (slot >= 0) {
       NPX_HIP(hipGraphLaunch(graphs[slot].exec, stream));
-      swap_committed(((intptr_t)key[3] & 1) != 0);
+      swap_committed(((intptr_t)key[4] & 1) != 0);
       xs_final = graph_xs_final;
       return NPX_OK;
     }

@@ -17,6 +17,11 @@
 
 #define NPX_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
 
+// Global slot of this thread.  Scene segments are NPX_BLOCK-aligned, so the
+// scene of a slot is blk_scene[slot / NPX_BLOCK] whatever block size a kernel
+// is launched with (64..NPX_MAX_BLOCK), and a wave never straddles scenes.
+__device__ __forceinline__ int npx_slot() { return (int)(blockIdx.x * blockDim.x + threadIdx.x); }
+
 // Device error flags (bit set = condition seen).
 #define NPX_FLAG_WINDOW 1     // a particle's cell fell outside the window interior
 #define NPX_FLAG_NONFINITE 2  // a position or velocity is NaN / inf
@@ -164,10 +169,10 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
               const float4* __restrict__ V, const int* __restrict__ clsmap, int* __restrict__ ccount,
               int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of, int* __restrict__ flags) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const float4 x = X[q];
   const float4 v = V[q];
@@ -196,10 +201,10 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
           const int* __restrict__ cls_of, const int* __restrict__ off_of, const int* __restrict__ id,
           int* __restrict__ tperm, int* __restrict__ tkey) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const int shift = cstart[S.cls_base] - S.seg_off;
   const int t = cstart[cls_of[q]] - shift + off_of[q];
@@ -215,10 +220,10 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
        int* __restrict__ perm) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int t = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int t = npx_slot();
   if (t >= S.seg_off + S.n) return;
   const int shift = cstart[S.cls_base] - S.seg_off;
   const int p = tperm[t];
@@ -236,10 +241,10 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
          const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ id,
          const int* __restrict__ cell_of, float4* __restrict__ X2, float4* __restrict__ V2, float4* __restrict__ XS,
          int* __restrict__ id2, int* __restrict__ cell2) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const int p = perm[q];
   const float4 x = X[p];
@@ -258,10 +263,10 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
                  const float* __restrict__ A, const float* __restrict__ B, float* __restrict__ A2,
                  float* __restrict__ B2) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const int p = perm[q];
   A2[q] = A[p];
@@ -270,39 +275,211 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 
 // ------------------------------------------------------ PBF passes
 #define NPX_SCENE_PROLOGUE()                                 \
-  const int s = blk_scene[blockIdx.x];                       \
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];                       \
   if (s < 0) return;                                         \
   const SceneC& S = scenes[s];                               \
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;        \
+  const int q = npx_slot();        \
   if (q >= S.seg_off + S.n) return;                          \
   const int shift = cstart[S.cls_base] - S.seg_off;          \
   const int wxy = S.wx * S.wy;                               \
   const int* __restrict__ cm = clsmap + S.cls_base;          \
   const int l = cell[q];
 
+// Row staging.  The particles of a workgroup occupy a run of cells
+// [cmin, cmax] (slots are sorted by cell), so for every (dz, dy) offset the
+// candidates any of them visits lie in one slot range: the cells
+// [cmin + off - 1, cmax + off + 1].  The block copies those nine ranges of x*
+// into LDS once (about ten float4 per particle instead of ~216 cached global
+// loads per particle) and each thread sweeps its own ranges from LDS in the
+// reference's order.  A particle whose ranges are not inside the staged ones
+// (cells whose Teschner keys alias, or a block whose rows exceed the LDS
+// budget) reads from global memory instead; the values are the same.
+struct RowStage {
+  int gb[9], ge[9], lo[10];
+  int ok;
+  int cmin, cmax;
+};
+
+// Every thread of the block must call this (it synchronises).
+__device__ __forceinline__ void stage_rows(RowStage& st, float4* __restrict__ lds, int cap, bool active, int l,
+                                           int wx, int wxy, const int* __restrict__ cm,
+                                           const int* __restrict__ cstart, int shift,
+                                           const float4* __restrict__ XS) {
+  if (threadIdx.x == 0) {
+    st.cmin = 0x7FFFFFFF;
+    st.cmax = -1;
+  }
+  __syncthreads();
+  if (active) {
+    atomicMin(&st.cmin, l);
+    atomicMax(&st.cmax, l);
+  }
+  __syncthreads();
+  const bool any = st.cmin <= st.cmax;
+  if (threadIdx.x < 9) {
+    const int r = threadIdx.x;
+    const int off = (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    int gb = 0, ge = 0;
+    if (any) {
+      gb = cstart[cm[st.cmin + off - 1]] - shift;
+      ge = cstart[cm[st.cmax + off + 1] + 1] - shift;
+      if (ge < gb) ge = gb;
+    }
+    st.gb[r] = gb;
+    st.ge[r] = ge;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int r = 0; r < 9; ++r) {
+      st.lo[r] = t;
+      t += st.ge[r] - st.gb[r];
+    }
+    st.lo[9] = t;
+    st.ok = any && t <= cap;
+  }
+  __syncthreads();
+  if (st.ok) {
+    const int total = st.lo[9];
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+      int r = 0;
+      while (i >= st.lo[r + 1]) ++r;
+      lds[i] = XS[st.gb[r] + (i - st.lo[r])];
+    }
+  }
+  __syncthreads();
+}
+
+// The 27-cell sweep of for_each_candidate with x*_j supplied to f from LDS
+// when staged: f(j, x*_j).
+template <class F>
+__device__ __forceinline__ void staged_sweep(const RowStage& st, const float4* __restrict__ lds,
+                                             const float4* __restrict__ XS, int l, int wx, int wxy,
+                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                             F&& f) {
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1];
+    const int c1 = cm[base];
+    const int c2 = cm[base + 1];
+    if (c1 == c0 + 1 && c2 == c0 + 2) {
+      const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
+      if (st.ok && b >= st.gb[r] && e <= st.ge[r]) {
+        const int d = st.lo[r] - st.gb[r];
+        for (int j = b; j < e; ++j) f(j, lds[j + d]);
+      } else {
+        for (int j = b; j < e; ++j) f(j, XS[j]);
+      }
+    } else {
+      const int cc[3] = {c0, c1, c2};
+#pragma unroll 1
+      for (int k = 0; k < 3; ++k) {
+        const int b = cstart[cc[k]] - shift, e = cstart[cc[k] + 1] - shift;
+        for (int j = b; j < e; ++j) f(j, XS[j]);
+      }
+    }
+  }
+}
+
+// Staged LDS capacity (float4 entries) for a block size: nine rows of about
+// (B/8 + 2) cells of 8 particles, with headroom for compressed regions.
+__host__ __device__ constexpr int stage_cap(int block) { return 12 * (block + 16); }
+
+// Neighbour lists.  The first density pass of a step stores, in the
+// reference's visiting order (duplicates and self kept), every candidate
+// within the skin radius; later passes visit only those.  Terms the list
+// drops are candidates beyond cut2, whose poly6 and spiky_grad are exact
+// zeros, as long as no particle moved more than half the skin since the list
+// was built -- checked here from the per-iteration maximum displacements of
+// the scene.  Otherwise (or when a particle overflowed NPX_NBR_CAP) the full
+// candidate sweep runs, so results are identical either way.
+struct NbrList {
+  const int* __restrict__ idx;  // entry k of slot q at idx[k * stride + q]
+  const int* __restrict__ cnt;
+  const int* __restrict__ disp; // float bits, [scene][iteration]
+  int stride;
+};
+
+__device__ __forceinline__ bool list_usable(const NbrList& L, const SceneC& S, int s, int upto) {
+  if (S.skin2 <= 0.0f || S.iters < 1 || upto > NPX_MAX_DISP_IT) return false;
+  float sum = 0.0f;
+  for (int t = 0; t < upto; ++t) sum += __int_as_float(L.disp[s * NPX_MAX_DISP_IT + t]);
+  return sum <= S.disp_limit;
+}
+
+template <class F>
+__device__ __forceinline__ void for_each_neighbour(bool use_list, const NbrList& L, int q, int l, int wx, int wxy,
+                                                   const int* __restrict__ cm, const int* __restrict__ cstart,
+                                                   int shift, F&& f) {
+  if (use_list) {
+    const int n = L.cnt[q];
+    if (n <= NPX_NBR_CAP) {
+      const int* p = L.idx + q;
+      for (int k = 0; k < n; ++k) f(p[(size_t)k * L.stride]);
+      return;
+    }
+  }
+  for_each_candidate(l, wx, wxy, cm, cstart, shift, f);
+}
+
+// Block-wide maximum of a non-negative float into disp[slot] (one atomic per
+// wave).  Every thread of the block must reach it.
+__device__ __forceinline__ void block_max_disp(float d, int* slot) {
+  for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o, 64));
+  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax(slot, __float_as_int(d));
+}
+
+// Clears the displacement slots at the start of a step.
+__global__ void k_clear_disp(int* __restrict__ disp, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) disp[i] = 0;
+}
+
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
 // (:144-179): one sweep accumulates rho_i over all candidates (self and
 // duplicates included) and, for j != i, sum |grad_j|^2 and grad_i; lambda is
-// then 0 when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).
+// then 0 when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  Iteration 0
+// sweeps the 27 cells and writes the neighbour list.
 template <bool FAST>
-__global__ void __launch_bounds__(NPX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
-                 float* __restrict__ RHO, float* __restrict__ LAM, int iter) {
-  NPX_SCENE_PROLOGUE();
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, NbrList L, int* __restrict__ nbr_out,
+                 int* __restrict__ cnt_out, int scap) {
+  extern __shared__ float4 lds[];
+  __shared__ RowStage st;
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
   if (iter >= S.iters) return;
+  const int q = npx_slot();
+  const bool active = q < S.seg_off + S.n;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int wxy = S.wx * S.wy;
+  const int* __restrict__ cm = clsmap + S.cls_base;
+  const int l = active ? cell[q] : 0;
+  const bool use_list = iter > 0 && list_usable(L, S, s, iter);
+  if (!use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
+  if (!active) return;
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
   const float cut = FAST ? h2 : S.cut2;
+  const float skin2 = S.skin2;
+  const bool build = (iter == 0) && skin2 > 0.0f;
   float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
-    const float4 xj = XS[j];
+  int nn = 0;
+  auto body = [&](int j, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     float r_sq;
     if (FAST)
       r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
     else
       r_sq = sqn3(rx, ry, rz);
+    if (build && r_sq < skin2) {
+      if (nn < NPX_NBR_CAP) nbr_out[(size_t)nn * L.stride + q] = j;
+      ++nn;
+    }
     if (r_sq >= cut) return;  // poly6 and spiky_grad both vanish: adding 0 changes nothing
     rho += m * poly6(r_sq, h2, c6);
     if (j == q) return;
@@ -314,7 +491,21 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     gx += ax;
     gy += ay;
     gz += az;
-  });
+  };
+  if (use_list) {
+    const int n = L.cnt[q];
+    if (n <= NPX_NBR_CAP) {
+      for (int k = 0; k < n; ++k) {
+        const int j = L.idx[(size_t)k * L.stride + q];
+        body(j, XS[j]);
+      }
+    } else {
+      for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
+    }
+  } else {
+    staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
+  }
+  if (build) cnt_out[q] = nn;
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
   float lam = 0.0f;
@@ -349,81 +540,126 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 // Jacobi apply (:71-77) into the other x* buffer.  On the last iteration it
 // also performs the velocity update with the h/dt speed clamp (:81-93) and
 // stores v in VT with w = m / rho~ (rho~ = rho if > 1e-6 else rho0) for XSPH.
+// Records the scene's maximum displacement of this iteration.
 template <bool FAST>
-__global__ void __launch_bounds__(NPX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
-        float4* __restrict__ VT) {
-  NPX_SCENE_PROLOGUE();
-  const float4 xi = XS[q];
-  const bool last = (iter == max(S.iters, 1) - 1);
-  if (iter >= S.iters) {
-    // scenes with fewer iterations than the batch maximum pass x* through so
-    // every scene ends in the same ping-pong buffer
-    XSn[q] = xi;
-    if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
-    return;
-  }
-  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
-  const float wdq = S.wdq, negk = -S.corr_k;
-  const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
-  const float cut = FAST ? h2 : S.cut2;
-  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
-    if (j == q) return;
-    const float4 xj = XS[j];
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    float r_sq;
-    if (FAST)
-      r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-    else
-      r_sq = sqn3(rx, ry, rz);
-    if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
-    bool in;
-    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
-    if (!in) return;
-    const float ls = xi.w + xj.w;
-    float sc = 0.0f;
-    if (ls < -1e-10f && scorr_on) {
-      float ratio = poly6(r_sq, h2, c6) / wdq;
-      ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
-      float rp = ratio * ratio;
-      if (n4) rp = rp * rp;
-      sc = negk * rp;
-    }
-    const float coef = (ls + sc) * m;
-    if (FAST) {
-      const float cc = coef * co;
-      dx = __builtin_fmaf(cc, rx, dx);
-      dy = __builtin_fmaf(cc, ry, dy);
-      dz = __builtin_fmaf(cc, rz, dz);
+        float4* __restrict__ VT, NbrList L, int* __restrict__ disp, int scap) {
+  extern __shared__ float4 lds[];
+  __shared__ RowStage st;
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int q = npx_slot();
+  const bool active = q < S.seg_off + S.n;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int wxy = S.wx * S.wy;
+  const int* __restrict__ cm = clsmap + S.cls_base;
+  const int l = active ? cell[q] : 0;
+  const bool iterate = iter < S.iters;
+  const bool use_list = iterate && list_usable(L, S, s, iter);
+  if (iterate && !use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
+  float moved = 0.0f;
+  if (active) {
+    const float4 xi = XS[q];
+    const bool last = (iter == max(S.iters, 1) - 1);
+    if (!iterate) {
+      // scenes with fewer iterations than the batch maximum pass x* through
+      // so every scene ends in the same ping-pong buffer
+      XSn[q] = xi;
+      if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
     } else {
-      dx += coef * (co * rx);
-      dy += coef * (co * ry);
-      dz += coef * (co * rz);
+      const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
+      const float wdq = S.wdq, negk = -S.corr_k;
+      const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
+      const float cut = FAST ? h2 : S.cut2;
+      float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+      auto body = [&](int j, float4 xj) {
+        if (j == q) return;
+        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        float r_sq;
+        if (FAST)
+          r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+        else
+          r_sq = sqn3(rx, ry, rz);
+        if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
+        bool in;
+        const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
+        if (!in) return;
+        const float ls = xi.w + xj.w;
+        float sc = 0.0f;
+        if (ls < -1e-10f && scorr_on) {
+          float ratio = poly6(r_sq, h2, c6) / wdq;
+          ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
+          float rp = ratio * ratio;
+          if (n4) rp = rp * rp;
+          sc = negk * rp;
+        }
+        const float coef = (ls + sc) * m;
+        if (FAST) {
+          const float cc = coef * co;
+          dx = __builtin_fmaf(cc, rx, dx);
+          dy = __builtin_fmaf(cc, ry, dy);
+          dz = __builtin_fmaf(cc, rz, dz);
+        } else {
+          dx += coef * (co * rx);
+          dy += coef * (co * ry);
+          dz += coef * (co * rz);
+        }
+      };
+      if (use_list) {
+        const int n = L.cnt[q];
+        if (n <= NPX_NBR_CAP) {
+          for (int k = 0; k < n; ++k) {
+            const int j = L.idx[(size_t)k * L.stride + q];
+            body(j, XS[j]);
+          }
+        } else {
+          for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
+        }
+      } else {
+        staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
+      }
+      const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
+      XSn[q] = make_float4(px, py, pz, 0.0f);
+      if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+      const float ex = px - xi.x, ey = py - xi.y, ez = pz - xi.z;
+      moved = sqrtf(sqn3(ex, ey, ez));
+      if (!(moved <= 3.0e38f)) moved = 3.0e38f;  // NaN / inf: make the lists unusable
     }
-  });
-  const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
-  XSn[q] = make_float4(px, py, pz, 0.0f);
-  if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+  }
+  if (S.skin2 > 0.0f && iter < NPX_MAX_DISP_IT) block_max_disp(moved, disp + s * NPX_MAX_DISP_IT + iter);
 }
 
 // Vorticity (this project's definition; the reference declares
 // vorticity_epsilon but never uses it): omega_i = sum_{j!=i} (m/rho~_j)
 // (v_j - v_i) x gradW(x*_j - x*_i), stored with |omega_i| in w.
-__global__ void __launch_bounds__(NPX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_omega(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-        const float4* __restrict__ VT, float4* __restrict__ OM) {
-  NPX_SCENE_PROLOGUE();
+        const float4* __restrict__ VT, float4* __restrict__ OM, NbrList L, int scap) {
+  extern __shared__ float4 lds[];
+  __shared__ RowStage st;
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = npx_slot();
+  const bool active = q < S.seg_off + S.n;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int wxy = S.wx * S.wy;
+  const int* __restrict__ cm = clsmap + S.cls_base;
+  const int l = active ? cell[q] : 0;
+  const bool use_list = list_usable(L, S, s, S.iters);
+  if (!use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
+  if (!active) return;
   const float4 xi = XS[q];
   const float4 vi = VT[q];
   const float h = S.h, cs = S.cs;
   float wx = 0.0f, wy = 0.0f, wz = 0.0f;
-  for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+  auto body = [&](int j, float4 xj) {
     if (j == q) return;
-    const float4 xj = XS[j];
     const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
     const float r_sq = sqn3(rx, ry, rz);
     bool in;
@@ -436,27 +672,50 @@ k_omega(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     wx += vj.w * cx;
     wy += vj.w * cy;
     wz += vj.w * cz;
-  });
+  };
+  if (use_list) {
+    for_each_neighbour(true, L, q, l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
+  } else {
+    staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
+  }
   OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
 }
 
 // apply_xsph_viscosity (pbf_solver.cpp:225-250) + vorticity force + commit
 // with the domain clamp (:102-126), writing the new committed X / V.
-__global__ void __launch_bounds__(NPX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_xsph_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
               const float4* __restrict__ VT, const float4* __restrict__ OM, float4* __restrict__ X,
-              float4* __restrict__ V) {
-  NPX_SCENE_PROLOGUE();
+              float4* __restrict__ V, NbrList L, int scap) {
+  extern __shared__ float4 lds[];
+  __shared__ RowStage st;
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = npx_slot();
+  const bool active = q < S.seg_off + S.n;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int wxy = S.wx * S.wy;
+  const int* __restrict__ cm = clsmap + S.cls_base;
+  const int l = active ? cell[q] : 0;
+  const bool use_list = list_usable(L, S, s, S.iters);
+  if (!use_list && (S.xsph_on || S.vort_on)) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
+  if (!active) return;
   const float4 xi = XS[q];
   const float4 vi = VT[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs;
   float vx = vi.x, vy = vi.y, vz = vi.z;
+  auto sweep = [&](auto&& body) {
+    if (use_list)
+      for_each_neighbour(true, L, q, l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
+    else
+      staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
+  };
   if (S.xsph_on) {
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    sweep([&](int j, float4 xj) {
       if (j == q) return;
-      const float4 xj = XS[j];
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r_sq = sqn3(rx, ry, rz);
       if (r_sq >= h2) return;
@@ -473,9 +732,8 @@ k_xsph_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   if (S.vort_on) {
     const float4 oi = OM[q];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) {
+    sweep([&](int j, float4 xj) {
       if (j == q) return;
-      const float4 xj = XS[j];
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
       const float r_sq = sqn3(rx, ry, rz);
       bool in;
@@ -545,10 +803,10 @@ k_debug_lists(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_scene_health(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
                const float4* __restrict__ V, int* __restrict__ flags) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const float4 x = X[q], v = V[q];
   const bool ok = isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(v.x) && isfinite(v.y) &&
@@ -667,10 +925,10 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_boundary_density(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, GridWin W,
                    const float4* __restrict__ X, const int* __restrict__ bclsmap, const int* __restrict__ bcstart,
                    const float4* __restrict__ BPS, float* __restrict__ RHO, int* __restrict__ flags) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const float4 xi = X[q];
   const int l = aux_cell(W, xi.x, xi.y, xi.z);
@@ -691,10 +949,10 @@ k_boundary_density(const SceneC* __restrict__ scenes, const int* __restrict__ bl
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_fluid_payload(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
                 const float* __restrict__ RHO, float4* __restrict__ P) {
-  const int s = blk_scene[blockIdx.x];
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
-  const int q = blockIdx.x * NPX_BLOCK + threadIdx.x;
+  const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const float4 x = X[q];
   P[q] = make_float4(x.x, x.y, x.z, RHO[q]);

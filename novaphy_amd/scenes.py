@@ -152,7 +152,7 @@ def batched_scene_sizes(count: int = 1024, seed: int = 3, lo: int = 2000, hi: in
     aspect ratios in [1, 4], box sized for a 25-50 % initial fill."""
     rng = np.random.default_rng(seed)
     out = []
-    for _ in range(count):
+    while len(out) < count:
         n_target = math.exp(rng.uniform(math.log(lo), math.log(hi)))
         ax = rng.uniform(1.0, 4.0)   # x : z aspect
         ay = rng.uniform(1.0, 4.0)   # y : z aspect
@@ -160,21 +160,26 @@ def batched_scene_sizes(count: int = 1024, seed: int = 3, lo: int = 2000, hi: in
         nx = max(4, int(round(ax * nz)))
         ny = max(4, int(round(ay * nz)))
         fill = rng.uniform(0.25, 0.50)
-        out.append((nx, ny, nz, fill))
+        if lo <= nx * ny * nz <= hi:  # lattice rounding can step outside the range
+            out.append((nx, ny, nz, fill))
     return out
 
 
 def batched_scene(nx: int, ny: int, nz: int, fill: float, seed: int, steps: int = 200,
                   name: str = "batched") -> Scene:
-    """One scene of the batched workload: a corner block whose box keeps the
-    block's footprint in z, doubles height headroom and stretches x so that the
-    block volume is ``fill`` of the box."""
+    """One scene of the batched / served workloads: a corner block whose box
+    keeps the block's footprint in z, doubles height headroom and stretches x
+    so that the block volume is ``fill`` of the box.  These workloads fix r, h,
+    FP32 and 4 iterations only, so s_corr keeps PBFSettings' defaults
+    (k = 0.001, dq = 0.3h; pbf_solver.h:21-23)."""
     s = 0.05
     bx, by, bz = nx * s, ny * s, nz * s
     box_z = bz + 0.05
     box_y = by * 1.5 + 0.05
     box_x = max(bx + 0.05, (bx * by * bz) / (fill * box_y * box_z))
-    return dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=seed, steps=steps, name=name)
+    sc = dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=seed, steps=steps, name=name)
+    sc.params = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
+    return sc
 
 
 def batched_1024(count: int = 1024, steps: int = 200) -> List[Scene]:

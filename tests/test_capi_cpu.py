@@ -44,7 +44,8 @@ def test_invalid_params_rejected_before_device():
 def test_scene_bytes_admission_estimate():
     sc = scenes.dam_break_8k()
     b = capi.scene_bytes(sc.params, sc.n)
-    # 8000 particles -> 8192 slots * 180 B, plus the cell window (cells -4..24, -4..14,
-    # -4..14 for the 2x1x1 m domain at h = 0.1) * 8 B
-    assert b == 8192 * 180 + 29 * 19 * 19 * 8
+    # 8000 particles -> 8192 slots * (184 B of state + a 64-entry neighbour list),
+    # plus the cell window (cells -4..24, -4..14, -4..14 for the 2x1x1 m domain
+    # at h = 0.1) * 8 B
+    assert b == 8192 * (184 + 256) + 29 * 19 * 19 * 8
     assert capi.scene_bytes(sc.params, 2 * sc.n) > b

@@ -163,3 +163,22 @@ def test_grid_query_matches_oracle(gpu):
     oc, ol = O.grid_query(pts, 0.1, qs)
     np.testing.assert_array_equal(gc, oc)
     np.testing.assert_array_equal(gl, ol)
+
+
+def test_tunables_do_not_change_results(gpu):
+    """Block size and neighbour-list skin change speed only: a rollout that
+    switches tunables every step is bitwise equal to the oracle."""
+    sc = scenes.dam_break_8k()
+    prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
+    sc = dataclasses.replace(sc, params=prm)
+    _say("case: tunables (block size / skin) rollout")
+    c = _ctx(sc)
+    for k, (b, skin) in enumerate([(64, 0.0), (128, 0.1), (256, 0.2), (64, 0.3), (128, 0.0), (256, 0.1)] * 2):
+        c.set_tuning(b, skin)
+        c.step(sc.dt, sc.gravity, sc.mass, 1)
+    gp, gv, gr, gl = c.get_particles()
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 12)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    c.close()
