@@ -30,17 +30,19 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PBF particle-steps/s at 1M + % HBM roofline; batched p50/p99 step latency"
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic (compulsory) FP32/int32 bytes per particle for each pass; a
-# step is 161 + 48*I bytes (+48 with vorticity): predict+key 40, sort 13,
-# reorder 56, density+lambda 20 per iteration, delta-p+apply 28 per
-# iteration, velocity+XSPH+clamp+commit 52, vorticity omega pass 36 (+12 read
-# in the final pass).
-PASS_BYTES = {"density_lambda": 20, "delta": 28, "xsph_commit": 52, "omega": 36, "predict_bin": 40,
-              "gather": 56}
+# Algorithmic (compulsory) FP32/int32 bytes per particle for each pass, 12 B
+# per 3-vector: predict+key 40, sort 13, reorder 56; per iteration
+# density+lambda 20 (x* in, rho and lambda out) and delta-p+apply 28 (x* and
+# lambda in, x* out); the XSPH+curl pass 52 (x*, v, m/rho in; omega, v out;
+# 40 without vorticity) and the vorticity-force+commit pass 68 (x*, v,
+# omega, |omega|, m/rho in; x, v out; 48 without vorticity).
+def pass_bytes(vort: bool):
+    return {"density_lambda": 20, "delta": 28, "omega_xsph": 52 if vort else 40,
+            "vort_commit": 68 if vort else 48, "predict_bin": 40, "gather": 56}
 
 
 def step_bytes(iters: int, vort: bool) -> int:
-    return 161 + 48 * iters + (48 if vort else 0)
+    return 109 + 48 * iters + (120 if vort else 88)
 
 
 def log(*a):
@@ -68,12 +70,13 @@ def run_single(sc, steps, warmup, flags, cpu_seconds):
     wall = t1 - t0
     value = n * steps / wall
     # dominant kernel by total time
+    iters = sc.params.solver_iterations
+    vort = sc.params.vorticity_epsilon != 0
+    PASS_BYTES = pass_bytes(vort)
     main = max((k for k in kt if k in PASS_BYTES), key=lambda k: kt[k][0])
     ms_tot, launches = kt[main]
     avg_s = ms_tot / 1e3 / launches
     achieved = PASS_BYTES[main] * n / avg_s / 1e9
-    iters = sc.params.solver_iterations
-    vort = sc.params.vorticity_epsilon != 0
     B = step_bytes(iters, vort)
     out = {
         "value": value,
@@ -140,6 +143,70 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     return out
 
 
+def coupled_world(seed=2):
+    """500k-particle block (100x50x100) in a 10x5x5 m box, a ground plane
+    sampled with boundary particles over the box, and 64 rigid bodies: 32
+    spheres (r ~ U(0.1, 0.2)) and 32 boxes (half extents ~ U(0.05, 0.15)^3),
+    density ~ U(300, 2000), seeded positions over the block footprint 0.3-1.0
+    m above its surface."""
+    import novaphy_amd as nv
+    from novaphy_amd import scenes
+    sc = scenes.coupled_500k_fluid()
+    rng = np.random.default_rng(seed)
+    b = nv.ModelBuilder()
+    b.add_ground_plane(y=0.0)
+    top = float(sc.positions[:, 1].max())
+    lo = sc.positions.min(0)
+    hi = sc.positions.max(0)
+    for k in range(64):
+        x = rng.uniform(lo[0] + 0.3, hi[0] - 0.3)
+        z = rng.uniform(lo[2] + 0.3, hi[2] - 0.3)
+        y = top + rng.uniform(0.3, 1.0)
+        dens = rng.uniform(300.0, 2000.0)
+        t = nv.Transform.from_translation(np.array([x, y, z], np.float32))
+        if k < 32:
+            r = rng.uniform(0.1, 0.2)
+            i = b.add_body(nv.RigidBody.from_sphere(float(dens * 4.0 / 3.0 * np.pi * r ** 3), float(r)), t)
+            b.add_shape(nv.CollisionShape.make_sphere(float(r), i))
+        else:
+            he = rng.uniform(0.05, 0.15, 3).astype(np.float32)
+            i = b.add_body(nv.RigidBody.from_box(float(dens * 8.0 * np.prod(he)), he), t)
+            b.add_shape(nv.CollisionShape.make_box(he, i))
+    model = b.build()
+    prm = nv.PBFSettings()
+    p = sc.params
+    for f in ("rest_density", "kernel_radius", "particle_radius", "solver_iterations", "epsilon",
+              "xsph_viscosity", "vorticity_epsilon", "corr_delta_q", "corr_k", "corr_n", "use_domain_bounds"):
+        setattr(prm, f, getattr(p, f))
+    prm.domain_lower = np.array(p.domain_lower, np.float32)
+    prm.domain_upper = np.array(p.domain_upper, np.float32)
+    blk = nv.FluidBlockDef()
+    blk.particle_spacing = 0.05
+    blk.lower = np.zeros(3, np.float32)
+    blk.upper = np.zeros(3, np.float32)
+    world = nv.FluidWorld(model, [blk], nv.SolverSettings(), prm, boundary_extent=10.0)
+    world.fluid_state.init_arrays(sc.positions, sc.velocities)
+    return world, sc
+
+
+def run_coupled(steps, warmup):
+    world, sc = coupled_world()
+    log(f"[bench] coupled: {sc.n} fluid particles, {world.num_boundary_particles} boundary particles, 64 bodies")
+    dt = np.float32(sc.dt)
+    for _ in range(warmup):
+        world.step(dt)
+    mon = world.performance_monitor
+    mon.enabled = True
+    mon.reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        world.step(dt)
+    wall = time.perf_counter() - t0
+    phases = {p.name: round(p.avg_ms, 3) for p in mon.phase_stats()}
+    return {"value": sc.n * steps / wall, "ms_per_step": wall * 1e3 / steps, "particles": sc.n,
+            "boundary_particles": world.num_boundary_particles, "phases_avg_ms": phases}
+
+
 def scene_window_dims(sc):
     h = sc.params.kernel_radius
     inv = np.float32(1.0) / np.float32(h)
@@ -179,7 +246,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="1m", choices=["1m", "8k", "batch"])
+    ap.add_argument("--workload", default="1m", choices=["1m", "8k", "batch", "coupled"])
     ap.add_argument("--scenes", type=int, default=1024, help="scene count of the batch workload")
     ap.add_argument("--loop-steps", type=int, default=5, help="timed steps of the per-scene loop baseline")
     ap.add_argument("--fast-math", action="store_true",
@@ -211,6 +278,20 @@ def main():
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"]},
             "speedup_vs_per_scene_loop": round(r["speedup_vs_loop"], 2),
+        }
+        print(json.dumps(line), flush=True)
+        return
+    if args.workload == "coupled":
+        r = run_coupled(args.steps, args.warmup)
+        line = {
+            "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"coupled_500k: {r['particles']} fluid particles + 64 rigid bodies "
+                                   f"({r['boundary_particles']} Akinci boundary particles incl. ground plane), "
+                                   "FluidWorld.step with host rigid pipeline", "particles": r["particles"]},
+            "phases_avg_ms": r["phases_avg_ms"],
         }
         print(json.dumps(line), flush=True)
         return

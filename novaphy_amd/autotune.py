@@ -1,9 +1,9 @@
 """Autotuner for the launch tunables of a device context.
 
-The tunables are the block size of the neighbour passes (64 / 128 / 256; a
-block stages its particles' candidate rows in LDS, so the block size trades
-LDS occupancy against staging redundancy) and the neighbour-list skin (0 = off,
-else a fraction of h).  The cell size is not tunable: the reference's
+The tunables are the block size of the neighbour passes (64 / 128 / 256) and
+the XCD-aware block order (each of the 8 XCDs sweeps one contiguous run of
+particles so the neighbour rows its blocks share stay in its own L2, or the
+hardware's round-robin order).  The cell size is not tunable: the reference's
 candidate order is defined on cells of size h, and the GPU reproduces it
 exactly.  The sort is a counting sort with a per-cell rank, so there is no
 radix to choose.
@@ -21,8 +21,8 @@ from typing import Dict, Iterable, Optional, Tuple
 
 import numpy as np
 
-DEFAULT_CANDIDATES: Tuple[Tuple[int, float], ...] = (
-    (256, 0.0), (128, 0.0), (64, 0.0), (256, 0.1), (128, 0.2))
+DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
+    (256, 1), (128, 1), (64, 1), (256, 0), (128, 0))
 
 
 def _timed_steps(target, nsteps: int, step_args) -> float:
@@ -40,9 +40,9 @@ def _timed_steps(target, nsteps: int, step_args) -> float:
     return float(np.median(times))
 
 
-def autotune(target, candidates: Iterable[Tuple[int, float]] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
+def autotune(target, candidates: Iterable[Tuple[int, int]] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
              step_args=None, cache_path: Optional[str] = None, key: Optional[str] = None):
-    """Times each (block_size, skin) candidate on ``target`` (a capi.Batch, or
+    """Times each (block_size, xcd_remap) candidate on ``target`` (a capi.Batch, or
     a capi.Context with ``step_args`` = (dt, gravity, mass)), applies the
     fastest and returns (best, {candidate: ms_per_step}).  With ``cache_path``
     and ``key`` the choice is stored / reused."""
@@ -51,13 +51,13 @@ def autotune(target, candidates: Iterable[Tuple[int, float]] = DEFAULT_CANDIDATE
             cache = json.load(f)
         if key in cache:
             b, sk = cache[key]["best"]
-            target.set_tuning(int(b), float(sk))
-            return (int(b), float(sk)), {}
-    results: Dict[Tuple[int, float], float] = {}
+            target.set_tuning(int(b), int(sk))
+            return (int(b), int(sk)), {}
+    results: Dict[Tuple[int, int], float] = {}
     for b, sk in candidates:
-        target.set_tuning(int(b), float(sk))
+        target.set_tuning(int(b), int(sk))
         _timed_steps(target, 1, step_args)           # graph capture / warm caches
-        results[(int(b), float(sk))] = _timed_steps(target, steps_per_candidate, step_args)
+        results[(int(b), int(sk))] = _timed_steps(target, steps_per_candidate, step_args)
     best = min(results, key=results.get)
     target.set_tuning(*best)
     if cache_path and key:

@@ -282,17 +282,18 @@ void destroy(Ctx* c) {
     (void)hipEventDestroy(t.b);
   }
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
-  npx::DBuf<float4>* f4[] = {&c->X, &c->X2, &c->V, &c->V2, &c->XS0, &c->XS1, &c->VT, &c->OM,
+  npx::DBuf<float4>* f4[] = {&c->X, &c->X2, &c->V, &c->V2, &c->XS0, &c->XS1, &c->VT, &c->OM, &c->VX,
                              &c->BL, &c->BP, &c->BPS, &c->FP, &c->FS, &c->FB};
   for (auto* b : f4) b->release();
   npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->PSI, &c->POSE, &c->OUT6};
   for (auto* b : f1) b->release();
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
-                          &c->PERM, &c->NBR, &c->NCNT, &c->DISP, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->PERM, &c->RBASE, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
                           &c->ATKEY, &c->APERM, &c->aux_clsmap};
   for (auto* b : i1) b->release();
   c->htk.release();
+  c->RMASK.release();
   c->d_scenes.release();
   c->STATIC.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -586,16 +587,15 @@ int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {
   return c->step_n(nsteps);
 }
 
-int npx_set_tuning(npx_ctx* ctx, int32_t block_size, float skin_fraction) {
+int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return NPX_EINVAL;
   if (block_size != 64 && block_size != 128 && block_size != 256)
     return c->fail(NPX_EINVAL, "block_size must be 64, 128 or 256");
-  if (!(skin_fraction >= 0.0f && skin_fraction <= 1.0f))
-    return c->fail(NPX_EINVAL, "skin_fraction must be in [0, 1] (0 turns neighbour lists off)");
+  if (xcd_remap != 0 && xcd_remap != 1) return c->fail(NPX_EINVAL, "xcd_remap must be 0 or 1");
   NPX_HIP_C(c, hipStreamSynchronize(c->stream));
   c->nbr_block = block_size;
-  c->skin_frac = skin_fraction;
+  c->xcd_remap = xcd_remap;
   c->table_dirty = true;
   return NPX_OK;
 }
@@ -621,7 +621,7 @@ int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
   const long long slots = (long long)cdiv(std::max(n, 1), NPX_BLOCK) * NPX_BLOCK;
   // 8 float4 + 4 float + 10 int arrays and the neighbour list per slot;
   // class map + start per cell
-  return slots * (8 * 16 + 4 * 4 + 10 * 4 + 4 * NPX_NBR_CAP) + cells * 8;
+  return slots * (9 * 16 + 4 * 4 + 9 * 4 + NPX_ROWS * 12) + cells * 8;
 }
 
 int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {

@@ -28,15 +28,8 @@ struct __align__(16) SceneC {
   int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
   int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
   int seg_off, n, seg_blocks, vort_on;
-  float skin2, disp_limit, pad1, pad2;  // neighbour-list skin radius^2, allowed summed displacement
+  float sp_lo, sp_hi, pad2, pad3;  // spiky support on r^2: (sp_lo, sp_hi), i.e. 1e-7 < |r| < h
 };
-
-// Per-step neighbour lists: NPX_NBR_CAP entries per particle, column-major
-// (entry k of slot q at k * stride + q) so a wave reads one entry per lane
-// with consecutive addresses.  Skin = 0.1 h; a list is used while twice the
-// summed per-iteration maximum displacement stays below half the skin.
-#define NPX_NBR_CAP 64
-#define NPX_MAX_DISP_IT 32
 
 struct __align__(16) GridWin {  // window of an auxiliary grid (boundary / committed fluid)
   float inv_h;
@@ -66,6 +59,20 @@ __device__ __forceinline__ float spiky_coeff(float r_sq, float h, float cs, bool
   }
   *in = true;
   float diff = h - r_len;
+  return (-cs) * diff * diff / r_len;
+}
+
+// spiky_coeff with the support test done on r^2 against the exact bounds
+// sp_lo = max{x : sqrtf(x) <= 1e-7} and sp_hi = min{x : sqrtf(x) >= h}, so the
+// square root is taken only inside the support; same results bit for bit.
+__device__ __forceinline__ float spiky_coeff_sq(float r_sq, float sp_lo, float sp_hi, float h, float cs, bool* in) {
+  if (r_sq <= sp_lo || r_sq >= sp_hi) {
+    *in = false;
+    return 0.0f;
+  }
+  *in = true;
+  const float r_len = sqrtf(r_sq);
+  const float diff = h - r_len;
   return (-cs) * diff * diff / r_len;
 }
 

@@ -116,7 +116,16 @@ static float spiky_cut(float h) {
   return x;
 }
 
-static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
+// Largest float x >= 0 with sqrtf(x) <= c.  sqrtf is correctly rounded on the
+// host and in the kernels (monotone), so sqrtf(r^2) <= c  <=>  r^2 <= x.
+static float sqrt_le_max(float c) {
+  float x = c * c;
+  while (std::sqrt(std::nextafter(x, 1e30f)) <= c) x = std::nextafter(x, 1e30f);
+  while (x > 0.0f && std::sqrt(x) > c) x = std::nextafter(x, 0.0f);
+  return x;
+}
+
+static SceneC make_scene_consts(const SceneH& H) {
   const npx_pbf_params& P = H.prm;
   SceneC S;
   std::memset(&S, 0, sizeof(S));
@@ -165,7 +174,9 @@ static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   S.chi0 = P.domain_upper[0] - pr;
   S.chi1 = P.domain_upper[1] - pr;
   S.chi2 = P.domain_upper[2] - pr;
-  S.cut2 = std::max(S.h2, spiky_cut(h));
+  S.sp_hi = spiky_cut(h);
+  S.sp_lo = sqrt_le_max(1e-7f);
+  S.cut2 = std::max(S.h2, S.sp_hi);
   S.iters = P.solver_iterations;
   S.bounds = P.use_domain_bounds ? 1 : 0;
   S.wlo0 = H.wlo[0];
@@ -179,11 +190,6 @@ static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   S.seg_off = H.seg_off;
   S.n = H.n;
   S.seg_blocks = H.seg_blocks;
-  if (skin_frac > 0.0f) {
-    const float hs = h + skin_frac * h;
-    S.skin2 = hs * hs;
-    S.disp_limit = 0.25f * (skin_frac * h);  // 2 * sum(max |dx|) <= skin / 2
-  }
   return S;
 }
 
@@ -211,15 +217,16 @@ struct Ctx {
   int hw_blocks = 0;
   bool table_dirty = true;
   // tunables (results are bitwise independent of them)
-  int nbr_block = 256;     // block size of the neighbour passes (64..NPX_MAX_BLOCK)
-  float skin_frac = 0.0f;  // neighbour-list skin as a fraction of h (0 = lists off)
+  int nbr_block = 256;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
+  int xcd_remap = 1;    // 1: XCD-aware block order in the neighbour passes
   std::vector<int> h_blk_scene;
 
   // particle arrays (slot order)
-  DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM;
+  DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM, VX;
   DBuf<float> RHO, RHO2, LAM, LAM2;
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
-  DBuf<int> NBR, NCNT, DISP;  // per-step neighbour lists and displacement bounds
+  DBuf<unsigned long long> RMASK;  // row masks (NPX_ROWS per slot)
+  DBuf<int> RBASE;                 // row start slots (NPX_ROWS per slot)
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -313,7 +320,6 @@ struct Ctx {
     scenes.assign(max_scenes, SceneH());
     NPX_HIP(d_scenes.ensure(max_scenes));
     NPX_HIP(d_flags.ensure(1 + max_scenes));
-    NPX_HIP(DISP.ensure((size_t)max_scenes * NPX_MAX_DISP_IT));
     NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
     NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));
@@ -337,6 +343,7 @@ struct Ctx {
     NPX_HIP(XS1.ensure(n));
     NPX_HIP(VT.ensure(n));
     NPX_HIP(OM.ensure(n));
+    NPX_HIP(VX.ensure(n));
     NPX_HIP(RHO.ensure(n));
     NPX_HIP(RHO2.ensure(n));
     NPX_HIP(LAM.ensure(n));
@@ -350,8 +357,8 @@ struct Ctx {
     NPX_HIP(TPERM.ensure(n));
     NPX_HIP(TKEY.ensure(n));
     NPX_HIP(PERM.ensure(n));
-    NPX_HIP(NBR.ensure(n * NPX_NBR_CAP));
-    NPX_HIP(NCNT.ensure(n));
+    NPX_HIP(RMASK.ensure(n * NPX_ROWS));
+    NPX_HIP(RBASE.ensure(n * NPX_ROWS));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -409,7 +416,7 @@ struct Ctx {
     std::vector<SceneC> sc(max_scenes);
     for (int s = 0; s < max_scenes; ++s) {
       if (scenes[s].used)
-        sc[s] = make_scene_consts(scenes[s], skin_frac);
+        sc[s] = make_scene_consts(scenes[s]);
       else
         std::memset(&sc[s], 0, sizeof(SceneC));
     }
@@ -646,11 +653,10 @@ struct Ctx {
   // Enqueues one step (no host synchronisation, so it can be captured).
   int enqueue_step() {
     int imax = 0;
-    bool any_vort = false, carry_rho = false;
+    bool carry_rho = false;
     for (const auto& H : scenes) {
       if (!H.used || H.n == 0) continue;
       imax = std::max(imax, H.prm.solver_iterations);
-      any_vort = any_vort || H.prm.vorticity_epsilon != 0.0f;
       carry_rho = carry_rho || H.prm.solver_iterations <= 0;
     }
     NPX_TRY(grid_pass(carry_rho));
@@ -659,36 +665,32 @@ struct Ctx {
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
-    const NbrList L{NBR.p, NCNT.p, DISP.p, slot_cap};
-    const int scap = stage_cap(nbr_block);
-    const size_t shm = sizeof(float4) * (size_t)scap;
-    NPX_TRY(launch("clear_disp", k_clear_disp, cdiv(max_scenes * NPX_MAX_DISP_IT, 256), 256, DISP.p,
-                   max_scenes * NPX_MAX_DISP_IT));
+    const RowMasks R{RMASK.p, RBASE.p};
+    const int* cm = clsmap.p;
+    const int* cst = cstart.p;
+    const int* cel = CELL.p;
+    const int rm = xcd_remap;
     int cur = 0;
     const int npass = std::max(imax, 1);
     for (int it = 0; it < npass; ++it) {
       if (fast)
-        NPX_TRY(launch_shm("density_lambda", k_density_lambda<true>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
-                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it, L, NBR.p, NCNT.p, scap));
+        NPX_TRY(launch("density_lambda", k_density_lambda<true>, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p,
+                       RHO.p, LAM.p, it, R, rm));
       else
-        NPX_TRY(launch_shm("density_lambda", k_density_lambda<false>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
-                       (const int*)cstart.p, (const int*)CELL.p, xs[cur]->p, RHO.p, LAM.p, it, L, NBR.p, NCNT.p, scap));
+        NPX_TRY(launch("density_lambda", k_density_lambda<false>, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p,
+                       RHO.p, LAM.p, it, R, rm));
       if (fast)
-        NPX_TRY(launch_shm("delta", k_delta<true>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
-                       (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
-                       (const float*)RHO.p, VT.p, L, DISP.p, scap));
+        NPX_TRY(launch("delta", k_delta<true>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
+                       xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
       else
-        NPX_TRY(launch_shm("delta", k_delta<false>, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
-                       (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, xs[1 - cur]->p, it,
-                       (const float4*)X.p, (const float*)RHO.p, VT.p, L, DISP.p, scap));
+        NPX_TRY(launch("delta", k_delta<false>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
+                       xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
       cur = 1 - cur;
     }
-    if (any_vort)
-      NPX_TRY(launch_shm("omega", k_omega, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p, (const int*)cstart.p,
-                     (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, L, scap));
-    NPX_TRY(launch_shm("xsph_commit", k_xsph_commit, blocks, nbr_block, shm, sc, bs, (const int*)clsmap.p,
-                   (const int*)cstart.p, (const int*)CELL.p, (const float4*)xs[cur]->p, (const float4*)VT.p,
-                   (const float4*)OM.p, X.p, V.p, L, scap));
+    NPX_TRY(launch("omega_xsph", k_omega_xsph, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
+                   (const float4*)VT.p, OM.p, VX.p, R, rm));
+    NPX_TRY(launch("vort_commit", k_vort_commit, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
+                   (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p, V.p, R, rm));
     xs_final = cur;
     return NPX_OK;
   }
@@ -718,12 +720,12 @@ struct Ctx {
     // pointer pairs are order-insensitive (they swap every step)
     auto lo = [](const void* a, const void* b) { return a < b ? a : b; };
     auto hi = [](const void* a, const void* b) { return a < b ? b : a; };
-    return {(const void*)(intptr_t)hw_blocks, (const void*)(intptr_t)nbr_block, (const void*)(intptr_t)total_cls(),
+    return {(const void*)(intptr_t)hw_blocks, (const void*)(intptr_t)nbr_block, (const void*)(intptr_t)xcd_remap, (const void*)(intptr_t)total_cls(),
             (const void*)(intptr_t)imax,
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
-            lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, NBR.p, NCNT.p, DISP.p};
+            lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, RMASK.p, RBASE.p};
   }
 
   void drop_graphs() {

@@ -275,237 +275,244 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 
 // ------------------------------------------------------ PBF passes
 #define NPX_SCENE_PROLOGUE()                                 \
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];                       \
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];           \
   if (s < 0) return;                                         \
   const SceneC& S = scenes[s];                               \
-  const int q = npx_slot();        \
+  const int q = npx_slot();                                  \
   if (q >= S.seg_off + S.n) return;                          \
   const int shift = cstart[S.cls_base] - S.seg_off;          \
   const int wxy = S.wx * S.wy;                               \
   const int* __restrict__ cm = clsmap + S.cls_base;          \
   const int l = cell[q];
 
-// Row staging.  The particles of a workgroup occupy a run of cells
-// [cmin, cmax] (slots are sorted by cell), so for every (dz, dy) offset the
-// candidates any of them visits lie in one slot range: the cells
-// [cmin + off - 1, cmax + off + 1].  The block copies those nine ranges of x*
-// into LDS once (about ten float4 per particle instead of ~216 cached global
-// loads per particle) and each thread sweeps its own ranges from LDS in the
-// reference's order.  A particle whose ranges are not inside the staged ones
-// (cells whose Teschner keys alias, or a block whose rows exceed the LDS
-// budget) reads from global memory instead; the values are the same.
-struct RowStage {
-  int gb[9], ge[9], lo[10];
-  int ok;
-  int cmin, cmax;
-};
-
-// Every thread of the block must call this (it synchronises).
-__device__ __forceinline__ void stage_rows(RowStage& st, float4* __restrict__ lds, int cap, bool active, int l,
-                                           int wx, int wxy, const int* __restrict__ cm,
-                                           const int* __restrict__ cstart, int shift,
-                                           const float4* __restrict__ XS) {
-  if (threadIdx.x == 0) {
-    st.cmin = 0x7FFFFFFF;
-    st.cmax = -1;
-  }
-  __syncthreads();
-  if (active) {
-    atomicMin(&st.cmin, l);
-    atomicMax(&st.cmax, l);
-  }
-  __syncthreads();
-  const bool any = st.cmin <= st.cmax;
-  if (threadIdx.x < 9) {
-    const int r = threadIdx.x;
-    const int off = (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
-    int gb = 0, ge = 0;
-    if (any) {
-      gb = cstart[cm[st.cmin + off - 1]] - shift;
-      ge = cstart[cm[st.cmax + off + 1] + 1] - shift;
-      if (ge < gb) ge = gb;
-    }
-    st.gb[r] = gb;
-    st.ge[r] = ge;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = 0;
-    for (int r = 0; r < 9; ++r) {
-      st.lo[r] = t;
-      t += st.ge[r] - st.gb[r];
-    }
-    st.lo[9] = t;
-    st.ok = any && t <= cap;
-  }
-  __syncthreads();
-  if (st.ok) {
-    const int total = st.lo[9];
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-      int r = 0;
-      while (i >= st.lo[r + 1]) ++r;
-      lds[i] = XS[st.gb[r] + (i - st.lo[r])];
-    }
-  }
-  __syncthreads();
+// XCD-aware block order.  The hardware hands workgroups to the 8 XCDs round
+// robin (workgroup i to XCD i mod 8), and each XCD has its own L2.  With
+// `remap` the neighbour passes process logical block
+//   x * (G / 8) + min(x, G mod 8) + i / 8,   x = i mod 8,
+// a bijection of [0, G) that gives XCD x one contiguous run of slots, so the
+// candidate rows its blocks share stay in its L2.
+__device__ __forceinline__ int npx_lblock(int remap) {
+  const int i = (int)blockIdx.x;
+  if (!remap) return i;
+  const int g = (int)gridDim.x, x = i & 7;
+  return x * (g >> 3) + min(x, g & 7) + (i >> 3);
 }
 
-// The 27-cell sweep of for_each_candidate with x*_j supplied to f from LDS
-// when staged: f(j, x*_j).
-template <class F>
-__device__ __forceinline__ void staged_sweep(const RowStage& st, const float4* __restrict__ lds,
-                                             const float4* __restrict__ XS, int l, int wx, int wxy,
-                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
-                                             F&& f) {
+#define NPX_NBR_PROLOGUE()                                   \
+  const int q = npx_lblock(remap) * (int)blockDim.x + (int)threadIdx.x; \
+  const int s = blk_scene[q / NPX_BLOCK];                    \
+  if (s < 0) return;                                         \
+  const SceneC& S = scenes[s];                               \
+  if (q >= S.seg_off + S.n) return;                          \
+  const int shift = cstart[S.cls_base] - S.seg_off;          \
+  const int wxy = S.wx * S.wy;                               \
+  const int* __restrict__ cm = clsmap + S.cls_base;          \
+  const int l = cell[q];
+
+// Row masks.  The 27-cell sweep of a particle is nine rows (dz outer, dy
+// inner), and a row whose three cells are consecutive classes is one slot
+// range [b, b + 64) at most.  The density pass tests every candidate with the
+// cheap distance check and records, per row, b and a 64-bit mask whose bit t
+// is set when candidate b + t has r^2 < cut.  The SPH arithmetic then runs
+// over the set bits only, in row and bit order, which is the reference's
+// candidate order with the out-of-range candidates (exact zero terms) left
+// out, so every lane of a wave works on a real pair.  Delta-p of the same
+// iteration sees the same x* and reuses the masks; the vorticity-force pass
+// reuses those built at the final x*.  In memory, word
+// ((q / 64) * 9 + row) * 64 + q % 64 keeps a wave's accesses coalesced.  A
+// particle with a row of more than 64 candidates, or a row whose cells alias
+// (not one range), stores b[row 0] = -1 and sweeps its candidates inline.
+// Both visitors fetch the data of G pairs before evaluating any of them, so
+// G loads are in flight per wave instead of one.
+#define NPX_ROWS 9
+
+struct RowMasks {
+  unsigned long long* __restrict__ m;
+  int* __restrict__ b;
+};
+
+struct Rows {
+  unsigned long long m[NPX_ROWS];
+  int b[NPX_ROWS];
+};
+
+__device__ __forceinline__ size_t rm_at(int q, int r) {
+  return ((size_t)(q >> 6) * NPX_ROWS + r) * 64 + (q & 63);
+}
+
+template <bool FAST>
+__device__ __forceinline__ float dist2(float rx, float ry, float rz) {
+  return FAST ? __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)) : sqn3(rx, ry, rz);
+}
+
+// Builds the row masks of a particle at xi in registers; false when it must
+// use the inline sweep.
+template <bool FAST>
+__device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, const float4* __restrict__ XS, int l,
+                                           int wx, int wxy, const int* __restrict__ cm,
+                                           const int* __restrict__ cstart, int shift) {
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1];
+    const int c1 = cm[base];
+    const int c2 = cm[base + 1];
+    const int b = cstart[c0] - shift;
+    const int e = cstart[c0 + 3] - shift;
+    ok = ok && (c1 == c0 + 1) && (c2 == c0 + 2) && (e - b <= 64);
+    unsigned long long mk = 0;
+    if (ok) {
+      for (int j = b; j < e; j += 4) {
+        const int j1 = min(j + 1, e - 1), j2 = min(j + 2, e - 1), j3 = min(j + 3, e - 1);
+        const float4 a0 = XS[j], a1 = XS[j1], a2 = XS[j2], a3 = XS[j3];
+        const unsigned long long t0 = dist2<FAST>(xi.x - a0.x, xi.y - a0.y, xi.z - a0.z) < cut;
+        const unsigned long long t1 = (j + 1 < e) && dist2<FAST>(xi.x - a1.x, xi.y - a1.y, xi.z - a1.z) < cut;
+        const unsigned long long t2 = (j + 2 < e) && dist2<FAST>(xi.x - a2.x, xi.y - a2.y, xi.z - a2.z) < cut;
+        const unsigned long long t3 = (j + 3 < e) && dist2<FAST>(xi.x - a3.x, xi.y - a3.y, xi.z - a3.z) < cut;
+        const int o = j - b;
+        mk |= (t0 << o) | (t1 << ((o + 1) & 63)) | (t2 << ((o + 2) & 63)) | (t3 << ((o + 3) & 63));
+      }
+    }
+    w.m[r] = mk;
+    w.b[r] = b;
+  }
+  if (!ok) w.b[0] = -1;
+  return ok;
+}
+
+__device__ __forceinline__ void store_rows(const RowMasks& R, int q, const Rows& w) {
+  const size_t a = rm_at(q, 0);
+#pragma unroll
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    R.m[a + 64 * r] = w.m[r];
+    R.b[a + 64 * r] = w.b[r];
+  }
+}
+
+__device__ __forceinline__ bool load_rows(const RowMasks& R, int q, Rows& w) {
+  const size_t a = rm_at(q, 0);
+#pragma unroll
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    w.m[r] = R.m[a + 64 * r];
+    w.b[r] = R.b[a + 64 * r];
+  }
+  return w.b[0] >= 0;
+}
+
+// f(j, ld(j)) over the set bits of the row masks in order (consumes w).
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_rows(Rows& w, L&& ld, F&& f) {
+  unsigned long long mk = w.m[0];
+  int b = w.b[0];
+  int left = NPX_ROWS - 1;
+  auto next = [&]() -> int {
+    while (mk == 0 && left > 0) {
+      mk = w.m[1];
+      b = w.b[1];
+#pragma unroll
+      for (int r = 1; r < NPX_ROWS - 1; ++r) {
+        w.m[r] = w.m[r + 1];
+        w.b[r] = w.b[r + 1];
+      }
+      --left;
+    }
+    if (mk == 0) return -1;
+    const int j = b + (int)__builtin_ctzll(mk);
+    mk &= mk - 1;
+    return j;
+  };
+  for (;;) {
+    int js[G];
+    js[0] = next();
+    if (js[0] < 0) break;
+#pragma unroll
+    for (int u = 1; u < G; ++u) js[u] = next();
+    decltype(ld(0)) v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = ld(js[u] < 0 ? js[0] : js[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      if (js[u] >= 0) f(js[u], v[u]);
+  }
+}
+
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_range(int b, int e, L&& ld, F&& f) {
+  int j = b;
+  for (; j + G <= e; j += G) {
+    decltype(ld(0)) v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = ld(j + u);
+#pragma unroll
+    for (int u = 0; u < G; ++u) f(j + u, v[u]);
+  }
+  for (; j < e; ++j) f(j, ld(j));
+}
+
+// f(j, ld(j)) over every candidate in the reference's order (the 27-cell
+// sweep of for_each_candidate); f applies the cut itself.
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_all(int l, int wx, int wxy, const int* __restrict__ cm,
+                                          const int* __restrict__ cstart, int shift, L&& ld, F&& f) {
 #pragma unroll 1
-  for (int r = 0; r < 9; ++r) {
+  for (int r = 0; r < NPX_ROWS; ++r) {
     const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
     const int c0 = cm[base - 1];
     const int c1 = cm[base];
     const int c2 = cm[base + 1];
     if (c1 == c0 + 1 && c2 == c0 + 2) {
-      const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
-      if (st.ok && b >= st.gb[r] && e <= st.ge[r]) {
-        const int d = st.lo[r] - st.gb[r];
-        for (int j = b; j < e; ++j) f(j, lds[j + d]);
-      } else {
-        for (int j = b; j < e; ++j) f(j, XS[j]);
-      }
+      visit_range<G>(cstart[c0] - shift, cstart[c0 + 3] - shift, ld, f);
     } else {
-      const int cc[3] = {c0, c1, c2};
 #pragma unroll 1
       for (int k = 0; k < 3; ++k) {
-        const int b = cstart[cc[k]] - shift, e = cstart[cc[k] + 1] - shift;
-        for (int j = b; j < e; ++j) f(j, XS[j]);
+        const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
+        visit_range<G>(cstart[c] - shift, cstart[c + 1] - shift, ld, f);
       }
     }
   }
-}
-
-// Staged LDS capacity (float4 entries) for a block size: nine rows of about
-// (B/8 + 2) cells of 8 particles, with headroom for compressed regions.
-__host__ __device__ constexpr int stage_cap(int block) { return 12 * (block + 16); }
-
-// Neighbour lists.  The first density pass of a step stores, in the
-// reference's visiting order (duplicates and self kept), every candidate
-// within the skin radius; later passes visit only those.  Terms the list
-// drops are candidates beyond cut2, whose poly6 and spiky_grad are exact
-// zeros, as long as no particle moved more than half the skin since the list
-// was built -- checked here from the per-iteration maximum displacements of
-// the scene.  Otherwise (or when a particle overflowed NPX_NBR_CAP) the full
-// candidate sweep runs, so results are identical either way.
-struct NbrList {
-  const int* __restrict__ idx;  // entry k of slot q at idx[k * stride + q]
-  const int* __restrict__ cnt;
-  const int* __restrict__ disp; // float bits, [scene][iteration]
-  int stride;
-};
-
-__device__ __forceinline__ bool list_usable(const NbrList& L, const SceneC& S, int s, int upto) {
-  if (S.skin2 <= 0.0f || S.iters < 1 || upto > NPX_MAX_DISP_IT) return false;
-  float sum = 0.0f;
-  for (int t = 0; t < upto; ++t) sum += __int_as_float(L.disp[s * NPX_MAX_DISP_IT + t]);
-  return sum <= S.disp_limit;
-}
-
-template <class F>
-__device__ __forceinline__ void for_each_neighbour(bool use_list, const NbrList& L, int q, int l, int wx, int wxy,
-                                                   const int* __restrict__ cm, const int* __restrict__ cstart,
-                                                   int shift, F&& f) {
-  if (use_list) {
-    const int n = L.cnt[q];
-    if (n <= NPX_NBR_CAP) {
-      const int* p = L.idx + q;
-      for (int k = 0; k < n; ++k) f(p[(size_t)k * L.stride]);
-      return;
-    }
-  }
-  for_each_candidate(l, wx, wxy, cm, cstart, shift, f);
-}
-
-// Block-wide maximum of a non-negative float into disp[slot] (one atomic per
-// wave).  Every thread of the block must reach it.
-__device__ __forceinline__ void block_max_disp(float d, int* slot) {
-  for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o, 64));
-  if ((threadIdx.x & 63) == 0 && d > 0.0f) atomicMax(slot, __float_as_int(d));
-}
-
-// Clears the displacement slots at the start of a step.
-__global__ void k_clear_disp(int* __restrict__ disp, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) disp[i] = 0;
 }
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
-// (:144-179): one sweep accumulates rho_i over all candidates (self and
-// duplicates included) and, for j != i, sum |grad_j|^2 and grad_i; lambda is
-// then 0 when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  Iteration 0
-// sweeps the 27 cells and writes the neighbour list.
+// (:144-179): rho_i sums m W over all in-range candidates (self and
+// duplicates included); for j != i, sum |grad_j|^2 and grad_i; lambda is 0
+// when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  Builds the iteration's
+// row masks.
 template <bool FAST>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
-                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, NbrList L, int* __restrict__ nbr_out,
-                 int* __restrict__ cnt_out, int scap) {
-  extern __shared__ float4 lds[];
-  __shared__ RowStage st;
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, RowMasks R, int remap) {
+  NPX_NBR_PROLOGUE();
   if (iter >= S.iters) return;
-  const int q = npx_slot();
-  const bool active = q < S.seg_off + S.n;
-  const int shift = cstart[S.cls_base] - S.seg_off;
-  const int wxy = S.wx * S.wy;
-  const int* __restrict__ cm = clsmap + S.cls_base;
-  const int l = active ? cell[q] : 0;
-  const bool use_list = iter > 0 && list_usable(L, S, s, iter);
-  if (!use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
-  if (!active) return;
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
+  const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
   const float cut = FAST ? h2 : S.cut2;
-  const float skin2 = S.skin2;
-  const bool build = (iter == 0) && skin2 > 0.0f;
   float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-  int nn = 0;
+  auto ld = [&](int j) { return XS[j]; };
   auto body = [&](int j, float4 xj) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    float r_sq;
-    if (FAST)
-      r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-    else
-      r_sq = sqn3(rx, ry, rz);
-    if (build && r_sq < skin2) {
-      if (nn < NPX_NBR_CAP) nbr_out[(size_t)nn * L.stride + q] = j;
-      ++nn;
-    }
-    if (r_sq >= cut) return;  // poly6 and spiky_grad both vanish: adding 0 changes nothing
+    const float r_sq = dist2<FAST>(rx, ry, rz);
+    if (r_sq >= cut) return;  // only reached on the inline sweep
     rho += m * poly6(r_sq, h2, c6);
     if (j == q) return;
     bool in;
-    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
     if (!in) return;
     const float ax = mr * (co * rx), ay = mr * (co * ry), az = mr * (co * rz);
-    sgs += FAST ? __builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az)) : sqn3(ax, ay, az);
+    sgs += dist2<FAST>(ax, ay, az);
     gx += ax;
     gy += ay;
     gz += az;
   };
-  if (use_list) {
-    const int n = L.cnt[q];
-    if (n <= NPX_NBR_CAP) {
-      for (int k = 0; k < n; ++k) {
-        const int j = L.idx[(size_t)k * L.stride + q];
-        body(j, XS[j]);
-      }
-    } else {
-      for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
-    }
-  } else {
-    staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
-  }
-  if (build) cnt_out[q] = nn;
+  Rows w;
+  const bool ok = sweep_rows<FAST>(w, xi, cut, XS, l, S.wx, wxy, cm, cstart, shift);
+  store_rows(R, q, w);
+  if (ok)
+    visit_rows<4>(w, ld, body);
+  else
+    visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
   float lam = 0.0f;
@@ -537,215 +544,179 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 }
 
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
-// Jacobi apply (:71-77) into the other x* buffer.  On the last iteration it
-// also performs the velocity update with the h/dt speed clamp (:81-93) and
-// stores v in VT with w = m / rho~ (rho~ = rho if > 1e-6 else rho0) for XSPH.
-// Records the scene's maximum displacement of this iteration.
+// Jacobi apply (:71-77) into the other x* buffer, over the density pass's row
+// masks.  On the last iteration it also performs the velocity update with
+// the h/dt speed clamp (:81-93).
 template <bool FAST>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
-        float4* __restrict__ VT, NbrList L, int* __restrict__ disp, int scap) {
-  extern __shared__ float4 lds[];
-  __shared__ RowStage st;
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
-  const int q = npx_slot();
-  const bool active = q < S.seg_off + S.n;
-  const int shift = cstart[S.cls_base] - S.seg_off;
-  const int wxy = S.wx * S.wy;
-  const int* __restrict__ cm = clsmap + S.cls_base;
-  const int l = active ? cell[q] : 0;
-  const bool iterate = iter < S.iters;
-  const bool use_list = iterate && list_usable(L, S, s, iter);
-  if (iterate && !use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
-  float moved = 0.0f;
-  if (active) {
-    const float4 xi = XS[q];
-    const bool last = (iter == max(S.iters, 1) - 1);
-    if (!iterate) {
-      // scenes with fewer iterations than the batch maximum pass x* through
-      // so every scene ends in the same ping-pong buffer
-      XSn[q] = xi;
-      if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
-    } else {
-      const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
-      const float wdq = S.wdq, negk = -S.corr_k;
-      const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
-      const float cut = FAST ? h2 : S.cut2;
-      float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-      auto body = [&](int j, float4 xj) {
-        if (j == q) return;
-        const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-        float r_sq;
-        if (FAST)
-          r_sq = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-        else
-          r_sq = sqn3(rx, ry, rz);
-        if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
-        bool in;
-        const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff(r_sq, h, cs, &in);
-        if (!in) return;
-        const float ls = xi.w + xj.w;
-        float sc = 0.0f;
-        if (ls < -1e-10f && scorr_on) {
-          float ratio = poly6(r_sq, h2, c6) / wdq;
-          ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
-          float rp = ratio * ratio;
-          if (n4) rp = rp * rp;
-          sc = negk * rp;
-        }
-        const float coef = (ls + sc) * m;
-        if (FAST) {
-          const float cc = coef * co;
-          dx = __builtin_fmaf(cc, rx, dx);
-          dy = __builtin_fmaf(cc, ry, dy);
-          dz = __builtin_fmaf(cc, rz, dz);
-        } else {
-          dx += coef * (co * rx);
-          dy += coef * (co * ry);
-          dz += coef * (co * rz);
-        }
-      };
-      if (use_list) {
-        const int n = L.cnt[q];
-        if (n <= NPX_NBR_CAP) {
-          for (int k = 0; k < n; ++k) {
-            const int j = L.idx[(size_t)k * L.stride + q];
-            body(j, XS[j]);
-          }
-        } else {
-          for_each_candidate(l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
-        }
-      } else {
-        staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
-      }
-      const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
-      XSn[q] = make_float4(px, py, pz, 0.0f);
-      if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
-      const float ex = px - xi.x, ey = py - xi.y, ez = pz - xi.z;
-      moved = sqrtf(sqn3(ex, ey, ez));
-      if (!(moved <= 3.0e38f)) moved = 3.0e38f;  // NaN / inf: make the lists unusable
-    }
-  }
-  if (S.skin2 > 0.0f && iter < NPX_MAX_DISP_IT) block_max_disp(moved, disp + s * NPX_MAX_DISP_IT + iter);
-}
-
-// Vorticity (this project's definition; the reference declares
-// vorticity_epsilon but never uses it): omega_i = sum_{j!=i} (m/rho~_j)
-// (v_j - v_i) x gradW(x*_j - x*_i), stored with |omega_i| in w.
-__global__ void __launch_bounds__(NPX_MAX_BLOCK)
-k_omega(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-        const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-        const float4* __restrict__ VT, float4* __restrict__ OM, NbrList L, int scap) {
-  extern __shared__ float4 lds[];
-  __shared__ RowStage st;
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;
-  const SceneC& S = scenes[s];
-  const int q = npx_slot();
-  const bool active = q < S.seg_off + S.n;
-  const int shift = cstart[S.cls_base] - S.seg_off;
-  const int wxy = S.wx * S.wy;
-  const int* __restrict__ cm = clsmap + S.cls_base;
-  const int l = active ? cell[q] : 0;
-  const bool use_list = list_usable(L, S, s, S.iters);
-  if (!use_list) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
-  if (!active) return;
+        float4* __restrict__ VT, RowMasks R, int remap) {
+  NPX_NBR_PROLOGUE();
   const float4 xi = XS[q];
-  const float4 vi = VT[q];
-  const float h = S.h, cs = S.cs;
-  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  const bool last = (iter == max(S.iters, 1) - 1);
+  if (iter >= S.iters) {
+    // scenes with fewer iterations than the batch maximum pass x* through so
+    // every scene ends in the same ping-pong buffer
+    XSn[q] = xi;
+    if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
+    return;
+  }
+  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
+  const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+  const float wdq = S.wdq, negk = -S.corr_k;
+  const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
+  const float cut = FAST ? h2 : S.cut2;
+  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+  auto ld = [&](int j) { return XS[j]; };
   auto body = [&](int j, float4 xj) {
+    const float ls = xi.w + xj.w;
     if (j == q) return;
-    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-    const float r_sq = sqn3(rx, ry, rz);
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r_sq = dist2<FAST>(rx, ry, rz);
+    if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
     bool in;
-    const float co = spiky_coeff(r_sq, h, cs, &in);
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
     if (!in) return;
-    const float4 vj = VT[j];
-    const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
-    const float gx = co * rx, gy = co * ry, gz = co * rz;
-    const float cx = ay * gz - az * gy, cy = az * gx - ax * gz, cz = ax * gy - ay * gx;
-    wx += vj.w * cx;
-    wy += vj.w * cy;
-    wz += vj.w * cz;
+    float sc = 0.0f;
+    if (ls < -1e-10f && scorr_on) {
+      float ratio = poly6(r_sq, h2, c6) / wdq;
+      ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
+      float rp = ratio * ratio;
+      if (n4) rp = rp * rp;
+      sc = negk * rp;
+    }
+    const float coef = (ls + sc) * m;
+    if (FAST) {
+      const float cc = coef * co;
+      dx = __builtin_fmaf(cc, rx, dx);
+      dy = __builtin_fmaf(cc, ry, dy);
+      dz = __builtin_fmaf(cc, rz, dz);
+    } else {
+      dx += coef * (co * rx);
+      dy += coef * (co * ry);
+      dz += coef * (co * rz);
+    }
   };
-  if (use_list) {
-    for_each_neighbour(true, L, q, l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
-  } else {
-    staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
-  }
-  OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
+  Rows w;
+  if (load_rows(R, q, w))
+    visit_rows<4>(w, ld, body);
+  else
+    visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+  const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
+  XSn[q] = make_float4(px, py, pz, 0.0f);
+  if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
 }
 
-// apply_xsph_viscosity (pbf_solver.cpp:225-250) + vorticity force + commit
-// with the domain clamp (:102-126), writing the new committed X / V.
+struct XV {
+  float4 x, v;
+};
+
+// apply_xsph_viscosity (pbf_solver.cpp:225-250) fused with the vorticity
+// curl (this project's definition; the reference declares vorticity_epsilon
+// but never uses it): one sweep at the final x* gives
+//   c_i     = sum_{j!=i, r^2<h^2} (m/rho~_j) (v_j - v_i) W(x*_i - x*_j)
+//   omega_i = sum_{j!=i} (m/rho~_j) (v_j - v_i) x gradW(x*_j - x*_i)
+// VX = v_i + xsph_c * c_i (v_i when XSPH is off); OM = (omega_i, |omega_i|).
+// With vorticity the row masks are stored for the force pass.
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
-k_xsph_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-              const float4* __restrict__ VT, const float4* __restrict__ OM, float4* __restrict__ X,
-              float4* __restrict__ V, NbrList L, int scap) {
-  extern __shared__ float4 lds[];
-  __shared__ RowStage st;
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;
-  const SceneC& S = scenes[s];
-  const int q = npx_slot();
-  const bool active = q < S.seg_off + S.n;
-  const int shift = cstart[S.cls_base] - S.seg_off;
-  const int wxy = S.wx * S.wy;
-  const int* __restrict__ cm = clsmap + S.cls_base;
-  const int l = active ? cell[q] : 0;
-  const bool use_list = list_usable(L, S, s, S.iters);
-  if (!use_list && (S.xsph_on || S.vort_on)) stage_rows(st, lds, scap, active, l, S.wx, wxy, cm, cstart, shift, XS);
-  if (!active) return;
+k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+             const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+             const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, RowMasks R,
+             int remap) {
+  NPX_NBR_PROLOGUE();
   const float4 xi = XS[q];
   const float4 vi = VT[q];
-  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs;
-  float vx = vi.x, vy = vi.y, vz = vi.z;
-  auto sweep = [&](auto&& body) {
-    if (use_list)
-      for_each_neighbour(true, L, q, l, S.wx, wxy, cm, cstart, shift, [&](int j) { body(j, XS[j]); });
-    else
-      staged_sweep(st, lds, XS, l, S.wx, wxy, cm, cstart, shift, body);
-  };
-  if (S.xsph_on) {
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    sweep([&](int j, float4 xj) {
+  const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
+  if (xs || vort) {
+    const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, cut = S.cut2;
+    const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
+    auto body = [&](int j, const XV& d) {
       if (j == q) return;
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r_sq = sqn3(rx, ry, rz);
-      if (r_sq >= h2) return;
-      const float w = poly6(r_sq, h2, c6);
-      const float4 vj = VT[j];
-      cx += (vj.w * (vj.x - vi.x)) * w;
-      cy += (vj.w * (vj.y - vi.y)) * w;
-      cz += (vj.w * (vj.z - vi.z)) * w;
-    });
-    vx = vx + S.xsph_c * cx;
-    vy = vy + S.xsph_c * cy;
-    vz = vz + S.xsph_c * cz;
+      const float rx = d.x.x - xi.x, ry = d.x.y - xi.y, rz = d.x.z - xi.z;
+      const float r_sq = sqn3(rx, ry, rz);  // equals |x*_i - x*_j|^2 bit for bit
+      if (r_sq >= cut) return;              // both terms are exact zeros beyond cut2
+      const float4 vj = d.v;
+      const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
+      if (xs && r_sq < h2) {
+        const float wv = poly6(r_sq, h2, c6);
+        cx += (vj.w * ax) * wv;
+        cy += (vj.w * ay) * wv;
+        cz += (vj.w * az) * wv;
+      }
+      if (vort) {
+        bool in;
+        const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+        if (!in) return;
+        const float gx = co * rx, gy = co * ry, gz = co * rz;
+        wx += vj.w * (ay * gz - az * gy);
+        wy += vj.w * (az * gx - ax * gz);
+        wz += vj.w * (ax * gy - ay * gx);
+      }
+    };
+    Rows w;
+    const bool ok = sweep_rows<false>(w, xi, cut, XS, l, S.wx, wxy, cm, cstart, shift);
+    if (vort) store_rows(R, q, w);
+    if (ok)
+      visit_rows<2>(w, ld, body);
+    else
+      visit_all<2>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+    if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
+    float vx = vi.x, vy = vi.y, vz = vi.z;
+    if (xs) {
+      vx = vx + S.xsph_c * cx;
+      vy = vy + S.xsph_c * cy;
+      vz = vz + S.xsph_c * cz;
+    }
+    VX[q] = make_float4(vx, vy, vz, 0.0f);
+  } else {
+    VX[q] = make_float4(vi.x, vi.y, vi.z, 0.0f);
   }
+}
+
+struct XOW {
+  float4 x;
+  float vw, ow;
+};
+
+// Vorticity force (N = eta / |eta|, eta_i = sum_{j!=i} (m/rho~_j)
+// (|omega_j| - |omega_i|) gradW(x*_i - x*_j); v += dt * eps (N x omega_i))
+// after XSPH, then the commit with the domain clamp (pbf_solver.cpp:102-126)
+// into the new committed X / V.
+__global__ void __launch_bounds__(NPX_MAX_BLOCK)
+k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+              const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
+              float4* __restrict__ X, float4* __restrict__ V, RowMasks R, int remap) {
+  NPX_NBR_PROLOGUE();
+  const float4 xi = XS[q];
+  const float4 vx4 = VX[q];
+  float vx = vx4.x, vy = vx4.y, vz = vx4.z;
   if (S.vort_on) {
+    const float h = S.h, cs = S.cs, sp_lo = S.sp_lo, sp_hi = S.sp_hi;
     const float4 oi = OM[q];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
-    sweep([&](int j, float4 xj) {
+    auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
+    auto body = [&](int j, const XOW& d) {
       if (j == q) return;
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
       const float r_sq = sqn3(rx, ry, rz);
       bool in;
-      const float co = spiky_coeff(r_sq, h, cs, &in);
+      const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
       if (!in) return;
-      const float4 vj = VT[j];
-      const float4 oj = OM[j];
-      const float sc = vj.w * (oj.w - oi.w);
+      const float sc = d.vw * (d.ow - oi.w);
       ex += sc * (co * rx);
       ey += sc * (co * ry);
       ez += sc * (co * rz);
-    });
+    };
+    Rows w;
+    if (load_rows(R, q, w))
+      visit_rows<4>(w, ld, body);
+    else
+      visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
     const float el = sqrtf(sqn3(ex, ey, ez));
     if (el > 1e-9f) {
       const float inv = 1.0f / el;

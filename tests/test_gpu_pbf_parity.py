@@ -166,19 +166,73 @@ def test_grid_query_matches_oracle(gpu):
 
 
 def test_tunables_do_not_change_results(gpu):
-    """Block size and neighbour-list skin change speed only: a rollout that
+    """Block size and XCD block order change speed only: a rollout that
     switches tunables every step is bitwise equal to the oracle."""
     sc = scenes.dam_break_8k()
     prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
     sc = dataclasses.replace(sc, params=prm)
-    _say("case: tunables (block size / skin) rollout")
+    _say("case: tunables (block size / XCD order) rollout")
     c = _ctx(sc)
-    for k, (b, skin) in enumerate([(64, 0.0), (128, 0.1), (256, 0.2), (64, 0.3), (128, 0.0), (256, 0.1)] * 2):
-        c.set_tuning(b, skin)
+    for k, (b, remap) in enumerate([(64, 0), (128, 1), (256, 1), (64, 1), (128, 0), (256, 0)] * 2):
+        c.set_tuning(b, remap)
         c.step(sc.dt, sc.gravity, sc.mass, 1)
     gp, gv, gr, gl = c.get_particles()
     op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 12)
     np.testing.assert_array_equal(gp, op)
     np.testing.assert_array_equal(gv, ov)
     np.testing.assert_array_equal(gr, orho)
+    c.close()
+
+
+def test_dense_cluster_list_overflow(gpu):
+    """A cluster whose particles all lie within h of each other has more
+    in-range candidates than an in-range list holds, so every pass takes its
+    inline-sweep path; the state stays bitwise equal to the oracle (with
+    vorticity on, so the curl and XSPH passes are covered too)."""
+    rng = np.random.default_rng(11)
+    pos = (np.array([0.5, 0.5, 0.5], np.float32) + rng.uniform(0.0, 0.04, (160, 3))).astype(np.float32)
+    vel = rng.uniform(-0.2, 0.2, (160, 3)).astype(np.float32)
+    prm = scenes.PBFParams(corr_k=0.001, corr_delta_q=0.3, vorticity_epsilon=1e-3, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.05)
+    _say("case: dense cluster n=160 (list overflow), 3 steps")
+    c = capi.Context(prm, capacity=160)
+    c.set_particles(pos, vel)
+    c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+    c.close()
+
+
+def test_coincident_and_near_coincident_pairs(gpu):
+    """Pairs at r = 0 and around the spiky support's inner edge |r| = 1e-7
+    (the state the configured s_corr drives clusters into): the GPU tests
+    the support on r^2 against exact bounds and must agree bit for bit."""
+    rng = np.random.default_rng(12)
+    base = (np.array([0.5, 0.5, 0.5], np.float32) + rng.uniform(0.0, 0.06, (60, 3))).astype(np.float32)
+    pos = [base]
+    pos.append(np.repeat(base[:1], 12, axis=0))                     # 12 copies of particle 0
+    for off in (5e-8, 9.9e-8, 1.0e-7, 1.01e-7, 2e-7, 1e-6):         # offsets across |r| = 1e-7
+        p = base[1].copy()
+        p[0] = np.float32(p[0] + np.float32(off))
+        pos.append(p[None, :])
+    pos = np.concatenate(pos).astype(np.float32)
+    vel = rng.uniform(-0.1, 0.1, pos.shape).astype(np.float32)
+    prm = scenes.PBFParams(corr_k=0.1, corr_delta_q=0.2, vorticity_epsilon=1e-3, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.05)
+    _say("case: coincident / near-coincident pairs n=", pos.shape[0])
+    c = capi.Context(prm, capacity=pos.shape[0])
+    c.set_particles(pos, vel)
+    c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
     c.close()
