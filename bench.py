@@ -5,7 +5,12 @@ Default workload: the single large dam break -- 100x100x100 = 1M FP32
 particles, r = 0.025 m, h = 0.1 m, rest density 1000, 10x5x5 m box (the
 100-particle lattice spans 4.95 m, so the box keeps the 2:1:1 aspect at
 10x5x5 m), seeded +-1 mm jitter, dt = 1/120 s, 4 iterations, XSPH c = 0.01,
-vorticity confinement eps = 1e-4, s_corr k = 0.1, n = 4, dq = 0.2h.
+vorticity confinement eps = 1e-4, s_corr k = 0.1, n = 4, dq = 0.2h (the
+headline `value`).  At k = 0.1 the tensile correction overshoots and the
+fluid collapses into clusters of coincident particles, so the step cost
+grows with the cluster size; the same scene with PBFSettings' default s_corr
+(k = 0.001, dq = 0.3h) stays a liquid and is reported beside it under
+"default_s_corr" (--s-corr picks one or both).
 
 A step is one PBFSolver::step over all particles (predict, grid, 4 x
 (density+lambda, delta-p), velocity update, vorticity, XSPH, commit).  Inputs
@@ -13,7 +18,8 @@ are resident on the device before the timed region.  The timed region is
 bracketed by device synchronisation on both sides; per-kernel durations come
 from HIP events recorded on the context's stream during the same region.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch|coupled]
+                  [--s-corr both|configured|default] [--copy-calibration]
 """
 from __future__ import annotations
 
@@ -37,7 +43,7 @@ HBM_PEAK_GBPS = 8000.0
 # 40 without vorticity) and the vorticity-force+commit pass 68 (x*, v,
 # omega, |omega|, m/rho in; x, v out; 48 without vorticity).
 def pass_bytes(vort: bool):
-    return {"density_lambda": 20, "delta": 28, "omega_xsph": 52 if vort else 40,
+    return {"density_lambda": 20, "density_lambda_build": 20, "delta": 28, "omega_xsph": 52 if vort else 40,
             "vort_commit": 68 if vort else 48, "predict_bin": 40, "gather": 56}
 
 
@@ -96,6 +102,29 @@ def run_single(sc, steps, warmup, flags, cpu_seconds):
     return out
 
 
+def copy_calibration(nbytes=256 << 20, reps=3):
+    """Device-to-device copies of a known size through the HIP runtime: the
+    reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "copies": reps,
+            "copy_GBps_read_plus_write": round(2 * nbytes * reps / wall / 1e9, 1)}
+
+
 def run_batch(count, steps, warmup, flags, loop_steps):
     """Batched scenes (one launch per pass over every scene, graph-replayed)
     against the plain per-scene loop of single-scene contexts."""
@@ -108,6 +137,12 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=count, flags=flags)
     for s in scs:
         b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities)
+    from novaphy_amd import autotune
+    # tuning steps are real steps of the workload (results are bitwise
+    # independent of the tunables), so they count towards the warm-up
+    best, tried = autotune.autotune(b, steps_per_candidate=1)
+    log(f"[bench] autotuner: block {best[0]}, xcd order {best[1]}, skin {best[2]}h "
+        f"({', '.join(f'{k[0]}/{k[1]}/{k[2]}: {v:.1f} ms' for k, v in tried.items())})")
     b.step(warmup)
     b.synchronize()
     t0 = time.perf_counter()
@@ -116,7 +151,8 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     b.close()
     value = total * steps / wall
     out = {"value": value, "ms_per_step": wall * 1e3 / steps, "particles": total, "scenes": count,
-           "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99))}
+           "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99)),
+           "tuning": {"block": best[0], "xcd_order": best[1], "skin": best[2]}}
     # plain per-scene loop: one single-scene context per scene, stepped in turn
     log(f"[bench] per-scene loop: {count} contexts, {loop_steps} steps")
     ctxs = []
@@ -253,13 +289,21 @@ def main():
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
     ap.add_argument("--corr-k", type=float, default=None,
                     help="override the s_corr k of the workload (the configured value is 0.1)")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0,
-                    help="budget of the host CPU baseline sample (0 disables)")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="budget of the host CPU baseline samples (0 disables)")
+    ap.add_argument("--s-corr", default="both", choices=["both", "configured", "default"],
+                    help="single-scene workloads: the configured s_corr (k=0.1, dq=0.2h; the headline), "
+                         "PBFSettings' default (k=0.001, dq=0.3h), or both")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 256 MiB device-to-device copies (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(independent scenes shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration()), flush=True)
+        return
     from novaphy_amd import capi, scenes
     if capi.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
@@ -278,6 +322,7 @@ def main():
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"]},
             "speedup_vs_per_scene_loop": round(r["speedup_vs_loop"], 2),
+            "tuning": r["tuning"],
         }
         print(json.dumps(line), flush=True)
         return
@@ -295,12 +340,28 @@ def main():
         }
         print(json.dumps(line), flush=True)
         return
-    sc = scenes.dam_break_1m() if args.workload == "1m" else scenes.dam_break_8k()
+    import dataclasses
+    base = scenes.dam_break_1m() if args.workload == "1m" else scenes.dam_break_8k()
     if args.corr_k is not None:
-        import dataclasses
-        sc = dataclasses.replace(sc, params=dataclasses.replace(sc.params, corr_k=args.corr_k))
-    r = run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds)
-    p = sc.params
+        base = dataclasses.replace(base, params=dataclasses.replace(base.params, corr_k=args.corr_k))
+    variants = []
+    if args.s_corr in ("both", "configured"):
+        variants.append(("configured", base))
+    if args.s_corr in ("both", "default"):
+        dflt = scenes.PBFParams()
+        variants.append(("default", dataclasses.replace(base, params=dataclasses.replace(
+            base.params, corr_k=dflt.corr_k, corr_delta_q=dflt.corr_delta_q, corr_n=dflt.corr_n))))
+    cpu_s = args.cpu_seconds / len(variants)
+    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, cpu_s)) for name, sc in variants]
+
+    def describe(sc):
+        p = sc.params
+        return (f"{sc.name}: {sc.n} FP32 particles, box {p.domain_upper}, h={p.kernel_radius}, "
+                f"{p.solver_iterations} iterations, vorticity {p.vorticity_epsilon}, "
+                f"s_corr k={p.corr_k} n={p.corr_n} dq={p.corr_delta_q}h, dt=1/120, "
+                f"{'fast-math' if args.fast_math else 'reference operation order'}")
+
+    name, sc, r = results[0]
     line = {
         "metric": METRIC,
         "value": r["value"],
@@ -314,13 +375,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"{sc.name}: {sc.n} FP32 particles, box {p.domain_upper}, h={p.kernel_radius}, "
-                        f"{p.solver_iterations} iterations, vorticity {p.vorticity_epsilon}, "
-                        f"s_corr k={p.corr_k} n={p.corr_n} dq={p.corr_delta_q}h, dt=1/120, "
-                        f"{'fast-math' if args.fast_math else 'reference operation order'}",
-            "particles": sc.n,
-        },
+        "config": {"workload": describe(sc), "particles": sc.n, "s_corr": name},
         "roofline": r["roofline"],
         "step_roofline": r["step_roofline"],
         "kernels": r["kernels"],
@@ -328,6 +383,13 @@ def main():
     if "cpu_baseline" in r:
         line["cpu_baseline"] = r["cpu_baseline"]
         line["speedup_vs_cpu"] = round(r["value"] / r["cpu_baseline"]["value"], 1)
+    for name2, sc2, r2 in results[1:]:
+        extra = {"workload": describe(sc2), "value": r2["value"], "ms_per_step": r2["ms_per_step"],
+                 "roofline": r2["roofline"], "step_roofline": r2["step_roofline"], "kernels": r2["kernels"]}
+        if "cpu_baseline" in r2:
+            extra["cpu_baseline"] = r2["cpu_baseline"]
+            extra["speedup_vs_cpu"] = round(r2["value"] / r2["cpu_baseline"]["value"], 1)
+        line[f"{name2}_s_corr"] = extra
     print(json.dumps(line), flush=True)
 
 

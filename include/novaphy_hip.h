@@ -170,12 +170,15 @@ int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, floa
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
 
 /* Launch tunables of a context (any context kind): block size of the
- * neighbour passes (64, 128 or 256; default 256) and the XCD-aware block
- * order (1, the default: each of the 8 XCDs sweeps one contiguous run of
+ * neighbour passes (64, 128 or 256; default 256), the XCD-aware block order
+ * (1, the default: each of the 8 XCDs sweeps one contiguous run of
  * particles, keeping shared neighbour rows in its own L2; 0: hardware
- * round-robin order).  Results are bitwise independent of both; only speed
- * changes.  No reference counterpart (used by the autotuner). */
-int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap);
+ * round-robin order) and the skin of the per-step candidate lists as a
+ * fraction of the kernel radius ([0.02, 0.5], default 0.1: a larger skin
+ * keeps the lists valid through larger per-step motion, a smaller one
+ * visits fewer candidates).  Results are bitwise independent of all three;
+ * only speed changes.  No reference counterpart (used by the autotuner). */
+int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float skin_fraction);
 
 /* npx_batch_step with a device timestamp after every step: step_ms[k] is the
  * duration of step k between consecutive HIP events on the context's stream

@@ -1,9 +1,11 @@
 """Autotuner for the launch tunables of a device context.
 
-The tunables are the block size of the neighbour passes (64 / 128 / 256) and
-the XCD-aware block order (each of the 8 XCDs sweeps one contiguous run of
+The tunables are the block size of the neighbour passes (64 / 128 / 256), the
+XCD-aware block order (each of the 8 XCDs sweeps one contiguous run of
 particles so the neighbour rows its blocks share stay in its own L2, or the
-hardware's round-robin order).  The cell size is not tunable: the reference's
+hardware's round-robin order) and the skin of the per-step candidate lists
+(a fraction of h: larger keeps the lists valid through more motion per step,
+smaller visits fewer candidates).  The cell size is not tunable: the reference's
 candidate order is defined on cells of size h, and the GPU reproduces it
 exactly.  The sort is a counting sort with a per-cell rank, so there is no
 radix to choose.
@@ -21,8 +23,13 @@ from typing import Dict, Iterable, Optional, Tuple
 
 import numpy as np
 
-DEFAULT_CANDIDATES: Tuple[Tuple[int, int], ...] = (
-    (256, 1), (128, 1), (64, 1), (256, 0), (128, 0))
+Candidate = Tuple[int, int, float]
+DEFAULT_CANDIDATES: Tuple[Candidate, ...] = (
+    (256, 1, 0.1), (128, 1, 0.1), (256, 0, 0.1), (256, 1, 0.05), (256, 1, 0.2))
+
+
+def _norm(c) -> Candidate:
+    return (int(c[0]), int(c[1]), float(c[2]))
 
 
 def _timed_steps(target, nsteps: int, step_args) -> float:
@@ -40,9 +47,9 @@ def _timed_steps(target, nsteps: int, step_args) -> float:
     return float(np.median(times))
 
 
-def autotune(target, candidates: Iterable[Tuple[int, int]] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
+def autotune(target, candidates: Iterable[Candidate] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
              step_args=None, cache_path: Optional[str] = None, key: Optional[str] = None):
-    """Times each (block_size, xcd_remap) candidate on ``target`` (a capi.Batch, or
+    """Times each (block_size, xcd_remap, skin_fraction) candidate on ``target`` (a capi.Batch, or
     a capi.Context with ``step_args`` = (dt, gravity, mass)), applies the
     fastest and returns (best, {candidate: ms_per_step}).  With ``cache_path``
     and ``key`` the choice is stored / reused."""
@@ -50,14 +57,15 @@ def autotune(target, candidates: Iterable[Tuple[int, int]] = DEFAULT_CANDIDATES,
         with open(cache_path) as f:
             cache = json.load(f)
         if key in cache:
-            b, sk = cache[key]["best"]
-            target.set_tuning(int(b), int(sk))
-            return (int(b), int(sk)), {}
-    results: Dict[Tuple[int, int], float] = {}
-    for b, sk in candidates:
-        target.set_tuning(int(b), int(sk))
+            best = _norm(cache[key]["best"])
+            target.set_tuning(*best)
+            return best, {}
+    results: Dict[Candidate, float] = {}
+    for c in candidates:
+        c = _norm(c)
+        target.set_tuning(*c)
         _timed_steps(target, 1, step_args)           # graph capture / warm caches
-        results[(int(b), int(sk))] = _timed_steps(target, steps_per_candidate, step_args)
+        results[c] = _timed_steps(target, steps_per_candidate, step_args)
     best = min(results, key=results.get)
     target.set_tuning(*best)
     if cache_path and key:
@@ -65,7 +73,7 @@ def autotune(target, candidates: Iterable[Tuple[int, int]] = DEFAULT_CANDIDATES,
         if os.path.exists(cache_path):
             with open(cache_path) as f:
                 cache = json.load(f)
-        cache[key] = {"best": list(best), "ms": {f"{b}/{sk}": v for (b, sk), v in results.items()}}
+        cache[key] = {"best": list(best), "ms": {"/".join(str(x) for x in c): v for c, v in results.items()}}
         with open(cache_path, "w") as f:
             json.dump(cache, f, indent=1)
     return best, results

@@ -125,7 +125,7 @@ def load() -> C.CDLL:
         "npx_kernel_times": (C.c_int, [vp, C.c_char_p, i32, vp, vp, i32, P(i32)]),
         "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "npx_scene_status": (C.c_int, [vp, vp, i32]),
-        "npx_set_tuning": (C.c_int, [vp, i32, i32]),
+        "npx_set_tuning": (C.c_int, [vp, i32, i32, f32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -274,8 +274,8 @@ class Context:
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
 
-    def set_tuning(self, block_size: int, xcd_remap: int = 1):
-        self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap)))
+    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1):
+        self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap), C.c_float(skin_fraction)))
 
 
 def _kernel_times(L, h):
@@ -381,8 +381,8 @@ class Batch:
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
 
-    def set_tuning(self, block_size: int, xcd_remap: int = 1):
-        self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap)))
+    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1):
+        self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap), C.c_float(skin_fraction)))
 
 
 def scene_bytes(params, n: int) -> int:

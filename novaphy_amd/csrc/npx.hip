@@ -288,12 +288,11 @@ void destroy(Ctx* c) {
   npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->PSI, &c->POSE, &c->OUT6};
   for (auto* b : f1) b->release();
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
-                          &c->PERM, &c->RBASE, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->PERM, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->DISPB, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
                           &c->ATKEY, &c->APERM, &c->aux_clsmap};
   for (auto* b : i1) b->release();
   c->htk.release();
-  c->RMASK.release();
   c->d_scenes.release();
   c->STATIC.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -587,15 +586,18 @@ int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {
   return c->step_n(nsteps);
 }
 
-int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap) {
+int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float skin_fraction) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return NPX_EINVAL;
   if (block_size != 64 && block_size != 128 && block_size != 256)
     return c->fail(NPX_EINVAL, "block_size must be 64, 128 or 256");
   if (xcd_remap != 0 && xcd_remap != 1) return c->fail(NPX_EINVAL, "xcd_remap must be 0 or 1");
+  if (!(skin_fraction >= 0.02f && skin_fraction <= 0.5f))
+    return c->fail(NPX_EINVAL, "skin_fraction must be in [0.02, 0.5]");
   NPX_HIP_C(c, hipStreamSynchronize(c->stream));
   c->nbr_block = block_size;
   c->xcd_remap = xcd_remap;
+  c->skin_frac = skin_fraction;
   c->table_dirty = true;
   return NPX_OK;
 }
@@ -619,9 +621,9 @@ int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
                           NPX_WINDOW_MARGIN, wlo, wdim);
   const long long cells = (long long)wdim[0] * wdim[1] * wdim[2];
   const long long slots = (long long)cdiv(std::max(n, 1), NPX_BLOCK) * NPX_BLOCK;
-  // 8 float4 + 4 float + 10 int arrays and the neighbour list per slot;
-  // class map + start per cell
-  return slots * (9 * 16 + 4 * 4 + 9 * 4 + NPX_ROWS * 12) + cells * 8;
+  // 9 float4 + 4 float + 11 int arrays and the skin and in-range lists per
+  // slot; class map + start per cell
+  return slots * (9 * 16 + 4 * 4 + 11 * 4 + 8 * NPX_SKIN_CAP) + cells * 8;
 }
 
 int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {

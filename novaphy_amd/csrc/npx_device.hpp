@@ -28,7 +28,8 @@ struct __align__(16) SceneC {
   int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
   int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
   int seg_off, n, seg_blocks, vort_on;
-  float sp_lo, sp_hi, pad2, pad3;  // spiky support on r^2: (sp_lo, sp_hi), i.e. 1e-7 < |r| < h
+  float sp_lo, sp_hi;               // spiky support on r^2: (sp_lo, sp_hi), i.e. 1e-7 < |r| < h
+  float skin_cut2, skin_allow;      // skin radius^2; largest summed displacement a skin list allows
 };
 
 struct __align__(16) GridWin {  // window of an auxiliary grid (boundary / committed fluid)

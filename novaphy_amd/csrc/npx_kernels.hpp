@@ -298,46 +298,78 @@ __device__ __forceinline__ int npx_lblock(int remap) {
   return x * (g >> 3) + min(x, g & 7) + (i >> 3);
 }
 
+// Neighbour-pass prologue.  `active` lanes own a particle; the others stay
+// alive to the end (the delta pass reduces over whole waves).
 #define NPX_NBR_PROLOGUE()                                   \
   const int q = npx_lblock(remap) * (int)blockDim.x + (int)threadIdx.x; \
   const int s = blk_scene[q / NPX_BLOCK];                    \
-  if (s < 0) return;                                         \
+  if (s < 0) return; /* uniform per block */                 \
   const SceneC& S = scenes[s];                               \
-  if (q >= S.seg_off + S.n) return;                          \
+  const bool active = q < S.seg_off + S.n;                   \
   const int shift = cstart[S.cls_base] - S.seg_off;          \
   const int wxy = S.wx * S.wy;                               \
   const int* __restrict__ cm = clsmap + S.cls_base;          \
-  const int l = cell[q];
+  const int l = active ? cell[q] : 0;
 
-// Row masks.  The 27-cell sweep of a particle is nine rows (dz outer, dy
-// inner), and a row whose three cells are consecutive classes is one slot
-// range [b, b + 64) at most.  The density pass tests every candidate with the
-// cheap distance check and records, per row, b and a 64-bit mask whose bit t
-// is set when candidate b + t has r^2 < cut.  The SPH arithmetic then runs
-// over the set bits only, in row and bit order, which is the reference's
-// candidate order with the out-of-range candidates (exact zero terms) left
-// out, so every lane of a wave works on a real pair.  Delta-p of the same
-// iteration sees the same x* and reuses the masks; the vorticity-force pass
-// reuses those built at the final x*.  In memory, word
-// ((q / 64) * 9 + row) * 64 + q % 64 keeps a wave's accesses coalesced.  A
-// particle with a row of more than 64 candidates, or a row whose cells alias
-// (not one range), stores b[row 0] = -1 and sweeps its candidates inline.
-// Both visitors fetch the data of G pairs before evaluating any of them, so
-// G loads are in flight per wave instead of one.
+// Skin lists.  The first density pass of a step sweeps every candidate (the
+// 27-cell sweep, reference order) and keeps, per particle, the candidates
+// within the skin radius h_s = (1 + f) h, in order.  Later passes of the step
+// visit only that list: a candidate left out had |r| >= h_s at x*_0, and
+// while every particle of the scene has moved by at most D since then (D =
+// the sum over finished iterations of the largest |delta x*|), |r| >= h_s -
+// 2D stays beyond the cut (skin_allow bounds D with a relative margin for
+// rounding), so the list holds every in-range candidate.  Each density pass
+// (and the XSPH / curl pass at the final x*) first tests the list entries
+// with the cheap distance check and records a 64-bit mask of the entries in
+// range, then runs the SPH arithmetic over the set bits only, writing the
+// in-range candidates as a compacted list that the delta-p pass of the same
+// iteration (and the vorticity-force pass) visits.
+// Skipped entries are exact zero terms of the reference's sums, so the
+// results are the reference's bit for bit, and every lane of a wave works on
+// a real pair.  When the bound fails, or a particle has more than
+// NPX_SKIN_CAP skin candidates, the pass sweeps the candidates inline
+// instead.  Lists are
+// stored per 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64,
+// so entry k of all lanes is one coalesced access (the compacted list is
+// written entry by entry in lockstep across the wave, so its stores are
+// coalesced too).  Visitors fetch the data
+// of G candidates before evaluating any of them, so G loads are in flight
+// per wave.
 #define NPX_ROWS 9
+#define NPX_SKIN_CAP 64
+#define NPX_DISP_SLOTS 32  // per-scene displacement maxima (one per iteration)
+#define NPX_DISP_LANES 16  // each spread over 16 words to keep the atomics uncontended
 
-struct RowMasks {
-  unsigned long long* __restrict__ m;
-  int* __restrict__ b;
+struct Skin {
+  int* __restrict__ idx;        // [slots / 64][NPX_SKIN_CAP][64] skin list
+  int* __restrict__ cnt;        // [slots]; -1: sweep inline
+  int* __restrict__ ex;         // same layout: in-range list of the current testing pass
+  int* __restrict__ ecnt;       // [slots]
+  unsigned* __restrict__ disp;  // [scenes][NPX_DISP_SLOTS][NPX_DISP_LANES], float bits of max |delta x*|
 };
 
-struct Rows {
-  unsigned long long m[NPX_ROWS];
-  int b[NPX_ROWS];
-};
+__device__ __forceinline__ size_t sk_at(int q, int k) {
+  return ((size_t)(q >> 6) * NPX_SKIN_CAP + k) * 64 + (q & 63);
+}
 
-__device__ __forceinline__ size_t rm_at(int q, int r) {
-  return ((size_t)(q >> 6) * NPX_ROWS + r) * 64 + (q & 63);
+// The skin list is usable for a pass after `upto` finished iterations.
+__device__ __forceinline__ bool skin_valid(const SceneC& S, const unsigned* __restrict__ disp, int s, int upto) {
+  float D = 0.0f;
+  const int nk = min(upto, NPX_DISP_SLOTS);  // skin_allow < 0 when iters > NPX_DISP_SLOTS
+  for (int k = 0; k < nk; ++k) {
+    const unsigned* p = disp + ((size_t)s * NPX_DISP_SLOTS + k) * NPX_DISP_LANES;
+    float mx = 0.0f;
+#pragma unroll
+    for (int u = 0; u < NPX_DISP_LANES; ++u) mx = fmaxf(mx, __uint_as_float(p[u]));
+    D += mx;
+  }
+  return D <= S.skin_allow;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
 }
 
 template <bool FAST>
@@ -345,11 +377,28 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
   return FAST ? __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)) : sqn3(rx, ry, rz);
 }
 
-// Builds the row masks of a particle at xi in registers; false when it must
-// use the inline sweep.
+// Pins loaded values in registers at this point, so the G gathers of a batch
+// are all issued before the first pair is evaluated (the compiler would
+// otherwise sink a load into the branch that uses it).  The asm statement is
+// empty: it only consumes the registers.
+__device__ __forceinline__ void npx_keep(const float4& a) {
+  __asm__ volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z), "v"(a.w));
+}
+
+__device__ __forceinline__ void keep(const float4& a) { npx_keep(a); }
+
+struct Rows {
+  unsigned long long m[NPX_ROWS];  // skin: r^2 < skin_cut2
+  unsigned long long e[NPX_ROWS];  // in range: r^2 < cut
+  int b[NPX_ROWS];
+};
+
+// Sweeps the nine candidate rows of a particle at xi; per row, bit t of m
+// (of e) is set when candidate b + t is inside the skin (in range).  False
+// when a row is not one range of at most 64 slots.
 template <bool FAST>
-__device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, const float4* __restrict__ XS, int l,
-                                           int wx, int wxy, const int* __restrict__ cm,
+__device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, float skin_cut, const float4* __restrict__ XS,
+                                           int l, int wx, int wxy, const int* __restrict__ cm,
                                            const int* __restrict__ cstart, int shift) {
   bool ok = true;
 #pragma unroll
@@ -361,79 +410,163 @@ __device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, const 
     const int b = cstart[c0] - shift;
     const int e = cstart[c0 + 3] - shift;
     ok = ok && (c1 == c0 + 1) && (c2 == c0 + 2) && (e - b <= 64);
-    unsigned long long mk = 0;
+    unsigned long long mk = 0, ek = 0;
     if (ok) {
       for (int j = b; j < e; j += 4) {
         const int j1 = min(j + 1, e - 1), j2 = min(j + 2, e - 1), j3 = min(j + 3, e - 1);
         const float4 a0 = XS[j], a1 = XS[j1], a2 = XS[j2], a3 = XS[j3];
-        const unsigned long long t0 = dist2<FAST>(xi.x - a0.x, xi.y - a0.y, xi.z - a0.z) < cut;
-        const unsigned long long t1 = (j + 1 < e) && dist2<FAST>(xi.x - a1.x, xi.y - a1.y, xi.z - a1.z) < cut;
-        const unsigned long long t2 = (j + 2 < e) && dist2<FAST>(xi.x - a2.x, xi.y - a2.y, xi.z - a2.z) < cut;
-        const unsigned long long t3 = (j + 3 < e) && dist2<FAST>(xi.x - a3.x, xi.y - a3.y, xi.z - a3.z) < cut;
-        const int o = j - b;
-        mk |= (t0 << o) | (t1 << ((o + 1) & 63)) | (t2 << ((o + 2) & 63)) | (t3 << ((o + 3) & 63));
+        const float d0 = dist2<FAST>(xi.x - a0.x, xi.y - a0.y, xi.z - a0.z);
+        const float d1 = (j + 1 < e) ? dist2<FAST>(xi.x - a1.x, xi.y - a1.y, xi.z - a1.z) : INFINITY;
+        const float d2 = (j + 2 < e) ? dist2<FAST>(xi.x - a2.x, xi.y - a2.y, xi.z - a2.z) : INFINITY;
+        const float d3 = (j + 3 < e) ? dist2<FAST>(xi.x - a3.x, xi.y - a3.y, xi.z - a3.z) : INFINITY;
+        const unsigned sb = (unsigned)(d0 < skin_cut) | ((unsigned)(d1 < skin_cut) << 1) |
+                            ((unsigned)(d2 < skin_cut) << 2) | ((unsigned)(d3 < skin_cut) << 3);
+        const unsigned eb = (unsigned)(d0 < cut) | ((unsigned)(d1 < cut) << 1) | ((unsigned)(d2 < cut) << 2) |
+                            ((unsigned)(d3 < cut) << 3);
+        mk |= (unsigned long long)sb << (j - b);  // j - b <= 60
+        ek |= (unsigned long long)eb << (j - b);
       }
     }
     w.m[r] = mk;
+    w.e[r] = ek & mk;
     w.b[r] = b;
   }
-  if (!ok) w.b[0] = -1;
   return ok;
 }
 
-__device__ __forceinline__ void store_rows(const RowMasks& R, int q, const Rows& w) {
-  const size_t a = rm_at(q, 0);
-#pragma unroll
-  for (int r = 0; r < NPX_ROWS; ++r) {
-    R.m[a + 64 * r] = w.m[r];
-    R.b[a + 64 * r] = w.b[r];
-  }
-}
-
-__device__ __forceinline__ bool load_rows(const RowMasks& R, int q, Rows& w) {
-  const size_t a = rm_at(q, 0);
-#pragma unroll
-  for (int r = 0; r < NPX_ROWS; ++r) {
-    w.m[r] = R.m[a + 64 * r];
-    w.b[r] = R.b[a + 64 * r];
-  }
-  return w.b[0] >= 0;
-}
-
-// f(j, ld(j)) over the set bits of the row masks in order (consumes w).
-template <int G, class L, class F>
-__device__ __forceinline__ void visit_rows(Rows& w, L&& ld, F&& f) {
-  unsigned long long mk = w.m[0];
+// Writes the skin entries of the row masks to the list of slot q, in order
+// (consumes w); returns the entry count (> NPX_SKIN_CAP: overflow) and the
+// mask of in-range entries.
+__device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigned long long* rel_out) {
+  unsigned long long mk = w.m[0], ek = w.e[0], rel = 0;
   int b = w.b[0];
   int left = NPX_ROWS - 1;
-  auto next = [&]() -> int {
+  int k = 0;
+  for (;;) {
     while (mk == 0 && left > 0) {
       mk = w.m[1];
+      ek = w.e[1];
       b = w.b[1];
 #pragma unroll
       for (int r = 1; r < NPX_ROWS - 1; ++r) {
         w.m[r] = w.m[r + 1];
+        w.e[r] = w.e[r + 1];
         w.b[r] = w.b[r + 1];
       }
       --left;
     }
-    if (mk == 0) return -1;
-    const int j = b + (int)__builtin_ctzll(mk);
+    if (mk == 0) break;
+    const int t = (int)__builtin_ctzll(mk);
     mk &= mk - 1;
-    return j;
-  };
-  for (;;) {
-    int js[G];
-    js[0] = next();
-    if (js[0] < 0) break;
+    if (k < NPX_SKIN_CAP) {
+      K.idx[sk_at(q, k)] = b + t;
+      rel |= ((ek >> t) & 1ull) << k;
+    }
+    ++k;
+  }
+  *rel_out = rel;
+  return k;
+}
+
+// Builds the skin list of slot q by sweeping every candidate in order (any
+// row structure, aliased cells included) and appending those inside the
+// skin; returns the entry count (> NPX_SKIN_CAP: overflow) and the mask of
+// in-range entries.  Used for the particles whose rows are not single slot
+// ranges of at most 64 candidates.
+template <bool FAST>
+__device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, float cut, float skin_cut,
+                                            const float4* __restrict__ XS, int l, int wx, int wxy,
+                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                            unsigned long long* rel_out) {
+  int k = 0;
+  unsigned long long rel = 0;
+  for_each_candidate(l, wx, wxy, cm, cstart, shift, [&](int j) {
+    const float4 a = XS[j];
+    const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
+    if (d < skin_cut) {
+      if (k < NPX_SKIN_CAP) {
+        K.idx[sk_at(q, k)] = j;
+        rel |= (unsigned long long)(d < cut) << k;
+      }
+      ++k;
+    }
+  });
+  *rel_out = rel;
+  return k;
+}
+
+// Mask of the skin-list entries of slot q (n of them) with r^2 < cut at xi.
+template <bool FAST>
+__device__ __forceinline__ unsigned long long test_list(const Skin& K, int q, int n, float4 xi, float cut,
+                                                        const float4* __restrict__ XS) {
+  unsigned long long rel = 0;
+  for (int k = 0; k < n; k += 4) {
+    int js[4];
 #pragma unroll
-    for (int u = 1; u < G; ++u) js[u] = next();
+    for (int u = 0; u < 4; ++u) js[u] = K.idx[sk_at(q, min(k + u, n - 1))];
+    float4 a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = XS[js[u]];
+    unsigned bits = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float d = dist2<FAST>(xi.x - a[u].x, xi.y - a[u].y, xi.z - a[u].z);
+      bits |= (unsigned)((d < cut) & (k + u < n)) << u;
+    }
+    rel |= (unsigned long long)bits << k;  // k <= 60
+  }
+  return rel;
+}
+
+// f(j, ld(j)) over the skin-list entries of slot q whose bit is set in rel,
+// in list order; writes them as the compacted in-range list and returns its
+// length.
+template <int G, class L, class F>
+__device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long long rel, L&& ld, F&& f) {
+  int e = 0;
+  for (;;) {
+    int ks[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      ks[u] = rel ? (int)__builtin_ctzll(rel) : -1;
+      rel &= rel - 1;
+    }
+    if (ks[0] < 0) break;
+    int js[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) js[u] = K.idx[sk_at(q, ks[u] < 0 ? ks[0] : ks[u])];
     decltype(ld(0)) v[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u) v[u] = ld(js[u] < 0 ? js[0] : js[u]);
+    for (int u = 0; u < G; ++u) v[u] = ld(js[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) keep(v[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (ks[u] >= 0) {
+        K.ex[sk_at(q, e)] = js[u];  // e = G * group + u for every lane still in the loop
+        ++e;
+        f(js[u], v[u]);
+      }
+    }
+  }
+  return e;
+}
+
+// f(j, ld(j)) over the n entries of a list (skin or in-range layout).
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_list(const int* __restrict__ list, int q, int n, L&& ld, F&& f) {
+  for (int k = 0; k < n; k += G) {
+    int js[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) js[u] = list[sk_at(q, min(k + u, n - 1))];
+    decltype(ld(0)) v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = ld(js[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) keep(v[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      if (js[u] >= 0) f(js[u], v[u]);
+      if (k + u < n) f(js[u], v[u]);
   }
 }
 
@@ -444,6 +577,8 @@ __device__ __forceinline__ void visit_range(int b, int e, L&& ld, F&& f) {
     decltype(ld(0)) v[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) v[u] = ld(j + u);
+#pragma unroll
+    for (int u = 0; u < G; ++u) keep(v[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u) f(j + u, v[u]);
   }
@@ -473,18 +608,51 @@ __device__ __forceinline__ void visit_all(int l, int wx, int wxy, const int* __r
   }
 }
 
+// A pass after `upto` finished iterations may use the skin list of slot q.
+__device__ __forceinline__ bool use_skin(const SceneC& S, const Skin& K, int s, int q, int upto) {
+  return K.cnt[q] >= 0 && (upto == 0 || skin_valid(S, K.disp, s, upto));
+}
+
+// Testing pass (density, XSPH / curl): visits the in-range candidates of
+// slot q at xi through the skin list (recording the in-range mask for the
+// pass that follows) or the inline sweep.
+template <bool FAST, int G, class L, class F>
+__device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int s, int q, int upto, float4 xi,
+                                           float cut, const float4* __restrict__ XS, int l, int wxy,
+                                           const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                           L&& ld, F&& f) {
+  if (use_skin(S, K, s, q, upto)) {
+    const unsigned long long rel = test_list<FAST>(K, q, K.cnt[q], xi, cut, XS);
+    K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
+  } else {
+    visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
+  }
+}
+
+// Reusing pass (delta-p, vorticity force): same positions as the testing
+// pass before it, so its in-range list applies.
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int s, int q, int upto, int l, int wxy,
+                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                            L&& ld, F&& f) {
+  if (use_skin(S, K, s, q, upto))
+    visit_list<G>(K.ex, q, K.ecnt[q], ld, f);
+  else
+    visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
+}
+
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
 // (:144-179): rho_i sums m W over all in-range candidates (self and
 // duplicates included); for j != i, sum |grad_j|^2 and grad_i; lambda is 0
-// when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  Builds the iteration's
-// row masks.
-template <bool FAST>
+// when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
+// iteration) builds the step's skin lists; later iterations test them.
+template <bool FAST, bool BUILD>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
-                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, RowMasks R, int remap) {
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, int remap) {
   NPX_NBR_PROLOGUE();
-  if (iter >= S.iters) return;
+  if (iter >= S.iters || !active) return;
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
   const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -506,13 +674,24 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     gy += ay;
     gz += az;
   };
-  Rows w;
-  const bool ok = sweep_rows<FAST>(w, xi, cut, XS, l, S.wx, wxy, cm, cstart, shift);
-  store_rows(R, q, w);
-  if (ok)
-    visit_rows<4>(w, ld, body);
-  else
-    visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+  if (BUILD) {
+    Rows w;
+    int n = -1;
+    unsigned long long rel = 0;
+    if (sweep_rows<FAST>(w, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
+      n = store_skin(w, K, q, &rel);
+    else
+      n = sweep_direct<FAST>(K, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+    if (n > NPX_SKIN_CAP) n = -1;
+    K.cnt[q] = n;
+    if (n >= 0) {
+      K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
+    } else {
+      visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+    }
+  } else {
+    visit_test<FAST, 4>(S, K, s, q, iter, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+  }
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
   float lam = 0.0f;
@@ -544,91 +723,111 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 }
 
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
-// Jacobi apply (:71-77) into the other x* buffer, over the density pass's row
-// masks.  On the last iteration it also performs the velocity update with
-// the h/dt speed clamp (:81-93).
+// Jacobi apply (:71-77) into the other x* buffer.  On the last iteration it
+// also performs the velocity update with the h/dt speed clamp (:81-93).
+// Records the scene's largest |delta x*| of the iteration for the skin test.
 template <bool FAST>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
-        float4* __restrict__ VT, RowMasks R, int remap) {
+        float4* __restrict__ VT, Skin K, int remap) {
   NPX_NBR_PROLOGUE();
-  const float4 xi = XS[q];
   const bool last = (iter == max(S.iters, 1) - 1);
   if (iter >= S.iters) {
     // scenes with fewer iterations than the batch maximum pass x* through so
     // every scene ends in the same ping-pong buffer
+    if (!active) return;
+    const float4 xi = XS[q];
     XSn[q] = xi;
     if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
     return;
   }
-  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
-  const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
-  const float wdq = S.wdq, negk = -S.corr_k;
-  const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
-  const float cut = FAST ? h2 : S.cut2;
-  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  auto ld = [&](int j) { return XS[j]; };
-  auto body = [&](int j, float4 xj) {
-    const float ls = xi.w + xj.w;
-    if (j == q) return;
-    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-    const float r_sq = dist2<FAST>(rx, ry, rz);
-    if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
-    bool in;
-    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
-    if (!in) return;
-    float sc = 0.0f;
-    if (ls < -1e-10f && scorr_on) {
-      float ratio = poly6(r_sq, h2, c6) / wdq;
-      ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
-      float rp = ratio * ratio;
-      if (n4) rp = rp * rp;
-      sc = negk * rp;
-    }
-    const float coef = (ls + sc) * m;
-    if (FAST) {
-      const float cc = coef * co;
-      dx = __builtin_fmaf(cc, rx, dx);
-      dy = __builtin_fmaf(cc, ry, dy);
-      dz = __builtin_fmaf(cc, rz, dz);
-    } else {
-      dx += coef * (co * rx);
-      dy += coef * (co * ry);
-      dz += coef * (co * rz);
-    }
-  };
-  Rows w;
-  if (load_rows(R, q, w))
-    visit_rows<4>(w, ld, body);
-  else
-    visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
-  const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
-  XSn[q] = make_float4(px, py, pz, 0.0f);
-  if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+  float moved = 0.0f;
+  if (active) {
+    const float4 xi = XS[q];
+    const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
+    const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+    const float wdq = S.wdq, negk = -S.corr_k;
+    const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
+    const float cut = FAST ? h2 : S.cut2;
+    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    auto ld = [&](int j) { return XS[j]; };
+    auto body = [&](int j, float4 xj) {
+      const float ls = xi.w + xj.w;
+      if (j == q) return;
+      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+      const float r_sq = dist2<FAST>(rx, ry, rz);
+      if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
+      bool in;
+      const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+      if (!in) return;
+      float sc = 0.0f;
+      if (ls < -1e-10f && scorr_on) {
+        float ratio = poly6(r_sq, h2, c6) / wdq;
+        ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
+        float rp = ratio * ratio;
+        if (n4) rp = rp * rp;
+        sc = negk * rp;
+      }
+      const float coef = (ls + sc) * m;
+      if (FAST) {
+        const float cc = coef * co;
+        dx = __builtin_fmaf(cc, rx, dx);
+        dy = __builtin_fmaf(cc, ry, dy);
+        dz = __builtin_fmaf(cc, rz, dz);
+      } else {
+        dx += coef * (co * rx);
+        dy += coef * (co * ry);
+        dz += coef * (co * rz);
+      }
+    };
+    visit_reuse<4>(S, K, s, q, iter, l, wxy, cm, cstart, shift, ld, body);
+    const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
+    XSn[q] = make_float4(px, py, pz, 0.0f);
+    if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+    moved = sqrtf(sqn3(px - xi.x, py - xi.y, pz - xi.z));
+  }
+  // blocks never straddle scenes: one atomic per block, spread over the
+  // scene's NPX_DISP_LANES words for this iteration
+  __shared__ float bmax[NPX_MAX_BLOCK / 64];
+  moved = wave_max(moved);
+  if ((threadIdx.x & 63) == 0) bmax[threadIdx.x >> 6] = moved;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float mx = 0.0f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = fmaxf(mx, bmax[w]);
+    if (mx > 0.0f)
+      atomicMax(K.disp + ((size_t)s * NPX_DISP_SLOTS + min(iter, NPX_DISP_SLOTS - 1)) * NPX_DISP_LANES +
+                    (blockIdx.x & (NPX_DISP_LANES - 1)),
+                __float_as_uint(mx));
+  }
 }
 
 struct XV {
   float4 x, v;
 };
+__device__ __forceinline__ void keep(const XV& d) {
+  npx_keep(d.x);
+  npx_keep(d.v);
+}
 
 // apply_xsph_viscosity (pbf_solver.cpp:225-250) fused with the vorticity
 // curl (this project's definition; the reference declares vorticity_epsilon
-// but never uses it): one sweep at the final x* gives
+// but never uses it): one testing pass at the final x* gives
 //   c_i     = sum_{j!=i, r^2<h^2} (m/rho~_j) (v_j - v_i) W(x*_i - x*_j)
 //   omega_i = sum_{j!=i} (m/rho~_j) (v_j - v_i) x gradW(x*_j - x*_i)
 // VX = v_i + xsph_c * c_i (v_i when XSPH is off); OM = (omega_i, |omega_i|).
-// With vorticity the row masks are stored for the force pass.
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-             const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, RowMasks R,
-             int remap) {
+             const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, int remap) {
   NPX_NBR_PROLOGUE();
+  if (!active) return;
   const float4 xi = XS[q];
   const float4 vi = VT[q];
   const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
+  float vx = vi.x, vy = vi.y, vz = vi.z;
   if (xs || vort) {
     const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, cut = S.cut2;
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -657,30 +856,28 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
         wz += vj.w * (ax * gy - ay * gx);
       }
     };
-    Rows w;
-    const bool ok = sweep_rows<false>(w, xi, cut, XS, l, S.wx, wxy, cm, cstart, shift);
-    if (vort) store_rows(R, q, w);
-    if (ok)
-      visit_rows<2>(w, ld, body);
+    if (S.iters > 0)  // the skin lists exist once a density pass has run
+      visit_test<false, 2>(S, K, s, q, S.iters, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
     else
       visit_all<2>(l, S.wx, wxy, cm, cstart, shift, ld, body);
     if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
-    float vx = vi.x, vy = vi.y, vz = vi.z;
     if (xs) {
       vx = vx + S.xsph_c * cx;
       vy = vy + S.xsph_c * cy;
       vz = vz + S.xsph_c * cz;
     }
-    VX[q] = make_float4(vx, vy, vz, 0.0f);
-  } else {
-    VX[q] = make_float4(vi.x, vi.y, vi.z, 0.0f);
   }
+  VX[q] = make_float4(vx, vy, vz, 0.0f);
 }
 
 struct XOW {
   float4 x;
   float vw, ow;
 };
+__device__ __forceinline__ void keep(const XOW& d) {
+  npx_keep(make_float4(d.x.x, d.x.y, d.x.z, d.vw));
+  __asm__ volatile("" ::"v"(d.ow));
+}
 
 // Vorticity force (N = eta / |eta|, eta_i = sum_{j!=i} (m/rho~_j)
 // (|omega_j| - |omega_i|) gradW(x*_i - x*_j); v += dt * eps (N x omega_i))
@@ -690,8 +887,9 @@ __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
-              float4* __restrict__ X, float4* __restrict__ V, RowMasks R, int remap) {
+              float4* __restrict__ X, float4* __restrict__ V, Skin K, int remap) {
   NPX_NBR_PROLOGUE();
+  if (!active) return;
   const float4 xi = XS[q];
   const float4 vx4 = VX[q];
   float vx = vx4.x, vy = vx4.y, vz = vx4.z;
@@ -712,9 +910,8 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       ey += sc * (co * ry);
       ez += sc * (co * rz);
     };
-    Rows w;
-    if (load_rows(R, q, w))
-      visit_rows<4>(w, ld, body);
+    if (S.iters > 0 && (S.xsph_on || S.vort_on))  // the XSPH / curl pass recorded the mask
+      visit_reuse<4>(S, K, s, q, S.iters, l, wxy, cm, cstart, shift, ld, body);
     else
       visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
     const float el = sqrtf(sqn3(ex, ey, ez));

@@ -166,15 +166,16 @@ def test_grid_query_matches_oracle(gpu):
 
 
 def test_tunables_do_not_change_results(gpu):
-    """Block size and XCD block order change speed only: a rollout that
+    """Block size, XCD block order and skin change speed only: a rollout that
     switches tunables every step is bitwise equal to the oracle."""
     sc = scenes.dam_break_8k()
     prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
     sc = dataclasses.replace(sc, params=prm)
-    _say("case: tunables (block size / XCD order) rollout")
+    _say("case: tunables (block size / XCD order / skin) rollout")
     c = _ctx(sc)
-    for k, (b, remap) in enumerate([(64, 0), (128, 1), (256, 1), (64, 1), (128, 0), (256, 0)] * 2):
-        c.set_tuning(b, remap)
+    for k, (b, remap, skin) in enumerate([(64, 0, 0.02), (128, 1, 0.1), (256, 1, 0.5), (64, 1, 0.05),
+                                          (128, 0, 0.3), (256, 0, 0.1)] * 2):
+        c.set_tuning(b, remap, skin)
         c.step(sc.dt, sc.gravity, sc.mass, 1)
     gp, gv, gr, gl = c.get_particles()
     op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 12)
