@@ -19,6 +19,9 @@ Scene geometry (all in metres, FP32):
   * coupled_500k     100x50x100 block (y has 50 layers) in [0,0,0]-[10,5,5]
                      plus 64 rigid bodies, seed 2.
   * batched_1024     1024 dam breaks, N log-uniform in [2k, 64k], seed 3.
+  * served requests  dam breaks with N log-uniform in [4k, 256k] and 0-8 rigid
+                     spheres (r ~ U(0.1, 0.2), density ~ U(300, 2000)) dropped
+                     0.3-1.0 m above the block, seed 4.
 """
 from __future__ import annotations
 
@@ -202,3 +205,50 @@ def lattice_rest_density(spacing: float = 0.05, h: float = 0.1, rest_density: fl
                 if r2 < h * h:
                     tot += m * coef * (h * h - r2) ** 3
     return tot
+
+
+def block_dims(rng: np.random.Generator, lo: int, hi: int):
+    """Block dims (nx, ny, nz) with N = nx*ny*nz log-uniform in [lo, hi],
+    aspect ratios in [1, 4], and a fill fraction in [0.25, 0.5]."""
+    while True:
+        n_target = math.exp(rng.uniform(math.log(lo), math.log(hi)))
+        ax = rng.uniform(1.0, 4.0)
+        ay = rng.uniform(1.0, 4.0)
+        nz = max(4, int(round((n_target / (ax * ay)) ** (1.0 / 3.0))))
+        nx = max(4, int(round(ax * nz)))
+        ny = max(4, int(round(ay * nz)))
+        fill = rng.uniform(0.25, 0.50)
+        if lo <= nx * ny * nz <= hi:
+            return nx, ny, nz, fill
+
+
+@dataclasses.dataclass
+class Sphere:
+    radius: float
+    density: float
+    position: Tuple[float, float, float]
+
+
+def served_request(k: int, rng: np.random.Generator, lo: int = 4000, hi: int = 256000,
+                   max_spheres: int = 8) -> Scene:
+    """One served request: a dam break (as in the batched workload, with a box
+    tall enough for the drop) and 0..max_spheres rigid spheres above it.
+    The scene carries ``spheres`` (list of Sphere); the box floor is a ground
+    plane for the rigid bodies."""
+    nx, ny, nz, fill = block_dims(rng, lo, hi)
+    s = 0.05
+    bx, by, bz = nx * s, ny * s, nz * s
+    box_z = bz + 0.05
+    box_y = max(by * 1.5 + 0.05, by + 1.5)
+    box_x = max(bx + 0.05, (bx * by * bz) / (fill * box_y * box_z))
+    sc = dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=4 * 1000003 + k, steps=0, name=f"served_{k}")
+    sc.params = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
+    spheres = []
+    for _ in range(int(rng.integers(0, max_spheres + 1))):
+        r = float(rng.uniform(0.1, 0.2))
+        x = float(rng.uniform(min(r + 0.05, bx / 2), max(bx - r, bx / 2)))
+        z = float(rng.uniform(min(r + 0.05, bz / 2), max(bz - r, bz / 2)))
+        y = float(by + rng.uniform(0.3, 1.0))
+        spheres.append(Sphere(radius=r, density=float(rng.uniform(300.0, 2000.0)), position=(x, y, z)))
+    sc.spheres = spheres
+    return sc

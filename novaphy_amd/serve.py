@@ -8,19 +8,27 @@ two device steps the server
   * admits waiting requests in arrival order while the HBM budget, the scene
     slots and the batch capacities allow (a request larger than the whole
     budget is rejected at once),
-then advances every admitted request by one step in the same launches
-(npx_batch_step_timed).  Particle and grid state stay on the device for the
-whole rollout.  Per-step latency samples are device timestamps between
-consecutive steps; a request's result is bitwise the same alone or in any mix
-(per-scene arithmetic does not depend on the batch).
+then advances every admitted request by one step.  Particle and grid state
+stay on the device for the whole rollout.  A request's result is bitwise the
+same alone or in any mix (per-scene arithmetic does not depend on the batch).
 
-The backend is ``capi.Batch``; the scheduling logic takes any object with
-the same methods (add / remove / get / step_timed / status), which is how the
+Backends (``add(scene) -> key, step() -> ms, status() -> {key: code},
+get(key), remove(key)``):
+  * BatchBackend: fluid-only requests in one batch context; every request
+    advances in the same launches (npx_batch_step_timed), and a step's
+    latency is its device time.
+  * MixedBackend: fluid-only requests as above, and requests with rigid
+    bodies (Akinci coupling, host rigid pipeline) as FluidWorld instances,
+    each on its own device context and stream, stepped from a thread pool
+    while the batch step runs, so their device work overlaps.  A round's
+    latency is the host time of the synchronised round.
+The scheduling logic takes any object with those methods, which is how the
 CPU tests exercise admission, eviction and cancellation without a GPU.
 """
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import dataclasses
 import time
 from typing import Callable, Dict, List, Optional
@@ -39,7 +47,7 @@ class Request:
     arrival: float = 0.0          # arrival time (seconds on the server's clock)
     status: str = WAITING
     steps_done: int = 0
-    sid: int = -1
+    sid: object = None            # backend key while admitted
     bytes: int = 0
     step_ms: List[float] = dataclasses.field(default_factory=list)
     result: Optional[tuple] = None
@@ -107,9 +115,8 @@ class Server:
             r = self.pending[0]
             if len(self.active) >= self.max_scenes or self.used + r.bytes > self.budget:
                 return                       # FIFO: later arrivals wait behind the head
-            sc = r.scene
             try:
-                sid = self.b.add(sc.params, sc.dt, sc.gravity, sc.mass, sc.positions, sc.velocities)
+                sid = self.b.add(r.scene)
             except Exception as e:            # capacity exhausted on the device side
                 if "capacity" in str(e) or "slots" in str(e):
                     return
@@ -126,9 +133,9 @@ class Server:
             self.active[r.rid] = r
 
     def _check_health(self) -> None:
-        st = self.b.status(self.max_scenes)
+        st = self.b.status()
         for r in list(self.active.values()):
-            if st[r.sid]:
+            if st.get(r.sid, 0):
                 r.error = f"scene status {int(st[r.sid])} (1 = left its cell window, 2 = non-finite state)"
                 self._retire(r, FAILED, read=False)
 
@@ -140,7 +147,7 @@ class Server:
         if not self.active:
             return False
         t0 = time.perf_counter()
-        ms = float(self.b.step_timed(1)[0])
+        ms = float(self.b.step())
         self.busy_s += time.perf_counter() - t0
         self.steps += 1
         self.step_ms.append(ms)
@@ -178,6 +185,125 @@ class Server:
             return {"p50_ms": None, "p99_ms": None, "samples": 0}
         return {"p50_ms": float(np.percentile(samples, 50)), "p99_ms": float(np.percentile(samples, 99)),
                 "samples": int(samples.size)}
+
+
+class BatchBackend:
+    """Fluid-only requests in one batch context (capi.Batch)."""
+
+    def __init__(self, batch, max_scenes: int):
+        self.b = batch
+        self.max_scenes = int(max_scenes)
+
+    def add(self, sc):
+        return self.b.add(sc.params, sc.dt, sc.gravity, sc.mass, sc.positions, sc.velocities)
+
+    def remove(self, key):
+        self.b.remove(key)
+
+    def get(self, key):
+        return self.b.get(key)
+
+    def step(self) -> float:
+        return float(self.b.step_timed(1)[0])
+
+    def status(self) -> Dict[object, int]:
+        st = self.b.status(self.max_scenes)
+        return {k: int(v) for k, v in enumerate(st) if v}
+
+
+def fluid_world_for(sc, plane_extent: Optional[float] = None):
+    """FluidWorld of a served request: the scene's fluid particles, its rigid
+    spheres and a ground plane at y = 0 for them to land on."""
+    import novaphy_amd as nv
+    b = nv.ModelBuilder()
+    b.add_ground_plane(y=0.0)
+    for sp in getattr(sc, "spheres", []):
+        mass = float(sp.density * 4.0 / 3.0 * np.pi * sp.radius ** 3)
+        t = nv.Transform.from_translation(np.array(sp.position, np.float32))
+        i = b.add_body(nv.RigidBody.from_sphere(mass, float(sp.radius)), t)
+        b.add_shape(nv.CollisionShape.make_sphere(float(sp.radius), i))
+    model = b.build()
+    prm = nv.PBFSettings()
+    p = sc.params
+    for f in ("rest_density", "kernel_radius", "particle_radius", "solver_iterations", "epsilon",
+              "xsph_viscosity", "vorticity_epsilon", "corr_delta_q", "corr_k", "corr_n", "use_domain_bounds"):
+        setattr(prm, f, getattr(p, f))
+    prm.domain_lower = np.array(p.domain_lower, np.float32)
+    prm.domain_upper = np.array(p.domain_upper, np.float32)
+    blk = nv.FluidBlockDef()
+    blk.particle_spacing = 2.0 * p.particle_radius
+    blk.lower = np.zeros(3, np.float32)
+    blk.upper = np.zeros(3, np.float32)
+    if plane_extent is None:
+        plane_extent = float(max(p.domain_upper[0], p.domain_upper[2]))
+    world = nv.FluidWorld(model, [blk], nv.SolverSettings(), prm, boundary_extent=plane_extent)
+    world.fluid_state.init_arrays(sc.positions, sc.velocities)
+    return world
+
+
+class MixedBackend:
+    """Fluid-only requests in a batch context; requests with rigid bodies as
+    FluidWorld instances (own context and stream each) stepped concurrently
+    from a thread pool while the batch step runs."""
+
+    def __init__(self, batch, max_scenes: int, threads: int = 8):
+        self.batch = BatchBackend(batch, max_scenes)
+        self.n_batch = 0
+        self.worlds: Dict[int, tuple] = {}
+        self.next_world = 0
+        self.failed: Dict[object, int] = {}
+        self.pool = concurrent.futures.ThreadPoolExecutor(max_workers=max(1, threads))
+
+    def add(self, sc):
+        if not getattr(sc, "spheres", None):
+            key = ("b", self.batch.add(sc))
+            self.n_batch += 1
+            return key
+        key = ("w", self.next_world)
+        self.next_world += 1
+        self.worlds[key[1]] = (fluid_world_for(sc), float(sc.dt))
+        return key
+
+    def remove(self, key):
+        self.failed.pop(key, None)
+        if key[0] == "b":
+            self.batch.remove(key[1])
+            self.n_batch -= 1
+        else:
+            self.worlds.pop(key[1], None)
+
+    def get(self, key):
+        if key[0] == "b":
+            return self.batch.get(key[1])
+        w, _ = self.worlds[key[1]]
+        st = w.fluid_state
+        return (st.positions_numpy(), st.velocities_numpy(), st.densities_numpy(), st.lambdas_numpy(),
+                [np.array(t.position) for t in w.state.transforms])
+
+    def _step_world(self, k):
+        w, dt = self.worlds[k]
+        try:
+            w.step(dt)
+        except Exception:
+            self.failed[("w", k)] = 2
+
+    def step(self) -> float:
+        t0 = time.perf_counter()
+        futs = [self.pool.submit(self._step_world, k) for k in list(self.worlds)]
+        if self.n_batch:
+            self.batch.step()
+        for f in futs:
+            f.result()
+        return (time.perf_counter() - t0) * 1e3
+
+    def status(self) -> Dict[object, int]:
+        out = {("b", k): v for k, v in self.batch.status().items()}
+        out.update(self.failed)
+        return out
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.worlds.clear()
 
 
 def poisson_requests(n: int, rate_hz: float, seed: int, make_scene: Callable[[int, np.random.Generator], object],

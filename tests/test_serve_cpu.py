@@ -17,12 +17,12 @@ class FakeBatch:
         self.cap = capacity_scenes
         self.added = []
 
-    def add(self, params, dt, g, m, pos, vel):
+    def add(self, sc):
         if len(self.scenes) >= self.cap:
             raise RuntimeError("npx error -3: all scene slots are in use")
         sid = self.next
         self.next += 1
-        self.scenes[sid] = np.asarray(pos).shape[0]
+        self.scenes[sid] = np.asarray(sc.positions).shape[0]
         self.added.append(sid)
         return sid
 
@@ -32,15 +32,11 @@ class FakeBatch:
     def get(self, sid):
         return ("state", sid)
 
-    def step_timed(self, n):
-        return np.full(n, 2.0 + 0.01 * len(self.scenes), np.float32)
+    def step(self):
+        return 2.0 + 0.01 * len(self.scenes)
 
-    def status(self, max_scenes):
-        st = np.zeros(max_scenes, np.int32)
-        for s in self.bad:
-            if s in self.scenes:
-                st[s] = 2
-        return st
+    def status(self):
+        return {s: 2 for s in self.bad if s in self.scenes}
 
 
 def _scene(n):
