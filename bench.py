@@ -37,25 +37,44 @@ sys.path.insert(0, ROOT)
 METRIC = "PBF particle-steps/s at 1M + % HBM roofline; batched p50/p99 step latency"
 HBM_PEAK_GBPS = 8000.0
 # Algorithmic (compulsory) FP32/int32 bytes per particle for each pass, 12 B
-# per 3-vector: predict+key 40, sort 13, reorder 56; per iteration
-# density+lambda 20 (x* in, rho and lambda out) and delta-p+apply 28 (x* and
-# lambda in, x* out); the XSPH+curl pass 52 (x*, v, m/rho in; omega, v out;
-# 40 without vorticity) and the vorticity-force+commit pass 68 (x*, v,
-# omega, |omega|, m/rho in; x, v out; 48 without vorticity).
+# per 3-vector, each dependency-forced pass reading and writing each array
+# once: predict + cell key 40, stable counting sort 13, reorder 56; per
+# iteration density + lambda 20 (x* in; rho, lambda out) and delta-p + apply
+# 28 (x*, lambda in; x* out); velocity + XSPH + clamp + commit 52 (x*, x, rho
+# in; x, v out), plus 48 with vorticity (omega pass: x*, v in, omega out 36;
+# the final pass reads omega, +12).  The fused omega_xsph / vort_commit
+# kernels split the last two: omega_xsph 36 with vorticity (24 without: the
+# x*, v reads of the XSPH sum) and vort_commit 64 (28 without), so a step is
+# 161 + 48 I bytes (+48 with vorticity).
 def pass_bytes(vort: bool):
-    return {"density_lambda": 20, "density_lambda_build": 20, "delta": 28, "omega_xsph": 52 if vort else 40,
-            "vort_commit": 68 if vort else 48, "predict_bin": 40, "gather": 56}
+    return {"density_lambda": 20, "density_lambda_build": 20, "delta": 28,
+            "omega_xsph": 36 if vort else 24, "vort_commit": 64 if vort else 28,
+            "predict_bin": 40, "gather": 56}
 
 
 def step_bytes(iters: int, vort: bool) -> int:
-    return 109 + 48 * iters + (120 if vort else 88)
+    return 161 + 48 * iters + (48 if vort else 0)
+
+
+def profiled_traffic(variant: str, kernel: str):
+    """HBM bytes per launch of `kernel` in the 1M workload `variant`, from the
+    calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+    profiles/hbm_traffic_1m.md (None when not profiled)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_1m.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic" and f[1] == variant and f[2] == kernel:
+            return int(f[3])
+    return None
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_single(sc, steps, warmup, flags, cpu_seconds):
+def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
     from novaphy_amd import capi
     n = sc.n
     ctx = capi.Context(sc.params, capacity=n, flags=flags)
@@ -88,7 +107,8 @@ def run_single(sc, steps, warmup, flags, cpu_seconds):
         "value": value,
         "ms_per_step": wall * 1e3 / steps,
         "roofline": {"bound": "hbm", "kernel": main, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
+                     "traffic": profiled_traffic(variant, main) if (variant and n == 1000000 and not flags) else None,
                      "bytes_per_particle": PASS_BYTES[main], "particles_per_launch": n,
                      "avg_launch_us": round(avg_s * 1e6, 2)},
         "step_roofline": {"bytes_per_particle_step": B, "achieved": round(B * value / 1e9, 2),
@@ -408,7 +428,7 @@ def main():
         variants.append(("default", dataclasses.replace(base, params=dataclasses.replace(
             base.params, corr_k=dflt.corr_k, corr_delta_q=dflt.corr_delta_q, corr_n=dflt.corr_n))))
     cpu_s = args.cpu_seconds / len(variants)
-    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, cpu_s)) for name, sc in variants]
+    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, cpu_s, name)) for name, sc in variants]
 
     def describe(sc):
         p = sc.params
