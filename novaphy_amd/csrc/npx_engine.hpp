@@ -179,13 +179,13 @@ static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   S.cut2 = std::max(S.h2, S.sp_hi);
   {
     // skin radius hs = (1 + f) h; a pair outside the skin at x*_0 stays
-    // beyond the cut while 2 D <= hs - |cut|, D the summed per-iteration
-    // maximum displacement.  The relative margins cover the rounding of the
-    // float distance tests.
+    // beyond the cut while c_i + c_j <= hs - |cut|, c a particle's summed
+    // displacement since.  The relative margins cover the rounding of the
+    // float distance tests and sums.
     const double hs = (double)h * (1.0 + (double)skin_frac);
     S.skin_cut2 = (float)(hs * hs);
-    const double allow = 0.5 * (hs * (1.0 - 1e-5) - std::sqrt((double)S.cut2) * (1.0 + 1e-5));
-    S.skin_allow = (P.solver_iterations > NPX_DISP_SLOTS || allow <= 0.0) ? -1.0f : (float)allow;
+    const double allow = hs * (1.0 - 1e-5) - std::sqrt((double)S.cut2) * (1.0 + 1e-5);
+    S.skin_allow = allow > 0.0 ? (float)allow : -1.0f;
   }
   S.iters = P.solver_iterations;
   S.bounds = P.use_domain_bounds ? 1 : 0;
@@ -238,7 +238,8 @@ struct Ctx {
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
-  DBuf<int> DISPB;     // per-scene, per-iteration displacement maxima (float bits)
+  DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
+  DBuf<int> CMAX;   // per class: float bits of the largest summed displacement
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -332,7 +333,6 @@ struct Ctx {
     scenes.assign(max_scenes, SceneH());
     NPX_HIP(d_scenes.ensure(max_scenes));
     NPX_HIP(d_flags.ensure(1 + max_scenes));
-    NPX_HIP(DISPB.ensure((size_t)max_scenes * NPX_DISP_SLOTS * NPX_DISP_LANES));
     NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
     NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));
@@ -374,6 +374,7 @@ struct Ctx {
     NPX_HIP(SKN.ensure(n));
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(EXN.ensure(n));
+    NPX_HIP(CDB.ensure(n));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -386,6 +387,7 @@ struct Ctx {
     if (cells <= cls_cap) return NPX_OK;
     NPX_HIP(clsmap.ensure((size_t)cells));
     NPX_HIP(cstart.ensure((size_t)cells + 1));
+    NPX_HIP(CMAX.ensure((size_t)cells));
     NPX_HIP(bsum.ensure((size_t)cdiv(cells + 1, 1024) + 1));
     cls_cap = cells;
     return NPX_OK;
@@ -680,9 +682,8 @@ struct Ctx {
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
-    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, (unsigned*)DISPB.p};
-    NPX_TRY(launch("clear_disp", k_fill_i32, cdiv(max_scenes * NPX_DISP_SLOTS * NPX_DISP_LANES, 256), 256, DISPB.p,
-                   0, max_scenes * NPX_DISP_SLOTS * NPX_DISP_LANES));
+    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p};
+    NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(total_cls(), 256), 256, CMAX.p, 0, total_cls()));
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
@@ -740,7 +741,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, DISPB.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p};
   }
 
   void drop_graphs() {

@@ -314,56 +314,54 @@ __device__ __forceinline__ int npx_lblock(int remap) {
 // Skin lists.  The first density pass of a step sweeps every candidate (the
 // 27-cell sweep, reference order) and keeps, per particle, the candidates
 // within the skin radius h_s = (1 + f) h, in order.  Later passes of the step
-// visit only that list: a candidate left out had |r| >= h_s at x*_0, and
-// while every particle of the scene has moved by at most D since then (D =
-// the sum over finished iterations of the largest |delta x*|), |r| >= h_s -
-// 2D stays beyond the cut (skin_allow bounds D with a relative margin for
-// rounding), so the list holds every in-range candidate.  Each density pass
-// (and the XSPH / curl pass at the final x*) first tests the list entries
-// with the cheap distance check and records a 64-bit mask of the entries in
-// range, then runs the SPH arithmetic over the set bits only, writing the
-// in-range candidates as a compacted list that the delta-p pass of the same
-// iteration (and the vorticity-force pass) visits.
+// visit only that list.  A candidate j left out of i's list had |r| >= h_s at
+// x*_0, and |r| can since have shrunk by at most c_i + c_j, c the summed
+// per-iteration displacement of a particle.  The candidates of i are the
+// particles of the alias classes of its 27 cells, so with C_i the largest c
+// over those classes (kept per class with atomic maxima), i may use its list
+// while c_i + C_i <= h_s - |cut| (skin_allow, with relative margins for
+// rounding): then the list holds every in-range candidate.  Each testing pass (density,
+// and the XSPH / curl pass at the final x*) decides per particle, tests the
+// list entries with the cheap distance check, runs the SPH arithmetic over
+// the in-range entries only and writes them as a compacted list; the pass
+// that reuses the same positions (delta-p, vorticity force) follows that
+// list, or sweeps inline when the testing pass did (in-range count -1).
 // Skipped entries are exact zero terms of the reference's sums, so the
 // results are the reference's bit for bit, and every lane of a wave works on
-// a real pair.  When the bound fails, or a particle has more than
-// NPX_SKIN_CAP skin candidates, the pass sweeps the candidates inline
-// instead.  Lists are
-// stored per 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64,
-// so entry k of all lanes is one coalesced access (the compacted list is
-// written entry by entry in lockstep across the wave, so its stores are
-// coalesced too).  Visitors fetch the data
-// of G candidates before evaluating any of them, so G loads are in flight
-// per wave.
+// a real pair.  A particle with more than NPX_SKIN_CAP skin candidates, or
+// one whose bound fails, sweeps its candidates inline.  Lists are stored per
+// 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64, so entry k
+// of all lanes is one coalesced access (the compacted list is written entry
+// by entry in lockstep across the wave, so its stores are coalesced too).
+// Visitors fetch the data of G candidates before evaluating any of them, so
+// G loads are in flight per wave.
 #define NPX_ROWS 9
 #define NPX_SKIN_CAP 64
-#define NPX_DISP_SLOTS 32  // per-scene displacement maxima (one per iteration)
-#define NPX_DISP_LANES 16  // each spread over 16 words to keep the atomics uncontended
 
 struct Skin {
-  int* __restrict__ idx;        // [slots / 64][NPX_SKIN_CAP][64] skin list
-  int* __restrict__ cnt;        // [slots]; -1: sweep inline
-  int* __restrict__ ex;         // same layout: in-range list of the current testing pass
-  int* __restrict__ ecnt;       // [slots]
-  unsigned* __restrict__ disp;  // [scenes][NPX_DISP_SLOTS][NPX_DISP_LANES], float bits of max |delta x*|
+  int* __restrict__ idx;     // [slots / 64][NPX_SKIN_CAP][64] skin list
+  int* __restrict__ cnt;     // [slots]; -1: sweep inline
+  int* __restrict__ ex;      // same layout: in-range list of the current testing pass
+  int* __restrict__ ecnt;    // [slots]; -1: the testing pass swept inline
+  float* __restrict__ cd;      // [slots] summed displacement since the lists were built
+  unsigned* __restrict__ cmax; // [classes] float bits of the largest cd in each class
 };
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
   return ((size_t)(q >> 6) * NPX_SKIN_CAP + k) * 64 + (q & 63);
 }
 
-// The skin list is usable for a pass after `upto` finished iterations.
-__device__ __forceinline__ bool skin_valid(const SceneC& S, const unsigned* __restrict__ disp, int s, int upto) {
-  float D = 0.0f;
-  const int nk = min(upto, NPX_DISP_SLOTS);  // skin_allow < 0 when iters > NPX_DISP_SLOTS
-  for (int k = 0; k < nk; ++k) {
-    const unsigned* p = disp + ((size_t)s * NPX_DISP_SLOTS + k) * NPX_DISP_LANES;
-    float mx = 0.0f;
+// Largest summed displacement among the candidates of the particle in window
+// cell l (the classes of its 27 cells).
+__device__ __forceinline__ float candidate_disp(const Skin& K, int l, int wx, int wxy, const int* __restrict__ cm) {
+  float mx = 0.0f;
 #pragma unroll
-    for (int u = 0; u < NPX_DISP_LANES; ++u) mx = fmaxf(mx, __uint_as_float(p[u]));
-    D += mx;
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+#pragma unroll
+    for (int d = -1; d <= 1; ++d) mx = fmaxf(mx, __uint_as_float(K.cmax[cm[base + d]]));
   }
-  return D <= S.skin_allow;
+  return mx;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
@@ -608,23 +606,21 @@ __device__ __forceinline__ void visit_all(int l, int wx, int wxy, const int* __r
   }
 }
 
-// A pass after `upto` finished iterations may use the skin list of slot q.
-__device__ __forceinline__ bool use_skin(const SceneC& S, const Skin& K, int s, int q, int upto) {
-  return K.cnt[q] >= 0 && (upto == 0 || skin_valid(S, K.disp, s, upto));
-}
-
 // Testing pass (density, XSPH / curl): visits the in-range candidates of
-// slot q at xi through the skin list (recording the in-range mask for the
-// pass that follows) or the inline sweep.
+// slot q at xi through the skin list, recording them for the pass that
+// follows, or by the inline sweep (in-range count -1).  C is the particle's
+// candidate_disp.
 template <bool FAST, int G, class L, class F>
-__device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int s, int q, int upto, float4 xi,
-                                           float cut, const float4* __restrict__ XS, int l, int wxy,
+__device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int q, float C, float4 xi, float cut,
+                                           const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                            L&& ld, F&& f) {
-  if (use_skin(S, K, s, q, upto)) {
-    const unsigned long long rel = test_list<FAST>(K, q, K.cnt[q], xi, cut, XS);
+  const int n = K.cnt[q];
+  if (n >= 0 && K.cd[q] + C <= S.skin_allow) {
+    const unsigned long long rel = test_list<FAST>(K, q, n, xi, cut, XS);
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
   } else {
+    K.ecnt[q] = -1;
     visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
   }
 }
@@ -632,11 +628,12 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int s
 // Reusing pass (delta-p, vorticity force): same positions as the testing
 // pass before it, so its in-range list applies.
 template <int G, class L, class F>
-__device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int s, int q, int upto, int l, int wxy,
+__device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int q, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             L&& ld, F&& f) {
-  if (use_skin(S, K, s, q, upto))
-    visit_list<G>(K.ex, q, K.ecnt[q], ld, f);
+  const int e = K.ecnt[q];
+  if (e >= 0)
+    visit_list<G>(K.ex, q, e, ld, f);
   else
     visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
 }
@@ -687,10 +684,12 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
+      K.ecnt[q] = -1;
       visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
   } else {
-    visit_test<FAST, 4>(S, K, s, q, iter, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+    const float C = candidate_disp(K, l, S.wx, wxy, cm);
+    visit_test<FAST, 4>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
   }
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
@@ -782,26 +781,28 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
         dz += coef * (co * rz);
       }
     };
-    visit_reuse<4>(S, K, s, q, iter, l, wxy, cm, cstart, shift, ld, body);
+    visit_reuse<4>(S, K, q, l, wxy, cm, cstart, shift, ld, body);
     const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
     XSn[q] = make_float4(px, py, pz, 0.0f);
     if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
     moved = sqrtf(sqn3(px - xi.x, py - xi.y, pz - xi.z));
+    moved = (iter == 0 ? 0.0f : K.cd[q]) + moved;  // summed displacement since the lists were built
+    K.cd[q] = moved;
   }
-  // blocks never straddle scenes: one atomic per block, spread over the
-  // scene's NPX_DISP_LANES words for this iteration
-  __shared__ float bmax[NPX_MAX_BLOCK / 64];
-  moved = wave_max(moved);
-  if ((threadIdx.x & 63) == 0) bmax[threadIdx.x >> 6] = moved;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float mx = 0.0f;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = fmaxf(mx, bmax[w]);
-    if (mx > 0.0f)
-      atomicMax(K.disp + ((size_t)s * NPX_DISP_SLOTS + min(iter, NPX_DISP_SLOTS - 1)) * NPX_DISP_LANES +
-                    (blockIdx.x & (NPX_DISP_LANES - 1)),
-                __float_as_uint(mx));
+  // Per-class maxima: slots are sorted by class, so the lanes of a class form
+  // one run in the wave; a segmented max leaves the run's maximum in its first
+  // lane, which alone does the atomic.  (The own class of a particle is the
+  // class of its cell, cm[l].)
+  const int cls = active ? cm[l] : -1 - (int)(threadIdx.x & 63);
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const float mo = __shfl_down(moved, o, 64);
+    const int co = __shfl_down(cls, o, 64);
+    if (lane + o < 64 && co == cls) moved = fmaxf(moved, mo);
   }
+  const int prev = __shfl_up(cls, 1, 64);
+  if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
 }
 
 struct XV {
@@ -856,10 +857,13 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
         wz += vj.w * (ax * gy - ay * gx);
       }
     };
-    if (S.iters > 0)  // the skin lists exist once a density pass has run
-      visit_test<false, 2>(S, K, s, q, S.iters, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
-    else
+    if (S.iters > 0) {  // the skin lists exist once a density pass has run
+      const float C = candidate_disp(K, l, S.wx, wxy, cm);
+      visit_test<false, 2>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+    } else {
+      K.ecnt[q] = -1;
       visit_all<2>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+    }
     if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
     if (xs) {
       vx = vx + S.xsph_c * cx;
@@ -910,10 +914,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       ey += sc * (co * ry);
       ez += sc * (co * rz);
     };
-    if (S.iters > 0 && (S.xsph_on || S.vort_on))  // the XSPH / curl pass recorded the mask
-      visit_reuse<4>(S, K, s, q, S.iters, l, wxy, cm, cstart, shift, ld, body);
-    else
-      visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+    visit_reuse<4>(S, K, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
     const float el = sqrtf(sqn3(ex, ey, ez));
     if (el > 1e-9f) {
       const float inv = 1.0f / el;
