@@ -239,6 +239,8 @@ struct Ctx {
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
+  DBuf<float> BCOST;         // per 256-slot block: estimated neighbour-pass cost
+  DBuf<int> BORDER, BMODE;   // block order of the neighbour passes and its mode
   DBuf<int> CMAX;   // per class: float bits of the largest summed displacement
   // class tables
   DBuf<int> clsmap, cstart, bsum;
@@ -378,6 +380,9 @@ struct Ctx {
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
+    NPX_HIP(BCOST.ensure(n / NPX_BLOCK + 4));
+    NPX_HIP(BORDER.ensure(n / NPX_BLOCK + 4));
+    NPX_HIP(BMODE.ensure(1));
     slot_cap = cap;
     h_blk_scene.assign(slot_cap / NPX_BLOCK + 4, -1);
     return NPX_OK;
@@ -683,11 +688,14 @@ struct Ctx {
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p};
+    NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
+                   (const int*)cstart.p, (const int*)CELL.p, hw_blocks, BCOST.p));
+    NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p));
+    const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
     NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(total_cls(), 256), 256, CMAX.p, 0, total_cls()));
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
-    const int rm = xcd_remap;
     int cur = 0;
     const int npass = std::max(imax, 1);
     for (int it = 0; it < npass; ++it) {
@@ -741,7 +749,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, BCOST.p, BORDER.p, BMODE.p};
   }
 
   void drop_graphs() {

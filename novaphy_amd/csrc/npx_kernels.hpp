@@ -285,23 +285,113 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
   const int* __restrict__ cm = clsmap + S.cls_base;          \
   const int l = cell[q];
 
-// XCD-aware block order.  The hardware hands workgroups to the 8 XCDs round
-// robin (workgroup i to XCD i mod 8), and each XCD has its own L2.  With
-// `remap` the neighbour passes process logical block
-//   x * (G / 8) + min(x, G mod 8) + i / 8,   x = i mod 8,
-// a bijection of [0, G) that gives XCD x one contiguous run of slots, so the
-// candidate rows its blocks share stay in its L2.
-__device__ __forceinline__ int npx_lblock(int remap) {
+// Block order of the neighbour passes.  The hardware hands workgroups to the
+// 8 XCDs round robin (workgroup i to XCD i mod 8), and each XCD has its own
+// L2.  Two orders are used, chosen per step by k_block_plan:
+//  * XCD order (mode 0, with `remap`): launch block i processes logical block
+//      x * (G / 8) + min(x, G mod 8) + i / 8,   x = i mod 8,
+//    a bijection of [0, G) that gives XCD x one contiguous run of slots, so
+//    the candidate rows its blocks share stay in its L2;
+//  * cost order (mode 1): 256-slot blocks heaviest first, so the blocks of
+//    large clusters spread over all XCDs and start before the light ones
+//    (used when the XCD order would load the XCDs unevenly).
+// Results do not depend on the order: no pass reads what another block of
+// the same launch writes.
+struct BlockPlan {
+  const int* __restrict__ order;  // [256-slot blocks] cost order
+  const int* __restrict__ mode;   // 1: cost order
+  int remap;                      // 1: XCD order when the cost order is not used
+};
+
+__device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
   const int i = (int)blockIdx.x;
-  if (!remap) return i;
+  if (*bp.mode) {
+    const int k = NPX_BLOCK / (int)blockDim.x;  // launch blocks per 256-slot block
+    return bp.order[i / k] * k + (i % k);
+  }
+  if (!bp.remap) return i;
   const int g = (int)gridDim.x, x = i & 7;
   return x * (g >> 3) + min(x, g & 7) + (i >> 3);
+}
+
+// Estimated cost of each 256-slot block: candidates of its first particle
+// (the populations of the classes of its 27 cells) times its particle count.
+__global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
+                             const int* __restrict__ clsmap, const int* __restrict__ cstart,
+                             const int* __restrict__ cell, int nb, float* __restrict__ cost) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int s = blk_scene[b];
+  float c = 0.0f;
+  if (s >= 0) {
+    const SceneC& S = scenes[s];
+    const int q0 = b * NPX_BLOCK;
+    const int na = min(NPX_BLOCK, S.seg_off + S.n - q0);
+    if (na > 0) {
+      const int* cm = clsmap + S.cls_base;
+      const int l = cell[q0];
+      const int wxy = S.wx * S.wy;
+      int cand = 0;
+      for (int r = 0; r < 9; ++r) {
+        const int base = l + (r % 3 - 1) * S.wx + (r / 3 - 1) * wxy;
+        for (int d = -1; d <= 1; ++d) {
+          const int k = cm[base + d];
+          cand += cstart[k + 1] - cstart[k];
+        }
+      }
+      c = (float)cand * (float)na;
+    }
+  }
+  cost[b] = c;
+}
+
+// Chooses the block order of the step's neighbour passes (one workgroup):
+// the cost order when the XCD order's eight contiguous chunks differ by more
+// than 15 % in estimated cost, else the XCD order.  The cost order is a
+// bucket sort by floor(log2(cost)), descending.
+__global__ void __launch_bounds__(1024) k_block_plan(const float* __restrict__ cost, int nb, int* __restrict__ order,
+                                                     int* __restrict__ mode) {
+  __shared__ float chunk[8];
+  __shared__ int hist[64];
+  __shared__ int base[64];
+  __shared__ int lpt;
+  const int t = threadIdx.x;
+  if (t < 8) chunk[t] = 0.0f;
+  if (t < 64) hist[t] = 0;
+  __syncthreads();
+  for (int b = t; b < nb; b += blockDim.x) atomicAdd(&chunk[(int)((long long)b * 8 / nb)], cost[b]);
+  __syncthreads();
+  if (t == 0) {
+    float tot = 0.0f, mx = 0.0f;
+    for (int x = 0; x < 8; ++x) {
+      tot += chunk[x];
+      mx = fmaxf(mx, chunk[x]);
+    }
+    lpt = (nb >= 16 && mx > 1.15f * (tot / 8.0f)) ? 1 : 0;
+    if (!lpt) *mode = 0;
+  }
+  __syncthreads();
+  if (!lpt) return;
+  auto bucket = [](float c) { return 63 - min(62, max(0, (int)log2f(c + 1.0f))); };
+  for (int b = t; b < nb; b += blockDim.x) atomicAdd(&hist[bucket(cost[b])], 1);
+  __syncthreads();
+  if (t == 0) {
+    int acc = 0;
+    for (int k = 0; k < 64; ++k) {
+      base[k] = acc;
+      acc += hist[k];
+    }
+  }
+  __syncthreads();
+  for (int b = t; b < nb; b += blockDim.x) order[atomicAdd(&base[bucket(cost[b])], 1)] = b;
+  __syncthreads();
+  if (t == 0) *mode = 1;
 }
 
 // Neighbour-pass prologue.  `active` lanes own a particle; the others stay
 // alive to the end (the delta pass reduces over whole waves).
 #define NPX_NBR_PROLOGUE()                                   \
-  const int q = npx_lblock(remap) * (int)blockDim.x + (int)threadIdx.x; \
+  const int q = npx_lblock(bp) * (int)blockDim.x + (int)threadIdx.x; \
   const int s = blk_scene[q / NPX_BLOCK];                    \
   if (s < 0) return; /* uniform per block */                 \
   const SceneC& S = scenes[s];                               \
@@ -478,17 +568,30 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, flo
                                             unsigned long long* rel_out) {
   int k = 0;
   unsigned long long rel = 0;
-  for_each_candidate(l, wx, wxy, cm, cstart, shift, [&](int j) {
-    const float4 a = XS[j];
-    const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
-    if (d < skin_cut) {
-      if (k < NPX_SKIN_CAP) {
-        K.idx[sk_at(q, k)] = j;
-        rel |= (unsigned long long)(d < cut) << k;
+  // visits the candidates in the reference's order (for_each_candidate) and
+  // stops once the list has overflowed
+#pragma unroll 1
+  for (int r = 0; r < NPX_ROWS && k <= NPX_SKIN_CAP; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1], c1 = cm[base], c2 = cm[base + 1];
+    const bool one = (c1 == c0 + 1 && c2 == c0 + 2);
+#pragma unroll 1
+    for (int m = 0; m < (one ? 1 : 3) && k <= NPX_SKIN_CAP; ++m) {
+      const int c = (m == 0) ? c0 : (m == 1) ? c1 : c2;
+      const int b = cstart[c] - shift, e = cstart[one ? c0 + 3 : c + 1] - shift;
+      for (int j = b; j < e && k <= NPX_SKIN_CAP; ++j) {
+        const float4 a = XS[j];
+        const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
+        if (d < skin_cut) {
+          if (k < NPX_SKIN_CAP) {
+            K.idx[sk_at(q, k)] = j;
+            rel |= (unsigned long long)(d < cut) << k;
+          }
+          ++k;
+        }
       }
-      ++k;
     }
-  });
+  }
   *rel_out = rel;
   return k;
 }
@@ -647,7 +750,7 @@ template <bool FAST, bool BUILD>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
-                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, int remap) {
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   if (iter >= S.iters || !active) return;
   const float4 xi = XS[q];
@@ -730,7 +833,7 @@ __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
-        float4* __restrict__ VT, Skin K, int remap) {
+        float4* __restrict__ VT, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   const bool last = (iter == max(S.iters, 1) - 1);
   if (iter >= S.iters) {
@@ -822,7 +925,7 @@ __device__ __forceinline__ void keep(const XV& d) {
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-             const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, int remap) {
+             const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   if (!active) return;
   const float4 xi = XS[q];
@@ -891,7 +994,7 @@ __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
-              float4* __restrict__ X, float4* __restrict__ V, Skin K, int remap) {
+              float4* __restrict__ X, float4* __restrict__ V, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   if (!active) return;
   const float4 xi = XS[q];
