@@ -203,6 +203,13 @@ int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable);
 int npx_kernel_times(npx_ctx* ctx, char* names, int32_t names_cap, float* ms, int32_t* launches,
                      int32_t max_kernels, int32_t* count);
 
+/* The same timed launches one by one, in launch order: names ('\n'-joined),
+ * start offset (ms, from the first timed launch's start event) and duration
+ * of each; *count is the number of launches recorded.  Used for the GPU
+ * track of PerformanceMonitor's trace export.  No reference counterpart. */
+int npx_kernel_trace(npx_ctx* ctx, char* names, int32_t names_cap, float* start_ms, float* dur_ms,
+                     int32_t max_launches, int32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

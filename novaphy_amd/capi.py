@@ -36,7 +36,8 @@ SYMBOLS = (
     "npx_set_boundary", "npx_compute_boundary_volumes", "npx_boundary_world_positions",
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
-    "npx_set_kernel_timing", "npx_kernel_times", "npx_batch_step_timed", "npx_scene_status", "npx_set_tuning",
+    "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
+    "npx_set_tuning",
 )
 
 
@@ -123,6 +124,7 @@ def load() -> C.CDLL:
         "npx_scene_bytes": (i64, [P(NpxParams), i32]),
         "npx_set_kernel_timing": (C.c_int, [vp, i32]),
         "npx_kernel_times": (C.c_int, [vp, C.c_char_p, i32, vp, vp, i32, P(i32)]),
+        "npx_kernel_trace": (C.c_int, [vp, C.c_char_p, i32, vp, vp, i32, P(i32)]),
         "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "npx_scene_status": (C.c_int, [vp, vp, i32]),
         "npx_set_tuning": (C.c_int, [vp, i32, i32, f32]),
@@ -273,6 +275,18 @@ class Context:
 
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
+
+    def kernel_trace(self):
+        """Timed launches in order: [(name, start_ms, dur_ms)]."""
+        cnt = C.c_int32()
+        _check(self._L.npx_kernel_trace(self._h, None, 0, None, None, 0, C.byref(cnt)), self._h)
+        n = cnt.value
+        names = C.create_string_buffer(max(64, 48 * n))
+        st = np.zeros(max(n, 1), np.float32)
+        du = np.zeros(max(n, 1), np.float32)
+        _check(self._L.npx_kernel_trace(self._h, names, len(names), _p(st), _p(du), n, C.byref(cnt)), self._h)
+        keys = names.value.decode().split("\n")[:n]
+        return [(k, float(st[i]), float(du[i])) for i, k in enumerate(keys)]
 
     def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1):
         self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap), C.c_float(skin_fraction)))

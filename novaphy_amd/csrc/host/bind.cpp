@@ -186,6 +186,12 @@ PYBIND11_MODULE(_core, m) {
       .def_readonly("transforms", &SimState::transforms)
       .def_readonly("linear_velocities", &SimState::linear_velocities)
       .def_readonly("angular_velocities", &SimState::angular_velocities)
+      .def("set_transform",
+           [](SimState& s, int i, const Transform& t) {
+             if (i < 0 || i >= (int)s.transforms.size()) throw py::index_error("body index out of range");
+             s.transforms[i] = t;
+           },
+           py::arg("body_index"), py::arg("transform"))
       .def("set_linear_velocity", &SimState::set_linear_velocity, py::arg("body_index"), py::arg("velocity"))
       .def("set_angular_velocity", &SimState::set_angular_velocity, py::arg("body_index"), py::arg("velocity"))
       .def("apply_force", &SimState::apply_force, py::arg("body_index"), py::arg("force"))

@@ -176,6 +176,25 @@ void PBFSolver::step(ParticleState& state, float dt, const Vec3f& gravity, float
   step_n(state, dt, gravity, particle_mass, 1);
 }
 
+std::vector<PBFSolver::KernelLaunch> PBFSolver::kernel_trace() {
+  std::vector<KernelLaunch> out;
+  if (!ctx_ || !timing_) return out;
+  int cnt = 0;
+  ctx_->check(npx_kernel_trace(ctx_->get(), nullptr, 0, nullptr, nullptr, 0, &cnt));
+  if (cnt <= 0) return out;
+  std::vector<char> names((size_t)cnt * 40 + 64);
+  std::vector<float> st(cnt), du(cnt);
+  ctx_->check(npx_kernel_trace(ctx_->get(), names.data(), (int32_t)names.size(), st.data(), du.data(), cnt, &cnt));
+  const char* p = names.data();
+  for (int k = 0; k < cnt && *p; ++k) {
+    const char* e = std::strchr(p, '\n');
+    std::string nm = e ? std::string(p, e - p) : std::string(p);
+    out.push_back({nm, st[k], du[k]});
+    p = e ? e + 1 : p + nm.size();
+  }
+  return out;
+}
+
 std::map<std::string, std::pair<double, int>> PBFSolver::take_kernel_times(bool keep_on) {
   std::map<std::string, std::pair<double, int>> out;
   if (!ctx_) return out;
@@ -443,8 +462,12 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
       }
       state_.bind_device(solver_.context());
       if (profile) {
-        // GPU kernel time under the reference's phase names (passes fused
-        // on the GPU are reported under the first phase they cover)
+        // GPU kernel time under the reference's phase names.  Fused passes
+        // are reported under the first phase they cover: density + lambda
+        // under compute_density, delta-p + apply (+ the velocity update on
+        // the last iteration) under compute_delta_position, XSPH + vorticity
+        // curl under xsph_viscosity, vorticity force + commit under
+        // commit_positions.
         static const std::pair<const char*, const char*> map[] = {
             {"predict_bin", "fluid.pbf.predict"},
             {"fill_counts", "fluid.pbf.grid_build"},
@@ -454,11 +477,28 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"scatter", "fluid.pbf.grid_build"},
             {"rank", "fluid.pbf.neighbor_query"},
             {"gather", "fluid.pbf.neighbor_query"},
+            {"clear_cmax", "fluid.pbf.neighbor_query"},
+            {"block_cost", "fluid.pbf.neighbor_query"},
+            {"block_plan", "fluid.pbf.neighbor_query"},
+            {"density_lambda_build", "fluid.pbf.iteration.compute_density"},
             {"density_lambda", "fluid.pbf.iteration.compute_density"},
             {"delta", "fluid.pbf.iteration.compute_delta_position"},
-            {"omega", "fluid.pbf.vorticity"},
-            {"xsph_commit", "fluid.pbf.xsph_viscosity"},
+            {"omega_xsph", "fluid.pbf.xsph_viscosity"},
+            {"vort_commit", "fluid.pbf.commit_positions"},
         };
+        // algorithmic bytes per particle of each pass (FP32/int32, 12 B per
+        // 3-vector), for the GB/s counters of the trace's GPU track
+        const bool vort = solver_.settings().vorticity_epsilon != 0.0f;
+        const std::map<std::string, double> bytes = {
+            {"predict_bin", 40}, {"gather", 56}, {"density_lambda_build", 20}, {"density_lambda", 20},
+            {"delta", 28}, {"omega_xsph", vort ? 36.0 : 24.0}, {"vort_commit", vort ? 64.0 : 28.0}};
+        for (const auto& k : solver_.kernel_trace()) {
+          auto b = bytes.find(k.name);
+          const double gbps = (b != bytes.end() && k.dur_ms > 0.0f)
+                                  ? b->second * (double)n_fluid / ((double)k.dur_ms * 1e-3) / 1e9
+                                  : 0.0;
+          monitor_.record_gpu_kernel(k.name, k.start_ms, k.dur_ms, gbps);
+        }
         auto kt = solver_.take_kernel_times(false);
         for (const auto& [k, ph] : map) {
           auto it = kt.find(k);

@@ -288,6 +288,10 @@ class PerformanceMonitor {
   void record_phase(const std::string& name, Clock::time_point a, Clock::time_point b);
   void record_phase_ms(const std::string& name, double ms);
   void record_metric(std::string_view name, double v);
+  // One GPU kernel of the current frame for the trace's GPU track: start
+  // offset from the frame's first kernel and duration (ms), and its
+  // algorithmic bandwidth (GB/s; 0 when not stated).
+  void record_gpu_kernel(const std::string& name, double start_ms, double dur_ms, double gbps);
   std::vector<PerformancePhaseStat> phase_stats() const;
   std::vector<PerformanceMetric> last_frame_metrics() const { return last_metrics_; }
   int frame_count() const { return frame_count_; }
@@ -303,9 +307,14 @@ class PerformanceMonitor {
     std::string name;
     long long ts, dur;
   };
+  struct GpuK {
+    std::string name;
+    double start_ms, dur_ms, gbps;
+  };
   struct Frame {
     std::vector<Dur> dur;
-    long long ts = 0;
+    std::vector<GpuK> gpu;
+    long long ts = 0, gpu_ts = 0;
     std::vector<PerformanceMetric> metrics;
   };
   long long us(Clock::time_point t) const {
@@ -318,6 +327,8 @@ class PerformanceMonitor {
   std::map<std::string, double> cur_phase_;
   std::vector<PerformanceMetric> cur_metrics_, last_metrics_;
   std::vector<Dur> cur_dur_;
+  std::vector<GpuK> cur_gpu_;
+  long long cur_gpu_ts_ = 0;
   std::map<std::string, Agg> agg_;
   std::deque<Frame> frames_;
 };
@@ -499,6 +510,12 @@ class PBFSolver {  // pbf_solver.h:46-114
   const std::shared_ptr<DeviceContext>& context() const { return ctx_; }
   void ensure_bound(ParticleState& state);
   std::map<std::string, std::pair<double, int>> take_kernel_times(bool keep_on);
+  struct KernelLaunch {
+    std::string name;
+    float start_ms, dur_ms;
+  };
+  // Timed launches in order (kernel timing must be on), before take_kernel_times.
+  std::vector<KernelLaunch> kernel_trace();
 
  private:
   PBFSettings settings_;

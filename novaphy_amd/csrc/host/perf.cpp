@@ -28,8 +28,12 @@ void PerformanceMonitor::begin_frame() {
   cur_phase_.clear();
   cur_metrics_.clear();
   cur_dur_.clear();
+  cur_gpu_.clear();
   frame_active_ = enabled || trace_enabled;
-  if (frame_active_) frame_start_ = Clock::now();
+  if (frame_active_) {
+    frame_start_ = Clock::now();
+    cur_gpu_ts_ = us(frame_start_);
+  }
 }
 
 void PerformanceMonitor::end_frame() {
@@ -48,6 +52,8 @@ void PerformanceMonitor::end_frame() {
   if (trace_enabled && trace_frame_capacity > 0) {
     Frame f;
     f.dur = cur_dur_;
+    f.gpu = cur_gpu_;
+    f.gpu_ts = cur_gpu_ts_;
     f.ts = us(end);
     f.metrics = last_metrics_;
     f.metrics.push_back({"frame_count", (double)frame_count_});
@@ -69,6 +75,11 @@ void PerformanceMonitor::record_phase_ms(const std::string& name, double ms) {
   if (!frame_active_) return;
   cur_phase_[name] += ms;
   if (trace_enabled && trace_frame_capacity > 0) cur_dur_.push_back({name, us(Clock::now()), (long long)(ms * 1e3)});
+}
+
+void PerformanceMonitor::record_gpu_kernel(const std::string& name, double start_ms, double dur_ms, double gbps) {
+  if (!frame_active_ || !trace_enabled || trace_frame_capacity <= 0) return;
+  cur_gpu_.push_back({name, start_ms, dur_ms, gbps});
 }
 
 void PerformanceMonitor::record_metric(std::string_view name, double v) {
@@ -112,11 +123,31 @@ void PerformanceMonitor::write_trace_json(const std::string& path) const {
     if (!first) out << ",\n";
     first = false;
   };
+  bool named = false;
   for (const Frame& f : frames_) {
     for (const Dur& d : f.dur) {
       sep();
       out << "    {\"name\":\"" << esc(d.name) << "\",\"ph\":\"X\",\"ts\":" << d.ts << ",\"dur\":" << d.dur
           << ",\"pid\":1,\"tid\":0}";
+    }
+    if (!f.gpu.empty() && !named) {  // name the GPU track once
+      sep();
+      out << "    {\"name\":\"thread_name\",\"ph\":\"M\",\"pid\":1,\"tid\":1,\"args\":{\"name\":\"gpu\"}}";
+      named = true;
+    }
+    for (const GpuK& k : f.gpu) {
+      // GPU kernels on their own track, placed from the frame start at their
+      // device offsets; algorithmic GB/s as a counter
+      const long long ts = f.gpu_ts + (long long)(k.start_ms * 1e3);
+      sep();
+      out << "    {\"name\":\"" << esc(k.name) << "\",\"ph\":\"X\",\"ts\":" << ts << ",\"dur\":"
+          << std::max(1LL, (long long)(k.dur_ms * 1e3)) << ",\"pid\":1,\"tid\":1,\"args\":{\"GB/s\":"
+          << std::setprecision(6) << k.gbps << "}}";
+      if (k.gbps > 0.0) {
+        sep();
+        out << "    {\"name\":\"gpu.GB/s\",\"ph\":\"C\",\"ts\":" << ts << ",\"pid\":1,\"tid\":1,\"args\":{\""
+            << esc(k.name) << "\":" << std::setprecision(6) << k.gbps << "}}";
+      }
     }
     sep();
     out << "    {\"name\":\"frame.metrics\",\"ph\":\"C\",\"ts\":" << f.ts << ",\"pid\":1,\"tid\":0,\"args\":{";

@@ -670,4 +670,31 @@ int npx_kernel_times(npx_ctx* ctx, char* names, int32_t names_cap, float* ms, in
   return NPX_OK;
 }
 
+int npx_kernel_trace(npx_ctx* ctx, char* names, int32_t names_cap, float* start_ms, float* dur_ms,
+                     int32_t max_launches, int32_t* count) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !count) return NPX_EINVAL;
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  std::string joined;
+  int k = 0;
+  for (auto& t : c->timers) {
+    if (k < max_launches) {
+      float st = 0.0f, d = 0.0f;
+      NPX_HIP_C(c, hipEventElapsedTime(&st, c->timers.front().a, t.a));
+      NPX_HIP_C(c, hipEventElapsedTime(&d, t.a, t.b));
+      if (start_ms) start_ms[k] = st;
+      if (dur_ms) dur_ms[k] = d;
+      joined += t.name;
+      joined += '\n';
+    }
+    ++k;
+  }
+  *count = k;
+  if (names && names_cap > 0) {
+    std::strncpy(names, joined.c_str(), names_cap - 1);
+    names[names_cap - 1] = '\0';
+  }
+  return NPX_OK;
+}
+
 }  // extern "C"

@@ -35,8 +35,8 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-DONE, CANCELLED, REJECTED, FAILED, RUNNING, WAITING = (
-    "done", "cancelled", "rejected", "failed", "running", "waiting")
+DONE, CANCELLED, REJECTED, FAILED, RUNNING, WAITING, PREEMPTED = (
+    "done", "cancelled", "rejected", "failed", "running", "waiting", "preempted")
 
 
 @dataclasses.dataclass
@@ -54,6 +54,7 @@ class Request:
     error: str = ""
     admitted_at: float = 0.0
     finished_at: float = 0.0
+    snapshot: Optional[dict] = None  # state to resume from (novaphy_amd.snapshot layout)
 
 
 class Server:
@@ -69,6 +70,7 @@ class Server:
         self.pending: "collections.deque[Request]" = collections.deque()
         self.active: Dict[int, Request] = {}
         self.finished: Dict[int, Request] = {}
+        self.preempted: Dict[int, Request] = {}
         self.used = 0
         self.steps = 0
         self.step_ms: List[float] = []
@@ -99,6 +101,38 @@ class Server:
             return True
         return False
 
+    def preempt(self, rid: int) -> bool:
+        """Takes a running request off the device: its state is kept as a
+        snapshot (novaphy_amd.snapshot layout; save it with
+        snapshot.save to survive a restart) and its slots and budget are
+        freed.  resume() queues it again; it continues bit for bit."""
+        r = self.active.get(rid)
+        if r is None:
+            return False
+        r.snapshot = self.b.snapshot(r.sid)
+        self.b.remove(r.sid)
+        self.used -= r.bytes
+        del self.active[rid]
+        r.status = PREEMPTED
+        self.preempted[rid] = r
+        return True
+
+    def resume(self, rid: int, snapshot: Optional[dict] = None, front: bool = True) -> bool:
+        """Queues a preempted request again (ahead of the others with
+        front=True), optionally from a snapshot loaded from disk."""
+        r = self.preempted.pop(rid, None)
+        if r is None:
+            return False
+        if snapshot is not None:
+            r.snapshot = snapshot
+        r.status = WAITING
+        r.arrival = min(r.arrival, self.clock())
+        if front:
+            self.pending.appendleft(r)
+        else:
+            self.pending.append(r)
+        return True
+
     # ---------------------------------------------------------- scheduling
     def _retire(self, r: Request, status: str, read: bool = True) -> None:
         if read:
@@ -116,7 +150,7 @@ class Server:
             if len(self.active) >= self.max_scenes or self.used + r.bytes > self.budget:
                 return                       # FIFO: later arrivals wait behind the head
             try:
-                sid = self.b.add(r.scene)
+                sid = self.b.restore(r.scene, r.snapshot) if r.snapshot is not None else self.b.add(r.scene)
             except Exception as e:            # capacity exhausted on the device side
                 if "capacity" in str(e) or "slots" in str(e):
                     return
@@ -127,6 +161,7 @@ class Server:
                 continue
             self.pending.popleft()
             r.sid = sid
+            r.snapshot = None
             r.status = RUNNING
             r.admitted_at = now
             self.used += r.bytes
@@ -202,6 +237,13 @@ class BatchBackend:
 
     def get(self, key):
         return self.b.get(key)
+
+    def snapshot(self, key) -> dict:
+        p, v, r, l = self.b.get(key)
+        return {"positions": p, "velocities": v, "densities": r, "lambdas": l}
+
+    def restore(self, sc, snap):
+        return self.b.add(sc.params, sc.dt, sc.gravity, sc.mass, snap["positions"], snap["velocities"])
 
     def step(self) -> float:
         return float(self.b.step_timed(1)[0])
@@ -279,6 +321,37 @@ class MixedBackend:
         st = w.fluid_state
         return (st.positions_numpy(), st.velocities_numpy(), st.densities_numpy(), st.lambdas_numpy(),
                 [np.array(t.position) for t in w.state.transforms])
+
+    def snapshot(self, key) -> dict:
+        if key[0] == "b":
+            return self.batch.snapshot(key[1])
+        w, _ = self.worlds[key[1]]
+        st = w.fluid_state
+        ss = w.state
+        return {"positions": st.positions_numpy(), "velocities": st.velocities_numpy(),
+                "densities": st.densities_numpy(), "lambdas": st.lambdas_numpy(),
+                "body_positions": np.array([np.array(t.position) for t in ss.transforms], np.float32),
+                "body_rotations": np.array([np.array(t.rotation) for t in ss.transforms], np.float32),
+                "body_linear_velocities": np.array([np.array(v) for v in ss.linear_velocities], np.float32),
+                "body_angular_velocities": np.array([np.array(v) for v in ss.angular_velocities], np.float32)}
+
+    def restore(self, sc, snap):
+        if not getattr(sc, "spheres", None):
+            key = ("b", self.batch.restore(sc, snap))
+            self.n_batch += 1
+            return key
+        import novaphy_amd as nv
+        w = fluid_world_for(sc)  # boundary particles and psi as at the start of the rollout
+        w.fluid_state.init_arrays(snap["positions"], snap["velocities"])
+        ss = w.state
+        for i in range(len(snap.get("body_positions", []))):
+            ss.set_transform(i, nv.Transform(snap["body_positions"][i], snap["body_rotations"][i]))
+            ss.set_linear_velocity(i, snap["body_linear_velocities"][i])
+            ss.set_angular_velocity(i, snap["body_angular_velocities"][i])
+        key = ("w", self.next_world)
+        self.next_world += 1
+        self.worlds[key[1]] = (w, float(sc.dt))
+        return key
 
     def _step_world(self, k):
         w, dt = self.worlds[k]

@@ -157,7 +157,9 @@ def test_performance_monitor_fluid_phases(gpu, tmp_path):  # test_performance_mo
     world.step(1.0 / 60.0)
     phases = {p.name: p for p in mon.phase_stats()}
     for name in ("fluid.total", "fluid.pbf.total", "fluid.rigid_step", "world.total", "world.broadphase.sap",
-                 "fluid.pbf.iteration.compute_density", "fluid.pbf.iteration.compute_delta_position"):
+                 "fluid.pbf.predict", "fluid.pbf.grid_build", "fluid.pbf.neighbor_query",
+                 "fluid.pbf.iteration.compute_density", "fluid.pbf.iteration.compute_delta_position",
+                 "fluid.pbf.xsph_viscosity", "fluid.pbf.commit_positions"):
         assert name in phases, name
     metrics = {m.name: m.value for m in mon.last_frame_metrics()}
     assert metrics["fluid_particles"] > 0 and metrics["boundary_particles"] >= 0
@@ -167,3 +169,12 @@ def test_performance_monitor_fluid_phases(gpu, tmp_path):  # test_performance_mo
     mon.write_trace_json(str(out))
     ev = json.loads(out.read_text())["traceEvents"]
     assert any(e["ph"] == "X" for e in ev) and any(e["ph"] == "C" for e in ev)
+    # GPU track: every kernel of the step with its device duration, and the
+    # algorithmic GB/s of the PBF passes as counters
+    gpu = [e for e in ev if e.get("tid") == 1 and e["ph"] == "X"]
+    names = {e["name"] for e in gpu}
+    for k in ("predict_bin", "density_lambda_build", "delta", "omega_xsph", "vort_commit"):
+        assert k in names, k
+    assert sum(e["name"] == "delta" for e in gpu) == 2  # one per solver iteration
+    bw = [e for e in ev if e["name"] == "gpu.GB/s"]
+    assert bw and all(v > 0 for e in bw for v in e["args"].values())

@@ -32,6 +32,13 @@ class FakeBatch:
     def get(self, sid):
         return ("state", sid)
 
+    def snapshot(self, sid):
+        return {"positions": np.full((self.scenes[sid], 3), float(sid), np.float32)}
+
+    def restore(self, sc, snap):
+        self.restored = getattr(self, "restored", []) + [snap]
+        return self.add(sc)
+
     def step(self):
         return 2.0 + 0.01 * len(self.scenes)
 
@@ -110,3 +117,43 @@ def test_poisson_requests_seeded():
     assert [r.arrival for r in a] == [r.arrival for r in c]
     assert all(50 <= r.horizon <= 500 for r in a)
     assert all(b.arrival > a_.arrival for a_, b in zip(a, a[1:]))
+
+
+def test_preempt_and_resume_keep_progress():
+    b = FakeBatch()
+    clk = Clock()
+    srv = serve.Server(b, hbm_budget_bytes=1000, max_scenes=8, bytes_fn=lambda p, n: n, clock=clk)
+    srv.submit(serve.Request(rid=0, scene=_scene(100), horizon=6))
+    srv.submit(serve.Request(rid=1, scene=_scene(100), horizon=3))
+    srv.step()
+    srv.step()
+    assert srv.preempt(0) and srv.finished.get(0) is None and srv.used == 100
+    assert srv.active[1].steps_done == 2 and srv.preempted[0].steps_done == 2
+    srv.step()                                   # request 1 finishes alone
+    assert srv.finished[1].status == serve.DONE
+    assert srv.resume(0)
+    srv.run()
+    r = srv.finished[0]
+    assert r.status == serve.DONE and r.steps_done == 6 and len(r.step_ms) == 6
+    assert len(b.restored) == 1 and b.restored[0]["positions"].shape == (100, 3)
+    assert srv.used == 0
+
+
+def test_snapshot_file_round_trip(tmp_path):
+    from novaphy_amd import snapshot
+    rng = np.random.default_rng(0)
+    snap = {"positions": rng.random((7, 3), np.float32), "velocities": rng.random((7, 3), np.float32),
+            "densities": rng.random(7, np.float32), "lambdas": rng.random(7, np.float32),
+            "body_positions": rng.random((2, 3), np.float32), "body_rotations": rng.random((2, 4), np.float32),
+            "body_linear_velocities": rng.random((2, 3), np.float32),
+            "body_angular_velocities": rng.random((2, 3), np.float32)}
+    path = tmp_path / "s.npz"
+    snapshot.save(path, snap, steps_done=42, meta={"rid": 3})
+    got, steps, meta = snapshot.load(path)
+    assert steps == 42 and meta == {"rid": 3} and set(got) == set(snap)
+    for k in snap:
+        np.testing.assert_array_equal(got[k], snap[k])
+    np.savez(tmp_path / "bad.npz", format=np.array("other"))
+    import pytest
+    with pytest.raises(ValueError):
+        snapshot.load(tmp_path / "bad.npz")
