@@ -263,7 +263,7 @@ def run_coupled(steps, warmup):
             "boundary_particles": world.num_boundary_particles, "phases_avg_ms": phases}
 
 
-def run_serve(seconds, rate_hz, threads):
+def run_serve(seconds, rate_hz, threads, backend="fused"):
     """Served rollouts: Poisson arrivals (seed 4) of dam breaks with N
     log-uniform in [4k, 256k], horizons uniform in [50, 500] and 0-8 rigid
     spheres, admitted against 80 % of 288 GB; fluid-only requests share one
@@ -277,7 +277,7 @@ def run_serve(seconds, rate_hz, threads):
     log(f"[bench] serve: {n_req} requests over {seconds:.0f} s, N {min(r.scene.n for r in reqs)}-"
         f"{max(r.scene.n for r in reqs)}, {sum(bool(r.scene.spheres) for r in reqs)} with spheres")
     b = capi.Batch(slot_capacity=1 << 24, cell_capacity=1 << 26, max_scenes=256)
-    be = serve.MixedBackend(b, 256, threads=threads)
+    be = serve.FusedBackend(b, 256) if backend == "fused" else serve.MixedBackend(b, 256, threads=threads)
     srv = serve.Server(be, hbm_budget_bytes=int(0.8 * 288e9), max_scenes=256, bytes_fn=capi.scene_bytes)
     for r in reqs:
         srv.submit(r)
@@ -289,7 +289,8 @@ def run_serve(seconds, rate_hz, threads):
             time.sleep(0.005)
         active.append(len(srv.active))
     wall = time.perf_counter() - t0
-    be.close()
+    if backend != "fused":
+        be.close()
     b.close()
     done = [r for r in srv.finished.values() if r.status == serve.DONE]
     lat = srv.latency_percentiles()
@@ -342,7 +343,9 @@ def main():
     ap.add_argument("--workload", default="1m", choices=["1m", "8k", "batch", "coupled", "serve"])
     ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the served sample")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-threads", type=int, default=8, help="host threads stepping coupled requests")
+    ap.add_argument("--serve-threads", type=int, default=8, help="host threads stepping coupled requests (mixed)")
+    ap.add_argument("--serve-backend", default="fused", choices=["fused", "mixed"],
+                    help="coupling of requests with spheres: fused into the batch, or separate FluidWorlds")
     ap.add_argument("--scenes", type=int, default=1024, help="scene count of the batch workload")
     ap.add_argument("--loop-steps", type=int, default=5, help="timed steps of the per-scene loop baseline")
     ap.add_argument("--fast-math", action="store_true",
@@ -387,7 +390,7 @@ def main():
         print(json.dumps(line), flush=True)
         return
     if args.workload == "serve":
-        r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads)
+        r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend)
         line = {
             "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
             "steps": r["rounds"], "warmup": 0, "ms_per_step": r["p50_step_ms"],
@@ -395,7 +398,7 @@ def main():
             "data": "synthetic",
             "config": {"workload": f"served: {r['requests']} Poisson requests at {args.serve_rate}/s (seed 4), "
                                    "N log-uniform 4k-256k, horizon 50-500, 0-8 spheres with Akinci coupling, "
-                                   "admission against 80% of 288 GB"},
+                                   f"admission against 80% of 288 GB, {args.serve_backend} coupling"},
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "serve": {k: r[k] for k in ("requests", "done", "unfinished", "failed", "wall_s", "latency_samples",
                                         "mean_active")},

@@ -166,6 +166,28 @@ int npx_scene_remove(npx_ctx* ctx, int32_t scene_id);
 int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, float* rho,
                   float* lambda);
 
+/* Akinci boundary of a batched scene (FluidWorld's boundary particles for
+ * that scene: local positions, owning body or -1 for world-owned, volumes
+ * psi as npx_compute_boundary_volumes gives them, and the static flag of each
+ * of its nbodies bodies).  Replaces the scene's previous boundary; nb = 0
+ * removes it.  Batched counterpart of npx_set_boundary (boundary.cpp,
+ * fluid_world.cpp:36-44). */
+int npx_scene_set_boundary(npx_ctx* ctx, int32_t scene_id, const float* local3, const int32_t* body,
+                           const float* psi, int32_t nb, int32_t nbodies, const uint8_t* is_static);
+
+/* Number of bodies over all scenes with a boundary (the length of the pose
+ * and force tables of npx_batch_coupling), or -1. */
+int32_t npx_batch_num_bodies(npx_ctx* ctx);
+
+/* After npx_batch_step: boundary world positions from the bodies' poses,
+ * boundary density and coupling forces for every scene with a boundary, in
+ * one launch per pass over all scenes (FluidWorld::step after the solve,
+ * fluid_world.cpp:58-82).  pose7 / out6 hold the bodies of the scenes with a
+ * boundary by ascending scene id, each scene's bodies in its own order:
+ * position + quaternion (w, x, y, z) in, force + torque out.  Each scene's
+ * result is the single-scene FluidWorld's, bit for bit. */
+int npx_batch_coupling(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out6);
+
 /* Advances every scene by nsteps (each with its own dt / gravity / mass). */
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
 

@@ -37,7 +37,7 @@ SYMBOLS = (
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
     "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
-    "npx_set_tuning",
+    "npx_set_tuning", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
 )
 
 
@@ -128,6 +128,9 @@ def load() -> C.CDLL:
         "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "npx_scene_status": (C.c_int, [vp, vp, i32]),
         "npx_set_tuning": (C.c_int, [vp, i32, i32, f32]),
+        "npx_scene_set_boundary": (C.c_int, [vp, i32, vp, vp, vp, i32, i32, vp]),
+        "npx_batch_num_bodies": (i32, [vp]),
+        "npx_batch_coupling": (C.c_int, [vp, vp, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -357,6 +360,31 @@ class Batch:
                                       _p(pos), _p(vel), pos.shape[0], C.byref(sid)))
         self._n[sid.value] = pos.shape[0]
         return sid.value
+
+    def set_boundary(self, sid: int, local, body, psi, nbodies: int, is_static):
+        """Akinci boundary of scene sid (npx_scene_set_boundary)."""
+        local = _f32(local).reshape(-1, 3)
+        body = np.ascontiguousarray(body, np.int32).reshape(-1)
+        psi = _f32(psi).reshape(-1)
+        st = np.ascontiguousarray(is_static, np.uint8).reshape(-1)
+        if not (local.shape[0] == body.shape[0] == psi.shape[0]) or st.shape[0] != nbodies:
+            raise ValueError("boundary array sizes do not match")
+        self._c(self._L.npx_scene_set_boundary(self._h, int(sid), _p(local), _p(body), _p(psi), local.shape[0],
+                                               int(nbodies), _p(st)))
+
+    def num_bodies(self) -> int:
+        n = int(self._L.npx_batch_num_bodies(self._h))
+        if n < 0:
+            self._c(-1)
+        return n
+
+    def coupling(self, poses7) -> np.ndarray:
+        """Boundary density and coupling forces of every scene with a boundary
+        after a step; returns (bodies, 6) force + torque (npx_batch_coupling)."""
+        poses7 = _f32(poses7).reshape(-1, 7)
+        out = np.zeros((max(poses7.shape[0], 1), 6), np.float32)
+        self._c(self._L.npx_batch_coupling(self._h, _p(poses7), poses7.shape[0], _p(out)))
+        return out[:poses7.shape[0]]
 
     def remove(self, sid: int):
         self._c(self._L.npx_scene_remove(self._h, int(sid)))

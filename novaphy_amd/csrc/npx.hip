@@ -293,6 +293,18 @@ void destroy(Ctx* c) {
                           &c->ATKEY, &c->APERM, &c->aux_clsmap};
   for (auto* b : i1) b->release();
   c->htk.release();
+  for (auto* ps : {&c->bsort, &c->fsort}) {
+    ps->cst.release();
+    ps->cls.release();
+    ps->off.release();
+    ps->tperm.release();
+    ps->tkey.release();
+    ps->perm.release();
+  }
+  for (auto* b : {&c->BBL, &c->BBP, &c->BBPS, &c->BFP, &c->BFS, &c->BFB}) b->release();
+  for (auto* b : {&c->BBODY, &c->BSCN, &c->BBPTR, &c->BBIDX, &c->FPSC}) b->release();
+  for (auto* b : {&c->BPSI, &c->BPOSE, &c->BOUT6}) b->release();
+  c->BSTAT.release();
   c->d_scenes.release();
   c->STATIC.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
@@ -567,6 +579,10 @@ int npx_scene_remove(npx_ctx* ctx, int32_t scene_id) {
   for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = -1;
   c->slot_alloc.release(b0, H.seg_blocks);
   if (H.window_valid) c->cls_alloc.release(H.cls_base, H.ncell);
+  if (H.nbnd > 0) {
+    c->sbnd[scene_id] = Ctx::SceneBnd();
+    c->bnd_dirty = true;
+  }
   H = SceneH();
   c->table_dirty = true;
   return c->upload_tables();
@@ -578,6 +594,44 @@ int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, floa
   if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
     return c->fail(NPX_EINVAL, "no scene %d", scene_id);
   return c->download(c->scenes[scene_id], pos3, vel3, rho, lambda);
+}
+
+int npx_scene_set_boundary(npx_ctx* ctx, int32_t scene_id, const float* local3, const int32_t* body,
+                           const float* psi, int32_t nb, int32_t nbodies, const uint8_t* is_static) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_set_boundary needs a batch context");
+  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+  if (nb < 0 || nbodies < 0) return c->fail(NPX_EINVAL, "negative boundary size");
+  if (nb > 0 && (!local3 || !body || !psi)) return c->fail(NPX_EINVAL, "boundary arrays are NULL");
+  if (nbodies > 0 && !is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
+  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+  if (c->sbnd.size() < (size_t)c->max_scenes) c->sbnd.resize(c->max_scenes);
+  Ctx::SceneBnd& B = c->sbnd[scene_id];
+  B.local3.assign(local3, local3 + 3 * (size_t)nb);
+  B.body.assign(body, body + nb);
+  B.psi.assign(psi, psi + nb);
+  B.is_static.assign(is_static, is_static + nbodies);
+  B.nbodies = nbodies;
+  c->scenes[scene_id].nbnd = nb;
+  c->bnd_dirty = true;
+  c->table_dirty = true;
+  return c->upload_tables();
+}
+
+int32_t npx_batch_num_bodies(npx_ctx* ctx) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return -1;
+  if (c->bnd_dirty && c->rebuild_boundaries() != NPX_OK) return -1;
+  return c->bb_bodies;
+}
+
+int npx_batch_coupling(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out6) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return NPX_EINVAL;
+  if (!c->batch) return c->fail(NPX_ESTATE, "npx_batch_coupling needs a batch context");
+  return c->batch_coupling(pose7, nbodies, out6);
 }
 
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {

@@ -28,6 +28,7 @@ struct __align__(16) SceneC {
   int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
   int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
   int seg_off, n, seg_blocks, vort_on;
+  int nbnd, pad_i1, pad_i2, pad_i3;  // nbnd: boundary particles of the scene (batched coupling)
   float sp_lo, sp_hi;               // spiky support on r^2: (sp_lo, sp_hi), i.e. 1e-7 < |r| < h
   float skin_cut2, skin_allow;      // skin radius^2; largest summed displacement a skin list allows
 };

@@ -97,6 +97,7 @@ struct RangeAlloc {
 
 struct SceneH {
   bool used = false;
+  int nbnd = 0;  // boundary particles (batched coupling)
   npx_pbf_params prm{};
   float dt = 0.0f, g[3] = {0, 0, 0}, mass = 0.0f;
   int n = 0;
@@ -200,6 +201,7 @@ static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   S.seg_off = H.seg_off;
   S.n = H.n;
   S.seg_blocks = H.seg_blocks;
+  S.nbnd = H.nbnd;
   return S;
 }
 
@@ -262,6 +264,145 @@ struct Ctx {
   DBuf<float> OUT6;
   DBuf<int> Acstart, ACLS, AOFF, ATPERM, ATKEY, APERM;
   DBuf<int> aux_clsmap;
+
+  // batched coupling: per-scene boundaries gathered into batch-wide arrays
+  // (scenes with a boundary by ascending id, each scene's bodies in its own
+  // order); rebuilt when a boundary is set or a scene removed
+  struct SceneBnd {
+    std::vector<float> local3, psi;
+    std::vector<int> body;
+    std::vector<uint8_t> is_static;
+    int nbodies = 0;
+  };
+  std::vector<SceneBnd> sbnd;
+  bool bnd_dirty = false;
+  int bb_n = 0, bb_bodies = 0;
+  struct PointSort {
+    DBuf<int> cst, cls, off, tperm, tkey, perm;
+  };
+  PointSort bsort, fsort;
+  DBuf<float4> BBL, BBP, BBPS, BFP, BFS, BFB;
+  DBuf<int> BBODY, BSCN, BBPTR, BBIDX, FPSC;
+  DBuf<float> BPSI, BPOSE, BOUT6;
+  DBuf<uint8_t> BSTAT;
+
+  // Sorts points of known scene (pscene, -1: none) into the global classes:
+  // ps.cst holds the class starts, PS the payload in class order, ascending
+  // key inside a class (key = point index when NULL).
+  int sort_scene_points(PointSort& ps, const float4* P, const int* pscene, int n, const int* key, float4* PS) {
+    const int ncls = total_cls();
+    NPX_HIP(ps.cst.ensure((size_t)ncls + 1));
+    const size_t m = (size_t)std::max(n, 1);
+    NPX_HIP(ps.cls.ensure(m));
+    NPX_HIP(ps.off.ensure(m));
+    NPX_HIP(ps.tperm.ensure(m));
+    NPX_HIP(ps.tkey.ensure(m));
+    NPX_HIP(ps.perm.ensure(m));
+    NPX_TRY(launch("bsort_fill", k_fill_i32, cdiv(ncls + 1, 256), 256, ps.cst.p, 0, ncls + 1));
+    if (n == 0) return NPX_OK;
+    NPX_TRY(launch("bsort_bin", k_bin_scene_points, cdiv(n, 256), 256, (const SceneC*)d_scenes.p, P, pscene, n,
+                   (const int*)clsmap.p, ps.cst.p, ps.cls.p, ps.off.p));
+    NPX_TRY(scan(ps.cst.p, ncls + 1));
+    NPX_TRY(launch("bsort_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)ps.cst.p, (const int*)ps.cls.p,
+                   (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p));
+    NPX_TRY(launch("bsort_rank", k_aux_rank, cdiv(n, 256), 256, n, ncls, (const int*)ps.cst.p, (const int*)ps.cls.p,
+                   (const int*)ps.tperm.p, (const int*)ps.tkey.p, P, ps.perm.p, PS));
+    return NPX_OK;
+  }
+
+  // Gathers the scenes' boundaries into the batch-wide arrays.
+  int rebuild_boundaries() {
+    if (sbnd.size() < (size_t)max_scenes) sbnd.resize(max_scenes);
+    std::vector<float4> bl;
+    std::vector<int> bd, sc, cnt(1, 0);
+    std::vector<float> ps;
+    std::vector<uint8_t> st;
+    int boff = 0;
+    for (int s = 0; s < max_scenes; ++s) {
+      const SceneBnd& B = sbnd[s];
+      const int nb = (int)B.body.size();
+      if (!scenes[s].used || nb == 0) continue;
+      for (int b = 0; b < nb; ++b) {
+        bl.push_back(make_float4(B.local3[3 * b], B.local3[3 * b + 1], B.local3[3 * b + 2], 0.0f));
+        const int k = B.body[b];
+        bd.push_back((k >= 0 && k < B.nbodies) ? boff + k : -1);
+        sc.push_back(s);
+        ps.push_back(B.psi[b]);
+      }
+      for (int k = 0; k < B.nbodies; ++k) st.push_back(B.is_static[k]);
+      boff += B.nbodies;
+    }
+    bb_n = (int)bl.size();
+    bb_bodies = boff;
+    // per-body lists of boundary particles, ascending
+    cnt.assign(bb_bodies + 1, 0);
+    for (int b = 0; b < bb_n; ++b)
+      if (bd[b] >= 0) cnt[bd[b] + 1]++;
+    for (int k = 0; k < bb_bodies; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int> idx(std::max(bb_n, 1), 0), fillp(cnt.begin(), cnt.end());
+    for (int b = 0; b < bb_n; ++b)
+      if (bd[b] >= 0) idx[fillp[bd[b]]++] = b;
+    const size_t n = (size_t)std::max(bb_n, 1), nbod = (size_t)std::max(bb_bodies, 1);
+    NPX_HIP(BBL.ensure(n));
+    NPX_HIP(BBP.ensure(n));
+    NPX_HIP(BBPS.ensure(n));
+    NPX_HIP(BFB.ensure(n));
+    NPX_HIP(BBODY.ensure(n));
+    NPX_HIP(BSCN.ensure(n));
+    NPX_HIP(BPSI.ensure(n));
+    NPX_HIP(BBIDX.ensure(n));
+    NPX_HIP(BBPTR.ensure(nbod + 1));
+    NPX_HIP(BPOSE.ensure(nbod * 7));
+    NPX_HIP(BOUT6.ensure(nbod * 6));
+    NPX_HIP(BSTAT.ensure(nbod));
+    if (bb_n > 0) {
+      NPX_TRY(copy_sync(BBL.p, bl.data(), sizeof(float4) * bb_n, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BBODY.p, bd.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BSCN.p, sc.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BPSI.p, ps.data(), sizeof(float) * bb_n, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BBIDX.p, idx.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
+    }
+    NPX_TRY(copy_sync(BBPTR.p, cnt.data(), sizeof(int) * (bb_bodies + 1), hipMemcpyHostToDevice));
+    if (bb_bodies > 0) NPX_TRY(copy_sync(BSTAT.p, st.data(), bb_bodies, hipMemcpyHostToDevice));
+    bnd_dirty = false;
+    return NPX_OK;
+  }
+
+  // Boundary positions, boundary density and coupling forces of every scene
+  // with a boundary (FluidWorld::step after the solve, fluid_world.cpp:58-82),
+  // out6: per body force and torque.
+  int batch_coupling(const float* pose7, int nbodies, float* out6) {
+    if (bnd_dirty) NPX_TRY(rebuild_boundaries());
+    if (nbodies != bb_bodies) return fail(NPX_EINVAL, "expected %d body poses, got %d", bb_bodies, nbodies);
+    if (bb_n == 0) return NPX_OK;
+    if (nbodies > 0 && (!pose7 || !out6)) return fail(NPX_EINVAL, "pose7 / out6 is NULL");
+    for (const auto& H : scenes)
+      if (H.used && H.nbnd > 0 && !H.window_valid) return fail(NPX_ESTATE, "no step has run yet");
+    if (nbodies > 0) NPX_TRY(copy_sync(BPOSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice));
+    NPX_TRY(launch("bpos", k_bpos, cdiv(bb_n, 256), 256, (const float4*)BBL.p, (const int*)BBODY.p,
+                   (const float*)BPSI.p, bb_n, (const float*)BPOSE.p, nbodies, BBP.p));
+    NPX_TRY(sort_scene_points(bsort, BBP.p, BSCN.p, bb_n, nullptr, BBPS.p));
+    const SceneC* sc = d_scenes.p;
+    const int* bs = d_blk_scene.p;
+    NPX_TRY(launch("boundary_density", k_boundary_density_b, hw_blocks, NPX_BLOCK, sc, bs, (const float4*)X.p,
+                   (const int*)clsmap.p, (const int*)bsort.cst.p, (const float4*)BBPS.p, RHO.p, d_flags.p));
+    NPX_HIP(BFP.ensure(slot_cap));
+    NPX_HIP(BFS.ensure(slot_cap));
+    NPX_HIP(FPSC.ensure(slot_cap));
+    NPX_TRY(launch("fluid_payload", k_fluid_payload, hw_blocks, NPX_BLOCK, sc, bs, (const float4*)X.p,
+                   (const float*)RHO.p, BFP.p));
+    NPX_TRY(launch("coupled_slots", k_coupled_slots, hw_blocks, NPX_BLOCK, sc, bs, FPSC.p));
+    NPX_TRY(sort_scene_points(fsort, BFP.p, FPSC.p, hw_blocks * NPX_BLOCK, ID.p, BFS.p));
+    NPX_TRY(launch("coupling", k_coupling_b, cdiv(bb_n, 256), 256, sc, (const float4*)BBP.p, (const int*)BSCN.p,
+                   (const int*)BBODY.p, bb_n, (const uint8_t*)BSTAT.p, nbodies, (const int*)clsmap.p,
+                   (const int*)fsort.cst.p, (const float4*)BFS.p, BFB.p));
+    if (nbodies > 0) {
+      NPX_TRY(launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
+                     nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p, BOUT6.p));
+      NPX_TRY(copy_sync(out6, BOUT6.p, sizeof(float) * 6 * nbodies, hipMemcpyDeviceToHost));
+    }
+    return NPX_OK;  // a scene whose particles left its window is reported by npx_scene_status
+  }
 
   // timing
   bool timing = false;

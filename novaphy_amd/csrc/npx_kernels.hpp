@@ -1301,6 +1301,121 @@ __global__ void k_body_reduce(const int* __restrict__ body_ptr, const int* __res
   out6[6 * k + 5] = Tz;
 }
 
+// ------------------------------------------------- batched Akinci coupling
+// The boundary particles of every scene of a batch context are binned into
+// their own scene's cell window and alias classes, the same buckets the
+// single-scene path uses (its auxiliary grids also take the scene's window),
+// so boundary density, coupling forces and their float order are the
+// single-scene ones.
+
+// Window-local cell of a point in scene S's window, or -1 when its 27-cell
+// neighbourhood leaves the window.
+__device__ __forceinline__ int scene_cell(const SceneC& S, float x, float y, float z) {
+  const int lx = cell_coord(x, S.inv_h) - S.wlo0;
+  const int ly = cell_coord(y, S.inv_h) - S.wlo1;
+  const int lz = cell_coord(z, S.inv_h) - S.wlo2;
+  if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) return -1;
+  return lx + S.wx * (ly + S.wy * lz);
+}
+
+// Bins points of known scene (pscene[i], -1: none) into global classes.
+__global__ void k_bin_scene_points(const SceneC* __restrict__ scenes, const float4* __restrict__ P,
+                                   const int* __restrict__ pscene, int n, const int* __restrict__ clsmap,
+                                   int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = pscene[i];
+  int c = -1;
+  if (s >= 0) {
+    const SceneC& S = scenes[s];
+    const float4 p = P[i];
+    const int l = scene_cell(S, p.x, p.y, p.z);
+    if (l >= 0) c = clsmap[S.cls_base + l];
+  }
+  cls_of[i] = c;
+  if (c >= 0) off_of[i] = atomicAdd(&ccount[c], 1);
+}
+
+// Scene of each fluid slot when its scene has a boundary, else -1.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_coupled_slots(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, int* __restrict__ pscene) {
+  const int q = npx_slot();
+  const int s = blk_scene[q / NPX_BLOCK];
+  int r = -1;
+  if (s >= 0) {
+    const SceneC& S = scenes[s];
+    if (q < S.seg_off + S.n && S.nbnd > 0) r = s;
+  }
+  pscene[q] = r;
+}
+
+// FluidWorld::apply_boundary_density (fluid_world.cpp:101-123) for every
+// scene with a boundary: rho_i += sum_b (rho0 psi_b) W(x_i - x_b), committed x.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_boundary_density_b(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
+                     const float4* __restrict__ X, const int* __restrict__ clsmap, const int* __restrict__ bcstart,
+                     const float4* __restrict__ BPS, float* __restrict__ RHO, int* __restrict__ flags) {
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = npx_slot();
+  if (q >= S.seg_off + S.n || S.nbnd == 0) return;
+  const float4 xi = X[q];
+  const int l = scene_cell(S, xi.x, xi.y, xi.z);
+  if (l < 0) {
+    raise_flag(flags, s, NPX_FLAG_WINDOW);
+    return;
+  }
+  float bd = 0.0f;
+  const float h2 = S.h2, c6 = S.c6, rho0 = S.rho0;
+  for_each_candidate(l, S.wx, S.wx * S.wy, clsmap + S.cls_base, bcstart, 0, [&](int j) {
+    const float4 xb = BPS[j];
+    bd += rho0 * xb.w * poly6(sqn3(xi.x - xb.x, xi.y - xb.y, xi.z - xb.z), h2, c6);
+  });
+  RHO[q] = RHO[q] + bd;
+}
+
+// FluidWorld::apply_coupling_forces per boundary particle
+// (fluid_world.cpp:140-160), every scene: F_b = sum_f ((m p_f / rho_f^2)
+// rho0 psi_b) gradW over the committed-fluid buckets of b's scene.
+__global__ void k_coupling_b(const SceneC* __restrict__ scenes, const float4* __restrict__ BP,
+                             const int* __restrict__ bscene, const int* __restrict__ body, int nb,
+                             const uint8_t* __restrict__ is_static, int nbodies, const int* __restrict__ clsmap,
+                             const int* __restrict__ fcstart, const float4* __restrict__ FS,
+                             float4* __restrict__ FB) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int bi = body[b];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (bi >= 0 && bi < nbodies && !is_static[bi]) {
+    const SceneC& S = scenes[bscene[b]];
+    const float4 xb = BP[b];
+    const float psi_b = xb.w;
+    const float h = S.h, h2 = S.h2, cs = S.cs, rho0 = S.rho0, mass = S.mass;
+    const int l = scene_cell(S, xb.x, xb.y, xb.z);
+    if (l >= 0) {
+      for_each_candidate(l, S.wx, S.wx * S.wy, clsmap + S.cls_base, fcstart, 0, [&](int j) {
+        const float4 xf = FS[j];
+        const float rx = xb.x - xf.x, ry = xb.y - xf.y, rz = xb.z - xf.z;
+        const float r_sq = sqn3(rx, ry, rz);
+        if (r_sq >= h2) return;
+        const float rho_f = (xf.w < 1.0f) ? 1.0f : xf.w;
+        const float pr = rho_f - rho0;
+        const float pressure = (0.0f < pr) ? pr : 0.0f;
+        const float pc = mass * pressure / (rho_f * rho_f);
+        bool in;
+        const float co = spiky_coeff(r_sq, h, cs, &in);
+        if (!in) return;
+        const float k = pc * rho0 * psi_b;
+        fx += k * (co * rx);
+        fy += k * (co * ry);
+        fz += k * (co * rz);
+      });
+    }
+  }
+  FB[b] = make_float4(fx, fy, fz, 0.0f);
+}
+
 // query_neighbors for arbitrary points (standalone SpatialHashGrid API).
 __global__ void k_query_count(GridWin W, const float4* __restrict__ Q, int nq, const int* __restrict__ clsmap,
                               const int* __restrict__ cstart, int* __restrict__ counts) {

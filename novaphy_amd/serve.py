@@ -17,11 +17,14 @@ get(key), remove(key)``):
   * BatchBackend: fluid-only requests in one batch context; every request
     advances in the same launches (npx_batch_step_timed), and a step's
     latency is its device time.
-  * MixedBackend: fluid-only requests as above, and requests with rigid
-    bodies (Akinci coupling, host rigid pipeline) as FluidWorld instances,
-    each on its own device context and stream, stepped from a thread pool
-    while the batch step runs, so their device work overlaps.  A round's
+  * FusedBackend: every request in one batch context; the Akinci coupling
+    of requests with rigid bodies (boundary density, coupling forces) runs
+    for all of them in the same launches (npx_batch_coupling), and their
+    rigid bodies step on the host (World), as FluidWorld does.  A round's
     latency is the host time of the synchronised round.
+  * MixedBackend: fluid-only requests as above, and requests with rigid
+    bodies as separate FluidWorld instances, each on its own device context
+    and stream, stepped from a thread pool while the batch step runs.
 The scheduling logic takes any object with those methods, which is how the
 CPU tests exercise admission, eviction and cancellation without a GPU.
 """
@@ -281,6 +284,135 @@ def fluid_world_for(sc, plane_extent: Optional[float] = None):
     world = nv.FluidWorld(model, [blk], nv.SolverSettings(), prm, boundary_extent=plane_extent)
     world.fluid_state.init_arrays(sc.positions, sc.velocities)
     return world
+
+
+def _model_for(sc):
+    """Rigid model of a served request: its spheres and a ground plane at
+    y = 0 (the same model fluid_world_for builds)."""
+    import novaphy_amd as nv
+    b = nv.ModelBuilder()
+    b.add_ground_plane(y=0.0)
+    for sp in getattr(sc, "spheres", []):
+        mass = float(sp.density * 4.0 / 3.0 * np.pi * sp.radius ** 3)
+        t = nv.Transform.from_translation(np.array(sp.position, np.float32))
+        i = b.add_body(nv.RigidBody.from_sphere(mass, float(sp.radius)), t)
+        b.add_shape(nv.CollisionShape.make_sphere(float(sp.radius), i))
+    return b.build()
+
+
+def _sqn32(v) -> np.float32:
+    v = np.asarray(v, np.float32)
+    return v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])
+
+
+def _poses7(transforms):
+    out = np.zeros((len(transforms), 7), np.float32)
+    for k, t in enumerate(transforms):
+        out[k, :3] = np.array(t.position, np.float32)
+        x, y, z, w = np.array(t.rotation, np.float32)  # xyzw
+        out[k, 3:] = (w, x, y, z)
+    return out
+
+
+class FusedBackend:
+    """Every request in one batch context.  A request with rigid spheres
+    gets its Akinci boundary in the batch (npx_scene_set_boundary, volumes
+    computed as FluidWorld computes them) and a host World for its bodies;
+    each round is one batch step, one npx_batch_coupling over all coupled
+    scenes and the host rigid steps (FluidWorld::step order)."""
+
+    def __init__(self, batch, max_scenes: int):
+        self.batch = BatchBackend(batch, max_scenes)
+        self.coupled: Dict[int, tuple] = {}   # sid -> (World, nbodies, is_static, dt)
+
+    def _attach(self, sid, sc, snap=None):
+        import novaphy_amd as nv
+        from novaphy_amd import capi
+        model = _model_for(sc)
+        p = sc.params
+        extent = float(max(p.domain_upper[0], p.domain_upper[2]))
+        bnd = nv.sample_model_boundaries(model, 2.0 * p.particle_radius, extent)
+        local = np.array([np.array(b.local_position, np.float32) for b in bnd], np.float32).reshape(-1, 3)
+        body = np.array([b.body_index for b in bnd], np.int32)
+        nbod = model.num_bodies
+        # volumes at the model's initial transforms, on a scratch context as
+        # FluidWorld computes them (boundary.cpp:152-176)
+        tmp = capi.Context(p, capacity=1)
+        tmp.set_boundary(local, body, None, nbod)
+        psi = tmp.compute_boundary_volumes(_poses7(model.initial_transforms))
+        tmp.close()
+        static = np.array([b.is_static() for b in model.bodies], np.uint8)
+        self.batch.b.set_boundary(sid, local, body, psi, nbod, static)
+        world = nv.World(model, nv.SolverSettings())
+        if snap is not None and "body_positions" in snap:
+            ss = world.state
+            for i in range(len(snap["body_positions"])):
+                ss.set_transform(i, nv.Transform(snap["body_positions"][i], snap["body_rotations"][i]))
+                ss.set_linear_velocity(i, snap["body_linear_velocities"][i])
+                ss.set_angular_velocity(i, snap["body_angular_velocities"][i])
+        self.coupled[sid] = (world, nbod, static, float(sc.dt))
+
+    def add(self, sc):
+        sid = self.batch.add(sc)
+        if getattr(sc, "spheres", None):
+            self._attach(sid, sc)
+        return sid
+
+    def restore(self, sc, snap):
+        sid = self.batch.restore(sc, snap)
+        if getattr(sc, "spheres", None):
+            self._attach(sid, sc, snap)
+        return sid
+
+    def remove(self, key):
+        self.coupled.pop(key, None)
+        self.batch.remove(key)
+
+    def get(self, key):
+        res = self.batch.get(key)
+        if key in self.coupled:
+            w = self.coupled[key][0]
+            res = tuple(res) + ([np.array(t.position) for t in w.state.transforms],)
+        return res
+
+    def snapshot(self, key) -> dict:
+        snap = self.batch.snapshot(key)
+        if key in self.coupled:
+            ss = self.coupled[key][0].state
+            snap.update({
+                "body_positions": np.array([np.array(t.position) for t in ss.transforms], np.float32),
+                "body_rotations": np.array([np.array(t.rotation) for t in ss.transforms], np.float32),
+                "body_linear_velocities": np.array([np.array(v) for v in ss.linear_velocities], np.float32),
+                "body_angular_velocities": np.array([np.array(v) for v in ss.angular_velocities], np.float32)})
+        return snap
+
+    def step(self) -> float:
+        t0 = time.perf_counter()
+        self.batch.step()
+        if self.coupled:
+            keys = sorted(self.coupled)  # the body table order of npx_batch_coupling
+            poses = np.concatenate([_poses7(self.coupled[k][0].state.transforms) for k in keys])
+            out = self.batch.b.coupling(poses)
+            off = 0
+            for k in keys:
+                world, nb, static, dt = self.coupled[k]
+                for i in range(nb):
+                    if static[i]:
+                        continue
+                    f, t = out[off + i, :3], out[off + i, 3:]
+                    # FluidWorld skips a body when both squared norms are 0 in
+                    # float32 (Eigen order x*x + (y*y + z*z)); tiny forces
+                    # whose squares underflow are skipped too
+                    if _sqn32(f) == 0.0 and _sqn32(t) == 0.0:
+                        continue
+                    world.apply_force(i, f)
+                    world.apply_torque(i, t)
+                world.step(dt)
+                off += nb
+        return (time.perf_counter() - t0) * 1e3
+
+    def status(self) -> Dict[object, int]:
+        return self.batch.status()
 
 
 class MixedBackend:

@@ -1,8 +1,9 @@
 """Served rollouts on the GPU: requests that join and leave a running batch
 (continuous batching) end bitwise equal to the same request run alone, for
 fluid-only requests (one batch context) and for requests with rigid spheres
-(FluidWorld with Akinci coupling and the host rigid pipeline, stepped
-concurrently with the batch)."""
+(Akinci coupling and the host rigid pipeline), both with the coupling fused
+into the batch (FusedBackend) and with separate FluidWorld contexts stepped
+concurrently (MixedBackend)."""
 import numpy as np
 import pytest
 
@@ -76,18 +77,24 @@ def test_served_fluid_requests_equal_alone(gpu):
     b.close()
 
 
-def test_served_coupled_requests_equal_alone(gpu):
+def _backend(kind, b):
+    return serve.FusedBackend(b, 8) if kind == "fused" else serve.MixedBackend(b, 8, threads=4)
+
+
+@pytest.mark.parametrize("kind", ["fused", "mixed"])
+def test_served_coupled_requests_equal_alone(gpu, kind):
     reqs = _requests(2, 3, seed=22)
-    _say("case: served mixed requests", [(r.scene.n, len(r.scene.spheres)) for r in reqs])
+    _say("case: served coupled requests,", kind, [(r.scene.n, len(r.scene.spheres)) for r in reqs])
     clock = Clock()
     b = _batch_for(reqs)
-    be = serve.MixedBackend(b, 8, threads=4)
+    be = _backend(kind, b)
     srv = serve.Server(be, hbm_budget_bytes=1 << 40, max_scenes=8, bytes_fn=capi.scene_bytes, clock=clock,
                        health_every=1)
     for r in reqs:
         srv.submit(r)
     _run(srv, clock)
-    be.close()
+    if kind == "mixed":
+        be.close()
     assert all(srv.finished[r.rid].status == serve.DONE for r in reqs), \
         [(r.rid, srv.finished[r.rid].status, srv.finished[r.rid].error) for r in reqs]
     for r in reqs:
@@ -114,7 +121,8 @@ def test_served_coupled_requests_equal_alone(gpu):
     b.close()
 
 
-def test_preempted_requests_continue_bitwise(gpu, tmp_path):
+@pytest.mark.parametrize("kind", ["fused", "mixed"])
+def test_preempted_requests_continue_bitwise(gpu, tmp_path, kind):
     """Requests taken off the device mid-rollout (snapshot written to disk and
     read back) and resumed end bitwise equal to an uninterrupted run, for a
     fluid-only request and one with rigid spheres."""
@@ -123,10 +131,10 @@ def test_preempted_requests_continue_bitwise(gpu, tmp_path):
     for r in reqs:
         r.arrival = 0.0
         r.horizon = 8
-    _say("case: preempt / resume", [(r.scene.n, len(r.scene.spheres)) for r in reqs])
+    _say("case: preempt / resume,", kind, [(r.scene.n, len(r.scene.spheres)) for r in reqs])
     clock = Clock()
     b = _batch_for(reqs)
-    be = serve.MixedBackend(b, 8, threads=2)
+    be = _backend(kind, b)
     srv = serve.Server(be, hbm_budget_bytes=1 << 40, max_scenes=8, bytes_fn=capi.scene_bytes, clock=clock)
     for r in reqs:
         srv.submit(r)
@@ -139,7 +147,8 @@ def test_preempted_requests_continue_bitwise(gpu, tmp_path):
         assert done == 3
         assert srv.resume(r.rid, snapshot=snap)
     _run(srv, clock, rounds=20)
-    be.close()
+    if kind == "mixed":
+        be.close()
     for r in reqs:
         got = srv.finished[r.rid]
         assert got.status == serve.DONE and got.steps_done == 8
