@@ -475,6 +475,7 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"scan_bsum", "fluid.pbf.grid_build"},
             {"scan_down", "fluid.pbf.grid_build"},
             {"scatter", "fluid.pbf.grid_build"},
+            {"rank_sort", "fluid.pbf.neighbor_query"},
             {"rank", "fluid.pbf.neighbor_query"},
             {"gather", "fluid.pbf.neighbor_query"},
             {"clear_cmax", "fluid.pbf.neighbor_query"},

@@ -243,7 +243,8 @@ struct Ctx {
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
   DBuf<float> BCOST;         // per 256-slot block: estimated neighbour-pass cost
   DBuf<int> BORDER, BMODE;   // block order of the neighbour passes and its mode
-  DBuf<int> CMAX;   // per class: float bits of the largest summed displacement
+  DBuf<int> CMAX;   // per class: float bits of the largest summed displacement; then per class a run flag
+  DBuf<uint8_t> RUNL;  // per slot: length - 1 of the run of identical state starting there
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -305,6 +306,8 @@ struct Ctx {
     NPX_TRY(scan(ps.cst.p, ncls + 1));
     NPX_TRY(launch("bsort_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)ps.cst.p, (const int*)ps.cls.p,
                    (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p));
+    NPX_TRY(launch("bsort_rank_sort", k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, ncls, (const int*)ps.cst.p,
+                   (const int*)ps.cls.p, ps.tperm.p, ps.tkey.p));
     NPX_TRY(launch("bsort_rank", k_aux_rank, cdiv(n, 256), 256, n, ncls, (const int*)ps.cst.p, (const int*)ps.cls.p,
                    (const int*)ps.tperm.p, (const int*)ps.tkey.p, P, ps.perm.p, PS));
     return NPX_OK;
@@ -518,6 +521,7 @@ struct Ctx {
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(EXN.ensure(n));
     NPX_HIP(CDB.ensure(n));
+    NPX_HIP(RUNL.ensure(n));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -533,7 +537,7 @@ struct Ctx {
     if (cells <= cls_cap) return NPX_OK;
     NPX_HIP(clsmap.ensure((size_t)cells));
     NPX_HIP(cstart.ensure((size_t)cells + 1));
-    NPX_HIP(CMAX.ensure((size_t)cells));
+    NPX_HIP(CMAX.ensure(2 * (size_t)cells));
     NPX_HIP(bsum.ensure((size_t)cdiv(cells + 1, 1024) + 1));
     cls_cap = cells;
     return NPX_OK;
@@ -783,11 +787,15 @@ struct Ctx {
     NPX_TRY(scan(cstart.p, ncls + 1));
     NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p));
+    NPX_TRY(launch("rank_sort", k_rank_sort, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const int*)CLS.p, TPERM.p, TKEY.p));
     NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p));
+    // per-class displacement maxima and run flags start at zero each step
+    NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls, 256), 256, CMAX.p, 0, 2 * ncls));
     NPX_TRY(launch("gather", k_gather, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)PERM.p, (const float4*)X.p, (const float4*)V.p, (const int*)ID.p, (const int*)CELL.p,
-                   X2.p, V2.p, XS0.p, ID2.p, CELL2.p));
+                   (const int*)clsmap.p, X2.p, V2.p, XS0.p, ID2.p, CELL2.p, RUNL.p, CMAX.p + ncls));
     if (carry_rho) {
       NPX_TRY(launch("gather_rho", k_gather_scalar2, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)PERM.p, (const float*)RHO.p, (const float*)LAM.p, RHO2.p,
@@ -828,12 +836,11 @@ struct Ctx {
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
-    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p};
+    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RUNL.p, CMAX.p + total_cls()};
     NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
                    (const int*)cstart.p, (const int*)CELL.p, hw_blocks, BCOST.p));
     NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p));
     const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
-    NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(total_cls(), 256), 256, CMAX.p, 0, total_cls()));
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
@@ -890,7 +897,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, BCOST.p, BORDER.p, BMODE.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RUNL.p, BCOST.p, BORDER.p, BMODE.p};
   }
 
   void drop_graphs() {
@@ -1052,6 +1059,8 @@ struct Ctx {
     NPX_TRY(scan(Acstart.p, W.ncell + 1));
     NPX_TRY(launch("aux_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)ACLS.p,
                    (const int*)AOFF.p, key, ATPERM.p, ATKEY.p));
+    NPX_TRY(launch("aux_rank_sort", k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, W.ncell + cls_base,
+                   (const int*)cst, (const int*)ACLS.p, ATPERM.p, ATKEY.p));
     NPX_TRY(launch("aux_rank", k_aux_rank, cdiv(n, 256), 256, n, W.ncell + cls_base, (const int*)cst,
                    (const int*)ACLS.p, (const int*)ATPERM.p, (const int*)ATKEY.p, P, APERM.p, PS));
     return NPX_OK;

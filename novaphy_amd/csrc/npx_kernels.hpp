@@ -14,6 +14,7 @@
 #pragma once
 
 #include "npx_device.hpp"
+#include "npx_repadd.hpp"
 
 #define NPX_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
 
@@ -170,30 +171,50 @@ k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
               const float4* __restrict__ V, const int* __restrict__ clsmap, int* __restrict__ ccount,
               int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of, int* __restrict__ flags) {
   const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;
+  if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
   const int q = npx_slot();
-  if (q >= S.seg_off + S.n) return;
-  const float4 x = X[q];
-  const float4 v = V[q];
-  const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
-  const float sx = x.x + vx * S.dt, sy = x.y + vy * S.dt, sz = x.z + vz * S.dt;
-  int lx = cell_coord(sx, S.inv_h) - S.wlo0;
-  int ly = cell_coord(sy, S.inv_h) - S.wlo1;
-  int lz = cell_coord(sz, S.inv_h) - S.wlo2;
-  // The 27-cell neighbourhood must lie inside the window; outside it the
-  // class table is undefined, so the step is flagged as failed.
-  if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
-    raise_flag(flags, s, NPX_FLAG_WINDOW);
-    lx = min(max(lx, 1), S.wx - 2);
-    ly = min(max(ly, 1), S.wy - 2);
-    lz = min(max(lz, 1), S.wz - 2);
+  const bool active = q < S.seg_off + S.n;
+  int c = -1;
+  if (active) {
+    const float4 x = X[q];
+    const float4 v = V[q];
+    const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
+    const float sx = x.x + vx * S.dt, sy = x.y + vy * S.dt, sz = x.z + vz * S.dt;
+    int lx = cell_coord(sx, S.inv_h) - S.wlo0;
+    int ly = cell_coord(sy, S.inv_h) - S.wlo1;
+    int lz = cell_coord(sz, S.inv_h) - S.wlo2;
+    // The 27-cell neighbourhood must lie inside the window; outside it the
+    // class table is undefined, so the step is flagged as failed.
+    if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
+      raise_flag(flags, s, NPX_FLAG_WINDOW);
+      lx = min(max(lx, 1), S.wx - 2);
+      ly = min(max(ly, 1), S.wy - 2);
+      lz = min(max(lz, 1), S.wz - 2);
+    }
+    const int l = lx + S.wx * (ly + S.wy * lz);
+    c = clsmap[S.cls_base + l];
+    cls_of[q] = c;
+    cell_of[q] = l;
   }
-  const int l = lx + S.wx * (ly + S.wy * lz);
-  const int c = clsmap[S.cls_base + l];
-  cls_of[q] = c;
-  cell_of[q] = l;
-  off_of[q] = atomicAdd(&ccount[c], 1);
+  // One atomic per distinct class of the wave (slots arrive in the previous
+  // step's class order, so a wave holds few classes, and a stack of
+  // particles on one point would otherwise serialise on one counter).  The
+  // offsets only place particles provisionally; k_rank fixes the order.
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long todo = __ballot(active);
+  int off = 0;
+  while (todo) {
+    const int leader = (int)__builtin_ctzll(todo);
+    const int cl = __shfl(c, leader, 64);
+    const unsigned long long same = __ballot(active && c == cl) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ccount[cl], (int)__popcll(same));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) off = base + (int)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  if (active) off_of[q] = off;
 }
 
 // Pass 2: provisional slot = class start + atomic offset.
@@ -212,10 +233,59 @@ k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, 
   tkey[t] = id[q];
 }
 
-// Pass 3: final slot = class start + rank of the particle's original index
+// Pass 3a: inside each 256-slot block, sorts the provisional slots by
+// (class, original index).  Slots are already grouped by class in ascending
+// class order, so every class keeps its slot range and the part of it inside
+// the block becomes sorted by index.  Bitonic sort in LDS.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cls_of,
+            int* __restrict__ tperm, int* __restrict__ tkey) {
+  __shared__ unsigned long long sk[NPX_BLOCK];
+  __shared__ int sp[NPX_BLOCK];
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int i = (int)threadIdx.x;
+  const int t = npx_slot();
+  const bool active = t < S.seg_off + S.n;
+  int p = 0;
+  unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
+  if (active) {
+    p = tperm[t];
+    key = ((unsigned long long)(unsigned)cls_of[p] << 32) | (unsigned)tkey[t];
+  }
+  sk[i] = key;
+  sp[i] = p;
+  __syncthreads();
+  for (int size = 2; size <= NPX_BLOCK; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int j = i ^ stride;
+      if (j > i) {
+        const unsigned long long a = sk[i], b = sk[j];
+        if ((a > b) == ((i & size) == 0)) {
+          sk[i] = b;
+          sk[j] = a;
+          const int tp = sp[i];
+          sp[i] = sp[j];
+          sp[j] = tp;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (active) {
+    tkey[t] = (int)(unsigned)sk[i];
+    tperm[t] = sp[i];
+  }
+}
+
+// Pass 3b: final slot = class start + rank of the particle's original index
 // inside its class, so each bucket lists particles in ascending index like
-// the reference's push_back order (neighbor_search.cpp:18-21).  The work per
-// class is count^2 compares, a small fraction of one neighbour pass.
+// the reference's push_back order (neighbor_search.cpp:18-21).  Indices are
+// unique within a scene; the rank is the slot's position in its own sorted
+// block segment plus, for every other block the class spans, a binary search
+// in that block's sorted segment: about (count / 256) log2(256) reads per
+// particle instead of count.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
@@ -230,31 +300,94 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
   const int c = cls_of[p];
   const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
   const int my = tkey[t];
-  int r = 0;
-  for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  const int own = t / NPX_BLOCK;
+  int r = t - max(b, own * NPX_BLOCK);
+  for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
+    if (blk == own) continue;
+    const int lo = max(b, blk * NPX_BLOCK);
+    int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
+    while (lb < ub) {
+      const int mid = (lb + ub) >> 1;
+      if (tkey[mid] < my)
+        lb = mid + 1;
+      else
+        ub = mid;
+    }
+    r += lb - lo;
+  }
   perm[b + r] = p;
 }
 
-// Pass 4: reorder the state into the new slot order and form x*.
+// Pass 4: reorder the state into the new slot order and form x*; also marks
+// runs of identical state.  Every quantity a step computes for a particle is
+// a function of its committed x, its v' = v + g dt and its cell's candidate
+// order, so particles with bitwise equal (x, v') in one cell (equal x puts
+// them in one cell, and the sort puts them in one class) get bitwise equal
+// values in every pass, and each contributes the same term to a neighbour's
+// sums.  runl[q] + 1 is the number of slots from q up to the next slot whose
+// (x, v') differs from its predecessor's, or to the end of q's 256-slot
+// block; runf[class] is set when the class holds a run longer than one.
+// The neighbour passes' inline sweeps add such a run's term once, k times
+// over (npx_rep_add).  Scenes with no solver iteration carry density from the
+// previous step into the velocity update, so they get no runs.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
          const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ id,
-         const int* __restrict__ cell_of, float4* __restrict__ X2, float4* __restrict__ V2, float4* __restrict__ XS,
-         int* __restrict__ id2, int* __restrict__ cell2) {
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;
+         const int* __restrict__ cell_of, const int* __restrict__ clsmap, float4* __restrict__ X2,
+         float4* __restrict__ V2, float4* __restrict__ XS, int* __restrict__ id2, int* __restrict__ cell2,
+         unsigned char* __restrict__ runl, int* __restrict__ runf) {
+  __shared__ float4 sx[NPX_BLOCK], sv[NPX_BLOCK];
+  __shared__ unsigned long long smask[NPX_BLOCK / 64];
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
+  const int t = (int)threadIdx.x;
   const int q = npx_slot();
-  if (q >= S.seg_off + S.n) return;
-  const int p = perm[q];
-  const float4 x = X[p];
-  const float4 v = V[p];
-  const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
-  X2[q] = x;
-  V2[q] = make_float4(vx, vy, vz, 0.0f);
-  XS[q] = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
-  id2[q] = id[p];
-  cell2[q] = cell_of[p];
+  const bool active = q < S.seg_off + S.n;
+  float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w = x;
+  int c = 0;
+  if (active) {
+    const int p = perm[q];
+    x = X[p];
+    const float4 v = V[p];
+    const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
+    w = make_float4(vx, vy, vz, 0.0f);
+    X2[q] = x;
+    V2[q] = w;
+    XS[q] = make_float4(x.x + vx * S.dt, x.y + vy * S.dt, x.z + vz * S.dt, 0.0f);
+    id2[q] = id[p];
+    c = cell_of[p];
+    cell2[q] = c;
+  }
+  sx[t] = x;
+  sv[t] = w;
+  __syncthreads();
+  bool start = true;  // active slots form a prefix of the block, so t - 1 is active
+  if (active && t > 0 && S.iters > 0) {
+    const float4 a = sx[t - 1], b = sv[t - 1];
+    start = !(__float_as_uint(a.x) == __float_as_uint(x.x) && __float_as_uint(a.y) == __float_as_uint(x.y) &&
+              __float_as_uint(a.z) == __float_as_uint(x.z) && __float_as_uint(b.x) == __float_as_uint(w.x) &&
+              __float_as_uint(b.y) == __float_as_uint(w.y) && __float_as_uint(b.z) == __float_as_uint(w.z));
+  }
+  const unsigned long long m = __ballot(start);
+  if ((t & 63) == 0) smask[t >> 6] = m;
+  __syncthreads();
+  if (!active) return;
+  const int lane = t & 63;
+  int next = NPX_BLOCK;
+  const unsigned long long above = (lane == 63) ? 0ull : (m >> (lane + 1)) << (lane + 1);
+  if (above) {
+    next = (t & ~63) + (int)__builtin_ctzll(above);
+  } else {
+    for (int u = (t >> 6) + 1; u < NPX_BLOCK / 64; ++u)
+      if (smask[u]) {
+        next = u * 64 + (int)__builtin_ctzll(smask[u]);
+        break;
+      }
+  }
+  const int len = next - t;  // 1 .. NPX_BLOCK
+  runl[q] = (unsigned char)(len - 1);
+  if (start && len > 1) runf[clsmap[S.cls_base + c]] = 1;
 }
 
 // Carries density and lambda into the new slot order (only needed when a
@@ -435,6 +568,8 @@ struct Skin {
   int* __restrict__ ecnt;    // [slots]; -1: the testing pass swept inline
   float* __restrict__ cd;      // [slots] summed displacement since the lists were built
   unsigned* __restrict__ cmax; // [classes] float bits of the largest cd in each class
+  const unsigned char* __restrict__ runl;  // [slots] run length - 1 (k_gather)
+  const int* __restrict__ runf;            // [classes] 1: the class holds a run longer than one
 };
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
@@ -560,8 +695,9 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
 // row structure, aliased cells included) and appending those inside the
 // skin; returns the entry count (> NPX_SKIN_CAP: overflow) and the mask of
 // in-range entries.  Used for the particles whose rows are not single slot
-// ranges of at most 64 candidates.
-template <bool FAST>
+// ranges of at most 64 candidates.  With RUNS, classes holding runs of
+// identical state are stepped run by run (one distance test per run).
+template <bool FAST, bool RUNS>
 __device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, float cut, float skin_cut,
                                             const float4* __restrict__ XS, int l, int wx, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
@@ -579,16 +715,22 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, flo
     for (int m = 0; m < (one ? 1 : 3) && k <= NPX_SKIN_CAP; ++m) {
       const int c = (m == 0) ? c0 : (m == 1) ? c1 : c2;
       const int b = cstart[c] - shift, e = cstart[one ? c0 + 3 : c + 1] - shift;
-      for (int j = b; j < e && k <= NPX_SKIN_CAP; ++j) {
+      const bool runs = RUNS && (one ? (K.runf[c0] | K.runf[c0 + 1] | K.runf[c0 + 2]) : K.runf[c]);
+      for (int j = b; j < e && k <= NPX_SKIN_CAP;) {
+        const int kk = runs ? min((int)K.runl[j] + 1, e - j) : 1;
         const float4 a = XS[j];
         const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
         if (d < skin_cut) {
-          if (k < NPX_SKIN_CAP) {
-            K.idx[sk_at(q, k)] = j;
+          if (k + kk > NPX_SKIN_CAP) {
+            k = NPX_SKIN_CAP + 1;  // overflow
+            break;
+          }
+          for (int u = 0; u < kk; ++u, ++k) {
+            K.idx[sk_at(q, k)] = j + u;
             rel |= (unsigned long long)(d < cut) << k;
           }
-          ++k;
         }
+        j += kk;
       }
     }
   }
@@ -646,7 +788,7 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
       if (ks[u] >= 0) {
         K.ex[sk_at(q, e)] = js[u];  // e = G * group + u for every lane still in the loop
         ++e;
-        f(js[u], v[u]);
+        f(js[u], v[u], 1);
       }
     }
   }
@@ -667,7 +809,7 @@ __device__ __forceinline__ void visit_list(const int* __restrict__ list, int q, 
     for (int u = 0; u < G; ++u) keep(v[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u)
-      if (k + u < n) f(js[u], v[u]);
+      if (k + u < n) f(js[u], v[u], 1);
   }
 }
 
@@ -681,15 +823,28 @@ __device__ __forceinline__ void visit_range(int b, int e, L&& ld, F&& f) {
 #pragma unroll
     for (int u = 0; u < G; ++u) keep(v[u]);
 #pragma unroll
-    for (int u = 0; u < G; ++u) f(j + u, v[u]);
+    for (int u = 0; u < G; ++u) f(j + u, v[u], 1);
   }
-  for (; j < e; ++j) f(j, ld(j));
+  for (; j < e; ++j) f(j, ld(j), 1);
 }
 
-// f(j, ld(j)) over every candidate in the reference's order (the 27-cell
-// sweep of for_each_candidate); f applies the cut itself.
-template <int G, class L, class F>
-__device__ __forceinline__ void visit_all(int l, int wx, int wxy, const int* __restrict__ cm,
+// visit_range over a slot range holding runs of identical state: one call
+// per run, f(j, ld(j), k) standing for the k equal terms of slots j..j+k-1.
+template <class L, class F>
+__device__ __forceinline__ void visit_runs(int b, int e, const unsigned char* __restrict__ runl, L&& ld, F&& f) {
+  for (int j = b; j < e;) {
+    const int k = min((int)runl[j] + 1, e - j);
+    f(j, ld(j), k);
+    j += k;
+  }
+}
+
+// f(j, ld(j), k) over every candidate in the reference's order (the 27-cell
+// sweep of for_each_candidate); f applies the cut itself.  With RUNS, classes
+// flagged by k_gather are walked run by run (k = run length), so a stack of
+// m coincident particles costs one term per run instead of m.
+template <int G, bool RUNS, class L, class F>
+__device__ __forceinline__ void visit_all(const Skin& K, int l, int wx, int wxy, const int* __restrict__ cm,
                                           const int* __restrict__ cstart, int shift, L&& ld, F&& f) {
 #pragma unroll 1
   for (int r = 0; r < NPX_ROWS; ++r) {
@@ -698,22 +853,33 @@ __device__ __forceinline__ void visit_all(int l, int wx, int wxy, const int* __r
     const int c1 = cm[base];
     const int c2 = cm[base + 1];
     if (c1 == c0 + 1 && c2 == c0 + 2) {
-      visit_range<G>(cstart[c0] - shift, cstart[c0 + 3] - shift, ld, f);
+      const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
+      if (RUNS && (K.runf[c0] | K.runf[c0 + 1] | K.runf[c0 + 2]))
+        visit_runs(b, e, K.runl, ld, f);
+      else
+        visit_range<G>(b, e, ld, f);
     } else {
 #pragma unroll 1
       for (int k = 0; k < 3; ++k) {
         const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
-        visit_range<G>(cstart[c] - shift, cstart[c + 1] - shift, ld, f);
+        const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+        if (RUNS && K.runf[c])
+          visit_runs(b, e, K.runl, ld, f);
+        else
+          visit_range<G>(b, e, ld, f);
       }
     }
   }
 }
 
+// acc += t, k times in a row (k equal terms of a run), bit for bit.
+__device__ __forceinline__ void add_k(float& acc, float t, int k) { acc = (k == 1) ? acc + t : npx_rep_add(acc, t, k); }
+
 // Testing pass (density, XSPH / curl): visits the in-range candidates of
 // slot q at xi through the skin list, recording them for the pass that
 // follows, or by the inline sweep (in-range count -1).  C is the particle's
 // candidate_disp.
-template <bool FAST, int G, class L, class F>
+template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int q, float C, float4 xi, float cut,
                                            const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
@@ -724,13 +890,13 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int q
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
   } else {
     K.ecnt[q] = -1;
-    visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all<G, RUNS>(K, l, S.wx, wxy, cm, cstart, shift, ld, f);
   }
 }
 
 // Reusing pass (delta-p, vorticity force): same positions as the testing
 // pass before it, so its in-range list applies.
-template <int G, class L, class F>
+template <int G, bool RUNS, class L, class F>
 __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int q, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             L&& ld, F&& f) {
@@ -738,7 +904,7 @@ __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int 
   if (e >= 0)
     visit_list<G>(K.ex, q, e, ld, f);
   else
-    visit_all<G>(l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all<G, RUNS>(K, l, S.wx, wxy, cm, cstart, shift, ld, f);
 }
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
@@ -759,20 +925,22 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
   const float cut = FAST ? h2 : S.cut2;
   float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
   auto ld = [&](int j) { return XS[j]; };
-  auto body = [&](int j, float4 xj) {
+  // k > 1: the k equal terms of a run of identical candidates (slot q can be
+  // one of them; its self term equals theirs: the density term, no gradient)
+  auto body = [&](int j, float4 xj, int k) {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r_sq = dist2<FAST>(rx, ry, rz);
     if (r_sq >= cut) return;  // only reached on the inline sweep
-    rho += m * poly6(r_sq, h2, c6);
+    add_k(rho, m * poly6(r_sq, h2, c6), k);
     if (j == q) return;
     bool in;
     const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
     if (!in) return;
     const float ax = mr * (co * rx), ay = mr * (co * ry), az = mr * (co * rz);
-    sgs += dist2<FAST>(ax, ay, az);
-    gx += ax;
-    gy += ay;
-    gz += az;
+    add_k(sgs, dist2<FAST>(ax, ay, az), k);
+    add_k(gx, ax, k);
+    add_k(gy, ay, k);
+    add_k(gz, az, k);
   };
   if (BUILD) {
     Rows w;
@@ -781,18 +949,18 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (sweep_rows<FAST>(w, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
       n = store_skin(w, K, q, &rel);
     else
-      n = sweep_direct<FAST>(K, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+      n = sweep_direct<FAST, !FAST>(K, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
     if (n > NPX_SKIN_CAP) n = -1;
     K.cnt[q] = n;
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
       K.ecnt[q] = -1;
-      visit_all<4>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+      visit_all<4, !FAST>(K, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+    visit_test<FAST, 4, !FAST>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
   }
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
@@ -855,7 +1023,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     const float cut = FAST ? h2 : S.cut2;
     float dx = 0.0f, dy = 0.0f, dz = 0.0f;
     auto ld = [&](int j) { return XS[j]; };
-    auto body = [&](int j, float4 xj) {
+    auto body = [&](int j, float4 xj, int k) {  // k > 1 only without FAST
       const float ls = xi.w + xj.w;
       if (j == q) return;
       const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
@@ -879,12 +1047,12 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
         dy = __builtin_fmaf(cc, ry, dy);
         dz = __builtin_fmaf(cc, rz, dz);
       } else {
-        dx += coef * (co * rx);
-        dy += coef * (co * ry);
-        dz += coef * (co * rz);
+        add_k(dx, coef * (co * rx), k);
+        add_k(dy, coef * (co * ry), k);
+        add_k(dz, coef * (co * rz), k);
       }
     };
-    visit_reuse<4>(S, K, q, l, wxy, cm, cstart, shift, ld, body);
+    visit_reuse<4, !FAST>(S, K, q, l, wxy, cm, cstart, shift, ld, body);
     const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
     XSn[q] = make_float4(px, py, pz, 0.0f);
     if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
@@ -937,7 +1105,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
     float cx = 0.0f, cy = 0.0f, cz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
-    auto body = [&](int j, const XV& d) {
+    auto body = [&](int j, const XV& d, int k) {
       if (j == q) return;
       const float rx = d.x.x - xi.x, ry = d.x.y - xi.y, rz = d.x.z - xi.z;
       const float r_sq = sqn3(rx, ry, rz);  // equals |x*_i - x*_j|^2 bit for bit
@@ -946,26 +1114,26 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
       const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
       if (xs && r_sq < h2) {
         const float wv = poly6(r_sq, h2, c6);
-        cx += (vj.w * ax) * wv;
-        cy += (vj.w * ay) * wv;
-        cz += (vj.w * az) * wv;
+        add_k(cx, (vj.w * ax) * wv, k);
+        add_k(cy, (vj.w * ay) * wv, k);
+        add_k(cz, (vj.w * az) * wv, k);
       }
       if (vort) {
         bool in;
         const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
         if (!in) return;
         const float gx = co * rx, gy = co * ry, gz = co * rz;
-        wx += vj.w * (ay * gz - az * gy);
-        wy += vj.w * (az * gx - ax * gz);
-        wz += vj.w * (ax * gy - ay * gx);
+        add_k(wx, vj.w * (ay * gz - az * gy), k);
+        add_k(wy, vj.w * (az * gx - ax * gz), k);
+        add_k(wz, vj.w * (ax * gy - ay * gx), k);
       }
     };
     if (S.iters > 0) {  // the skin lists exist once a density pass has run
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
-      visit_test<false, 2>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+      visit_test<false, 2, true>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
     } else {
       K.ecnt[q] = -1;
-      visit_all<2>(l, S.wx, wxy, cm, cstart, shift, ld, body);
+      visit_all<2, false>(K, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
     if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
     if (xs) {
@@ -1005,7 +1173,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
     const float4 oi = OM[q];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
-    auto body = [&](int j, const XOW& d) {
+    auto body = [&](int j, const XOW& d, int k) {
       if (j == q) return;
       const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
       const float r_sq = sqn3(rx, ry, rz);
@@ -1013,11 +1181,11 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
       if (!in) return;
       const float sc = d.vw * (d.ow - oi.w);
-      ex += sc * (co * rx);
-      ey += sc * (co * ry);
-      ez += sc * (co * rz);
+      add_k(ex, sc * (co * rx), k);
+      add_k(ey, sc * (co * ry), k);
+      add_k(ez, sc * (co * rz), k);
     };
-    visit_reuse<4>(S, K, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
+    visit_reuse<4, true>(S, K, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
     const float el = sqrtf(sqn3(ex, ey, ez));
     if (el > 1e-9f) {
       const float inv = 1.0f / el;
@@ -1120,7 +1288,50 @@ __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* 
   tkey[t] = key ? key[i] : i;
 }
 
-// Rank inside the class by key and write the sorted payload copy.
+// Sorts each 256-slot block of the binned slots by (class, key), as
+// k_rank_sort does for the fluid sort (keys are unique within a class).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of, int* __restrict__ tperm,
+                int* __restrict__ tkey) {
+  __shared__ unsigned long long sk[NPX_BLOCK];
+  __shared__ int sp[NPX_BLOCK];
+  const int i = (int)threadIdx.x;
+  const int t = (int)blockIdx.x * NPX_BLOCK + i;
+  const bool active = t < cstart[ncls];
+  int p = 0;
+  unsigned long long key = ~0ull;
+  if (active) {
+    p = tperm[t];
+    key = ((unsigned long long)(unsigned)cls_of[p] << 32) | (unsigned)tkey[t];
+  }
+  sk[i] = key;
+  sp[i] = p;
+  __syncthreads();
+  for (int size = 2; size <= NPX_BLOCK; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int j = i ^ stride;
+      if (j > i) {
+        const unsigned long long a = sk[i], b = sk[j];
+        if ((a > b) == ((i & size) == 0)) {
+          sk[i] = b;
+          sk[j] = a;
+          const int tp = sp[i];
+          sp[i] = sp[j];
+          sp[j] = tp;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (active) {
+    tkey[t] = (int)(unsigned)sk[i];
+    tperm[t] = sp[i];
+  }
+}
+
+// Rank inside the class by key (own block position plus binary searches in
+// the class's other sorted blocks, as k_rank) and write the sorted payload
+// copy.
 __global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
                            const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
                            int* __restrict__ perm, float4* __restrict__ PS) {
@@ -1130,8 +1341,21 @@ __global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, cons
   const int c = cls_of[i];
   const int b = cstart[c], e = cstart[c + 1];
   const int my = tkey[t];
-  int r = 0;
-  for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  const int own = t / NPX_BLOCK;
+  int r = t - max(b, own * NPX_BLOCK);
+  for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
+    if (blk == own) continue;
+    const int lo = max(b, blk * NPX_BLOCK);
+    int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
+    while (lb < ub) {
+      const int mid = (lb + ub) >> 1;
+      if (tkey[mid] < my)
+        lb = mid + 1;
+      else
+        ub = mid;
+    }
+    r += lb - lo;
+  }
   perm[b + r] = i;
   PS[b + r] = P[i];
 }

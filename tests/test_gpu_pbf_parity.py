@@ -239,6 +239,60 @@ def test_coincident_and_near_coincident_pairs(gpu):
     c.close()
 
 
+def test_stacked_identical_particles_bitexact(gpu):
+    """Stacks of particles with identical position and velocity (what the
+    domain clamp builds in box corners), interleaved in index order with a
+    stack at the same point but another velocity, a clamped corner stack at
+    rest and a fluid block around them.  The inline sweeps add such a run's
+    term once per run, k times over; the result must stay bit for bit the
+    reference's."""
+    rng = np.random.default_rng(31)
+    block = (np.array([0.03, 0.03, 0.03], np.float32) + rng.uniform(0.0, 0.35, (3000, 3))).astype(np.float32)
+    vblock = rng.uniform(-0.5, 0.5, block.shape).astype(np.float32)
+    pa = np.array([0.21, 0.13, 0.17], np.float32)
+    parts = [(block, vblock),
+             (np.repeat(pa[None], 2500, 0), np.repeat(np.array([[0.3, -0.2, 0.1]], np.float32), 2500, 0)),
+             (np.repeat(pa[None], 600, 0), np.repeat(np.array([[-0.1, 0.4, 0.0]], np.float32), 600, 0)),
+             (np.repeat(np.array([[0.025, 0.025, 0.025]], np.float32), 700, 0), np.zeros((700, 3), np.float32)),
+             (np.repeat(np.array([[0.3, 0.3, 0.3]], np.float32), 64, 0), np.zeros((64, 3), np.float32))]
+    pos = np.concatenate([p for p, _ in parts]).astype(np.float32)
+    vel = np.concatenate([v for _, v in parts]).astype(np.float32)
+    order = rng.permutation(pos.shape[0])  # interleave the stacks in index order
+    pos, vel = pos[order], vel[order]
+    prm = scenes.PBFParams(corr_k=0.1, corr_delta_q=0.2, vorticity_epsilon=1e-4, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.025)
+    _say("case: stacked identical particles n=", pos.shape[0])
+    c = capi.Context(prm, capacity=pos.shape[0])
+    c.set_particles(pos, vel)
+    c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
+def test_configured_collapse_20_steps_bitexact(gpu):
+    """The configured s_corr (k = 0.1, dq = 0.2h) over 20 steps, long enough
+    for the domain clamp to stack particles in the box corners (about 2800 on
+    one point)."""
+    sc = scenes.dam_break_8k()
+    _say("case: configured s_corr, 20 steps, dam_break_8k")
+    c = _ctx(sc)
+    c.step(sc.dt, sc.gravity, sc.mass, 20)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 20)
+    _say("  largest stack of identical positions:", int(np.unique(op, axis=0, return_counts=True)[1].max()))
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
 def _stats(pos, vel, rho, mass):
     return float(np.mean(rho)), float(0.5 * mass * np.sum(vel.astype(np.float64) ** 2))
 
