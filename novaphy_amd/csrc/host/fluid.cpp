@@ -486,6 +486,7 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"delta", "fluid.pbf.iteration.compute_delta_position"},
             {"omega_xsph", "fluid.pbf.xsph_viscosity"},
             {"vort_commit", "fluid.pbf.commit_positions"},
+            {"run_fill", "fluid.pbf.commit_positions"},
         };
         // algorithmic bytes per particle of each pass (FP32/int32, 12 B per
         // 3-vector), for the GB/s counters of the trace's GPU track

@@ -245,6 +245,8 @@ struct Ctx {
   DBuf<int> BORDER, BMODE;   // block order of the neighbour passes and its mode
   DBuf<int> CMAX;   // per class: float bits of the largest summed displacement; then per class a run flag
   DBuf<uint8_t> RUNL;  // per slot: length - 1 of the run of identical state starting there
+  DBuf<uint8_t> RUNH;  // per slot: distance back to the head of its run
+  DBuf<int> BHEADS;    // per 256-slot block: run heads
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -522,6 +524,8 @@ struct Ctx {
     NPX_HIP(EXN.ensure(n));
     NPX_HIP(CDB.ensure(n));
     NPX_HIP(RUNL.ensure(n));
+    NPX_HIP(RUNH.ensure(n));
+    NPX_HIP(BHEADS.ensure(n / NPX_BLOCK + 4));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -537,7 +541,7 @@ struct Ctx {
     if (cells <= cls_cap) return NPX_OK;
     NPX_HIP(clsmap.ensure((size_t)cells));
     NPX_HIP(cstart.ensure((size_t)cells + 1));
-    NPX_HIP(CMAX.ensure(2 * (size_t)cells));
+    NPX_HIP(CMAX.ensure(2 * (size_t)cells + (size_t)max_scenes));
     NPX_HIP(bsum.ensure((size_t)cdiv(cells + 1, 1024) + 1));
     cls_cap = cells;
     return NPX_OK;
@@ -788,14 +792,15 @@ struct Ctx {
     NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p));
     NPX_TRY(launch("rank_sort", k_rank_sort, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
-                   (const int*)d_blk_scene.p, (const int*)CLS.p, TPERM.p, TKEY.p));
+                   (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)CLS.p, TPERM.p, TKEY.p));
     NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p));
-    // per-class displacement maxima and run flags start at zero each step
-    NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls, 256), 256, CMAX.p, 0, 2 * ncls));
+    // per-class displacement maxima and run flags, per-scene run flags: zero each step
+    NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls + max_scenes, 256), 256, CMAX.p, 0, 2 * ncls + max_scenes));
     NPX_TRY(launch("gather", k_gather, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)PERM.p, (const float4*)X.p, (const float4*)V.p, (const int*)ID.p, (const int*)CELL.p,
-                   (const int*)clsmap.p, X2.p, V2.p, XS0.p, ID2.p, CELL2.p, RUNL.p, CMAX.p + ncls));
+                   (const int*)clsmap.p, X2.p, V2.p, XS0.p, ID2.p, CELL2.p, fast ? 0 : 1, RUNL.p, RUNH.p,
+                   (unsigned*)CMAX.p + ncls, CMAX.p + 2 * ncls, BHEADS.p));
     if (carry_rho) {
       NPX_TRY(launch("gather_rho", k_gather_scalar2, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)PERM.p, (const float*)RHO.p, (const float*)LAM.p, RHO2.p,
@@ -836,9 +841,10 @@ struct Ctx {
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
-    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RUNL.p, CMAX.p + total_cls()};
+    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RUNL.p, RUNH.p,
+                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls()};
     NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
-                   (const int*)cstart.p, (const int*)CELL.p, hw_blocks, BCOST.p));
+                   (const int*)cstart.p, (const int*)CELL.p, (const int*)BHEADS.p, hw_blocks, BCOST.p));
     NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p));
     const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
     const int* cm = clsmap.p;
@@ -863,6 +869,8 @@ struct Ctx {
                    (const float4*)VT.p, OM.p, VX.p, R, rm));
     NPX_TRY(launch("vort_commit", k_vort_commit, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                    (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p, V.p, R, rm));
+    NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, (const unsigned char*)RUNH.p, X.p, V.p,
+                   RHO.p, LAM.p));
     xs_final = cur;
     return NPX_OK;
   }
@@ -897,7 +905,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RUNL.p, BCOST.p, BORDER.p, BMODE.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RUNL.p, RUNH.p, BHEADS.p, BCOST.p, BORDER.p, BMODE.p};
   }
 
   void drop_graphs() {

@@ -233,13 +233,18 @@ k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, 
   tkey[t] = id[q];
 }
 
-// Pass 3a: inside each 256-slot block, sorts the provisional slots by
-// (class, original index).  Slots are already grouped by class in ascending
-// class order, so every class keeps its slot range and the part of it inside
-// the block becomes sorted by index.  Bitonic sort in LDS.
+// Classes with more particles than this are ranked through sorted blocks;
+// smaller ones by direct counting.
+#define NPX_RANK_DIRECT 64
+
+// Pass 3a: inside each 256-slot block holding part of a class larger than
+// NPX_RANK_DIRECT, sorts the provisional slots by (class, original index).
+// Slots are already grouped by class in ascending class order, so every
+// class keeps its slot range and the part of it inside the block becomes
+// sorted by index.  Bitonic sort in LDS.
 __global__ void __launch_bounds__(NPX_BLOCK)
-k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cls_of,
-            int* __restrict__ tperm, int* __restrict__ tkey) {
+k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
+            const int* __restrict__ cls_of, int* __restrict__ tperm, int* __restrict__ tkey) {
   __shared__ unsigned long long sk[NPX_BLOCK];
   __shared__ int sp[NPX_BLOCK];
   const int s = blk_scene[blockIdx.x];
@@ -248,12 +253,14 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
   const int i = (int)threadIdx.x;
   const int t = npx_slot();
   const bool active = t < S.seg_off + S.n;
-  int p = 0;
-  unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
+  int p = 0, c = 0;
   if (active) {
     p = tperm[t];
-    key = ((unsigned long long)(unsigned)cls_of[p] << 32) | (unsigned)tkey[t];
+    c = cls_of[p];
   }
+  if (!__syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT)) return;
+  unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
+  if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
   sk[i] = key;
   sp[i] = p;
   __syncthreads();
@@ -282,10 +289,10 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
 // Pass 3b: final slot = class start + rank of the particle's original index
 // inside its class, so each bucket lists particles in ascending index like
 // the reference's push_back order (neighbor_search.cpp:18-21).  Indices are
-// unique within a scene; the rank is the slot's position in its own sorted
-// block segment plus, for every other block the class spans, a binary search
-// in that block's sorted segment: about (count / 256) log2(256) reads per
-// particle instead of count.
+// unique within a scene.  Small classes count directly; for a larger class
+// the rank is the slot's position in its own sorted block segment plus, for
+// every other block the class spans, a binary search in that block's sorted
+// segment: about (count / 256) log2(256) reads per particle instead of count.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
@@ -300,20 +307,25 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
   const int c = cls_of[p];
   const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
   const int my = tkey[t];
-  const int own = t / NPX_BLOCK;
-  int r = t - max(b, own * NPX_BLOCK);
-  for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
-    if (blk == own) continue;
-    const int lo = max(b, blk * NPX_BLOCK);
-    int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
-    while (lb < ub) {
-      const int mid = (lb + ub) >> 1;
-      if (tkey[mid] < my)
-        lb = mid + 1;
-      else
-        ub = mid;
+  int r = 0;
+  if (e - b <= NPX_RANK_DIRECT) {
+    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  } else {
+    const int own = t / NPX_BLOCK;
+    r = t - max(b, own * NPX_BLOCK);
+    for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
+      if (blk == own) continue;
+      const int lo = max(b, blk * NPX_BLOCK);
+      int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
+      while (lb < ub) {
+        const int mid = (lb + ub) >> 1;
+        if (tkey[mid] < my)
+          lb = mid + 1;
+        else
+          ub = mid;
+      }
+      r += lb - lo;
     }
-    r += lb - lo;
   }
   perm[b + r] = p;
 }
@@ -326,16 +338,23 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
 // values in every pass, and each contributes the same term to a neighbour's
 // sums.  runl[q] + 1 is the number of slots from q up to the next slot whose
 // (x, v') differs from its predecessor's, or to the end of q's 256-slot
-// block; runf[class] is set when the class holds a run longer than one.
-// The neighbour passes' inline sweeps add such a run's term once, k times
-// over (npx_rep_add).  Scenes with no solver iteration carry density from the
-// previous step into the velocity update, so they get no runs.
+// block; runh[q] is the distance back to the first slot of q's run (its
+// head); bit class of runb and runs_s[scene] are set when the class (the
+// scene) holds a run longer than one, and bheads[block] counts the block's
+// heads.  Only heads compute in the
+// neighbour passes: every read of a candidate's data goes to its run's head
+// (the inline sweeps step run by run and add a run's term once, k times over,
+// with npx_rep_add; list entries store heads), and k_run_fill copies the
+// heads' final state to the other members.  Scenes with no solver iteration
+// carry density from the previous step into the velocity update, so they get
+// no runs, and runs_on = 0 (the FMA kernels) turns runs off.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
          const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ id,
          const int* __restrict__ cell_of, const int* __restrict__ clsmap, float4* __restrict__ X2,
          float4* __restrict__ V2, float4* __restrict__ XS, int* __restrict__ id2, int* __restrict__ cell2,
-         unsigned char* __restrict__ runl, int* __restrict__ runf) {
+         int runs_on, unsigned char* __restrict__ runl, unsigned char* __restrict__ runh, unsigned* __restrict__ runb,
+         int* __restrict__ runs_s, int* __restrict__ bheads) {
   __shared__ float4 sx[NPX_BLOCK], sv[NPX_BLOCK];
   __shared__ unsigned long long smask[NPX_BLOCK / 64];
   const int s = blk_scene[blockIdx.x];
@@ -363,7 +382,7 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
   sv[t] = w;
   __syncthreads();
   bool start = true;  // active slots form a prefix of the block, so t - 1 is active
-  if (active && t > 0 && S.iters > 0) {
+  if (active && t > 0 && S.iters > 0 && runs_on) {
     const float4 a = sx[t - 1], b = sv[t - 1];
     start = !(__float_as_uint(a.x) == __float_as_uint(x.x) && __float_as_uint(a.y) == __float_as_uint(x.y) &&
               __float_as_uint(a.z) == __float_as_uint(x.z) && __float_as_uint(b.x) == __float_as_uint(w.x) &&
@@ -372,8 +391,26 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
   const unsigned long long m = __ballot(start);
   if ((t & 63) == 0) smask[t >> 6] = m;
   __syncthreads();
+  if (t == 0) {
+    int nh = 0;
+    for (int u = 0; u < NPX_BLOCK / 64; ++u) nh += (int)__popcll(smask[u]);
+    const int na = max(0, min(NPX_BLOCK, S.seg_off + S.n - (int)blockIdx.x * NPX_BLOCK));
+    bheads[blockIdx.x] = nh - (NPX_BLOCK - na);  // inactive slots count as starts
+  }
   if (!active) return;
   const int lane = t & 63;
+  int head = 0;  // slot 0 of the block always starts a run
+  const unsigned long long below = (lane == 63) ? m : (m & ((2ull << lane) - 1ull));
+  if (below) {
+    head = (t & ~63) + 63 - (int)__builtin_clzll(below);
+  } else {
+    for (int u = (t >> 6) - 1; u >= 0; --u)
+      if (smask[u]) {
+        head = u * 64 + 63 - (int)__builtin_clzll(smask[u]);
+        break;
+      }
+  }
+  runh[q] = (unsigned char)(t - head);
   int next = NPX_BLOCK;
   const unsigned long long above = (lane == 63) ? 0ull : (m >> (lane + 1)) << (lane + 1);
   if (above) {
@@ -387,7 +424,11 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
   }
   const int len = next - t;  // 1 .. NPX_BLOCK
   runl[q] = (unsigned char)(len - 1);
-  if (start && len > 1) runf[clsmap[S.cls_base + c]] = 1;
+  if (start && len > 1) {
+    const unsigned cl = (unsigned)clsmap[S.cls_base + c];
+    atomicOr(runb + (cl >> 5), 1u << (cl & 31u));
+    runs_s[s] = 1;
+  }
 }
 
 // Carries density and lambda into the new slot order (only needed when a
@@ -448,10 +489,12 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
 }
 
 // Estimated cost of each 256-slot block: candidates of its first particle
-// (the populations of the classes of its 27 cells) times its particle count.
+// (the populations of the classes of its 27 cells) times its count of run
+// heads (the particles that compute).
 __global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
                              const int* __restrict__ clsmap, const int* __restrict__ cstart,
-                             const int* __restrict__ cell, int nb, float* __restrict__ cost) {
+                             const int* __restrict__ cell, const int* __restrict__ bheads, int nb,
+                             float* __restrict__ cost) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const int s = blk_scene[b];
@@ -472,7 +515,7 @@ __global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __res
           cand += cstart[k + 1] - cstart[k];
         }
       }
-      c = (float)cand * (float)na;
+      c = (float)cand * (float)max(bheads[b], 1);  // run members do no work
     }
   }
   cost[b] = c;
@@ -569,7 +612,9 @@ struct Skin {
   float* __restrict__ cd;      // [slots] summed displacement since the lists were built
   unsigned* __restrict__ cmax; // [classes] float bits of the largest cd in each class
   const unsigned char* __restrict__ runl;  // [slots] run length - 1 (k_gather)
-  const int* __restrict__ runf;            // [classes] 1: the class holds a run longer than one
+  const unsigned char* __restrict__ runh;  // [slots] distance back to the run's head; 0: a head
+  const unsigned* __restrict__ runb;       // [classes / 32] bit set: the class holds a run longer than one
+  const int* __restrict__ runs_s;          // [scenes] 1: the scene holds a run longer than one
 };
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
@@ -610,20 +655,31 @@ __device__ __forceinline__ void npx_keep(const float4& a) {
 
 __device__ __forceinline__ void keep(const float4& a) { npx_keep(a); }
 
+// Whether any of the n (1 or 3) consecutive classes from c holds a run.
+__device__ __forceinline__ bool cls_runs(const Skin& K, int c, int n) {
+  const unsigned sh = (unsigned)c & 31u, mask = (1u << n) - 1u;
+  const unsigned* w = K.runb + ((unsigned)c >> 5);
+  unsigned bits = w[0] >> sh;
+  if (sh + (unsigned)n > 32u) bits |= w[1] << (32u - sh);
+  return (bits & mask) != 0u;
+}
+
 struct Rows {
   unsigned long long m[NPX_ROWS];  // skin: r^2 < skin_cut2
   unsigned long long e[NPX_ROWS];  // in range: r^2 < cut
   int b[NPX_ROWS];
+  unsigned rf;                     // bit r: row r's classes hold runs
 };
 
 // Sweeps the nine candidate rows of a particle at xi; per row, bit t of m
 // (of e) is set when candidate b + t is inside the skin (in range).  False
 // when a row is not one range of at most 64 slots.
 template <bool FAST>
-__device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, float skin_cut, const float4* __restrict__ XS,
-                                           int l, int wx, int wxy, const int* __restrict__ cm,
-                                           const int* __restrict__ cstart, int shift) {
+__device__ __forceinline__ bool sweep_rows(Rows& w, const Skin& K, bool rs, float4 xi, float cut, float skin_cut,
+                                           const float4* __restrict__ XS, int l, int wx, int wxy,
+                                           const int* __restrict__ cm, const int* __restrict__ cstart, int shift) {
   bool ok = true;
+  w.rf = 0u;
 #pragma unroll
   for (int r = 0; r < NPX_ROWS; ++r) {
     const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
@@ -653,6 +709,7 @@ __device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, float 
     w.m[r] = mk;
     w.e[r] = ek & mk;
     w.b[r] = b;
+    if (rs && ok && cls_runs(K, c0, 3)) w.rf |= 1u << r;
   }
   return ok;
 }
@@ -663,6 +720,7 @@ __device__ __forceinline__ bool sweep_rows(Rows& w, float4 xi, float cut, float 
 __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigned long long* rel_out) {
   unsigned long long mk = w.m[0], ek = w.e[0], rel = 0;
   int b = w.b[0];
+  unsigned rf = w.rf;  // bit 0: the current row holds runs (its entries map to run heads)
   int left = NPX_ROWS - 1;
   int k = 0;
   for (;;) {
@@ -670,6 +728,7 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
       mk = w.m[1];
       ek = w.e[1];
       b = w.b[1];
+      rf >>= 1;
 #pragma unroll
       for (int r = 1; r < NPX_ROWS - 1; ++r) {
         w.m[r] = w.m[r + 1];
@@ -682,7 +741,7 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
     const int t = (int)__builtin_ctzll(mk);
     mk &= mk - 1;
     if (k < NPX_SKIN_CAP) {
-      K.idx[sk_at(q, k)] = b + t;
+      K.idx[sk_at(q, k)] = (rf & 1u) ? b + t - K.runh[b + t] : b + t;  // the run's head holds the data
       rel |= ((ek >> t) & 1ull) << k;
     }
     ++k;
@@ -698,7 +757,7 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
 // ranges of at most 64 candidates.  With RUNS, classes holding runs of
 // identical state are stepped run by run (one distance test per run).
 template <bool FAST, bool RUNS>
-__device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, float cut, float skin_cut,
+__device__ __forceinline__ int sweep_direct(const Skin& K, bool rs, int q, float4 xi, float cut, float skin_cut,
                                             const float4* __restrict__ XS, int l, int wx, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             unsigned long long* rel_out) {
@@ -715,22 +774,35 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, int q, float4 xi, flo
     for (int m = 0; m < (one ? 1 : 3) && k <= NPX_SKIN_CAP; ++m) {
       const int c = (m == 0) ? c0 : (m == 1) ? c1 : c2;
       const int b = cstart[c] - shift, e = cstart[one ? c0 + 3 : c + 1] - shift;
-      const bool runs = RUNS && (one ? (K.runf[c0] | K.runf[c0 + 1] | K.runf[c0 + 2]) : K.runf[c]);
-      for (int j = b; j < e && k <= NPX_SKIN_CAP;) {
-        const int kk = runs ? min((int)K.runl[j] + 1, e - j) : 1;
-        const float4 a = XS[j];
-        const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
-        if (d < skin_cut) {
-          if (k + kk > NPX_SKIN_CAP) {
-            k = NPX_SKIN_CAP + 1;  // overflow
-            break;
+      if (RUNS && rs && (one ? cls_runs(K, c0, 3) : cls_runs(K, c, 1))) {
+        for (int j = b; j < e && k <= NPX_SKIN_CAP;) {
+          const int kk = min((int)K.runl[j] + 1, e - j);
+          const float4 a = XS[j];
+          const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
+          if (d < skin_cut) {
+            if (k + kk > NPX_SKIN_CAP) {
+              k = NPX_SKIN_CAP + 1;  // overflow
+              break;
+            }
+            for (int u = 0; u < kk; ++u, ++k) {
+              K.idx[sk_at(q, k)] = j;  // j heads the run (a class start, a block start or a run start)
+              rel |= (unsigned long long)(d < cut) << k;
+            }
           }
-          for (int u = 0; u < kk; ++u, ++k) {
-            K.idx[sk_at(q, k)] = j + u;
-            rel |= (unsigned long long)(d < cut) << k;
+          j += kk;
+        }
+      } else {
+        for (int j = b; j < e && k <= NPX_SKIN_CAP; ++j) {
+          const float4 a = XS[j];
+          const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
+          if (d < skin_cut) {
+            if (k < NPX_SKIN_CAP) {
+              K.idx[sk_at(q, k)] = j;  // no runs here: every slot is a head
+              rel |= (unsigned long long)(d < cut) << k;
+            }
+            ++k;
           }
         }
-        j += kk;
       }
     }
   }
@@ -844,7 +916,7 @@ __device__ __forceinline__ void visit_runs(int b, int e, const unsigned char* __
 // flagged by k_gather are walked run by run (k = run length), so a stack of
 // m coincident particles costs one term per run instead of m.
 template <int G, bool RUNS, class L, class F>
-__device__ __forceinline__ void visit_all(const Skin& K, int l, int wx, int wxy, const int* __restrict__ cm,
+__device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx, int wxy, const int* __restrict__ cm,
                                           const int* __restrict__ cstart, int shift, L&& ld, F&& f) {
 #pragma unroll 1
   for (int r = 0; r < NPX_ROWS; ++r) {
@@ -854,7 +926,7 @@ __device__ __forceinline__ void visit_all(const Skin& K, int l, int wx, int wxy,
     const int c2 = cm[base + 1];
     if (c1 == c0 + 1 && c2 == c0 + 2) {
       const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
-      if (RUNS && (K.runf[c0] | K.runf[c0 + 1] | K.runf[c0 + 2]))
+      if (RUNS && rs && cls_runs(K, c0, 3))
         visit_runs(b, e, K.runl, ld, f);
       else
         visit_range<G>(b, e, ld, f);
@@ -863,7 +935,7 @@ __device__ __forceinline__ void visit_all(const Skin& K, int l, int wx, int wxy,
       for (int k = 0; k < 3; ++k) {
         const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
         const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
-        if (RUNS && K.runf[c])
+        if (RUNS && rs && cls_runs(K, c, 1))
           visit_runs(b, e, K.runl, ld, f);
         else
           visit_range<G>(b, e, ld, f);
@@ -880,7 +952,7 @@ __device__ __forceinline__ void add_k(float& acc, float t, int k) { acc = (k == 
 // follows, or by the inline sweep (in-range count -1).  C is the particle's
 // candidate_disp.
 template <bool FAST, int G, bool RUNS, class L, class F>
-__device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int q, float C, float4 xi, float cut,
+__device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool rs, int q, float C, float4 xi, float cut,
                                            const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                            L&& ld, F&& f) {
@@ -890,21 +962,21 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, int q
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
   } else {
     K.ecnt[q] = -1;
-    visit_all<G, RUNS>(K, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all<G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
   }
 }
 
 // Reusing pass (delta-p, vorticity force): same positions as the testing
 // pass before it, so its in-range list applies.
 template <int G, bool RUNS, class L, class F>
-__device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, int q, int l, int wxy,
+__device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool rs, int q, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             L&& ld, F&& f) {
   const int e = K.ecnt[q];
   if (e >= 0)
     visit_list<G>(K.ex, q, e, ld, f);
   else
-    visit_all<G, RUNS>(K, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all<G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
 }
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
@@ -918,7 +990,8 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                  float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
-  if (iter >= S.iters || !active) return;
+  const bool rs = K.runs_s[s] != 0;
+  if (iter >= S.iters || !active || (rs && K.runh[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
   const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -946,21 +1019,21 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     Rows w;
     int n = -1;
     unsigned long long rel = 0;
-    if (sweep_rows<FAST>(w, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
+    if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
       n = store_skin(w, K, q, &rel);
     else
-      n = sweep_direct<FAST, !FAST>(K, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+      n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
     if (n > NPX_SKIN_CAP) n = -1;
     K.cnt[q] = n;
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
       K.ecnt[q] = -1;
-      visit_all<4, !FAST>(K, l, S.wx, wxy, cm, cstart, shift, ld, body);
+      visit_all<4, !FAST>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4, !FAST>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
   }
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
@@ -997,7 +1070,7 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 // also performs the velocity update with the h/dt speed clamp (:81-93).
 // Records the scene's largest |delta x*| of the iteration for the skin test.
 template <bool FAST>
-__global__ void __launch_bounds__(NPX_MAX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
@@ -1014,7 +1087,8 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     return;
   }
   float moved = 0.0f;
-  if (active) {
+  const bool rs = K.runs_s[s] != 0;
+  if (active && !(rs && K.runh[q])) {  // run members take their head's values
     const float4 xi = XS[q];
     const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -1052,7 +1126,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
         add_k(dz, coef * (co * rz), k);
       }
     };
-    visit_reuse<4, !FAST>(S, K, q, l, wxy, cm, cstart, shift, ld, body);
+    visit_reuse<4, !FAST>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);
     const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
     XSn[q] = make_float4(px, py, pz, 0.0f);
     if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
@@ -1095,7 +1169,8 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
              const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
-  if (!active) return;
+  const bool rs = K.runs_s[s] != 0;
+  if (!active || (rs && K.runh[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float4 vi = VT[q];
   const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
@@ -1130,10 +1205,10 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     };
     if (S.iters > 0) {  // the skin lists exist once a density pass has run
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
-      visit_test<false, 2, true>(S, K, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+      visit_test<false, 2, true>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
     } else {
       K.ecnt[q] = -1;
-      visit_all<2, false>(K, l, S.wx, wxy, cm, cstart, shift, ld, body);
+      visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
     if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
     if (xs) {
@@ -1164,7 +1239,8 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
               float4* __restrict__ X, float4* __restrict__ V, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
-  if (!active) return;
+  const bool rs = K.runs_s[s] != 0;
+  if (!active || (rs && K.runh[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float4 vx4 = VX[q];
   float vx = vx4.x, vy = vx4.y, vz = vx4.z;
@@ -1185,7 +1261,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       add_k(ey, sc * (co * ry), k);
       add_k(ez, sc * (co * rz), k);
     };
-    visit_reuse<4, true>(S, K, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
+    visit_reuse<4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
     const float el = sqrtf(sqn3(ex, ey, ez));
     if (el > 1e-9f) {
       const float inv = 1.0f / el;
@@ -1206,6 +1282,25 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   }
   X[q] = make_float4(px, py, pz, 0.0f);
   V[q] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// End of a step: every run member takes its head's committed state, density
+// and lambda (the values the member would have computed, bit for bit).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_run_fill(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const unsigned char* __restrict__ runh,
+           float4* __restrict__ X, float4* __restrict__ V, float* __restrict__ RHO, float* __restrict__ LAM) {
+  const int s = blk_scene[npx_slot() / NPX_BLOCK];
+  if (s < 0) return;
+  const SceneC& S = scenes[s];
+  const int q = npx_slot();
+  if (q >= S.seg_off + S.n) return;
+  const int d = runh[q];
+  if (d == 0) return;
+  const int h = q - d;
+  X[q] = X[h];
+  V[q] = V[h];
+  RHO[q] = RHO[h];
+  LAM[q] = LAM[h];
 }
 
 // Grid diagnostics on x*_0: candidate count (duplicates kept) and in-range
@@ -1288,8 +1383,9 @@ __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* 
   tkey[t] = key ? key[i] : i;
 }
 
-// Sorts each 256-slot block of the binned slots by (class, key), as
-// k_rank_sort does for the fluid sort (keys are unique within a class).
+// Sorts each 256-slot block of the binned slots that holds part of a class
+// larger than NPX_RANK_DIRECT by (class, key), as k_rank_sort does for the
+// fluid sort (keys are unique within a class).
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of, int* __restrict__ tperm,
                 int* __restrict__ tkey) {
@@ -1298,12 +1394,14 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
   const int i = (int)threadIdx.x;
   const int t = (int)blockIdx.x * NPX_BLOCK + i;
   const bool active = t < cstart[ncls];
-  int p = 0;
-  unsigned long long key = ~0ull;
+  int p = 0, c = 0;
   if (active) {
     p = tperm[t];
-    key = ((unsigned long long)(unsigned)cls_of[p] << 32) | (unsigned)tkey[t];
+    c = cls_of[p];
   }
+  if (!__syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT)) return;
+  unsigned long long key = ~0ull;
+  if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
   sk[i] = key;
   sp[i] = p;
   __syncthreads();
@@ -1341,20 +1439,25 @@ __global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, cons
   const int c = cls_of[i];
   const int b = cstart[c], e = cstart[c + 1];
   const int my = tkey[t];
-  const int own = t / NPX_BLOCK;
-  int r = t - max(b, own * NPX_BLOCK);
-  for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
-    if (blk == own) continue;
-    const int lo = max(b, blk * NPX_BLOCK);
-    int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
-    while (lb < ub) {
-      const int mid = (lb + ub) >> 1;
-      if (tkey[mid] < my)
-        lb = mid + 1;
-      else
-        ub = mid;
+  int r = 0;
+  if (e - b <= NPX_RANK_DIRECT) {
+    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+  } else {
+    const int own = t / NPX_BLOCK;
+    r = t - max(b, own * NPX_BLOCK);
+    for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
+      if (blk == own) continue;
+      const int lo = max(b, blk * NPX_BLOCK);
+      int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
+      while (lb < ub) {
+        const int mid = (lb + ub) >> 1;
+        if (tkey[mid] < my)
+          lb = mid + 1;
+        else
+          ub = mid;
+      }
+      r += lb - lo;
     }
-    r += lb - lo;
   }
   perm[b + r] = i;
   PS[b + r] = P[i];

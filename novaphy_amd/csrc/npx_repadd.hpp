@@ -43,6 +43,14 @@ NPX_HD float npx_u2f(uint32_t u) {
   return x;
 }
 
+// Largest n >= 0 with m + (n - 1) d staying within bound (steps of size
+// |d| >= 1 from m towards the bound, all values below 2^25): n = k when the
+// whole remaining run fits, so the division is only taken at a binade edge.
+NPX_HD int npx_steps_within(int span, int d, int k) {  // span = |bound - m| >= 0, d > 0
+  if ((long long)(k - 1) * d <= span) return k;
+  return span / d + 1;
+}
+
 NPX_HD float npx_rep_add(float a, float t, int k) {
   for (;;) {
     if (k <= 3) {
@@ -59,27 +67,29 @@ NPX_HD float npx_rep_add(float a, float t, int k) {
     const uint32_t se = b1 >> 23;  // sign and exponent
     const uint32_t e = se & 0xffu;
     if ((b2 >> 23) != se || (b3 >> 23) != se || e == 0u || e == 255u) continue;
-    const double tu = (double)((b1 >> 31) ? -t : t) * ldexp(1.0, 150 - (int)e);  // t / u, exact
-    if (!(fabs(tu) < 4194304.0)) continue;
-    const int64_t m1 = (int64_t)((b1 & 0x7fffffu) | 0x800000u);
-    const int64_t m2 = (int64_t)((b2 & 0x7fffffu) | 0x800000u);
-    const int64_t m3 = (int64_t)((b3 & 0x7fffffu) | 0x800000u);
-    const double s1 = (double)m1 + tu, s2 = (double)m2 + tu;
-    if (s1 < 8388608.0 || s1 > 16777215.0 || s2 < 8388608.0 || s2 > 16777215.0) continue;
-    const int64_t d = m3 - m2;
-    int64_t n;  // further steps from m3 whose exact sums stay in [2^23, 2^24 - 1]
+    // t / u in the positive frame; exact: a2 != a3 gives |t| >= u / 2, and
+    // the range test keeps it far from overflow
+    const float tu = ldexpf((b1 >> 31) ? -t : t, 150 - (int)e);
+    if (!(fabsf(tu) < 4194304.0f)) continue;
+    const int tf = (int)floorf(tu), tc = (int)ceilf(tu);
+    const int m1 = (int)((b1 & 0x7fffffu) | 0x800000u);
+    const int m2 = (int)((b2 & 0x7fffffu) | 0x800000u);
+    const int m3 = (int)((b3 & 0x7fffffu) | 0x800000u);
+    // exact sums m1 + tu and m2 + tu within [2^23, 2^24 - 1]
+    if (m1 + tf < 8388608 || m1 + tc > 16777215 || m2 + tf < 8388608 || m2 + tc > 16777215) continue;
+    const int d = m3 - m2;
+    int n;  // further steps from m3 whose exact sums stay in [2^23, 2^24 - 1]
     if (d > 0) {
-      const int64_t lim = 16777215 - (int64_t)ceil(tu);  // m + tu <= 2^24 - 1
-      n = (m3 > lim) ? 0 : (lim - m3) / d + 1;
+      const int lim = 16777215 - tc;  // m + tu <= 2^24 - 1
+      n = (m3 > lim) ? 0 : npx_steps_within(lim - m3, d, k);
     } else {
-      const int64_t lo = 8388608 - (int64_t)floor(tu);  // m + tu >= 2^23
-      n = (m3 < lo) ? 0 : (m3 - lo) / (-d) + 1;
+      const int lo = 8388608 - tf;  // m + tu >= 2^23
+      n = (m3 < lo) ? 0 : npx_steps_within(m3 - lo, -d, k);
     }
-    if (n > k) n = k;
     if (n > 0) {
-      const int64_t m = m3 + n * d;
+      const int m = m3 + n * d;
       a = npx_u2f((se << 23) | (uint32_t)(m - 0x800000));
-      k -= (int)n;
+      k -= n;
     }
   }
 }
