@@ -478,6 +478,7 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"rank_sort", "fluid.pbf.neighbor_query"},
             {"rank", "fluid.pbf.neighbor_query"},
             {"gather", "fluid.pbf.neighbor_query"},
+            {"pieces", "fluid.pbf.neighbor_query"},
             {"clear_cmax", "fluid.pbf.neighbor_query"},
             {"block_cost", "fluid.pbf.neighbor_query"},
             {"block_plan", "fluid.pbf.neighbor_query"},

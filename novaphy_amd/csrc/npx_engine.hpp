@@ -244,9 +244,9 @@ struct Ctx {
   DBuf<float> BCOST;         // per 256-slot block: estimated neighbour-pass cost
   DBuf<int> BORDER, BMODE;   // block order of the neighbour passes and its mode
   DBuf<int> CMAX;   // per class: float bits of the largest summed displacement; then per class a run flag
-  DBuf<uint8_t> RUNL;  // per slot: length - 1 of the run of identical state starting there
-  DBuf<uint8_t> RUNH;  // per slot: distance back to the head of its run
-  DBuf<int> BHEADS;    // per 256-slot block: run heads
+  DBuf<uint8_t> RSTART;  // per slot: 1 if the slot starts a run of identical state
+  DBuf<int> PI, PS;      // per slot: its run; per run: its first slot
+  DBuf<int> BOFF;        // per 256-slot block: run starts, then their exclusive prefix
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -523,9 +523,10 @@ struct Ctx {
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(EXN.ensure(n));
     NPX_HIP(CDB.ensure(n));
-    NPX_HIP(RUNL.ensure(n));
-    NPX_HIP(RUNH.ensure(n));
-    NPX_HIP(BHEADS.ensure(n / NPX_BLOCK + 4));
+    NPX_HIP(RSTART.ensure(n));
+    NPX_HIP(PI.ensure(n));
+    NPX_HIP(PS.ensure(n));
+    NPX_HIP(BOFF.ensure(n / NPX_BLOCK + 5));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -799,8 +800,11 @@ struct Ctx {
     NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls + max_scenes, 256), 256, CMAX.p, 0, 2 * ncls + max_scenes));
     NPX_TRY(launch("gather", k_gather, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)PERM.p, (const float4*)X.p, (const float4*)V.p, (const int*)ID.p, (const int*)CELL.p,
-                   (const int*)clsmap.p, X2.p, V2.p, XS0.p, ID2.p, CELL2.p, fast ? 0 : 1, RUNL.p, RUNH.p,
-                   (unsigned*)CMAX.p + ncls, CMAX.p + 2 * ncls, BHEADS.p));
+                   (const int*)clsmap.p, X2.p, V2.p, XS0.p, ID2.p, CELL2.p, fast ? 0 : 1, RSTART.p,
+                   (unsigned*)CMAX.p + ncls, CMAX.p + 2 * ncls, BOFF.p));
+    NPX_TRY(scan(BOFF.p, blocks + 1));
+    NPX_TRY(launch("pieces", k_pieces, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
+                   (const unsigned char*)RSTART.p, (const int*)BOFF.p, PI.p, PS.p));
     if (carry_rho) {
       NPX_TRY(launch("gather_rho", k_gather_scalar2, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)PERM.p, (const float*)RHO.p, (const float*)LAM.p, RHO2.p,
@@ -841,10 +845,10 @@ struct Ctx {
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
-    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RUNL.p, RUNH.p,
+    const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
                  (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls()};
     NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
-                   (const int*)cstart.p, (const int*)CELL.p, (const int*)BHEADS.p, hw_blocks, BCOST.p));
+                   (const int*)cstart.p, (const int*)CELL.p, (const int*)BOFF.p, hw_blocks, BCOST.p));
     NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p));
     const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
     const int* cm = clsmap.p;
@@ -869,8 +873,7 @@ struct Ctx {
                    (const float4*)VT.p, OM.p, VX.p, R, rm));
     NPX_TRY(launch("vort_commit", k_vort_commit, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                    (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p, V.p, R, rm));
-    NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, (const unsigned char*)RUNH.p, X.p, V.p,
-                   RHO.p, LAM.p));
+    NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, R, X.p, V.p, RHO.p, LAM.p));
     xs_final = cur;
     return NPX_OK;
   }
@@ -905,7 +908,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RUNL.p, RUNH.p, BHEADS.p, BCOST.p, BORDER.p, BMODE.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, BCOST.p, BORDER.p, BMODE.p};
   }
 
   void drop_graphs() {

@@ -336,35 +336,46 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
 // order, so particles with bitwise equal (x, v') in one cell (equal x puts
 // them in one cell, and the sort puts them in one class) get bitwise equal
 // values in every pass, and each contributes the same term to a neighbour's
-// sums.  runl[q] + 1 is the number of slots from q up to the next slot whose
-// (x, v') differs from its predecessor's, or to the end of q's 256-slot
-// block; runh[q] is the distance back to the first slot of q's run (its
-// head); bit class of runb and runs_s[scene] are set when the class (the
-// scene) holds a run longer than one, and bheads[block] counts the block's
-// heads.  Only heads compute in the
+// sums.  A slot starts a run (rstart[q] = 1; the run's head) unless it holds
+// the same (x, v') as the slot before it in the same scene; bcount[block]
+// counts the block's run starts, the bit of a class in runb and
+// runs_s[scene] are set when the class (the scene) holds a run longer than
+// one.  k_pieces turns these into run lists.  Only heads compute in the
 // neighbour passes: every read of a candidate's data goes to its run's head
 // (the inline sweeps step run by run and add a run's term once, k times over,
 // with npx_rep_add; list entries store heads), and k_run_fill copies the
 // heads' final state to the other members.  Scenes with no solver iteration
 // carry density from the previous step into the velocity update, so they get
 // no runs, and runs_on = 0 (the FMA kernels) turns runs off.
+__device__ __forceinline__ bool same_state(const float4& a, const float4& b, const float4& x, const float4& w) {
+  return __float_as_uint(a.x) == __float_as_uint(x.x) && __float_as_uint(a.y) == __float_as_uint(x.y) &&
+         __float_as_uint(a.z) == __float_as_uint(x.z) && __float_as_uint(b.x) == __float_as_uint(w.x) &&
+         __float_as_uint(b.y) == __float_as_uint(w.y) && __float_as_uint(b.z) == __float_as_uint(w.z);
+}
+
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ perm,
          const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ id,
          const int* __restrict__ cell_of, const int* __restrict__ clsmap, float4* __restrict__ X2,
          float4* __restrict__ V2, float4* __restrict__ XS, int* __restrict__ id2, int* __restrict__ cell2,
-         int runs_on, unsigned char* __restrict__ runl, unsigned char* __restrict__ runh, unsigned* __restrict__ runb,
-         int* __restrict__ runs_s, int* __restrict__ bheads) {
+         int runs_on, unsigned char* __restrict__ rstart, unsigned* __restrict__ runb, int* __restrict__ runs_s,
+         int* __restrict__ bcount) {
   __shared__ float4 sx[NPX_BLOCK], sv[NPX_BLOCK];
   __shared__ unsigned long long smask[NPX_BLOCK / 64];
-  const int s = blk_scene[blockIdx.x];
-  if (s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
+  __shared__ int scount;
   const int t = (int)threadIdx.x;
+  if (blockIdx.x == 0 && t == 0) bcount[gridDim.x] = 0;  // the scan's last entry
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) {  // uniform per block
+    if (t == 0) bcount[blockIdx.x] = 0;
+    return;
+  }
+  const SceneC& S = scenes[s];
   const int q = npx_slot();
   const bool active = q < S.seg_off + S.n;
   float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w = x;
   int c = 0;
+  if (t == 0) scount = 0;
   if (active) {
     const int p = perm[q];
     x = X[p];
@@ -381,54 +392,64 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
   sx[t] = x;
   sv[t] = w;
   __syncthreads();
-  bool start = true;  // active slots form a prefix of the block, so t - 1 is active
-  if (active && t > 0 && S.iters > 0 && runs_on) {
-    const float4 a = sx[t - 1], b = sv[t - 1];
-    start = !(__float_as_uint(a.x) == __float_as_uint(x.x) && __float_as_uint(a.y) == __float_as_uint(x.y) &&
-              __float_as_uint(a.z) == __float_as_uint(x.z) && __float_as_uint(b.x) == __float_as_uint(w.x) &&
-              __float_as_uint(b.y) == __float_as_uint(w.y) && __float_as_uint(b.z) == __float_as_uint(w.z));
+  bool start = active;  // inactive slots (a suffix of the block) are not counted
+  if (active && S.iters > 0 && runs_on && q > S.seg_off) {
+    if (t > 0) {  // active slots form a prefix of the block, so t - 1 is active
+      start = !same_state(sx[t - 1], sv[t - 1], x, w);
+    } else {  // the last slot of the previous block (same scene, active)
+      const int p = perm[q - 1];
+      const float4 v = V[p];
+      start = !same_state(X[p], make_float4(v.x + S.gdt0, v.y + S.gdt1, v.z + S.gdt2, 0.0f), x, w);
+    }
   }
   const unsigned long long m = __ballot(start);
+  if ((t & 63) == 0) {
+    smask[t >> 6] = m;
+    atomicAdd(&scount, (int)__popcll(m));
+  }
+  __syncthreads();
+  if (active) {
+    rstart[q] = start ? 1 : 0;
+    // a run longer than one: flagged by its second slot (or by the block's
+    // first slot when the run comes from the previous block)
+    const int lane = t & 63;
+    const bool prev_start = (t == 0) || ((lane > 0) ? ((m >> (lane - 1)) & 1ull) : (smask[(t >> 6) - 1] >> 63));
+    if (!start && prev_start) {
+      const unsigned cl = (unsigned)clsmap[S.cls_base + c];
+      atomicOr(runb + (cl >> 5), 1u << (cl & 31u));
+      runs_s[s] = 1;
+    }
+  }
+  if (t == 0) bcount[blockIdx.x] = scount;
+}
+
+// Run lists, after the exclusive scan of bcount into boff: piece i (run i in
+// slot order) starts at slot ps[i], and pi[q] is the piece of slot q's run.
+// The runs of a slot range [b, e) are pieces pi[b] .. pi[e - 1] (b is always
+// a run start: a class start differs from the slot before it in cell), the
+// last ending at e.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_pieces(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
+         const unsigned char* __restrict__ rstart, const int* __restrict__ boff, int* __restrict__ pi,
+         int* __restrict__ ps) {
+  __shared__ unsigned long long smask[NPX_BLOCK / 64];
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int t = (int)threadIdx.x;
+  const int q = npx_slot();
+  const bool active = q < S.seg_off + S.n;
+  const bool st = active && rstart[q];
+  const unsigned long long m = __ballot(st);
   if ((t & 63) == 0) smask[t >> 6] = m;
   __syncthreads();
-  if (t == 0) {
-    int nh = 0;
-    for (int u = 0; u < NPX_BLOCK / 64; ++u) nh += (int)__popcll(smask[u]);
-    const int na = max(0, min(NPX_BLOCK, S.seg_off + S.n - (int)blockIdx.x * NPX_BLOCK));
-    bheads[blockIdx.x] = nh - (NPX_BLOCK - na);  // inactive slots count as starts
-  }
   if (!active) return;
   const int lane = t & 63;
-  int head = 0;  // slot 0 of the block always starts a run
-  const unsigned long long below = (lane == 63) ? m : (m & ((2ull << lane) - 1ull));
-  if (below) {
-    head = (t & ~63) + 63 - (int)__builtin_clzll(below);
-  } else {
-    for (int u = (t >> 6) - 1; u >= 0; --u)
-      if (smask[u]) {
-        head = u * 64 + 63 - (int)__builtin_clzll(smask[u]);
-        break;
-      }
-  }
-  runh[q] = (unsigned char)(t - head);
-  int next = NPX_BLOCK;
-  const unsigned long long above = (lane == 63) ? 0ull : (m >> (lane + 1)) << (lane + 1);
-  if (above) {
-    next = (t & ~63) + (int)__builtin_ctzll(above);
-  } else {
-    for (int u = (t >> 6) + 1; u < NPX_BLOCK / 64; ++u)
-      if (smask[u]) {
-        next = u * 64 + (int)__builtin_ctzll(smask[u]);
-        break;
-      }
-  }
-  const int len = next - t;  // 1 .. NPX_BLOCK
-  runl[q] = (unsigned char)(len - 1);
-  if (start && len > 1) {
-    const unsigned cl = (unsigned)clsmap[S.cls_base + c];
-    atomicOr(runb + (cl >> 5), 1u << (cl & 31u));
-    runs_s[s] = 1;
-  }
+  int before = (int)__popcll((lane == 63) ? m : (m & ((2ull << lane) - 1ull)));  // starts at or before q
+  for (int u = 0; u < (t >> 6); ++u) before += (int)__popcll(smask[u]);
+  const int i = boff[blockIdx.x] + before - 1;  // members before the block's first start: the previous piece
+  pi[q] = i;
+  if (st) ps[i] = q;
 }
 
 // Carries density and lambda into the new slot order (only needed when a
@@ -493,7 +514,7 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
 // heads (the particles that compute).
 __global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
                              const int* __restrict__ clsmap, const int* __restrict__ cstart,
-                             const int* __restrict__ cell, const int* __restrict__ bheads, int nb,
+                             const int* __restrict__ cell, const int* __restrict__ boff, int nb,
                              float* __restrict__ cost) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -515,7 +536,7 @@ __global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __res
           cand += cstart[k + 1] - cstart[k];
         }
       }
-      c = (float)cand * (float)max(bheads[b], 1);  // run members do no work
+      c = (float)cand * (float)max(boff[b + 1] - boff[b], 1);  // run members do no work
     }
   }
   cost[b] = c;
@@ -611,8 +632,9 @@ struct Skin {
   int* __restrict__ ecnt;    // [slots]; -1: the testing pass swept inline
   float* __restrict__ cd;      // [slots] summed displacement since the lists were built
   unsigned* __restrict__ cmax; // [classes] float bits of the largest cd in each class
-  const unsigned char* __restrict__ runl;  // [slots] run length - 1 (k_gather)
-  const unsigned char* __restrict__ runh;  // [slots] distance back to the run's head; 0: a head
+  const unsigned char* __restrict__ rstart;  // [slots] 1: the slot starts a run (a head)
+  const int* __restrict__ pi;               // [slots] piece (run) of the slot
+  const int* __restrict__ ps;               // [pieces] first slot of each run
   const unsigned* __restrict__ runb;       // [classes / 32] bit set: the class holds a run longer than one
   const int* __restrict__ runs_s;          // [scenes] 1: the scene holds a run longer than one
 };
@@ -741,7 +763,7 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
     const int t = (int)__builtin_ctzll(mk);
     mk &= mk - 1;
     if (k < NPX_SKIN_CAP) {
-      K.idx[sk_at(q, k)] = (rf & 1u) ? b + t - K.runh[b + t] : b + t;  // the run's head holds the data
+      K.idx[sk_at(q, k)] = (rf & 1u) ? K.ps[K.pi[b + t]] : b + t;  // the run's head holds the data
       rel |= ((ek >> t) & 1ull) << k;
     }
     ++k;
@@ -774,22 +796,30 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, bool rs, int q, float
     for (int m = 0; m < (one ? 1 : 3) && k <= NPX_SKIN_CAP; ++m) {
       const int c = (m == 0) ? c0 : (m == 1) ? c1 : c2;
       const int b = cstart[c] - shift, e = cstart[one ? c0 + 3 : c + 1] - shift;
-      if (RUNS && rs && (one ? cls_runs(K, c0, 3) : cls_runs(K, c, 1))) {
-        for (int j = b; j < e && k <= NPX_SKIN_CAP;) {
-          const int kk = min((int)K.runl[j] + 1, e - j);
-          const float4 a = XS[j];
-          const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
-          if (d < skin_cut) {
-            if (k + kk > NPX_SKIN_CAP) {
-              k = NPX_SKIN_CAP + 1;  // overflow
-              break;
-            }
-            for (int u = 0; u < kk; ++u, ++k) {
-              K.idx[sk_at(q, k)] = j;  // j heads the run (a class start, a block start or a run start)
-              rel |= (unsigned long long)(d < cut) << k;
+      if (RUNS && rs && b < e && (one ? cls_runs(K, c0, 3) : cls_runs(K, c, 1))) {
+        const int iend = K.pi[e - 1] + 1;
+        for (int i = K.pi[b]; i < iend && k <= NPX_SKIN_CAP; i += 2) {  // two runs per fetch
+          const int j0 = K.ps[i];
+          const int j1 = (i + 1 < iend) ? K.ps[i + 1] : e;
+          const int j2 = (i + 2 < iend) ? K.ps[i + 2] : e;
+          const float4 a0 = XS[j0], a1 = XS[(i + 1 < iend) ? j1 : j0];
+          const float d0 = dist2<FAST>(xi.x - a0.x, xi.y - a0.y, xi.z - a0.z);
+          const float d1 = dist2<FAST>(xi.x - a1.x, xi.y - a1.y, xi.z - a1.z);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int j = h ? j1 : j0, kk = (h ? j2 : j1) - j;
+            const float d = h ? d1 : d0;
+            if ((h == 0 || i + 1 < iend) && k <= NPX_SKIN_CAP && d < skin_cut) {
+              if (k + kk > NPX_SKIN_CAP) {
+                k = NPX_SKIN_CAP + 1;  // overflow
+              } else {
+                for (int u = 0; u < kk; ++u, ++k) {
+                  K.idx[sk_at(q, k)] = j;  // the run's head
+                  rel |= (unsigned long long)(d < cut) << k;
+                }
+              }
             }
           }
-          j += kk;
         }
       } else {
         for (int j = b; j < e && k <= NPX_SKIN_CAP; ++j) {
@@ -901,21 +931,33 @@ __device__ __forceinline__ void visit_range(int b, int e, L&& ld, F&& f) {
 }
 
 // visit_range over a slot range holding runs of identical state: one call
-// per run, f(j, ld(j), k) standing for the k equal terms of slots j..j+k-1.
-template <class L, class F>
-__device__ __forceinline__ void visit_runs(int b, int e, const unsigned char* __restrict__ runl, L&& ld, F&& f) {
-  for (int j = b; j < e;) {
-    const int k = min((int)runl[j] + 1, e - j);
-    f(j, ld(j), k);
-    j += k;
+// per run, f(j, ld(j), k) standing for the k equal terms of slots j..j+k-1,
+// with the starts and data of G runs fetched before any is evaluated.
+template <int G, class L, class F>
+__device__ __forceinline__ void visit_runs(const Skin& K, int b, int e, L&& ld, F&& f) {
+  if (b >= e) return;
+  const int iend = K.pi[e - 1] + 1;
+  for (int i = K.pi[b]; i < iend; i += G) {
+    int js[G + 1];
+#pragma unroll
+    for (int u = 0; u <= G; ++u) js[u] = (i + u < iend) ? K.ps[i + u] : e;
+    decltype(ld(0)) v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = ld((i + u < iend) ? js[u] : js[0]);
+#pragma unroll
+    for (int u = 0; u < G; ++u) keep(v[u]);
+#pragma unroll
+    for (int u = 0; u < G; ++u)
+      if (i + u < iend) f(js[u], v[u], js[u + 1] - js[u]);
   }
 }
 
 // f(j, ld(j), k) over every candidate in the reference's order (the 27-cell
 // sweep of for_each_candidate); f applies the cut itself.  With RUNS, classes
-// flagged by k_gather are walked run by run (k = run length), so a stack of
+// flagged by k_gather are walked run by run (k = run length, RG runs fetched
+// at a time), so a stack of
 // m coincident particles costs one term per run instead of m.
-template <int G, bool RUNS, class L, class F>
+template <int G, bool RUNS, int RG = 2, class L, class F>
 __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx, int wxy, const int* __restrict__ cm,
                                           const int* __restrict__ cstart, int shift, L&& ld, F&& f) {
 #pragma unroll 1
@@ -927,7 +969,7 @@ __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx,
     if (c1 == c0 + 1 && c2 == c0 + 2) {
       const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
       if (RUNS && rs && cls_runs(K, c0, 3))
-        visit_runs(b, e, K.runl, ld, f);
+        visit_runs<RG>(K, b, e, ld, f);
       else
         visit_range<G>(b, e, ld, f);
     } else {
@@ -936,7 +978,7 @@ __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx,
         const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
         const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
         if (RUNS && rs && cls_runs(K, c, 1))
-          visit_runs(b, e, K.runl, ld, f);
+          visit_runs<RG>(K, b, e, ld, f);
         else
           visit_range<G>(b, e, ld, f);
       }
@@ -968,7 +1010,7 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool 
 
 // Reusing pass (delta-p, vorticity force): same positions as the testing
 // pass before it, so its in-range list applies.
-template <int G, bool RUNS, class L, class F>
+template <int G, bool RUNS, int RG = 2, class L, class F>
 __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool rs, int q, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             L&& ld, F&& f) {
@@ -976,7 +1018,7 @@ __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool
   if (e >= 0)
     visit_list<G>(K.ex, q, e, ld, f);
   else
-    visit_all<G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all<G, RUNS, RG>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
 }
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
@@ -991,7 +1033,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
                  float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
-  if (iter >= S.iters || !active || (rs && K.runh[q])) return;  // run members take their head's values
+  if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
   const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -1088,7 +1130,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
   }
   float moved = 0.0f;
   const bool rs = K.runs_s[s] != 0;
-  if (active && !(rs && K.runh[q])) {  // run members take their head's values
+  if (active && !(rs && !K.rstart[q])) {  // run members take their head's values
     const float4 xi = XS[q];
     const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
@@ -1126,7 +1168,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
         add_k(dz, coef * (co * rz), k);
       }
     };
-    visit_reuse<4, !FAST>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);
+    visit_reuse<4, !FAST, 1>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);  // one run at a time: 64 VGPRs
     const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
     XSn[q] = make_float4(px, py, pz, 0.0f);
     if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
@@ -1170,7 +1212,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
              const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
-  if (!active || (rs && K.runh[q])) return;  // run members take their head's values
+  if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float4 vi = VT[q];
   const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
@@ -1240,7 +1282,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
               float4* __restrict__ X, float4* __restrict__ V, Skin K, BlockPlan bp) {
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
-  if (!active || (rs && K.runh[q])) return;  // run members take their head's values
+  if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
   const float4 vx4 = VX[q];
   float vx = vx4.x, vy = vx4.y, vz = vx4.z;
@@ -1287,16 +1329,14 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
 // End of a step: every run member takes its head's committed state, density
 // and lambda (the values the member would have computed, bit for bit).
 __global__ void __launch_bounds__(NPX_BLOCK)
-k_run_fill(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const unsigned char* __restrict__ runh,
+k_run_fill(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const Skin K,
            float4* __restrict__ X, float4* __restrict__ V, float* __restrict__ RHO, float* __restrict__ LAM) {
   const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
   const int q = npx_slot();
-  if (q >= S.seg_off + S.n) return;
-  const int d = runh[q];
-  if (d == 0) return;
-  const int h = q - d;
+  if (q >= S.seg_off + S.n || !K.runs_s[s] || K.rstart[q]) return;
+  const int h = K.ps[K.pi[q]];
   X[q] = X[h];
   V[q] = V[h];
   RHO[q] = RHO[h];
