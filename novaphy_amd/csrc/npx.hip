@@ -68,7 +68,7 @@ int upload_poses(Ctx* c, const float* pose7, int nbodies) {
   if (nbodies != c->nbodies) return c->fail(NPX_EINVAL, "expected %d body poses, got %d", c->nbodies, nbodies);
   if (nbodies > 0) {
     if (!pose7) return c->fail(NPX_EINVAL, "pose7 is NULL");
-    NPX_TRY(c->copy_sync(c->POSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice));
+    NPX_TRY(c->upload_bodies(c->POSE.p, pose7, nullptr, nullptr, nbodies));
   }
   return NPX_OK;
 }
@@ -150,9 +150,8 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
   std::vector<float> out(6 * nbodies, 0.0f);
   if (c->nb > 0 && H.used && H.n > 0) {
     if (!H.window_valid) return c->fail(NPX_ESTATE, "no step has run yet");
-    NPX_TRY(upload_poses(c, pose7, nbodies));
-    if (!is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
-    NPX_TRY(c->copy_sync(c->STATIC.p, is_static, nbodies, hipMemcpyHostToDevice));
+    if (!pose7 || !is_static) return c->fail(NPX_EINVAL, "pose7 / is_static is NULL");
+    NPX_TRY(c->upload_bodies(c->POSE.p, pose7, c->STATIC.p, is_static, nbodies));
     NPX_HIP_C(c, c->FP.ensure(c->slot_cap));
     NPX_HIP_C(c, c->FS.ensure(c->slot_cap));
     NPX_TRY(c->launch("fluid_payload", k_fluid_payload, c->hw_blocks, NPX_BLOCK, (const SceneC*)c->d_scenes.p,
@@ -169,9 +168,7 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
     NPX_TRY(c->launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)c->BODY_PTR.p,
                       (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
                       (const float*)c->POSE.p, c->OUT6.p));
-    NPX_HIP_C(c, hipMemcpyAsync(out.data(), c->OUT6.p, sizeof(float) * 6 * nbodies, hipMemcpyDeviceToHost,
-                                c->stream));
-    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    NPX_TRY(c->read_bodies(out.data(), c->OUT6.p, nbodies));
     NPX_TRY(c->check_flags());
   }
   for (int k = 0; k < nbodies; ++k) {
@@ -312,6 +309,10 @@ void destroy(Ctx* c) {
   c->d_scenes.release();
   c->STATIC.release();
   if (c->h_flags) (void)hipHostFree(c->h_flags);
+  if (c->h_pose) (void)hipHostFree(c->h_pose);
+  if (c->h_static) (void)hipHostFree(c->h_static);
+  if (c->h_out) (void)hipHostFree(c->h_out);
+  if (c->ev_up) (void)hipEventDestroy(c->ev_up);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -257,6 +257,17 @@ struct Ctx {
   int* h_flags = nullptr;  // pinned
   int xs_final = 0;
 
+  // Pinned staging of the per-step body transfers (the NB x 7 poses and
+  // is_static flags in, the NB x 6 forces and torques out), so uploads are
+  // asynchronous DMA on the stream and the host blocks once per coupling
+  // call, at the readback.  ev_up marks the last upload from the staging
+  // buffers; it is waited on before the host overwrites them.
+  float* h_pose = nullptr;
+  uint8_t* h_static = nullptr;
+  float* h_out = nullptr;
+  int h_body_cap = 0;
+  hipEvent_t ev_up = nullptr;
+
   // boundary (single-scene contexts)
   int nb = 0, nbodies = 0;
   DBuf<float4> BL, BP, BPS, FP, FS, FB;
@@ -383,7 +394,7 @@ struct Ctx {
     if (nbodies > 0 && (!pose7 || !out6)) return fail(NPX_EINVAL, "pose7 / out6 is NULL");
     for (const auto& H : scenes)
       if (H.used && H.nbnd > 0 && !H.window_valid) return fail(NPX_ESTATE, "no step has run yet");
-    if (nbodies > 0) NPX_TRY(copy_sync(BPOSE.p, pose7, sizeof(float) * 7 * nbodies, hipMemcpyHostToDevice));
+    NPX_TRY(upload_bodies(BPOSE.p, pose7, nullptr, nullptr, nbodies));
     NPX_TRY(launch("bpos", k_bpos, cdiv(bb_n, 256), 256, (const float4*)BBL.p, (const int*)BBODY.p,
                    (const float*)BPSI.p, bb_n, (const float*)BPOSE.p, nbodies, BBP.p));
     NPX_TRY(sort_scene_points(bsort, BBP.p, BSCN.p, bb_n, nullptr, BBPS.p));
@@ -404,7 +415,7 @@ struct Ctx {
     if (nbodies > 0) {
       NPX_TRY(launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
                      nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p, BOUT6.p));
-      NPX_TRY(copy_sync(out6, BOUT6.p, sizeof(float) * 6 * nbodies, hipMemcpyDeviceToHost));
+      NPX_TRY(read_bodies(out6, BOUT6.p, nbodies));
     }
     return NPX_OK;  // a scene whose particles left its window is reported by npx_scene_status
   }
@@ -465,6 +476,52 @@ struct Ctx {
     if (bytes == 0) return NPX_OK;
     NPX_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream));
     NPX_HIP(hipStreamSynchronize(stream));
+    return NPX_OK;
+  }
+
+  int stage_bodies(int nbodies) {
+    if (nbodies <= h_body_cap) return NPX_OK;
+    NPX_HIP(hipStreamSynchronize(stream));
+    if (h_pose) NPX_HIP(hipHostFree(h_pose));
+    if (h_static) NPX_HIP(hipHostFree(h_static));
+    if (h_out) NPX_HIP(hipHostFree(h_out));
+    h_pose = nullptr;
+    h_static = nullptr;
+    h_out = nullptr;
+    h_body_cap = 0;
+    const int cap = std::max(nbodies, 64);
+    NPX_HIP(hipHostMalloc((void**)&h_pose, sizeof(float) * 7 * cap, hipHostMallocDefault));
+    NPX_HIP(hipHostMalloc((void**)&h_static, (size_t)cap, hipHostMallocDefault));
+    NPX_HIP(hipHostMalloc((void**)&h_out, sizeof(float) * 6 * cap, hipHostMallocDefault));
+    if (!ev_up) NPX_HIP(hipEventCreateWithFlags(&ev_up, hipEventDisableTiming));
+    h_body_cap = cap;
+    return NPX_OK;
+  }
+
+  // Asynchronous upload of n bodies' poses (and is_static flags when given)
+  // through the pinned staging buffers.
+  int upload_bodies(float* d_pose, const float* pose7, uint8_t* d_static, const uint8_t* is_static, int n) {
+    if (n <= 0) return NPX_OK;
+    NPX_TRY(stage_bodies(n));
+    NPX_HIP(hipEventSynchronize(ev_up));  // the previous upload has left the staging buffers
+    std::memcpy(h_pose, pose7, sizeof(float) * 7 * n);
+    NPX_HIP(hipMemcpyAsync(d_pose, h_pose, sizeof(float) * 7 * n, hipMemcpyHostToDevice, stream));
+    if (is_static) {
+      std::memcpy(h_static, is_static, (size_t)n);
+      NPX_HIP(hipMemcpyAsync(d_static, h_static, (size_t)n, hipMemcpyHostToDevice, stream));
+    }
+    NPX_HIP(hipEventRecord(ev_up, stream));
+    return NPX_OK;
+  }
+
+  // Blocking readback of n bodies' force + torque (6 floats each) through the
+  // pinned staging buffer.
+  int read_bodies(float* out6, const float* d_out6, int n) {
+    if (n <= 0) return NPX_OK;
+    NPX_TRY(stage_bodies(n));
+    NPX_HIP(hipMemcpyAsync(h_out, d_out6, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, stream));
+    NPX_HIP(hipStreamSynchronize(stream));
+    std::memcpy(out6, h_out, sizeof(float) * 6 * n);
     return NPX_OK;
   }
 
