@@ -173,6 +173,9 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     out = {"value": value, "ms_per_step": wall * 1e3 / steps, "particles": total, "scenes": count,
            "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99)),
            "tuning": {"block": best[0], "xcd_order": best[1], "skin": best[2]}}
+    if loop_steps <= 0:
+        out.update(loop_value=None, loop_ms_per_step=None, speedup_vs_loop=None)
+        return out
     # plain per-scene loop: one single-scene context per scene, stepped in turn
     log(f"[bench] per-scene loop: {count} contexts, {loop_steps} steps")
     ctxs = []
@@ -347,7 +350,8 @@ def main():
     ap.add_argument("--serve-backend", default="fused", choices=["fused", "mixed"],
                     help="coupling of requests with spheres: fused into the batch, or separate FluidWorlds")
     ap.add_argument("--scenes", type=int, default=1024, help="scene count of the batch workload")
-    ap.add_argument("--loop-steps", type=int, default=5, help="timed steps of the per-scene loop baseline")
+    ap.add_argument("--loop-steps", type=int, default=5,
+                    help="timed steps of the per-scene loop baseline (0: skip it)")
     ap.add_argument("--fast-math", action="store_true",
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
     ap.add_argument("--corr-k", type=float, default=None,
@@ -381,10 +385,11 @@ def main():
             "data": "synthetic",
             "config": {"workload": f"batched_{args.scenes}: {args.scenes} dam breaks, N log-uniform "
                                    f"2k-64k ({r['particles']} particles), one launch per pass over all "
-                                   "scenes, hipGraph-replayed", "particles": r["particles"]},
+                                   "scenes (stream launches at this size; steps of at most 512k slots are "
+                                   "hipGraph-replayed)", "particles": r["particles"]},
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"]},
-            "speedup_vs_per_scene_loop": round(r["speedup_vs_loop"], 2),
+            "speedup_vs_per_scene_loop": None if r["speedup_vs_loop"] is None else round(r["speedup_vs_loop"], 2),
             "tuning": r["tuning"],
         }
         print(json.dumps(line), flush=True)
