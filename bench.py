@@ -269,18 +269,20 @@ def run_coupled(steps, warmup):
 def run_serve(seconds, rate_hz, threads, backend="fused"):
     """Served rollouts: Poisson arrivals (seed 4) of dam breaks with N
     log-uniform in [4k, 256k], horizons uniform in [50, 500] and 0-8 rigid
-    spheres, admitted against 80 % of 288 GB; fluid-only requests share one
-    batch context, requests with spheres run as FluidWorld instances stepped
-    concurrently.  The sample is the requests arriving in the first
-    `seconds` of server time; the run ends when they have all finished or at
-    3x that time."""
+    spheres, admitted against 80 % of 288 GB.  fused: every request shares one
+    batch context, with the coupling of all requests with spheres in one
+    launch per pass and their boundary particles sampled on the device; mixed:
+    requests with spheres run as FluidWorld instances stepped concurrently.
+    The sample is the requests arriving in the first `seconds` of server time;
+    the run ends when they have all finished or at 3x that time."""
     from novaphy_amd import capi, scenes, serve
     n_req = max(1, int(round(seconds * rate_hz)))
     reqs = serve.poisson_requests(n_req, rate_hz, seed=4, make_scene=lambda k, rng: scenes.served_request(k, rng))
     log(f"[bench] serve: {n_req} requests over {seconds:.0f} s, N {min(r.scene.n for r in reqs)}-"
         f"{max(r.scene.n for r in reqs)}, {sum(bool(r.scene.spheres) for r in reqs)} with spheres")
     b = capi.Batch(slot_capacity=1 << 24, cell_capacity=1 << 26, max_scenes=256)
-    be = serve.FusedBackend(b, 256) if backend == "fused" else serve.MixedBackend(b, 256, threads=threads)
+    be = (serve.FusedBackend(b, 256, device_sampling=True) if backend == "fused"
+          else serve.MixedBackend(b, 256, threads=threads))
     srv = serve.Server(be, hbm_budget_bytes=int(0.8 * 288e9), max_scenes=256, bytes_fn=capi.scene_bytes)
     for r in reqs:
         srv.submit(r)
@@ -403,7 +405,8 @@ def main():
             "data": "synthetic",
             "config": {"workload": f"served: {r['requests']} Poisson requests at {args.serve_rate}/s (seed 4), "
                                    "N log-uniform 4k-256k, horizon 50-500, 0-8 spheres with Akinci coupling, "
-                                   f"admission against 80% of 288 GB, {args.serve_backend} coupling"},
+                                   f"admission against 80% of 288 GB, {args.serve_backend} coupling"
+                                   + (", boundaries sampled on the device" if args.serve_backend == "fused" else "")},
             "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "serve": {k: r[k] for k in ("requests", "done", "unfinished", "failed", "wall_s", "latency_samples",
                                         "mean_active")},

@@ -118,6 +118,34 @@ int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* q
 
 /* ------------------------------------------------------ Akinci boundary */
 
+/* Boundary shapes for npx_sample_boundaries: the fields of CollisionShape
+   (include/novaphy/core/shape.h) that the samplers read. */
+#define NPX_SHAPE_BOX 0
+#define NPX_SHAPE_SPHERE 1
+#define NPX_SHAPE_PLANE 2
+typedef struct npx_shape {
+  int32_t type;            /* NPX_SHAPE_*                                     */
+  int32_t body;            /* owning body index, -1 for world-owned shapes    */
+  float half_extents[3];   /* box                                             */
+  float radius;            /* sphere                                          */
+  float normal[3];         /* plane (unit)                                    */
+  float offset;            /* plane                                           */
+  float position[3];       /* shape local transform: translation              */
+  float rotation[4];       /* shape local transform: unit quaternion w, x, y, z */
+} npx_shape;
+
+/* Replaces sample_model_boundaries (src/fluid/boundary.cpp:128-150) and the
+   box / sphere / plane samplers it calls (:15-127): the boundary particles of
+   every shape, shape by shape in the reference's loop order, in body-local
+   coordinates (world coordinates for planes), computed on the device.
+   *count receives the particle count; nothing is written when it exceeds cap
+   (call with cap = 0 to size local3 (3 floats each) and body).  Box and
+   plane samples equal the reference's float operations bit for bit; sphere
+   samples evaluate acos / sin / cos in double and round to float, which can
+   differ from a host libm's float functions in the last bit. */
+int npx_sample_boundaries(const npx_shape* shapes, int32_t nshapes, float spacing, float plane_extent,
+                          float* local3, int32_t* body, int64_t cap, int64_t* count);
+
 /* Uploads boundary particles (BoundaryParticle, boundary.h:21-25): body-local
  * positions, owning body (-1 = world) and psi volumes (may be NULL = 0). */
 int npx_set_boundary(npx_ctx* ctx, const float* local3, const int32_t* body, const float* psi,

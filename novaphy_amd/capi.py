@@ -38,7 +38,18 @@ SYMBOLS = (
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
     "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
     "npx_set_tuning", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
+    "npx_sample_boundaries",
 )
+
+
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_PLANE = 0, 1, 2
+
+
+class NpxShape(C.Structure):
+    """npx_shape: the CollisionShape fields the boundary samplers read."""
+    _fields_ = [("type", C.c_int32), ("body", C.c_int32), ("half_extents", C.c_float * 3), ("radius", C.c_float),
+                ("normal", C.c_float * 3), ("offset", C.c_float), ("position", C.c_float * 3),
+                ("rotation", C.c_float * 4)]
 
 
 class NpxError(RuntimeError):
@@ -131,6 +142,7 @@ def load() -> C.CDLL:
         "npx_scene_set_boundary": (C.c_int, [vp, i32, vp, vp, vp, i32, i32, vp]),
         "npx_batch_num_bodies": (i32, [vp]),
         "npx_batch_coupling": (C.c_int, [vp, vp, i32, vp]),
+        "npx_sample_boundaries": (C.c_int, [vp, i32, f32, f32, vp, vp, i64, P(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -319,6 +331,42 @@ def grid_query(points, cell_size: float, queries):
     _check(L.npx_grid_query(_p(pts), pts.shape[0], C.c_float(cell_size), _p(q), q.shape[0], _p(counts),
                             _p(out), tot.value, C.byref(tot)))
     return counts, out[:tot.value]
+
+
+def shapes_from_model(model):
+    """npx_shape records of a Model's collision shapes (ModelBuilder output):
+    type, body, box half extents, sphere radius, plane normal / offset and
+    the shape's local transform."""
+    kind = {"Box": SHAPE_BOX, "Sphere": SHAPE_SPHERE, "Plane": SHAPE_PLANE}
+    out = (NpxShape * max(len(model.shapes), 1))()
+    for k, sh in enumerate(model.shapes):
+        r = out[k]
+        r.type = kind[str(sh.type).split(".")[-1]]
+        r.body = int(sh.body_index)
+        r.half_extents[:] = [float(v) for v in np.asarray(sh.half_extents, np.float32)]
+        r.radius = float(sh.radius)
+        r.normal[:] = [float(v) for v in np.asarray(sh.normal, np.float32)]
+        r.offset = float(sh.offset)
+        t = sh.local_transform
+        r.position[:] = [float(v) for v in np.asarray(t.position, np.float32)]
+        x, y, z, w = np.asarray(t.rotation, np.float32)  # xyzw
+        r.rotation[:] = [float(w), float(x), float(y), float(z)]
+    return out, len(model.shapes)
+
+
+def sample_boundaries(shapes, nshapes: int, spacing: float, plane_extent: float):
+    """Boundary particles of the shapes on the device (npx_sample_boundaries):
+    returns (local positions (n, 3) float32, owning body (n,) int32)."""
+    L = load()
+    cnt = C.c_int64()
+    _check(L.npx_sample_boundaries(shapes, int(nshapes), C.c_float(spacing), C.c_float(plane_extent), None, None, 0,
+                                   C.byref(cnt)))
+    n = cnt.value
+    local = np.zeros((max(n, 1), 3), np.float32)
+    body = np.zeros(max(n, 1), np.int32)
+    _check(L.npx_sample_boundaries(shapes, int(nshapes), C.c_float(spacing), C.c_float(plane_extent), _p(local),
+                                   _p(body), n, C.byref(cnt)))
+    return local[:n], body[:n]
 
 
 class Batch:

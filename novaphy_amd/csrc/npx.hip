@@ -469,6 +469,136 @@ int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* q
   return r;
 }
 
+namespace {
+// Per-shape loop bounds and constants with the samplers' float expressions
+// (boundary.cpp:15-127).
+int prepare_shapes(const npx_shape* shapes, int nshapes, float spacing, float extent, std::vector<ShapeS>& out,
+                   long long* total) {
+  out.assign((size_t)nshapes, ShapeS{});
+  long long t = 0;
+  for (int k = 0; k < nshapes; ++k) {
+    const npx_shape& a = shapes[k];
+    ShapeS& S = out[(size_t)k];
+    S.type = a.type;
+    S.body = a.body;
+    S.first = (int)t;
+    for (int d = 0; d < 3; ++d) {
+      S.pos[d] = a.position[d];
+      S.he[d] = a.half_extents[d];
+      S.nrm[d] = a.normal[d];
+    }
+    for (int d = 0; d < 4; ++d) S.rot[d] = a.rotation[d];
+    long long cnt = 0;
+    if (a.type == NPX_SHAPE_BOX) {
+      for (int axis = 0; axis < 3; ++axis) {
+        const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+        S.n1[axis] = std::max(1, static_cast<int>(std::floor(2.0f * a.half_extents[a1] / spacing)) + 1);
+        S.n2[axis] = std::max(1, static_cast<int>(std::floor(2.0f * a.half_extents[a2] / spacing)) + 1);
+        S.axis_first[axis] = (int)cnt;
+        cnt += 2ll * S.n1[axis] * S.n2[axis];
+      }
+    } else if (a.type == NPX_SHAPE_SPHERE) {
+      const float r = a.radius;
+      const float pi = 3.14159265358979323846f;
+      const float area = 4.0f * pi * r * r;
+      S.n = std::max(12, static_cast<int>(area / (spacing * spacing)));
+      const float golden_ratio = (1.0f + std::sqrt(5.0f)) / 2.0f;
+      S.golden_angle = 2.0f * pi / (golden_ratio * golden_ratio);
+      S.r = r;
+      cnt = S.n;
+    } else if (a.type == NPX_SHAPE_PLANE) {
+      const float* n = a.normal;
+      float t1[3];
+      if (std::fabs(n[1]) < 0.9f) {  // (0, 1, 0) x n
+        t1[0] = 1.0f * n[2] - 0.0f * n[1];
+        t1[1] = 0.0f * n[0] - 0.0f * n[2];
+        t1[2] = 0.0f * n[1] - 1.0f * n[0];
+      } else {  // (1, 0, 0) x n
+        t1[0] = 0.0f * n[2] - 0.0f * n[1];
+        t1[1] = 0.0f * n[0] - 1.0f * n[2];
+        t1[2] = 1.0f * n[1] - 0.0f * n[0];
+      }
+      auto normalize = [](float* v) {
+        const float z = v[0] * v[0] + (v[1] * v[1] + v[2] * v[2]);
+        if (z > 0.0f) {
+          const float l = std::sqrt(z);
+          for (int d = 0; d < 3; ++d) v[d] = v[d] / l;
+        }
+      };
+      normalize(t1);
+      float t2[3] = {n[1] * t1[2] - n[2] * t1[1], n[2] * t1[0] - n[0] * t1[2], n[0] * t1[1] - n[1] * t1[0]};
+      normalize(t2);
+      for (int d = 0; d < 3; ++d) {
+        S.t1[d] = t1[d];
+        S.t2[d] = t2[d];
+      }
+      S.offset = a.offset;
+      S.extent = extent;
+      S.n = static_cast<int>(std::floor(2.0f * extent / spacing)) + 1;
+      cnt = (long long)S.n * S.n;
+    } else {
+      return NPX_EINVAL;
+    }
+    S.count = (int)cnt;
+    t += cnt;
+  }
+  *total = t;
+  return NPX_OK;
+}
+}  // namespace
+
+int npx_sample_boundaries(const npx_shape* shapes, int32_t nshapes, float spacing, float plane_extent,
+                          float* local3, int32_t* body, int64_t cap, int64_t* count) {
+  if (!count || nshapes < 0 || (nshapes > 0 && !shapes) || !(spacing > 0.0f)) {
+    g_create_error = "npx_sample_boundaries: bad arguments";
+    return NPX_EINVAL;
+  }
+  std::vector<ShapeS> hs;
+  long long total = 0;
+  if (prepare_shapes(shapes, nshapes, spacing, plane_extent, hs, &total) != NPX_OK) {
+    g_create_error = "npx_sample_boundaries: unknown shape type";
+    return NPX_EINVAL;
+  }
+  if (total > (1ll << 30)) {
+    g_create_error = "npx_sample_boundaries: too many boundary particles";
+    return NPX_EINVAL;
+  }
+  *count = total;
+  if (total == 0 || total > cap) return NPX_OK;
+  if (!local3 || !body) {
+    g_create_error = "npx_sample_boundaries: output is NULL";
+    return NPX_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    g_create_error = "no HIP device available";
+    return NPX_EDEVICE;
+  }
+  ShapeS* ds = nullptr;
+  float* dl = nullptr;
+  int* db = nullptr;
+  hipError_t e = hipMalloc((void**)&ds, sizeof(ShapeS) * hs.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&dl, sizeof(float) * 3 * (size_t)total);
+  if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof(int) * (size_t)total);
+  if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), sizeof(ShapeS) * hs.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const int n = (int)total;
+    hipLaunchKernelGGL(k_sample_boundaries, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, 0, (const ShapeS*)ds,
+                       nshapes, n, spacing, dl, db);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(local3, dl, sizeof(float) * 3 * (size_t)total, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(body, db, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost);
+  if (ds) (void)hipFree(ds);
+  if (dl) (void)hipFree(dl);
+  if (db) (void)hipFree(db);
+  if (e != hipSuccess) {
+    g_create_error = std::string("npx_sample_boundaries: ") + hipGetErrorString(e);
+    return NPX_EDEVICE;
+  }
+  return NPX_OK;
+}
+
 int npx_set_boundary(npx_ctx* ctx, const float* local3, const int32_t* body, const float* psi, int32_t nb,
                      int32_t nbodies) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);

@@ -1503,6 +1503,76 @@ __global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, cons
   PS[b + r] = P[i];
 }
 
+// ------------------------------------------------------ boundary sampling
+// One shape as prepared on the host for k_sample_boundaries: its first output
+// slot, its sample count and, per type, the loop bounds and constants the
+// reference's samplers compute (boundary.cpp:15-127), with the same float
+// operations.
+struct ShapeS {
+  int type, body, first, count;
+  int n1[3], n2[3], axis_first[3];  // box: per face axis, grid dims and first local index
+  int n;                            // sphere: point count; plane: points per side
+  float he[3], r, golden_angle, extent;
+  float nrm[3], offset, t1[3], t2[3];
+  float pos[3], rot[4];             // shape local transform (rot: w, x, y, z)
+};
+
+__device__ __forceinline__ void cross3(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Transform::transform_point: position + q v q*, as v + w (2 u x v) + u x (2 u x v).
+__device__ __forceinline__ void transform_point3(const float* pos, const float* rot, const float* v, float* o) {
+  const float u[3] = {rot[1], rot[2], rot[3]};
+  float uv[3], c[3];
+  cross3(u, v, uv);
+  for (int k = 0; k < 3; ++k) uv[k] = uv[k] + uv[k];
+  cross3(u, uv, c);
+  for (int k = 0; k < 3; ++k) o[k] = pos[k] + ((v[k] + rot[0] * uv[k]) + c[k]);
+}
+
+// One thread per boundary particle.
+__global__ void k_sample_boundaries(const ShapeS* __restrict__ shapes, int nshapes, int total, float spacing,
+                                    float* __restrict__ local3, int* __restrict__ body) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= total) return;
+  int s = 0;
+  while (s + 1 < nshapes && shapes[s + 1].first <= p) ++s;
+  const ShapeS& S = shapes[s];
+  const int i = p - S.first;
+  float lp[3], o[3];
+  if (S.type == NPX_SHAPE_BOX) {  // sample_box_boundary, boundary.cpp:15-53
+    int axis = 0;
+    while (axis < 2 && i >= S.axis_first[axis + 1]) ++axis;
+    const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    const int li = i - S.axis_first[axis], face = S.n1[axis] * S.n2[axis];
+    const int sign = (li / face == 0) ? -1 : 1;
+    const int i1 = (li % face) / S.n2[axis], i2 = (li % face) % S.n2[axis];
+    lp[axis] = (float)sign * S.he[axis];
+    lp[a1] = -S.he[a1] + (float)i1 * spacing;
+    lp[a2] = -S.he[a2] + (float)i2 * spacing;
+    transform_point3(S.pos, S.rot, lp, o);
+  } else if (S.type == NPX_SHAPE_SPHERE) {  // sample_sphere_boundary, boundary.cpp:55-91
+    const float theta = (float)acos((double)(1.0f - 2.0f * ((float)i + 0.5f) / (float)S.n));
+    const float phi = S.golden_angle * (float)i;
+    const float st = (float)sin((double)theta), ct = (float)cos((double)theta);
+    const float sp = (float)sin((double)phi), cp = (float)cos((double)phi);
+    lp[0] = S.r * st * cp;
+    lp[1] = S.r * st * sp;
+    lp[2] = S.r * ct;
+    transform_point3(S.pos, S.rot, lp, o);
+  } else {  // sample_plane_boundary, boundary.cpp:93-126 (world-owned: no transform)
+    const float a = -S.extent + (float)(i / S.n) * spacing, c = -S.extent + (float)(i % S.n) * spacing;
+    for (int k = 0; k < 3; ++k) o[k] = (S.nrm[k] * S.offset + S.t1[k] * a) + S.t2[k] * c;
+  }
+  local3[3 * (size_t)p] = o[0];
+  local3[3 * (size_t)p + 1] = o[1];
+  local3[3 * (size_t)p + 2] = o[2];
+  body[p] = S.body;
+}
+
 // Cell (local window index) of an arbitrary point, or -1 if its 27-cell
 // neighbourhood leaves the window.
 __device__ __forceinline__ int aux_cell(const GridWin& W, float x, float y, float z) {

@@ -319,11 +319,16 @@ class FusedBackend:
     gets its Akinci boundary in the batch (npx_scene_set_boundary, volumes
     computed as FluidWorld computes them) and a host World for its bodies;
     each round is one batch step, one npx_batch_coupling over all coupled
-    scenes and the host rigid steps (FluidWorld::step order)."""
+    scenes and the host rigid steps (FluidWorld::step order).
 
-    def __init__(self, batch, max_scenes: int):
+    device_sampling: sample the boundary particles on the device
+    (npx_sample_boundaries) instead of with the host samplers FluidWorld
+    uses; sphere samples can then differ from FluidWorld's in the last bit."""
+
+    def __init__(self, batch, max_scenes: int, device_sampling: bool = False):
         self.batch = BatchBackend(batch, max_scenes)
         self.coupled: Dict[int, tuple] = {}   # sid -> (World, nbodies, is_static, dt)
+        self.device_sampling = device_sampling
 
     def _attach(self, sid, sc, snap=None):
         import novaphy_amd as nv
@@ -331,9 +336,13 @@ class FusedBackend:
         model = _model_for(sc)
         p = sc.params
         extent = float(max(p.domain_upper[0], p.domain_upper[2]))
-        bnd = nv.sample_model_boundaries(model, 2.0 * p.particle_radius, extent)
-        local = np.array([np.array(b.local_position, np.float32) for b in bnd], np.float32).reshape(-1, 3)
-        body = np.array([b.body_index for b in bnd], np.int32)
+        if self.device_sampling:
+            shapes, nsh = capi.shapes_from_model(model)
+            local, body = capi.sample_boundaries(shapes, nsh, 2.0 * p.particle_radius, extent)
+        else:
+            bnd = nv.sample_model_boundaries(model, 2.0 * p.particle_radius, extent)
+            local = np.array([np.array(b.local_position, np.float32) for b in bnd], np.float32).reshape(-1, 3)
+            body = np.array([b.body_index for b in bnd], np.int32)
         nbod = model.num_bodies
         # volumes at the model's initial transforms, on a scratch context as
         # FluidWorld computes them (boundary.cpp:152-176)

@@ -49,3 +49,40 @@ def test_scene_bytes_admission_estimate():
     # at h = 0.1) * 16 B
     assert b == 8192 * (217 + 512) + 29 * 19 * 19 * 16
     assert capi.scene_bytes(sc.params, 2 * sc.n) > b
+
+
+def _shape_model():
+    """Boxes and spheres with rotated local transforms on rotated bodies, and
+    a tilted plane."""
+    import novaphy_amd as nv
+    b = nv.ModelBuilder()
+    n = np.array([0.1, 1.0, 0.2], np.float32)
+    b.add_shape(nv.CollisionShape.make_plane(n / np.linalg.norm(n), 0.05))
+    rot = nv.Transform(np.array([0.05, -0.02, 0.01], np.float32), np.array([0.1, 0.2, -0.3, 0.93], np.float32))
+    i = b.add_body(nv.RigidBody.from_box(2.0, np.array([0.1, 0.07, 0.13], np.float32)),
+                   nv.Transform(np.array([0.4, 0.6, 0.3], np.float32), np.array([0.0, 0.38, 0.0, 0.92], np.float32)))
+    b.add_shape(nv.CollisionShape.make_box(np.array([0.1, 0.07, 0.13], np.float32), i, rot))
+    j = b.add_body(nv.RigidBody.from_sphere(1.0, 0.17), nv.Transform.from_translation(np.array([0.1, 0.9, 0.2], np.float32)))
+    b.add_shape(nv.CollisionShape.make_sphere(0.17, j, rot))
+    b.add_shape(nv.CollisionShape.make_sphere(0.11, j))
+    return b.build()
+
+
+def test_boundary_sample_counts_match_host_sampler():
+    """npx_sample_boundaries sizes its output on the host with the samplers'
+    own expressions: the count equals sample_model_boundaries' (no device
+    needed for the size query)."""
+    import ctypes as C
+    import novaphy_amd as nv
+    m = _shape_model()
+    shapes, n = capi.shapes_from_model(m)
+    assert [shapes[k].type for k in range(n)] == [capi.SHAPE_PLANE, capi.SHAPE_BOX, capi.SHAPE_SPHERE,
+                                                  capi.SHAPE_SPHERE]
+    assert [shapes[k].body for k in range(n)] == [-1, 0, 1, 1]
+    cnt = C.c_int64()
+    assert capi.load().npx_sample_boundaries(shapes, n, C.c_float(0.05), C.c_float(0.7), None, None, 0,
+                                             C.byref(cnt)) == 0
+    assert cnt.value == len(nv.sample_model_boundaries(m, 0.05, 0.7))
+    bad = capi.NpxShape(type=7)
+    assert capi.load().npx_sample_boundaries(C.byref(bad), 1, C.c_float(0.05), C.c_float(0.7), None, None, 0,
+                                             C.byref(cnt)) == capi.NPX_EINVAL

@@ -1,9 +1,13 @@
 """Akinci boundary path on the GPU (through the C ABI) against the oracle:
 psi volumes, boundary world positions, boundary density and per-body coupling
 force / torque after a PBF step, all bitwise."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
+import novaphy_amd as nv
 from novaphy_amd import capi, scenes
 from oracle import oracle as O
 
@@ -121,3 +125,42 @@ def test_batched_scenes_equal_alone(gpu):
     np.testing.assert_array_equal(b.get(ids[0])[0], c.get_particles()[0])
     c.close()
     b.close()
+
+
+def test_device_boundary_sampling_matches_host(gpu):
+    """npx_sample_boundaries against the host samplers (boundary.cpp:15-150):
+    the same particles in the same order and bodies; box and plane samples
+    bit for bit; sphere samples (acos / sin / cos in double, rounded) within
+    two float ulps of the radius of the host libm's float functions."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_capi_cpu import _shape_model
+    m = _shape_model()
+    _say("case: device boundary sampling vs host, shapes", m.num_shapes)
+    shapes, n = capi.shapes_from_model(m)
+    local, body = capi.sample_boundaries(shapes, n, 0.05, 0.7)
+    host = nv.sample_model_boundaries(m, 0.05, 0.7)
+    hl = np.array([np.array(b.local_position, np.float32) for b in host], np.float32)
+    hb = np.array([b.body_index for b in host], np.int32)
+    assert local.shape == hl.shape
+    np.testing.assert_array_equal(body, hb)
+    kinds = []
+    for sh in m.shapes:
+        k = str(sh.type).split(".")[-1]
+        cnt = len(nv.sample_model_boundaries(_single(sh), 0.05, 0.7))
+        kinds += [k] * cnt
+    kinds = np.array(kinds)
+    exact = kinds != "Sphere"
+    np.testing.assert_array_equal(local[exact], hl[exact])
+    d = np.abs(local[~exact] - hl[~exact]).max()
+    _say("  sphere samples: max |device - host| %.3e m, %d of %d differ" % (
+        d, int(np.any(local[~exact] != hl[~exact], axis=1).sum()), int((~exact).sum())))
+    assert d <= 2 * np.spacing(np.float32(1.0)) * 1.0
+
+
+def _single(shape):
+    """A model holding only `shape` (and the bodies it refers to)."""
+    b = nv.ModelBuilder()
+    for _ in range(max(shape.body_index + 1, 0)):
+        b.add_body(nv.RigidBody.from_sphere(1.0, 0.1))
+    b.add_shape(shape)
+    return b.build()

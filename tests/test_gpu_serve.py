@@ -121,6 +121,37 @@ def test_served_coupled_requests_equal_alone(gpu, kind):
     b.close()
 
 
+def test_device_sampled_requests_equal_alone(gpu):
+    """Fused serving with the boundary particles sampled on the device
+    (npx_sample_boundaries): each coupled request run in the mix ends bitwise
+    equal to the same request served alone."""
+    reqs = _requests(1, 3, seed=24)
+    _say("case: device-sampled coupled requests", [(r.scene.n, len(r.scene.spheres)) for r in reqs])
+
+    def serve_all(rs):
+        clock = Clock()
+        b = _batch_for(rs)
+        srv = serve.Server(serve.FusedBackend(b, 8, device_sampling=True), hbm_budget_bytes=1 << 40, max_scenes=8,
+                           bytes_fn=capi.scene_bytes, clock=clock, health_every=1)
+        for r in rs:
+            srv.submit(r)
+        _run(srv, clock)
+        out = {r.rid: srv.finished[r.rid] for r in rs}
+        b.close()
+        return out
+
+    mix = serve_all(reqs)
+    for r in reqs:
+        alone = serve_all([serve.Request(rid=r.rid, scene=r.scene, horizon=r.horizon, arrival=0.0)])[r.rid]
+        got = mix[r.rid]
+        assert got.status == serve.DONE and alone.status == serve.DONE
+        for a, g in zip(alone.result[:3], got.result[:3]):
+            np.testing.assert_array_equal(g, a)
+        if r.scene.spheres:
+            for a, g in zip(alone.result[4], got.result[4]):
+                np.testing.assert_array_equal(g, a)
+
+
 @pytest.mark.parametrize("kind", ["fused", "mixed"])
 def test_preempted_requests_continue_bitwise(gpu, tmp_path, kind):
     """Requests taken off the device mid-rollout (snapshot written to disk and
