@@ -11,6 +11,9 @@
 //   ID / CELL       original particle index / local window cell of x*_0
 //   class tables    clsmap[window cell] -> class; cstart[class] exclusive prefix
 //                   of class populations (classes of scene s contiguous).
+//   skin lists      per-step candidate lists (struct Skin below)
+//   runs            runs of identical state: start flags, run of each slot,
+//                   first slot of each run (k_gather, k_pieces)
 #pragma once
 
 #include "npx_device.hpp"
@@ -1110,7 +1113,8 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
 // Jacobi apply (:71-77) into the other x* buffer.  On the last iteration it
 // also performs the velocity update with the h/dt speed clamp (:81-93).
-// Records the scene's largest |delta x*| of the iteration for the skin test.
+// Adds the particle's |delta x*| to its summed displacement and keeps the
+// largest sum of each class for the skin-list test.
 template <bool FAST>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
