@@ -78,8 +78,24 @@ def build_core(verbose: bool = True) -> str:
     return out
 
 
+def arith_check_path() -> str:
+    return os.path.join(ROOT, "tests", "hip", "libnpx_arith_check.so")
+
+
+def build_arith_check(verbose: bool = True) -> str:
+    """The device check of the in-range sqrt / division helpers (a test library)."""
+    src = os.path.join(ROOT, "tests", "hip", "arith_check.hip")
+    out = arith_check_path()
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = True) -> None:
     build_hip(verbose)
+    if os.path.isfile(os.path.join(ROOT, "tests", "hip", "arith_check.hip")):
+        build_arith_check(verbose)
     if os.path.isdir(os.path.join(CSRC, "host")):
         build_core(verbose)
 

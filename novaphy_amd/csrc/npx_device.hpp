@@ -28,7 +28,8 @@ struct __align__(16) SceneC {
   int wlo0, wlo1, wlo2, wx;        // cell window origin (global cell coords) and dims
   int wy, wz, cls_base, ncell;     // class ids of this scene: [cls_base, cls_base + ncell)
   int seg_off, n, seg_blocks, vort_on;
-  int nbnd, pad_i1, pad_i2, pad_i3;  // nbnd: boundary particles of the scene (batched coupling)
+  int nbnd, fdiv, pad_i2, pad_i3;  // nbnd: boundary particles of the scene (batched coupling);
+                                   // fdiv: h in [2^-6, 2^6], so npx_div_in serves the per-pair divisions
   float sp_lo, sp_hi;               // spiky support on r^2: (sp_lo, sp_hi), i.e. 1e-7 < |r| < h
   float skin_cut2, skin_allow;      // skin radius^2; largest summed displacement a skin list allows
 };
@@ -64,18 +65,67 @@ __device__ __forceinline__ float spiky_coeff(float r_sq, float h, float cs, bool
   return (-cs) * diff * diff / r_len;
 }
 
+// Correctly rounded square root of a finite x >= 2^-96.  This is the
+// sequence the compiler emits for a correctly rounded sqrtf (the hardware
+// estimate, within one ulp, corrected by the residuals of its two
+// neighbours) without the steps that only act on other inputs: the 2^32
+// pre-scaling of x < 2^-96 and the pass-through of zeros and infinities.
+// Same bits as sqrtf on that range (checked exhaustively on the device by
+// tests/hip/arith_check.hip).
+__device__ __forceinline__ float npx_sqrt_in(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x);
+  const float ru = __builtin_fmaf(-su, s, x);
+  const float t = (rd <= 0.0f) ? sd : s;
+  return (ru > 0.0f) ? su : t;
+}
+
+// Correctly rounded a / b for operands where the compiler's correctly
+// rounded division sequence scales nothing and fixes nothing up: b normal
+// with 2^-60 <= |b| <= 2^60, a = 0 or 2^-100 <= |a| <= 2^60, and
+// 2^-120 <= |a / b| <= 2^90.  There v_div_scale returns its input with VCC
+// clear, v_div_fmas is a plain fma and v_div_fixup returns the refined
+// quotient, so the Newton-Raphson refinement below is that sequence step for
+// step.  Same bits as a / b on that range (checked on the device by
+// tests/hip/arith_check.hip).  For h in [2^-6, 2^6] every per-pair division
+// of the neighbour passes is in range: spiky_grad's (-cs) diff^2 / |r| has
+// |r| in (1e-7, h), diff > h 2^-25 and cs = 45 / (PI h^6), and the s_corr
+// ratio poly6(r^2) / poly6((dq h)^2) has a numerator above 2^-93 or zero and a
+// denominator in (1e-10, 2^19).
+__device__ __forceinline__ float npx_div_in(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  const float e = __builtin_fmaf(-b, r, 1.0f);
+  r = __builtin_fmaf(e, r, r);
+  float q = a * r;
+  const float e1 = __builtin_fmaf(-b, q, a);
+  q = __builtin_fmaf(e1, r, q);
+  const float e2 = __builtin_fmaf(-b, q, a);
+  return __builtin_fmaf(e2, r, q);
+}
+
+// a / b, through npx_div_in when the scene's constants keep the operands in
+// its range (fd, uniform over a block).
+__device__ __forceinline__ float npx_div(float a, float b, bool fd) {
+  if (fd) return npx_div_in(a, b);
+  return a / b;
+}
+
 // spiky_coeff with the support test done on r^2 against the exact bounds
 // sp_lo = max{x : sqrtf(x) <= 1e-7} and sp_hi = min{x : sqrtf(x) >= h}, so the
-// square root is taken only inside the support; same results bit for bit.
-__device__ __forceinline__ float spiky_coeff_sq(float r_sq, float sp_lo, float sp_hi, float h, float cs, bool* in) {
+// square root is taken only inside the support (where r^2 > 1e-14 > 2^-96);
+// same results bit for bit.
+__device__ __forceinline__ float spiky_coeff_sq(float r_sq, float sp_lo, float sp_hi, float h, float cs, bool fd,
+                                                bool* in) {
   if (r_sq <= sp_lo || r_sq >= sp_hi) {
     *in = false;
     return 0.0f;
   }
   *in = true;
-  const float r_len = sqrtf(r_sq);
+  const float r_len = npx_sqrt_in(r_sq);
   const float diff = h - r_len;
-  return (-cs) * diff * diff / r_len;
+  return npx_div((-cs) * diff * diff, r_len, fd);
 }
 
 // Fast variant: one rsqrt, FMA-friendly; support tested on r_sq

@@ -177,6 +177,7 @@ static SceneC make_scene_consts(const SceneH& H, float skin_frac) {
   S.chi2 = P.domain_upper[2] - pr;
   S.sp_hi = spiky_cut(h);
   S.sp_lo = sqrt_le_max(1e-7f);
+  S.fdiv = (h >= 0.015625f && h <= 64.0f) ? 1 : 0;  // operand range of npx_div_in
   S.cut2 = std::max(S.h2, S.sp_hi);
   {
     // skin radius hs = (1 + f) h; a pair outside the skin at x*_0 stays

@@ -1040,6 +1040,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
   const float4 xi = XS[q];
   const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
   const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+  const bool fd = S.fdiv != 0;
   const float cut = FAST ? h2 : S.cut2;
   float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
   auto ld = [&](int j) { return XS[j]; };
@@ -1052,7 +1053,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     add_k(rho, m * poly6(r_sq, h2, c6), k);
     if (j == q) return;
     bool in;
-    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
     if (!in) return;
     const float ax = mr * (co * rx), ay = mr * (co * ry), az = mr * (co * rz);
     add_k(sgs, dist2<FAST>(ax, ay, az), k);
@@ -1138,6 +1139,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     const float4 xi = XS[q];
     const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+    const bool fd = S.fdiv != 0;
     const float wdq = S.wdq, negk = -S.corr_k;
     const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
     const float cut = FAST ? h2 : S.cut2;
@@ -1150,11 +1152,11 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
       const float r_sq = dist2<FAST>(rx, ry, rz);
       if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
       bool in;
-      const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+      const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
       if (!in) return;
       float sc = 0.0f;
       if (ls < -1e-10f && scorr_on) {
-        float ratio = poly6(r_sq, h2, c6) / wdq;
+        float ratio = npx_div(poly6(r_sq, h2, c6), wdq, fd);
         ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
         float rp = ratio * ratio;
         if (n4) rp = rp * rp;
@@ -1224,6 +1226,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
   if (xs || vort) {
     const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, cut = S.cut2;
     const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+    const bool fd = S.fdiv != 0;
     float cx = 0.0f, cy = 0.0f, cz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
     auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
     auto body = [&](int j, const XV& d, int k) {
@@ -1241,7 +1244,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
       }
       if (vort) {
         bool in;
-        const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+        const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
         if (!in) return;
         const float gx = co * rx, gy = co * ry, gz = co * rz;
         add_k(wx, vj.w * (ay * gz - az * gy), k);
@@ -1292,6 +1295,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   float vx = vx4.x, vy = vx4.y, vz = vx4.z;
   if (S.vort_on) {
     const float h = S.h, cs = S.cs, sp_lo = S.sp_lo, sp_hi = S.sp_hi;
+    const bool fd = S.fdiv != 0;
     const float4 oi = OM[q];
     float ex = 0.0f, ey = 0.0f, ez = 0.0f;
     auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
@@ -1300,7 +1304,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
       const float r_sq = sqn3(rx, ry, rz);
       bool in;
-      const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, &in);
+      const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
       if (!in) return;
       const float sc = d.vw * (d.ow - oi.w);
       add_k(ex, sc * (co * rx), k);
