@@ -16,6 +16,8 @@
 //                   first slot of each run (k_gather, k_pieces)
 #pragma once
 
+#include <type_traits>
+
 #include "npx_device.hpp"
 #include "npx_repadd.hpp"
 
@@ -989,22 +991,61 @@ __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx,
   }
 }
 
+// Position of a candidate from what the pass's loader fetched.
+template <class D>
+__device__ __forceinline__ float4 cand_pos(const D& d) {
+  if constexpr (std::is_same_v<D, float4>)
+    return d;
+  else
+    return d.x;
+}
+
+// visit_all for a testing pass whose particle has no valid skin list: the
+// in-range candidates it meets (dist2 < cut at xi, a run of k counting k
+// entries) are recorded, in order, as the particle's in-range list while
+// they fit in NPX_SKIN_CAP entries, so the reusing pass that follows walks
+// that list instead of sweeping again.  Returns the list length, or -1 when
+// it did not fit.
+template <bool FAST, int G, bool RUNS, class L, class F>
+__device__ __forceinline__ int visit_all_rec(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
+                                             int wxy, const int* __restrict__ cm, const int* __restrict__ cstart,
+                                             int shift, L&& ld, F&& f) {
+  int e = 0;
+  auto rec = [&](int j, const auto& d, int k) {
+    const float4 xj = cand_pos(d);
+    if (e >= 0 && dist2<FAST>(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) < cut) {
+      if (e + k <= NPX_SKIN_CAP) {
+        for (int t = 0; t < k; ++t) K.ex[sk_at(q, e + t)] = j;
+        e += k;
+      } else {
+        e = -1;
+      }
+    }
+    f(j, d, k);
+  };
+  visit_all<G, RUNS>(K, rs, l, wx, wxy, cm, cstart, shift, ld, rec);
+  return e;
+}
+
 // acc += t, k times in a row (k equal terms of a run), bit for bit.
 __device__ __forceinline__ void add_k(float& acc, float t, int k) { acc = (k == 1) ? acc + t : npx_rep_add(acc, t, k); }
 
 // Testing pass (density, XSPH / curl): visits the in-range candidates of
 // slot q at xi through the skin list, recording them for the pass that
-// follows, or by the inline sweep (in-range count -1).  C is the particle's
-// candidate_disp.
+// follows, or by the inline sweep, which records them when `record` (a
+// reusing pass follows) and they fit (in-range count -1 otherwise).  C is the
+// particle's candidate_disp.
 template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool rs, int q, float C, float4 xi, float cut,
                                            const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
-                                           L&& ld, F&& f) {
+                                           bool record, L&& ld, F&& f) {
   const int n = K.cnt[q];
   if (n >= 0 && K.cd[q] + C <= S.skin_allow) {
     const unsigned long long rel = test_list<FAST>(K, q, n, xi, cut, XS);
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
+  } else if (record) {
+    K.ecnt[q] = visit_all_rec<FAST, G, RUNS>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, ld, f);
   } else {
     K.ecnt[q] = -1;
     visit_all<G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
@@ -1074,12 +1115,11 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
-      K.ecnt[q] = -1;
-      visit_all<4, !FAST>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, body);
+      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, ld, body);
   }
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
@@ -1254,7 +1294,8 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     };
     if (S.iters > 0) {  // the skin lists exist once a density pass has run
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
-      visit_test<false, 2, true>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, ld, body);
+      // the vorticity force pass reuses the in-range list
+      visit_test<false, 2, true>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, vort, ld, body);
     } else {
       K.ecnt[q] = -1;
       visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
