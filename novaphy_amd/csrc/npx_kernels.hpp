@@ -991,6 +991,92 @@ __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx,
   }
 }
 
+// Compacted visit for the inline sweep: slots are tested 64 at a time (r^2 <
+// cut at xi, the test every pass body applies first) and only the in-range
+// ones are fetched again and evaluated, in order.  A wave whose lanes sweep
+// dense cells then evaluates each lane's in-range candidates side by side
+// instead of stepping every lane through every candidate.
+
+// Mask of the slots c .. c + m - 1 (1 <= m <= 64) with r^2 < cut at xi.
+template <bool FAST>
+__device__ __forceinline__ unsigned long long test_slots(int c, int m, float4 xi, float cut,
+                                                         const float4* __restrict__ XS) {
+  unsigned long long rel = 0;
+  for (int t = 0; t < m; t += 4) {
+    float4 a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = XS[c + min(t + u, m - 1)];
+    unsigned bits = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float d = dist2<FAST>(xi.x - a[u].x, xi.y - a[u].y, xi.z - a[u].z);
+      bits |= (unsigned)((d < cut) & (t + u < m)) << u;
+    }
+    rel |= (unsigned long long)bits << t;  // t <= 60
+  }
+  return rel;
+}
+
+// visit_range over b..e-1 restricted to the in-range slots.
+template <bool FAST, int G, class L, class F>
+__device__ __forceinline__ void visit_range_c(int b, int e, float4 xi, float cut, const float4* __restrict__ XS,
+                                              L&& ld, F&& f) {
+  for (int c = b; c < e; c += 64) {
+    unsigned long long rel = test_slots<FAST>(c, min(64, e - c), xi, cut, XS);
+    for (;;) {
+      int ks[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        ks[u] = rel ? (int)__builtin_ctzll(rel) : -1;
+        rel &= rel - 1;
+      }
+      if (ks[0] < 0) break;
+      decltype(ld(0)) v[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) v[u] = ld(c + (ks[u] < 0 ? ks[0] : ks[u]));
+#pragma unroll
+      for (int u = 0; u < G; ++u) keep(v[u]);
+#pragma unroll
+      for (int u = 0; u < G; ++u)
+        if (ks[u] >= 0) f(c + ks[u], v[u], 1);
+    }
+  }
+}
+
+// visit_all with the slot ranges restricted to their in-range candidates
+// (r^2 < cut at xi): the same calls of f in the same order, minus calls every
+// pass body returns from at once.  Classes holding runs keep the run walk,
+// whose dependent run-start loads would not pay for a separate test phase.
+template <bool FAST, int G, bool RUNS, int RG = 2, class L, class F>
+__device__ __forceinline__ void visit_all_c(const Skin& K, bool rs, int l, int wx, int wxy,
+                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                            float4 xi, float cut, const float4* __restrict__ XS, L&& ld, F&& f) {
+#pragma unroll 1
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1];
+    const int c1 = cm[base];
+    const int c2 = cm[base + 1];
+    if (c1 == c0 + 1 && c2 == c0 + 2) {
+      const int b = cstart[c0] - shift, e = cstart[c0 + 3] - shift;
+      if (RUNS && rs && cls_runs(K, c0, 3))
+        visit_runs<RG>(K, b, e, ld, f);
+      else
+        visit_range_c<FAST, G>(b, e, xi, cut, XS, ld, f);
+    } else {
+#pragma unroll 1
+      for (int k = 0; k < 3; ++k) {
+        const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
+        const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+        if (RUNS && rs && cls_runs(K, c, 1))
+          visit_runs<RG>(K, b, e, ld, f);
+        else
+          visit_range_c<FAST, G>(b, e, xi, cut, XS, ld, f);
+      }
+    }
+  }
+}
+
 // Position of a candidate from what the pass's loader fetched.
 template <class D>
 __device__ __forceinline__ float4 cand_pos(const D& d) {
@@ -1009,7 +1095,7 @@ __device__ __forceinline__ float4 cand_pos(const D& d) {
 template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ int visit_all_rec(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
                                              int wxy, const int* __restrict__ cm, const int* __restrict__ cstart,
-                                             int shift, L&& ld, F&& f) {
+                                             int shift, const float4* __restrict__ XS, L&& ld, F&& f) {
   int e = 0;
   auto rec = [&](int j, const auto& d, int k) {
     const float4 xj = cand_pos(d);
@@ -1023,7 +1109,7 @@ __device__ __forceinline__ int visit_all_rec(const Skin& K, bool rs, int q, floa
     }
     f(j, d, k);
   };
-  visit_all<G, RUNS>(K, rs, l, wx, wxy, cm, cstart, shift, ld, rec);
+  visit_all_c<FAST, G, RUNS>(K, rs, l, wx, wxy, cm, cstart, shift, xi, cut, XS, ld, rec);
   return e;
 }
 
@@ -1045,24 +1131,24 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool 
     const unsigned long long rel = test_list<FAST>(K, q, n, xi, cut, XS);
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
   } else if (record) {
-    K.ecnt[q] = visit_all_rec<FAST, G, RUNS>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    K.ecnt[q] = visit_all_rec<FAST, G, RUNS>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, f);
   } else {
     K.ecnt[q] = -1;
-    visit_all<G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all_c<FAST, G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, f);
   }
 }
 
 // Reusing pass (delta-p, vorticity force): same positions as the testing
 // pass before it, so its in-range list applies.
-template <int G, bool RUNS, int RG = 2, class L, class F>
+template <bool FAST, int G, bool RUNS, int RG = 2, int CG = G, class L, class F>
 __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool rs, int q, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
-                                            L&& ld, F&& f) {
+                                            float4 xi, float cut, const float4* __restrict__ XS, L&& ld, F&& f) {
   const int e = K.ecnt[q];
   if (e >= 0)
     visit_list<G>(K.ex, q, e, ld, f);
   else
-    visit_all<G, RUNS, RG>(K, rs, l, S.wx, wxy, cm, cstart, shift, ld, f);
+    visit_all_c<FAST, CG, RUNS, RG>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, f);
 }
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
@@ -1071,7 +1157,7 @@ __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool
 // when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
 // iteration) builds the step's skin lists; later iterations test them.
 template <bool FAST, bool BUILD>
-__global__ void __launch_bounds__(NPX_MAX_BLOCK)
+__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(BUILD ? 4 : 6)))
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                  float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
@@ -1115,7 +1201,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
-      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, ld, body);
+      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
@@ -1214,7 +1300,8 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
         add_k(dz, coef * (co * rz), k);
       }
     };
-    visit_reuse<4, !FAST, 1>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);  // one run at a time: 64 VGPRs
+    // one run / two slots at a time on the inline sweep: 64 VGPRs
+    visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, cut, XS, ld, body);
     const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
     XSn[q] = make_float4(px, py, pz, 0.0f);
     if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
@@ -1352,7 +1439,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       add_k(ey, sc * (co * ry), k);
       add_k(ez, sc * (co * rz), k);
     };
-    visit_reuse<4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, ld, body);  // list of the XSPH / curl pass
+    visit_reuse<false, 4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, S.cut2, XS, ld, body);  // list of the XSPH / curl pass
     const float el = sqrtf(sqn3(ex, ey, ez));
     if (el > 1e-9f) {
       const float inv = 1.0f / el;
