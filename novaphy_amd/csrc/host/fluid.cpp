@@ -484,9 +484,14 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"block_plan", "fluid.pbf.neighbor_query"},
             {"density_lambda_build", "fluid.pbf.iteration.compute_density"},
             {"density_lambda", "fluid.pbf.iteration.compute_density"},
+            {"density_lambda_build_dense", "fluid.pbf.iteration.compute_density"},
+            {"density_lambda_dense", "fluid.pbf.iteration.compute_density"},
             {"delta", "fluid.pbf.iteration.compute_delta_position"},
+            {"delta_dense", "fluid.pbf.iteration.compute_delta_position"},
             {"omega_xsph", "fluid.pbf.xsph_viscosity"},
+            {"omega_xsph_dense", "fluid.pbf.xsph_viscosity"},
             {"vort_commit", "fluid.pbf.commit_positions"},
+            {"vort_commit_dense", "fluid.pbf.commit_positions"},
             {"run_fill", "fluid.pbf.commit_positions"},
         };
         // algorithmic bytes per particle of each pass (FP32/int32, 12 B per

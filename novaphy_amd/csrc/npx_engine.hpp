@@ -248,6 +248,7 @@ struct Ctx {
   DBuf<uint8_t> RSTART;  // per slot: 1 if the slot starts a run of identical state
   DBuf<int> PI, PS;      // per slot: its run; per run: its first slot
   DBuf<int> BOFF;        // per 256-slot block: run starts, then their exclusive prefix
+  DBuf<int> DL, DN;      // dense list (heads whose skin list overflowed) and its length
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -585,6 +586,8 @@ struct Ctx {
     NPX_HIP(PI.ensure(n));
     NPX_HIP(PS.ensure(n));
     NPX_HIP(BOFF.ensure(n / NPX_BLOCK + 5));
+    NPX_HIP(DL.ensure(n));
+    NPX_HIP(DN.ensure(1));
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
@@ -904,10 +907,12 @@ struct Ctx {
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
-                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls()};
+                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1};
+    // one 64-lane block per dense particle, at most this many at a time
+    const int dense_blocks = std::max(1, std::min(8192, hw_blocks * (NPX_BLOCK / 64)));
     NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
                    (const int*)cstart.p, (const int*)CELL.p, (const int*)BOFF.p, hw_blocks, BCOST.p));
-    NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p));
+    NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p, DN.p));
     const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
@@ -919,18 +924,32 @@ struct Ctx {
                        : (it == 0 ? k_density_lambda<false, true> : k_density_lambda<false, false>);
       NPX_TRY(launch(it == 0 ? "density_lambda_build" : "density_lambda", dens, blocks, nbr_block, sc, bs, cm, cst,
                      cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
-      if (fast)
+      if (!fast)
+        NPX_TRY(launch(it == 0 ? "density_lambda_build_dense" : "density_lambda_dense", k_density_dense,
+                       dense_blocks, 64, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R));
+      if (fast) {
         NPX_TRY(launch("delta", k_delta<true>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                        xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
-      else
+      } else {
         NPX_TRY(launch("delta", k_delta<false>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                        xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
+        NPX_TRY(launch("delta_dense", k_delta_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
+                       (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p,
+                       R));
+      }
       cur = 1 - cur;
     }
     NPX_TRY(launch("omega_xsph", k_omega_xsph, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                    (const float4*)VT.p, OM.p, VX.p, R, rm));
+    if (!fast && npass > 0)
+      NPX_TRY(launch("omega_xsph_dense", k_omega_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
+                     (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, VX.p, R));
     NPX_TRY(launch("vort_commit", k_vort_commit, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                    (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p, V.p, R, rm));
+    if (!fast && npass > 0)
+      NPX_TRY(launch("vort_commit_dense", k_commit_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
+                     (const float4*)xs[cur]->p, (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p,
+                     V.p, R));
     NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, R, X.p, V.p, RHO.p, LAM.p));
     xs_final = cur;
     return NPX_OK;
@@ -966,7 +985,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, BCOST.p, BORDER.p, BMODE.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, BCOST.p, BORDER.p, BMODE.p, DL.p, DN.p};
   }
 
   void drop_graphs() {

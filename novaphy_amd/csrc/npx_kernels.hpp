@@ -550,14 +550,16 @@ __global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __res
 // Chooses the block order of the step's neighbour passes (one workgroup):
 // the cost order when the XCD order's eight contiguous chunks differ by more
 // than 15 % in estimated cost, else the XCD order.  The cost order is a
-// bucket sort by floor(log2(cost)), descending.
+// bucket sort by floor(log2(cost)), descending.  Also empties the step's
+// dense list (dn), which the build pass fills after it.
 __global__ void __launch_bounds__(1024) k_block_plan(const float* __restrict__ cost, int nb, int* __restrict__ order,
-                                                     int* __restrict__ mode) {
+                                                     int* __restrict__ mode, int* __restrict__ dn) {
   __shared__ float chunk[8];
   __shared__ int hist[64];
   __shared__ int base[64];
   __shared__ int lpt;
   const int t = threadIdx.x;
+  if (t == 0) *dn = 0;
   if (t < 8) chunk[t] = 0.0f;
   if (t < 64) hist[t] = 0;
   __syncthreads();
@@ -642,6 +644,9 @@ struct Skin {
   const int* __restrict__ ps;               // [pieces] first slot of each run
   const unsigned* __restrict__ runb;       // [classes / 32] bit set: the class holds a run longer than one
   const int* __restrict__ runs_s;          // [scenes] 1: the scene holds a run longer than one
+  int* __restrict__ dl;                    // [slots] dense list: heads whose skin list overflowed
+  int* __restrict__ dn;                    // [1] dense list length
+  int dense;                               // 1: dense particles go to the per-wave kernels
 };
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
@@ -1151,67 +1156,179 @@ __device__ __forceinline__ void visit_reuse(const SceneC& S, const Skin& K, bool
     visit_all_c<FAST, CG, RUNS, RG>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, f);
 }
 
-// compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
-// (:144-179): rho_i sums m W over all in-range candidates (self and
-// duplicates included); for j != i, sum |grad_j|^2 and grad_i; lambda is 0
-// when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
-// iteration) builds the step's skin lists; later iterations test them.
-template <bool FAST, bool BUILD>
-__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(BUILD ? 4 : 6)))
-k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-                 const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
-                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
-  NPX_NBR_PROLOGUE();
-  const bool rs = K.runs_s[s] != 0;
-  if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
-  const float4 xi = XS[q];
-  const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass, mr = S.m_inv_rho0;
-  const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
-  const bool fd = S.fdiv != 0;
-  const float cut = FAST ? h2 : S.cut2;
-  float rho = 0.0f, sgs = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
-  auto ld = [&](int j) { return XS[j]; };
-  // k > 1: the k equal terms of a run of identical candidates (slot q can be
-  // one of them; its self term equals theirs: the density term, no gradient)
-  auto body = [&](int j, float4 xj, int k) {
+// ------------------------------------------------------------ pair terms
+// The SPH terms of one candidate pair for each neighbour pass, shared by the
+// per-lane kernels and the per-wave sweep of dense particles (below).  f(j,
+// d, k, a) adds the pair's terms to the accumulators a[], k times over for a
+// run of k identical candidates.  Every term is added with add_k, so with
+// a[] at zero and k = 1 the call leaves the terms themselves in a[] (0 + t
+// == t, and a +0 in place of a -0 term adds the same to a sum that is never
+// -0).
+
+struct XV {
+  float4 x, v;
+};
+__device__ __forceinline__ void keep(const XV& d) {
+  npx_keep(d.x);
+  npx_keep(d.v);
+}
+
+struct XOW {
+  float4 x;
+  float vw, ow;
+};
+__device__ __forceinline__ void keep(const XOW& d) {
+  npx_keep(make_float4(d.x.x, d.x.y, d.x.z, d.vw));
+  __asm__ volatile("" ::"v"(d.ow));
+}
+
+// compute_density (pbf_solver.cpp:129-142) and compute_lambda (:144-179):
+// a = {rho, sum |grad_j|^2, grad_i}.  k > 1: the k equal terms of a run of
+// identical candidates (slot q can be one of them; its self term equals
+// theirs: the density term, no gradient).
+template <bool FAST>
+struct DensityPair {
+  float4 xi;
+  int q;
+  float h, h2, c6, cs, m, mr, sp_lo, sp_hi, cut;
+  bool fd;
+  __device__ __forceinline__ DensityPair(const SceneC& S, int q_, float4 xi_)
+      : xi(xi_), q(q_), h(S.h), h2(S.h2), c6(S.c6), cs(S.cs), m(S.mass), mr(S.m_inv_rho0), sp_lo(S.sp_lo),
+        sp_hi(S.sp_hi), cut(FAST ? S.h2 : S.cut2), fd(S.fdiv != 0) {}
+  static constexpr int NT = 5;
+  __device__ __forceinline__ void operator()(int j, const float4& xj, int k, float* a) const {
     const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
     const float r_sq = dist2<FAST>(rx, ry, rz);
-    if (r_sq >= cut) return;  // only reached on the inline sweep
-    add_k(rho, m * poly6(r_sq, h2, c6), k);
+    if (r_sq >= cut) return;  // only reached on inline sweeps
+    add_k(a[0], m * poly6(r_sq, h2, c6), k);
     if (j == q) return;
     bool in;
     const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
     if (!in) return;
     const float ax = mr * (co * rx), ay = mr * (co * ry), az = mr * (co * rz);
-    add_k(sgs, dist2<FAST>(ax, ay, az), k);
-    add_k(gx, ax, k);
-    add_k(gy, ay, k);
-    add_k(gz, az, k);
-  };
-  if (BUILD) {
-    Rows w;
-    int n = -1;
-    unsigned long long rel = 0;
-    if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
-      n = store_skin(w, K, q, &rel);
-    else
-      n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
-    if (n > NPX_SKIN_CAP) n = -1;
-    K.cnt[q] = n;
-    if (n >= 0) {
-      K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
-    } else {
-      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
-    }
-  } else {
-    const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, ld, body);
+    add_k(a[1], dist2<FAST>(ax, ay, az), k);
+    add_k(a[2], ax, k);
+    add_k(a[3], ay, k);
+    add_k(a[4], az, k);
   }
+};
+
+// compute_delta_position (pbf_solver.cpp:181-223) with s_corr: a = delta-p
+// before the 1/rho0 factor.  FAST accumulates with FMA instead.
+template <bool FAST>
+struct DeltaPair {
+  float4 xi;  // w: lambda_i
+  int q;
+  float h, h2, c6, cs, m, sp_lo, sp_hi, cut, wdq, negk;
+  bool n4, scorr_on, fd;
+  __device__ __forceinline__ DeltaPair(const SceneC& S, int q_, float4 xi_)
+      : xi(xi_), q(q_), h(S.h), h2(S.h2), c6(S.c6), cs(S.cs), m(S.mass), sp_lo(S.sp_lo), sp_hi(S.sp_hi),
+        cut(FAST ? S.h2 : S.cut2), wdq(S.wdq), negk(-S.corr_k), n4(S.corr_n == 4), scorr_on(S.wdq > 1e-10f),
+        fd(S.fdiv != 0) {}
+  static constexpr int NT = 3;
+  __device__ __forceinline__ void operator()(int j, const float4& xj, int k, float* a) const {
+    const float ls = xi.w + xj.w;
+    if (j == q) return;
+    const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+    const float r_sq = dist2<FAST>(rx, ry, rz);
+    if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
+    bool in;
+    const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
+    if (!in) return;
+    float sc = 0.0f;
+    if (ls < -1e-10f && scorr_on) {
+      float ratio = npx_div(poly6(r_sq, h2, c6), wdq, fd);
+      ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
+      float rp = ratio * ratio;
+      if (n4) rp = rp * rp;
+      sc = negk * rp;
+    }
+    const float coef = (ls + sc) * m;
+    if (FAST) {
+      const float cc = coef * co;
+      a[0] = __builtin_fmaf(cc, rx, a[0]);
+      a[1] = __builtin_fmaf(cc, ry, a[1]);
+      a[2] = __builtin_fmaf(cc, rz, a[2]);
+    } else {
+      add_k(a[0], coef * (co * rx), k);
+      add_k(a[1], coef * (co * ry), k);
+      add_k(a[2], coef * (co * rz), k);
+    }
+  }
+};
+
+// apply_xsph_viscosity (pbf_solver.cpp:225-250) and the vorticity curl:
+// a = {c_i, omega_i}.
+struct OmegaPair {
+  float4 xi, vi;
+  int q;
+  float h, h2, c6, cs, cut, sp_lo, sp_hi;
+  bool xs, vort, fd;
+  __device__ __forceinline__ OmegaPair(const SceneC& S, int q_, float4 xi_, float4 vi_)
+      : xi(xi_), vi(vi_), q(q_), h(S.h), h2(S.h2), c6(S.c6), cs(S.cs), cut(S.cut2), sp_lo(S.sp_lo), sp_hi(S.sp_hi),
+        xs(S.xsph_on != 0), vort(S.vort_on != 0), fd(S.fdiv != 0) {}
+  static constexpr int NT = 6;
+  __device__ __forceinline__ void operator()(int j, const XV& d, int k, float* a) const {
+    if (j == q) return;
+    const float rx = d.x.x - xi.x, ry = d.x.y - xi.y, rz = d.x.z - xi.z;
+    const float r_sq = sqn3(rx, ry, rz);  // equals |x*_i - x*_j|^2 bit for bit
+    if (r_sq >= cut) return;              // both terms are exact zeros beyond cut2
+    const float4 vj = d.v;
+    const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
+    if (xs && r_sq < h2) {
+      const float wv = poly6(r_sq, h2, c6);
+      add_k(a[0], (vj.w * ax) * wv, k);
+      add_k(a[1], (vj.w * ay) * wv, k);
+      add_k(a[2], (vj.w * az) * wv, k);
+    }
+    if (vort) {
+      bool in;
+      const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
+      if (!in) return;
+      const float gx = co * rx, gy = co * ry, gz = co * rz;
+      add_k(a[3], vj.w * (ay * gz - az * gy), k);
+      add_k(a[4], vj.w * (az * gx - ax * gz), k);
+      add_k(a[5], vj.w * (ax * gy - ay * gx), k);
+    }
+  }
+};
+
+// Vorticity force gradient eta_i: a = eta_i.
+struct VortPair {
+  float4 xi;
+  float oiw;  // |omega_i|
+  int q;
+  float h, cs, sp_lo, sp_hi;
+  bool fd;
+  __device__ __forceinline__ VortPair(const SceneC& S, int q_, float4 xi_, float oiw_)
+      : xi(xi_), oiw(oiw_), q(q_), h(S.h), cs(S.cs), sp_lo(S.sp_lo), sp_hi(S.sp_hi), fd(S.fdiv != 0) {}
+  static constexpr int NT = 3;
+  __device__ __forceinline__ void operator()(int j, const XOW& d, int k, float* a) const {
+    if (j == q) return;
+    const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
+    const float r_sq = sqn3(rx, ry, rz);
+    bool in;
+    const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
+    if (!in) return;
+    const float sc = d.vw * (d.ow - oiw);
+    add_k(a[0], sc * (co * rx), k);
+    add_k(a[1], sc * (co * ry), k);
+    add_k(a[2], sc * (co * rz), k);
+  }
+};
+
+// ------------------------------------------------------------ per-particle ends
+
+// lambda from the density sums (pbf_solver.cpp:144-179): 0 when C_i < 0,
+// else -C_i / (sum + |grad_i|^2 + eps).
+__device__ __forceinline__ void density_finish(const SceneC& S, int q, const float* a, float4* __restrict__ XS,
+                                               float* __restrict__ RHO, float* __restrict__ LAM) {
+  const float rho = a[0];
   RHO[q] = rho;
   const float Ci = rho * S.inv_rho0 - 1.0f;
   float lam = 0.0f;
   if (!(Ci < 0.0f)) {
-    sgs += sqn3(gx, gy, gz);
+    const float sgs = a[1] + sqn3(a[2], a[3], a[4]);
     lam = -Ci / (sgs + S.eps);
   }
   XS[q].w = lam;
@@ -1235,6 +1352,272 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
   const float r = RHO[q];
   const float rt = (r > 1e-6f) ? r : S.rho0;
   VT[q] = make_float4(vx, vy, vz, S.mass / rt);
+}
+
+// Jacobi apply (pbf_solver.cpp:71-77) into the other x* buffer, the velocity
+// update on the last iteration, and the particle's summed displacement since
+// the lists were built (returned).
+__device__ __forceinline__ float delta_finish(const SceneC& S, const Skin& K, int q, float4 xi, const float* a,
+                                              float4* __restrict__ XSn, int iter, bool last,
+                                              const float4* __restrict__ X, const float* __restrict__ RHO,
+                                              float4* __restrict__ VT) {
+  const float px = xi.x + a[0] * S.inv_rho0, py = xi.y + a[1] * S.inv_rho0, pz = xi.z + a[2] * S.inv_rho0;
+  XSn[q] = make_float4(px, py, pz, 0.0f);
+  if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+  float moved = sqrtf(sqn3(px - xi.x, py - xi.y, pz - xi.z));
+  moved = (iter == 0 ? 0.0f : K.cd[q]) + moved;
+  K.cd[q] = moved;
+  return moved;
+}
+
+// OM = (omega_i, |omega_i|); VX = v_i + xsph_c * c_i (v_i when XSPH is off).
+__device__ __forceinline__ void omega_finish(const SceneC& S, int q, float4 vi, const float* a,
+                                             float4* __restrict__ OM, float4* __restrict__ VX) {
+  float vx = vi.x, vy = vi.y, vz = vi.z;
+  if (S.vort_on) OM[q] = make_float4(a[3], a[4], a[5], sqrtf(sqn3(a[3], a[4], a[5])));
+  if (S.xsph_on) {
+    vx = vx + S.xsph_c * a[0];
+    vy = vy + S.xsph_c * a[1];
+    vz = vz + S.xsph_c * a[2];
+  }
+  VX[q] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// Vorticity force from eta (when on), then the commit with the domain clamp
+// (pbf_solver.cpp:102-126) into the new committed X / V.
+__device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi, float4 vx4, float4 oi, const float* a,
+                                              float4* __restrict__ X, float4* __restrict__ V) {
+  float vx = vx4.x, vy = vx4.y, vz = vx4.z;
+  if (S.vort_on) {
+    const float el = sqrtf(sqn3(a[0], a[1], a[2]));
+    if (el > 1e-9f) {
+      const float inv = 1.0f / el;
+      const float nx = a[0] * inv, ny = a[1] * inv, nz = a[2] * inv;
+      const float fx = S.vort_eps * (ny * oi.z - nz * oi.y);
+      const float fy = S.vort_eps * (nz * oi.x - nx * oi.z);
+      const float fz = S.vort_eps * (nx * oi.y - ny * oi.x);
+      vx = vx + S.dt * fx;
+      vy = vy + S.dt * fy;
+      vz = vz + S.dt * fz;
+    }
+  }
+  float px = xi.x, py = xi.y, pz = xi.z;
+  if (S.bounds) {
+    if (px < S.clo0) { px = S.clo0; vx = 0.0f; } else if (px > S.chi0) { px = S.chi0; vx = 0.0f; }
+    if (py < S.clo1) { py = S.clo1; vy = 0.0f; } else if (py > S.chi1) { py = S.chi1; vy = 0.0f; }
+    if (pz < S.clo2) { pz = S.clo2; vz = 0.0f; } else if (pz > S.chi2) { pz = S.chi2; vz = 0.0f; }
+  }
+  X[q] = make_float4(px, py, pz, 0.0f);
+  V[q] = make_float4(vx, vy, vz, 0.0f);
+}
+
+// ------------------------------------------------------------ dense particles
+// A particle whose skin list overflows (more than NPX_SKIN_CAP candidates
+// within the skin radius: a dense sheet or the surroundings of a stack) in
+// crowded cells sweeps thousands of candidates in every pass.  A lane per particle would
+// leave those sweeps to the few waves that hold such particles, each walking
+// its candidates one dependent fetch after another.  Instead the build pass
+// puts them on a list (heads only, exact FP32 path only), the per-lane
+// kernels skip them, and one wave per listed particle sweeps its candidates
+// 64 at a time: lane t fetches candidate t of the chunk and evaluates its
+// pair terms (a[] from zero, k = 1), the terms go to LDS, and lanes 0..NT-1
+// add term t of the in-range candidates in candidate order, k times for a
+// run of k, as the per-lane sweep does.  Sums, order and results are those
+// of the per-lane sweep bit for bit.
+
+// Particles whose skin list overflows and whose 27 cells hold more than
+// this many candidates go on the dense list; sparser overflows keep the
+// per-lane inline sweep.
+#define NPX_DENSE_MIN 1024
+
+// Skin-list count of a listed dense particle (other overflows count -1).
+#define NPX_CNT_DENSE (-2)
+
+// Whether slot q (a head of a scene with solver iterations, after this
+// step's build pass) is on the dense list.
+__device__ __forceinline__ bool is_dense(const SceneC& S, const Skin& K, int q) {
+  return K.dense && S.iters > 0 && K.cnt[q] == NPX_CNT_DENSE;
+}
+
+// Candidates in the 27 cells of window cell l (duplicates counted).
+__device__ __forceinline__ int candidate_total(int l, int wx, int wxy, const int* __restrict__ cm,
+                                               const int* __restrict__ cstart) {
+  int total = 0;
+#pragma unroll 1
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1], c1 = cm[base], c2 = cm[base + 1];
+    if (c1 == c0 + 1 && c2 == c0 + 2)
+      total += cstart[c0 + 3] - cstart[c0];
+    else
+      total += (cstart[c0 + 1] - cstart[c0]) + (cstart[c1 + 1] - cstart[c1]) + (cstart[c2 + 1] - cstart[c2]);
+  }
+  return total;
+}
+
+// Appends q to the dense list (one atomic per wave).
+__device__ __forceinline__ void dense_push(const Skin& K, int q) {
+  const unsigned long long m = __ballot(1);
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = (int)__builtin_ctzll(m);
+  const int rank = (int)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+  int base = 0;
+  if (lane == leader) base = atomicAdd(K.dn, (int)__builtin_popcountll(m));
+  base = __shfl(base, leader, 64);
+  K.dl[base + rank] = q;
+}
+
+// The candidates of one particle (window cell l, position xi) in the
+// reference's order, as visit_all walks them, swept by the whole 64-lane
+// block; lane i < NT returns accumulator i.
+template <int NT, class L, class P>
+__device__ __forceinline__ float wave_sweep(const Skin& K, bool rs, int l, int wx, int wxy, const int* __restrict__ cm,
+                                            const int* __restrict__ cstart, int shift, float4 xi, float cut, L&& ld,
+                                            const P& pair) {
+  __shared__ float tl[64 * NT];
+  __shared__ int kl[64];
+  const int t = (int)threadIdx.x;
+  float acc = 0.0f;
+  // one chunk: m <= 64 slots from c, or m runs from piece c (runs: iend is
+  // the range's end piece and e its end slot)
+  auto chunk = [&](int c, int m, bool runs, int iend, int e) {
+    const bool valid = t < m;
+    int j, k = 1;
+    if (runs) {
+      const int i = c + (valid ? t : 0);
+      j = K.ps[i];
+      k = ((i + 1 < iend) ? K.ps[i + 1] : e) - j;
+    } else {
+      j = c + (valid ? t : 0);
+    }
+    const auto d = ld(j);
+    const float4 xj = cand_pos(d);
+    const bool in = valid && dist2<false>(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) < cut;
+    float a[NT];
+#pragma unroll
+    for (int u = 0; u < NT; ++u) a[u] = 0.0f;
+    if (in) pair(j, d, 1, a);
+#pragma unroll
+    for (int u = 0; u < NT; ++u) tl[t * NT + u] = a[u];
+    kl[t] = k;
+    const unsigned long long mask = __ballot(in);
+    __syncthreads();
+    if (t < NT) {
+      for (unsigned long long mm = mask; mm; mm &= mm - 1ull) {
+        const int b = (int)__builtin_ctzll(mm);
+        add_k(acc, tl[b * NT + t], kl[b]);
+      }
+    }
+    __syncthreads();
+  };
+  auto range = [&](int b, int e, bool runs) {
+    if (b >= e) return;
+    if (runs) {
+      const int iend = K.pi[e - 1] + 1;
+      for (int c = K.pi[b]; c < iend; c += 64) chunk(c, min(64, iend - c), true, iend, e);
+    } else {
+      for (int c = b; c < e; c += 64) chunk(c, min(64, e - c), false, 0, e);
+    }
+  };
+  for (int r = 0; r < NPX_ROWS; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1];
+    const int c1 = cm[base];
+    const int c2 = cm[base + 1];
+    if (c1 == c0 + 1 && c2 == c0 + 2) {
+      range(cstart[c0] - shift, cstart[c0 + 3] - shift, rs && cls_runs(K, c0, 3));
+    } else {
+      for (int u = 0; u < 3; ++u) {
+        const int c = (u == 0) ? c0 : (u == 1) ? c1 : c2;
+        range(cstart[c] - shift, cstart[c + 1] - shift, rs && cls_runs(K, c, 1));
+      }
+    }
+  }
+  return acc;
+}
+
+// Scene data of dense-list particle q (uniform over the block).
+#define NPX_DENSE_PROLOGUE()                                 \
+  const int q = K.dl[w];                                     \
+  const int s = blk_scene[q / NPX_BLOCK];                    \
+  const SceneC& S = scenes[s];                               \
+  const int shift = cstart[S.cls_base] - S.seg_off;          \
+  const int wxy = S.wx * S.wy;                               \
+  const int* __restrict__ cm = clsmap + S.cls_base;          \
+  const int l = cell[q];                                     \
+  const bool rs = K.runs_s[s] != 0;
+
+// Broadcasts the NT accumulators of a wave_sweep to every lane.
+template <int NT>
+__device__ __forceinline__ void wave_sums(float acc, float* a) {
+#pragma unroll
+  for (int u = 0; u < NT; ++u) a[u] = __shfl(acc, u, 64);
+}
+
+// compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
+// (:144-179): rho_i sums m W over all in-range candidates (self and
+// duplicates included); for j != i, sum |grad_j|^2 and grad_i; lambda is 0
+// when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
+// iteration) builds the step's skin lists, and lists the particles whose
+// lists overflow for the dense kernels; later iterations test the lists.
+template <bool FAST, bool BUILD>
+__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(BUILD ? 4 : 6)))
+k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                 const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
+                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
+  NPX_NBR_PROLOGUE();
+  const bool rs = K.runs_s[s] != 0;
+  if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
+  if (!BUILD && !FAST && is_dense(S, K, q)) return;                  // the dense kernel's particle
+  const float4 xi = XS[q];
+  const DensityPair<FAST> P(S, q, xi);
+  const float cut = P.cut;
+  float a[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  auto ld = [&](int j) { return XS[j]; };
+  auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };
+  if (BUILD) {
+    Rows w;
+    int n = -1;
+    unsigned long long rel = 0;
+    if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
+      n = store_skin(w, K, q, &rel);
+    else
+      n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+    if (n > NPX_SKIN_CAP) n = -1;
+    if (n < 0 && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
+      K.cnt[q] = NPX_CNT_DENSE;
+      K.ecnt[q] = -1;
+      dense_push(K, q);
+      return;
+    }
+    K.cnt[q] = n;
+    if (n >= 0) {
+      K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
+    } else {
+      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
+    }
+  } else {
+    const float C = candidate_disp(K, l, S.wx, wxy, cm);
+    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, ld, body);
+  }
+  density_finish(S, q, a, XS, RHO, LAM);
+}
+
+// k_density_lambda for the dense list (iterations of the exact path).
+__global__ void __launch_bounds__(64)
+k_density_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
+                float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K) {
+  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+    NPX_DENSE_PROLOGUE();
+    if (iter >= S.iters) continue;
+    const float4 xi = XS[q];
+    const DensityPair<false> P(S, q, xi);
+    const float acc = wave_sweep<5>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut, [&](int j) { return XS[j]; }, P);
+    float a[5];
+    wave_sums<5>(acc, a);
+    if (threadIdx.x == 0) density_finish(S, q, a, XS, RHO, LAM);
+  }
 }
 
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
@@ -1261,53 +1644,16 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
   }
   float moved = 0.0f;
   const bool rs = K.runs_s[s] != 0;
-  if (active && !(rs && !K.rstart[q])) {  // run members take their head's values
+  // run members take their head's values; dense-list particles are the dense kernel's
+  if (active && !(rs && !K.rstart[q]) && !(!FAST && is_dense(S, K, q))) {
     const float4 xi = XS[q];
-    const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, m = S.mass;
-    const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
-    const bool fd = S.fdiv != 0;
-    const float wdq = S.wdq, negk = -S.corr_k;
-    const bool n4 = (S.corr_n == 4), scorr_on = (wdq > 1e-10f);
-    const float cut = FAST ? h2 : S.cut2;
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+    const DeltaPair<FAST> P(S, q, xi);
+    float a[3] = {0.0f, 0.0f, 0.0f};
     auto ld = [&](int j) { return XS[j]; };
-    auto body = [&](int j, float4 xj, int k) {  // k > 1 only without FAST
-      const float ls = xi.w + xj.w;
-      if (j == q) return;
-      const float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-      const float r_sq = dist2<FAST>(rx, ry, rz);
-      if (r_sq >= cut) return;  // spiky_grad is zero: the term adds exactly 0
-      bool in;
-      const float co = FAST ? spiky_coeff_fast(r_sq, h, h2, cs, &in) : spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
-      if (!in) return;
-      float sc = 0.0f;
-      if (ls < -1e-10f && scorr_on) {
-        float ratio = npx_div(poly6(r_sq, h2, c6), wdq, fd);
-        ratio = (1.0f < ratio) ? 1.0f : ratio;  // std::min(ratio, 1.0f)
-        float rp = ratio * ratio;
-        if (n4) rp = rp * rp;
-        sc = negk * rp;
-      }
-      const float coef = (ls + sc) * m;
-      if (FAST) {
-        const float cc = coef * co;
-        dx = __builtin_fmaf(cc, rx, dx);
-        dy = __builtin_fmaf(cc, ry, dy);
-        dz = __builtin_fmaf(cc, rz, dz);
-      } else {
-        add_k(dx, coef * (co * rx), k);
-        add_k(dy, coef * (co * ry), k);
-        add_k(dz, coef * (co * rz), k);
-      }
-    };
+    auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };  // k > 1 only without FAST
     // one run / two slots at a time on the inline sweep: 64 VGPRs
-    visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, cut, XS, ld, body);
-    const float px = xi.x + dx * S.inv_rho0, py = xi.y + dy * S.inv_rho0, pz = xi.z + dz * S.inv_rho0;
-    XSn[q] = make_float4(px, py, pz, 0.0f);
-    if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
-    moved = sqrtf(sqn3(px - xi.x, py - xi.y, pz - xi.z));
-    moved = (iter == 0 ? 0.0f : K.cd[q]) + moved;  // summed displacement since the lists were built
-    K.cd[q] = moved;
+    visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, P.cut, XS, ld, body);
+    moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
   }
   // Per-class maxima: slots are sorted by class, so the lanes of a class form
   // one run in the wave; a segmented max leaves the run's maximum in its first
@@ -1325,12 +1671,26 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
   if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
 }
 
-struct XV {
-  float4 x, v;
-};
-__device__ __forceinline__ void keep(const XV& d) {
-  npx_keep(d.x);
-  npx_keep(d.v);
+// k_delta for the dense list.
+__global__ void __launch_bounds__(64)
+k_delta_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+              float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
+              float4* __restrict__ VT, Skin K) {
+  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+    NPX_DENSE_PROLOGUE();
+    if (iter >= S.iters) continue;  // the pass-through is the per-lane kernel's
+    const bool last = (iter == max(S.iters, 1) - 1);
+    const float4 xi = XS[q];
+    const DeltaPair<false> P(S, q, xi);
+    const float acc = wave_sweep<3>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut, [&](int j) { return XS[j]; }, P);
+    float a[3];
+    wave_sums<3>(acc, a);
+    if (threadIdx.x == 0) {
+      const float moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
+      if (moved > 0.0f) atomicMax(K.cmax + cm[l], __float_as_uint(moved));
+    }
+  }
 }
 
 // apply_xsph_viscosity (pbf_solver.cpp:225-250) fused with the vorticity
@@ -1346,64 +1706,44 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
   if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
-  const float4 xi = XS[q];
   const float4 vi = VT[q];
   const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
-  float vx = vi.x, vy = vi.y, vz = vi.z;
+  float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (xs || vort) {
-    const float h = S.h, h2 = S.h2, c6 = S.c6, cs = S.cs, cut = S.cut2;
-    const float sp_lo = S.sp_lo, sp_hi = S.sp_hi;
-    const bool fd = S.fdiv != 0;
-    float cx = 0.0f, cy = 0.0f, cz = 0.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    if (is_dense(S, K, q)) return;  // the dense kernel's particle
+    const float4 xi = XS[q];
+    const OmegaPair P(S, q, xi, vi);
     auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
-    auto body = [&](int j, const XV& d, int k) {
-      if (j == q) return;
-      const float rx = d.x.x - xi.x, ry = d.x.y - xi.y, rz = d.x.z - xi.z;
-      const float r_sq = sqn3(rx, ry, rz);  // equals |x*_i - x*_j|^2 bit for bit
-      if (r_sq >= cut) return;              // both terms are exact zeros beyond cut2
-      const float4 vj = d.v;
-      const float ax = vj.x - vi.x, ay = vj.y - vi.y, az = vj.z - vi.z;
-      if (xs && r_sq < h2) {
-        const float wv = poly6(r_sq, h2, c6);
-        add_k(cx, (vj.w * ax) * wv, k);
-        add_k(cy, (vj.w * ay) * wv, k);
-        add_k(cz, (vj.w * az) * wv, k);
-      }
-      if (vort) {
-        bool in;
-        const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
-        if (!in) return;
-        const float gx = co * rx, gy = co * ry, gz = co * rz;
-        add_k(wx, vj.w * (ay * gz - az * gy), k);
-        add_k(wy, vj.w * (az * gx - ax * gz), k);
-        add_k(wz, vj.w * (ax * gy - ay * gx), k);
-      }
-    };
+    auto body = [&](int j, const XV& d, int k) { P(j, d, k, a); };
     if (S.iters > 0) {  // the skin lists exist once a density pass has run
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
       // the vorticity force pass reuses the in-range list
-      visit_test<false, 2, true>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, vort, ld, body);
+      visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, ld, body);
     } else {
       K.ecnt[q] = -1;
       visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
     }
-    if (vort) OM[q] = make_float4(wx, wy, wz, sqrtf(sqn3(wx, wy, wz)));
-    if (xs) {
-      vx = vx + S.xsph_c * cx;
-      vy = vy + S.xsph_c * cy;
-      vz = vz + S.xsph_c * cz;
-    }
   }
-  VX[q] = make_float4(vx, vy, vz, 0.0f);
+  omega_finish(S, q, vi, a, OM, VX);
 }
 
-struct XOW {
-  float4 x;
-  float vw, ow;
-};
-__device__ __forceinline__ void keep(const XOW& d) {
-  npx_keep(make_float4(d.x.x, d.x.y, d.x.z, d.vw));
-  __asm__ volatile("" ::"v"(d.ow));
+// k_omega_xsph for the dense list.
+__global__ void __launch_bounds__(64)
+k_omega_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+              const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K) {
+  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+    NPX_DENSE_PROLOGUE();
+    if (!S.xsph_on && !S.vort_on) continue;  // no neighbour sum: the per-lane kernel's
+    const float4 xi = XS[q];
+    const float4 vi = VT[q];
+    const OmegaPair P(S, q, xi, vi);
+    const float acc = wave_sweep<6>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut,
+                                    [&](int j) { return XV{XS[j], VT[j]}; }, P);
+    float a[6];
+    wave_sums<6>(acc, a);
+    if (threadIdx.x == 0) omega_finish(S, q, vi, a, OM, VX);
+  }
 }
 
 // Vorticity force (N = eta / |eta|, eta_i = sum_{j!=i} (m/rho~_j)
@@ -1419,47 +1759,37 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   const bool rs = K.runs_s[s] != 0;
   if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
   const float4 xi = XS[q];
-  const float4 vx4 = VX[q];
-  float vx = vx4.x, vy = vx4.y, vz = vx4.z;
+  float a[3] = {0.0f, 0.0f, 0.0f};
+  float4 oi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (S.vort_on) {
-    const float h = S.h, cs = S.cs, sp_lo = S.sp_lo, sp_hi = S.sp_hi;
-    const bool fd = S.fdiv != 0;
-    const float4 oi = OM[q];
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    if (is_dense(S, K, q)) return;  // the dense kernel's particle
+    oi = OM[q];
+    const VortPair P(S, q, xi, oi.w);
     auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
-    auto body = [&](int j, const XOW& d, int k) {
-      if (j == q) return;
-      const float rx = xi.x - d.x.x, ry = xi.y - d.x.y, rz = xi.z - d.x.z;
-      const float r_sq = sqn3(rx, ry, rz);
-      bool in;
-      const float co = spiky_coeff_sq(r_sq, sp_lo, sp_hi, h, cs, fd, &in);
-      if (!in) return;
-      const float sc = d.vw * (d.ow - oi.w);
-      add_k(ex, sc * (co * rx), k);
-      add_k(ey, sc * (co * ry), k);
-      add_k(ez, sc * (co * rz), k);
-    };
+    auto body = [&](int j, const XOW& d, int k) { P(j, d, k, a); };
     visit_reuse<false, 4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, S.cut2, XS, ld, body);  // list of the XSPH / curl pass
-    const float el = sqrtf(sqn3(ex, ey, ez));
-    if (el > 1e-9f) {
-      const float inv = 1.0f / el;
-      const float nx = ex * inv, ny = ey * inv, nz = ez * inv;
-      const float fx = S.vort_eps * (ny * oi.z - nz * oi.y);
-      const float fy = S.vort_eps * (nz * oi.x - nx * oi.z);
-      const float fz = S.vort_eps * (nx * oi.y - ny * oi.x);
-      vx = vx + S.dt * fx;
-      vy = vy + S.dt * fy;
-      vz = vz + S.dt * fz;
-    }
   }
-  float px = xi.x, py = xi.y, pz = xi.z;
-  if (S.bounds) {
-    if (px < S.clo0) { px = S.clo0; vx = 0.0f; } else if (px > S.chi0) { px = S.chi0; vx = 0.0f; }
-    if (py < S.clo1) { py = S.clo1; vy = 0.0f; } else if (py > S.chi1) { py = S.chi1; vy = 0.0f; }
-    if (pz < S.clo2) { pz = S.clo2; vz = 0.0f; } else if (pz > S.chi2) { pz = S.chi2; vz = 0.0f; }
+  commit_finish(S, q, xi, VX[q], oi, a, X, V);
+}
+
+// k_vort_commit for the dense list (scenes with vorticity).
+__global__ void __launch_bounds__(64)
+k_commit_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
+               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
+               float4* __restrict__ X, float4* __restrict__ V, Skin K) {
+  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+    NPX_DENSE_PROLOGUE();
+    if (!S.vort_on) continue;  // commit only: the per-lane kernel's
+    const float4 xi = XS[q];
+    const float4 oi = OM[q];
+    const VortPair P(S, q, xi, oi.w);
+    const float acc = wave_sweep<3>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, S.cut2,
+                                    [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; }, P);
+    float a[3];
+    wave_sums<3>(acc, a);
+    if (threadIdx.x == 0) commit_finish(S, q, xi, VX[q], oi, a, X, V);
   }
-  X[q] = make_float4(px, py, pz, 0.0f);
-  V[q] = make_float4(vx, vy, vz, 0.0f);
 }
 
 // End of a step: every run member takes its head's committed state, density
