@@ -810,9 +810,9 @@ int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
                           NPX_WINDOW_MARGIN, wlo, wdim);
   const long long cells = (long long)wdim[0] * wdim[1] * wdim[2];
   const long long slots = (long long)cdiv(std::max(n, 1), NPX_BLOCK) * NPX_BLOCK;
-  // 9 float4 + 5 float + 13 int + 1 byte arrays and the skin and in-range
+  // 9 float4 + 5 float + 14 int + 1 byte arrays and the skin and in-range
   // lists per slot; class map, start, displacement maximum and run flag per cell
-  return slots * (9 * 16 + 5 * 4 + 13 * 4 + 1 + 8 * NPX_SKIN_CAP) + cells * 16;
+  return slots * (9 * 16 + 5 * 4 + 14 * 4 + 1 + 8 * NPX_SKIN_CAP) + cells * 16;
 }
 
 int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {
