@@ -206,3 +206,42 @@ def test_batch_mixed_iterations_and_stacks(gpu):
     np.testing.assert_array_equal(zp, op)
     np.testing.assert_array_equal(zv, ov)
     b.close()
+
+
+def test_batch_crowded_scenes_equal_alone(gpu):
+    """Crowded wall sheets in a batch beside an ordinary scene, with 2 and 4
+    solver iterations: the dense list spans scenes, and the dense kernels
+    skip scenes past their iteration count.  Every scene matches itself
+    alone bitwise."""
+    rng = np.random.default_rng(61)
+
+    def sheet(n, iters, vort):
+        pos = np.empty((n, 3), np.float32)
+        pos[:, 0] = np.float32(0.025)
+        pos[:, 1:] = (0.2 + rng.uniform(0.0, 0.3, (n, 2))).astype(np.float32)
+        vel = rng.uniform(-0.3, 0.3, (n, 3)).astype(np.float32)
+        prm = scenes.PBFParams(corr_k=0.1, corr_delta_q=0.2, vorticity_epsilon=vort, solver_iterations=iters,
+                               use_domain_bounds=True, domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+        return scenes.Scene("sheet", pos, vel, prm, float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0),
+                            prm.particle_mass(0.025), 3, 61)
+
+    plain = scenes.batched_scene(9, 8, 7, 0.4, seed=311, steps=4)
+    scs = [sheet(2500, 2, 0.0), plain, sheet(2000, 4, 1e-3)]
+    _say("case: batch with crowded sheets, sizes", [s.n for s in scs])
+    slots = sum(((s.n + 255) // 256) * 256 for s in scs)
+    cells = sum(capi.scene_bytes(s.params, 1) for s in scs)
+    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=4)
+    ids = [b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities) for s in scs]
+    b.step(3)
+    for s, sid in zip(scs, ids):
+        bp, bv, br, bl = b.get(sid)
+        c = capi.Context(s.params, capacity=s.n)
+        c.set_particles(s.positions, s.velocities)
+        c.step(s.dt, s.gravity, s.mass, 3)
+        ap, av, ar, al = c.get_particles()
+        c.close()
+        np.testing.assert_array_equal(bp, ap)
+        np.testing.assert_array_equal(bv, av)
+        np.testing.assert_array_equal(br, ar)
+        np.testing.assert_array_equal(bl, al)
+    b.close()

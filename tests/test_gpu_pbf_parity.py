@@ -275,6 +275,52 @@ def test_stacked_identical_particles_bitexact(gpu):
     np.testing.assert_array_equal(gl, ol)
 
 
+def _sheet(n, seed):
+    """n particles pressed onto the wall x = r of a unit box (the clamp's
+    sheet) over 0.3 m x 0.3 m: about 330 per cell, 3000 in the 27 cells of
+    each, so every head is a crowded particle (skin list overflow, more than
+    1024 candidates) and runs on the per-wave dense kernels."""
+    rng = np.random.default_rng(seed)
+    pos = np.empty((n, 3), np.float32)
+    pos[:, 0] = np.float32(0.025)
+    pos[:, 1:] = (0.2 + rng.uniform(0.0, 0.3, (n, 2))).astype(np.float32)
+    vel = rng.uniform(-0.3, 0.3, (n, 3)).astype(np.float32)
+    vel[:, 0] = np.abs(vel[:, 0]) * -1.0  # into the wall: the clamp keeps the sheet
+    return pos, vel
+
+
+def _crowded_count(pos, h=0.1):
+    c = np.floor(pos.astype(np.float64) / h).astype(np.int64)
+    keys, inv, cnt = np.unique(c, axis=0, return_inverse=True, return_counts=True)
+    occ = {tuple(k): v for k, v in zip(keys, cnt)}
+    tot = np.array([sum(occ.get((k[0] + dx, k[1] + dy, k[2] + dz), 0) for dx in (-1, 0, 1) for dy in (-1, 0, 1)
+                        for dz in (-1, 0, 1)) for k in keys])
+    return int((tot[inv.ravel()] > 1024).sum())
+
+
+@pytest.mark.parametrize("corr_k,dq", [(0.001, 0.3), (0.1, 0.2)])
+def test_crowded_sheet_dense_path_bitexact(gpu, corr_k, dq):
+    """A wall sheet of 3000 particles: the dense kernels (one wave per crowded
+    particle, terms added in candidate order from LDS) must give the oracle's
+    state bit for bit, with vorticity on so all four passes run dense."""
+    pos, vel = _sheet(3000, 51)
+    assert _crowded_count(pos) == pos.shape[0]  # every particle is crowded at the start
+    prm = scenes.PBFParams(corr_k=corr_k, corr_delta_q=dq, vorticity_epsilon=1e-3, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.025)
+    _say("case: crowded wall sheet n=3000, k=", corr_k, ", 3 steps")
+    c = capi.Context(prm, capacity=pos.shape[0])
+    c.set_particles(pos, vel)
+    c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
 def test_configured_collapse_20_steps_bitexact(gpu):
     """The configured s_corr (k = 0.1, dq = 0.2h) over 20 steps, long enough
     for the domain clamp to stack particles in the box corners (about 2800 on
