@@ -294,45 +294,80 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
 // Pass 3b: final slot = class start + rank of the particle's original index
 // inside its class, so each bucket lists particles in ascending index like
 // the reference's push_back order (neighbor_search.cpp:18-21).  Indices are
-// unique within a scene.  Small classes count directly; for a larger class
+// unique within a scene.  Small classes count directly.  For a larger class
 // the rank is the slot's position in its own sorted block segment plus, for
-// every other block the class spans, a binary search in that block's sorted
-// segment: about (count / 256) log2(256) reads per particle instead of count.
+// every other block the class spans, the count of smaller indices in that
+// block's sorted segment.  The workgroup stages the blocks its large classes
+// span in LDS, eight at a time (coalesced loads), and each slot binary
+// searches its class's segment of each staged block there.
+#define NPX_RANK_STAGE 8
+
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
        int* __restrict__ perm) {
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;
+  __shared__ int sk[NPX_RANK_STAGE * NPX_BLOCK];
+  __shared__ int span_lo, span_hi;
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
   const int t = npx_slot();
-  if (t >= S.seg_off + S.n) return;
+  const bool active = t < S.seg_off + S.n;
   const int shift = cstart[S.cls_base] - S.seg_off;
-  const int p = tperm[t];
-  const int c = cls_of[p];
-  const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
-  const int my = tkey[t];
-  int r = 0;
-  if (e - b <= NPX_RANK_DIRECT) {
-    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
-  } else {
-    const int own = t / NPX_BLOCK;
-    r = t - max(b, own * NPX_BLOCK);
-    for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
-      if (blk == own) continue;
-      const int lo = max(b, blk * NPX_BLOCK);
-      int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
-      while (lb < ub) {
-        const int mid = (lb + ub) >> 1;
-        if (tkey[mid] < my)
-          lb = mid + 1;
-        else
-          ub = mid;
-      }
-      r += lb - lo;
-    }
+  int p = 0, b = 0, e = 0, my = 0;
+  if (active) {
+    p = tperm[t];
+    const int c = cls_of[p];
+    b = cstart[c] - shift;
+    e = cstart[c + 1] - shift;
+    my = tkey[t];
   }
-  perm[b + r] = p;
+  const bool big = active && e - b > NPX_RANK_DIRECT;
+  if (active && !big) {
+    int r = 0;
+    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+    perm[b + r] = p;
+  }
+  if (threadIdx.x == 0) {
+    span_lo = INT_MAX;
+    span_hi = -1;
+  }
+  if (!__syncthreads_or(big)) return;  // uniform per block
+  const int own = (int)blockIdx.x;
+  const int blo = big ? b / NPX_BLOCK : INT_MAX, bhi = big ? (e - 1) / NPX_BLOCK : -1;
+  if (big) {
+    atomicMin(&span_lo, blo);
+    atomicMax(&span_hi, bhi);
+  }
+  __syncthreads();
+  const int lo = span_lo, hi = span_hi;
+  int r = big ? t - max(b, own * NPX_BLOCK) : 0;
+  for (int B0 = lo; B0 <= hi; B0 += NPX_RANK_STAGE) {
+#pragma unroll
+    for (int u = 0; u < NPX_RANK_STAGE; ++u)
+      if (B0 + u <= hi) sk[u * NPX_BLOCK + (int)threadIdx.x] = tkey[(B0 + u) * NPX_BLOCK + (int)threadIdx.x];
+    __syncthreads();
+    if (big) {
+      for (int u = 0; u < NPX_RANK_STAGE; ++u) {
+        const int B = B0 + u;
+        if (B > hi || B < blo || B > bhi || B == own) continue;
+        const int base = B * NPX_BLOCK;
+        const int l0 = max(b, base) - base;
+        int lb = l0, ub = min(e, base + NPX_BLOCK) - base;
+        const int* __restrict__ seg = sk + u * NPX_BLOCK;
+        while (lb < ub) {
+          const int mid = (lb + ub) >> 1;
+          if (seg[mid] < my)
+            lb = mid + 1;
+          else
+            ub = mid;
+        }
+        r += lb - l0;
+      }
+    }
+    __syncthreads();
+  }
+  if (big) perm[b + r] = p;
 }
 
 // Pass 4: reorder the state into the new slot order and form x*; also marks
