@@ -1968,40 +1968,79 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
   }
 }
 
-// Rank inside the class by key (own block position plus binary searches in
-// the class's other sorted blocks, as k_rank) and write the sorted payload
-// copy.
-__global__ void k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
-                           const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
-                           int* __restrict__ perm, float4* __restrict__ PS) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cstart[ncls]) return;
-  const int i = tperm[t];
-  const int c = cls_of[i];
-  const int b = cstart[c], e = cstart[c + 1];
-  const int my = tkey[t];
-  int r = 0;
-  if (e - b <= NPX_RANK_DIRECT) {
-    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
-  } else {
-    const int own = t / NPX_BLOCK;
-    r = t - max(b, own * NPX_BLOCK);
-    for (int blk = b / NPX_BLOCK; blk <= (e - 1) / NPX_BLOCK; ++blk) {
-      if (blk == own) continue;
-      const int lo = max(b, blk * NPX_BLOCK);
-      int lb = lo, ub = min(e, (blk + 1) * NPX_BLOCK);
-      while (lb < ub) {
-        const int mid = (lb + ub) >> 1;
-        if (tkey[mid] < my)
-          lb = mid + 1;
-        else
-          ub = mid;
-      }
-      r += lb - lo;
-    }
+// Rank inside the class by key (own block position plus, for a class larger
+// than NPX_RANK_DIRECT, binary searches in the class's other sorted blocks
+// staged in LDS, as k_rank) and write the sorted payload copy.  256-thread
+// blocks.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
+           const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
+           int* __restrict__ perm, float4* __restrict__ PS) {
+  __shared__ int sk[NPX_RANK_STAGE * NPX_BLOCK];
+  __shared__ int span_lo, span_hi;
+  const int total = cstart[ncls];
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const bool active = t < total;
+  int i = 0, b = 0, e = 0, my = 0;
+  if (active) {
+    i = tperm[t];
+    const int c = cls_of[i];
+    b = cstart[c];
+    e = cstart[c + 1];
+    my = tkey[t];
   }
-  perm[b + r] = i;
-  PS[b + r] = P[i];
+  const bool big = active && e - b > NPX_RANK_DIRECT;
+  if (active && !big) {
+    int r = 0;
+    for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
+    perm[b + r] = i;
+    PS[b + r] = P[i];
+  }
+  if (threadIdx.x == 0) {
+    span_lo = INT_MAX;
+    span_hi = -1;
+  }
+  if (!__syncthreads_or(big)) return;  // uniform per block
+  const int own = (int)blockIdx.x;
+  const int blo = big ? b / NPX_BLOCK : INT_MAX, bhi = big ? (e - 1) / NPX_BLOCK : -1;
+  if (big) {
+    atomicMin(&span_lo, blo);
+    atomicMax(&span_hi, bhi);
+  }
+  __syncthreads();
+  const int lo = span_lo, hi = span_hi;
+  int r = big ? t - max(b, own * NPX_BLOCK) : 0;
+  for (int B0 = lo; B0 <= hi; B0 += NPX_RANK_STAGE) {
+#pragma unroll
+    for (int u = 0; u < NPX_RANK_STAGE; ++u) {
+      const int x = (B0 + u) * NPX_BLOCK + (int)threadIdx.x;
+      if (B0 + u <= hi) sk[u * NPX_BLOCK + (int)threadIdx.x] = (x < total) ? tkey[x] : INT_MAX;
+    }
+    __syncthreads();
+    if (big) {
+      for (int u = 0; u < NPX_RANK_STAGE; ++u) {
+        const int B = B0 + u;
+        if (B > hi || B < blo || B > bhi || B == own) continue;
+        const int base = B * NPX_BLOCK;
+        const int l0 = max(b, base) - base;
+        int lb = l0, ub = min(e, base + NPX_BLOCK) - base;
+        const int* __restrict__ seg = sk + u * NPX_BLOCK;
+        while (lb < ub) {
+          const int mid = (lb + ub) >> 1;
+          if (seg[mid] < my)
+            lb = mid + 1;
+          else
+            ub = mid;
+        }
+        r += lb - l0;
+      }
+    }
+    __syncthreads();
+  }
+  if (big) {
+    perm[b + r] = i;
+    PS[b + r] = P[i];
+  }
 }
 
 // ------------------------------------------------------ boundary sampling
