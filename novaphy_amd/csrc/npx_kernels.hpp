@@ -654,11 +654,14 @@ __global__ void __launch_bounds__(1024) k_block_plan(const float* __restrict__ c
 // list entries with the cheap distance check, runs the SPH arithmetic over
 // the in-range entries only and writes them as a compacted list; the pass
 // that reuses the same positions (delta-p, vorticity force) follows that
-// list, or sweeps inline when the testing pass did (in-range count -1).
-// Skipped entries are exact zero terms of the reference's sums, so the
-// results are the reference's bit for bit, and every lane of a wave works on
-// a real pair.  A particle with more than NPX_SKIN_CAP skin candidates, or
-// one whose bound fails, sweeps its candidates inline.  Lists are stored per
+// list, or sweeps inline when the testing pass found more in-range
+// candidates than a list holds (in-range count -1).  Skipped entries are
+// exact zero terms of the reference's sums, so the results are the
+// reference's bit for bit, and every lane of a wave works on a real pair.  A
+// particle with more than NPX_SKIN_CAP skin candidates, or one whose bound
+// fails, sweeps its candidates inline (an inline sweep of a testing pass
+// still records the in-range list when it fits), and a crowded one with an
+// overflowing list goes to the per-wave dense kernels.  Lists are stored per
 // 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64, so entry k
 // of all lanes is one coalesced access (the compacted list is written entry
 // by entry in lockstep across the wave, so its stores are coalesced too).
@@ -669,7 +672,7 @@ __global__ void __launch_bounds__(1024) k_block_plan(const float* __restrict__ c
 
 struct Skin {
   int* __restrict__ idx;     // [slots / 64][NPX_SKIN_CAP][64] skin list
-  int* __restrict__ cnt;     // [slots]; -1: sweep inline
+  int* __restrict__ cnt;     // [slots]; -1: sweep inline; NPX_CNT_DENSE: on the dense list
   int* __restrict__ ex;      // same layout: in-range list of the current testing pass
   int* __restrict__ ecnt;    // [slots]; -1: the testing pass swept inline
   float* __restrict__ cd;      // [slots] summed displacement since the lists were built
