@@ -165,7 +165,7 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
     NPX_TRY(c->launch("coupling", k_coupling_particles, cdiv(c->nb, 256), 256, W, h, h * h, cs, H.prm.rest_density,
                       H.mass, (const float4*)c->BP.p, (const int*)c->BODY.p, c->nb, (const uint8_t*)c->STATIC.p,
                       nbodies, cm, (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->FS.p, c->FB.p));
-    NPX_TRY(c->launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)c->BODY_PTR.p,
+    NPX_TRY(c->launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)c->BODY_PTR.p,
                       (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
                       (const float*)c->POSE.p, c->OUT6.p));
     NPX_TRY(c->read_bodies(out.data(), c->OUT6.p, nbodies));

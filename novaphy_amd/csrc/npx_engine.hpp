@@ -415,7 +415,7 @@ struct Ctx {
                    (const int*)BBODY.p, bb_n, (const uint8_t*)BSTAT.p, nbodies, (const int*)clsmap.p,
                    (const int*)fsort.cst.p, (const float4*)BFS.p, BFB.p));
     if (nbodies > 0) {
-      NPX_TRY(launch("body_reduce", k_body_reduce, cdiv(nbodies, 64), 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
+      NPX_TRY(launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
                      nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p, BOUT6.p));
       NPX_TRY(read_bodies(out6, BOUT6.p, nbodies));
     }

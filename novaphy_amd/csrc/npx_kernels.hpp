@@ -1898,21 +1898,37 @@ k_scene_health(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sc
 // SpatialHashGrid in the reference (fluid_world.cpp:106-107, 134-135,
 // boundary.cpp:162-163).  Points outside the window interior are left out
 // (cls = -1): no point inside the window can have them within h.
+// One atomic per distinct class of a wave, as in k_predict_bin: the fluid's
+// committed positions arrive in class order and stacks of particles on one
+// point would otherwise serialise on one counter.
 __global__ void k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const int* __restrict__ clsmap,
                           int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = P[i];
-  int lx = cell_coord(p.x, W.inv_h) - W.wlo0;
-  int ly = cell_coord(p.y, W.inv_h) - W.wlo1;
-  int lz = cell_coord(p.z, W.inv_h) - W.wlo2;
-  if (lx < 1 || ly < 1 || lz < 1 || lx > W.wx - 2 || ly > W.wy - 2 || lz > W.wz - 2) {
-    cls_of[i] = -1;
-    return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = -1;
+  if (i < n) {
+    const float4 p = P[i];
+    const int lx = cell_coord(p.x, W.inv_h) - W.wlo0;
+    const int ly = cell_coord(p.y, W.inv_h) - W.wlo1;
+    const int lz = cell_coord(p.z, W.inv_h) - W.wlo2;
+    if (!(lx < 1 || ly < 1 || lz < 1 || lx > W.wx - 2 || ly > W.wy - 2 || lz > W.wz - 2))
+      c = clsmap[lx + W.wx * (ly + W.wy * lz)];
+    cls_of[i] = c;
   }
-  const int c = clsmap[lx + W.wx * (ly + W.wy * lz)];
-  cls_of[i] = c;
-  off_of[i] = atomicAdd(&ccount[c], 1);
+  const bool active = c >= 0;
+  const int lane = (int)(threadIdx.x & 63);
+  unsigned long long todo = __ballot(active);
+  int off = 0;
+  while (todo) {
+    const int leader = (int)__builtin_ctzll(todo);
+    const int cl = __shfl(c, leader, 64);
+    const unsigned long long same = __ballot(active && c == cl) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&ccount[cl], (int)__popcll(same));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) off = base + (int)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  if (active) off_of[i] = off;
 }
 
 __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* __restrict__ cls_of,
@@ -2249,36 +2265,58 @@ __global__ void k_coupling_particles(GridWin W, float h, float h2, float cs, flo
 }
 
 // Per-body sums in boundary-particle order (World::apply_force / apply_torque,
-// world.cpp:189-216): one thread per body walks its particles ascending, so
+// world.cpp:189-216): one 64-lane block per body fetches 64 of its particles
+// at a time, each lane forming its particle's force and torque terms, and the
+// terms are added in ascending particle order (broadcast lane by lane), so
 // the float accumulation order is the reference's and the result is
 // deterministic.
-__global__ void k_body_reduce(const int* __restrict__ body_ptr, const int* __restrict__ body_idx, int nbodies,
-                              const float4* __restrict__ FB, const float4* __restrict__ BP,
-                              const float* __restrict__ pose7, float* __restrict__ out6) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nbodies) return;
+__global__ void __launch_bounds__(64)
+k_body_reduce(const int* __restrict__ body_ptr, const int* __restrict__ body_idx, int nbodies,
+              const float4* __restrict__ FB, const float4* __restrict__ BP, const float* __restrict__ pose7,
+              float* __restrict__ out6) {
+  const int k = (int)blockIdx.x;
+  if (k >= nbodies) return;  // uniform per block
+  const int t = (int)threadIdx.x;
   const float cxm = pose7[7 * k], cym = pose7[7 * k + 1], czm = pose7[7 * k + 2];
   float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
-  for (int u = body_ptr[k]; u < body_ptr[k + 1]; ++u) {
-    const int b = body_idx[u];
-    const float4 f = FB[b];
-    if (sqn3(f.x, f.y, f.z) > 1e-20f) {
-      Fx += f.x;
-      Fy += f.y;
-      Fz += f.z;
-      const float4 xb = BP[b];
-      const float ax = xb.x - cxm, ay = xb.y - cym, az = xb.z - czm;
-      Tx += ay * f.z - az * f.y;
-      Ty += az * f.x - ax * f.z;
-      Tz += ax * f.y - ay * f.x;
+  const int end = body_ptr[k + 1];
+  for (int u0 = body_ptr[k]; u0 < end; u0 += 64) {
+    const int u = u0 + t;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
+    bool use = false;
+    if (u < end) {
+      const int b = body_idx[u];
+      const float4 f = FB[b];
+      use = sqn3(f.x, f.y, f.z) > 1e-20f;
+      if (use) {
+        const float4 xb = BP[b];
+        const float ax = xb.x - cxm, ay = xb.y - cym, az = xb.z - czm;
+        fx = f.x;
+        fy = f.y;
+        fz = f.z;
+        tx = ay * f.z - az * f.y;
+        ty = az * f.x - ax * f.z;
+        tz = ax * f.y - ay * f.x;
+      }
+    }
+    for (unsigned long long m = __ballot(use); m; m &= m - 1ull) {
+      const int l = (int)__builtin_ctzll(m);
+      Fx += __shfl(fx, l, 64);
+      Fy += __shfl(fy, l, 64);
+      Fz += __shfl(fz, l, 64);
+      Tx += __shfl(tx, l, 64);
+      Ty += __shfl(ty, l, 64);
+      Tz += __shfl(tz, l, 64);
     }
   }
-  out6[6 * k + 0] = Fx;
-  out6[6 * k + 1] = Fy;
-  out6[6 * k + 2] = Fz;
-  out6[6 * k + 3] = Tx;
-  out6[6 * k + 4] = Ty;
-  out6[6 * k + 5] = Tz;
+  if (t == 0) {
+    out6[6 * k + 0] = Fx;
+    out6[6 * k + 1] = Fy;
+    out6[6 * k + 2] = Fz;
+    out6[6 * k + 3] = Tx;
+    out6[6 * k + 4] = Ty;
+    out6[6 * k + 5] = Tz;
+  }
 }
 
 // ------------------------------------------------- batched Akinci coupling
