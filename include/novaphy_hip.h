@@ -220,7 +220,7 @@ int npx_batch_coupling(npx_ctx* ctx, const float* pose7, int32_t nbodies, float*
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
 
 /* Launch tunables of a context (any context kind): block size of the
- * neighbour passes (64, 128 or 256; default 256), the XCD-aware block order
+ * neighbour passes (64, 128 or 256; default 128), the XCD-aware block order
  * (1, the default: each of the 8 XCDs sweeps one contiguous run of
  * particles, keeping shared neighbour rows in its own L2; 0: hardware
  * round-robin order) and the skin of the per-step candidate lists as a

@@ -230,7 +230,7 @@ struct Ctx {
   int hw_blocks = 0;
   bool table_dirty = true;
   // tunables (results are bitwise independent of them)
-  int nbr_block = 256;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
+  int nbr_block = 128;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
   int xcd_remap = 1;    // 1: XCD-aware block order in the neighbour passes
   float skin_frac = 0.1f;  // skin radius of the per-step candidate lists, as a fraction of h
   std::vector<int> h_blk_scene;
