@@ -294,6 +294,8 @@ void destroy(Ctx* c) {
   c->PI.release();
   c->PS.release();
   c->BOFF.release();
+  c->DL.release();
+  c->DN.release();
   for (auto* ps : {&c->bsort, &c->fsort}) {
     ps->cst.release();
     ps->cls.release();
