@@ -480,8 +480,7 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"gather", "fluid.pbf.neighbor_query"},
             {"pieces", "fluid.pbf.neighbor_query"},
             {"clear_cmax", "fluid.pbf.neighbor_query"},
-            {"block_cost", "fluid.pbf.neighbor_query"},
-            {"block_plan", "fluid.pbf.neighbor_query"},
+            {"dense_reset", "fluid.pbf.neighbor_query"},
             {"density_lambda_build", "fluid.pbf.iteration.compute_density"},
             {"density_lambda", "fluid.pbf.iteration.compute_density"},
             {"density_lambda_build_dense", "fluid.pbf.iteration.compute_density"},

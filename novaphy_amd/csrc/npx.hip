@@ -282,10 +282,10 @@ void destroy(Ctx* c) {
   npx::DBuf<float4>* f4[] = {&c->X, &c->X2, &c->V, &c->V2, &c->XS0, &c->XS1, &c->VT, &c->OM, &c->VX,
                              &c->BL, &c->BP, &c->BPS, &c->FP, &c->FS, &c->FB};
   for (auto* b : f4) b->release();
-  npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->CDB, &c->BCOST, &c->PSI, &c->POSE, &c->OUT6};
+  npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->CDB, &c->PSI, &c->POSE, &c->OUT6};
   for (auto* b : f1) b->release();
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
-                          &c->PERM, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->BORDER, &c->BMODE, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->PERM, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
                           &c->ATKEY, &c->APERM, &c->aux_clsmap};
   for (auto* b : i1) b->release();

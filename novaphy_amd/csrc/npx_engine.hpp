@@ -242,8 +242,6 @@ struct Ctx {
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
-  DBuf<float> BCOST;         // per 256-slot block: estimated neighbour-pass cost
-  DBuf<int> BORDER, BMODE;   // block order of the neighbour passes and its mode
   DBuf<int> CMAX;   // per class: float bits of the largest summed displacement; then per class a run flag
   DBuf<uint8_t> RSTART;  // per slot: 1 if the slot starts a run of identical state
   DBuf<int> PI, PS;      // per slot: its run; per run: its first slot
@@ -591,9 +589,6 @@ struct Ctx {
     // +4 entries of -1: neighbour passes with blocks larger than a segment
     // may run past the last segment
     NPX_HIP(d_blk_scene.ensure(n / NPX_BLOCK + 4));
-    NPX_HIP(BCOST.ensure(n / NPX_BLOCK + 4));
-    NPX_HIP(BORDER.ensure(n / NPX_BLOCK + 4));
-    NPX_HIP(BMODE.ensure(1));
     slot_cap = cap;
     h_blk_scene.assign(slot_cap / NPX_BLOCK + 4, -1);
     return NPX_OK;
@@ -910,10 +905,8 @@ struct Ctx {
                  (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1};
     // one 64-lane block per dense particle, at most this many at a time
     const int dense_blocks = std::max(1, std::min(8192, hw_blocks * (NPX_BLOCK / 64)));
-    NPX_TRY(launch("block_cost", k_block_cost, cdiv(hw_blocks, 256), 256, sc, bs, (const int*)clsmap.p,
-                   (const int*)cstart.p, (const int*)CELL.p, (const int*)BOFF.p, hw_blocks, BCOST.p));
-    NPX_TRY(launch("block_plan", k_block_plan, 1, 1024, (const float*)BCOST.p, hw_blocks, BORDER.p, BMODE.p, DN.p));
-    const BlockPlan rm{BORDER.p, BMODE.p, xcd_remap};
+    if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 64, DN.p));
+    const BlockPlan rm{xcd_remap};
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
@@ -985,7 +978,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, BCOST.p, BORDER.p, BMODE.p, DL.p, DN.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p};
   }
 
   void drop_graphs() {

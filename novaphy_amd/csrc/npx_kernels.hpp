@@ -522,109 +522,25 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 
 // Block order of the neighbour passes.  The hardware hands workgroups to the
 // 8 XCDs round robin (workgroup i to XCD i mod 8), and each XCD has its own
-// L2.  Two orders are used, chosen per step by k_block_plan:
-//  * XCD order (mode 0, with `remap`): launch block i processes logical block
-//      x * (G / 8) + min(x, G mod 8) + i / 8,   x = i mod 8,
-//    a bijection of [0, G) that gives XCD x one contiguous run of slots, so
-//    the candidate rows its blocks share stay in its L2;
-//  * cost order (mode 1): 256-slot blocks heaviest first, so the blocks of
-//    large clusters spread over all XCDs and start before the light ones
-//    (used when the XCD order would load the XCDs unevenly).
-// Results do not depend on the order: no pass reads what another block of
-// the same launch writes.
+// L2.  With `remap` (the XCD order) launch block i processes logical block
+//     x * (G / 8) + min(x, G mod 8) + i / 8,   x = i mod 8,
+// a bijection of [0, G) that gives XCD x one contiguous run of slots, so the
+// candidate rows its blocks share stay in its L2.  Results do not depend on
+// the order: no pass reads what another block of the same launch writes.
 struct BlockPlan {
-  const int* __restrict__ order;  // [256-slot blocks] cost order
-  const int* __restrict__ mode;   // 1: cost order
-  int remap;                      // 1: XCD order when the cost order is not used
+  int remap;  // 1: XCD order; 0: hardware order
 };
 
 __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
   const int i = (int)blockIdx.x;
-  if (*bp.mode) {
-    const int k = NPX_BLOCK / (int)blockDim.x;  // launch blocks per 256-slot block
-    return bp.order[i / k] * k + (i % k);
-  }
   if (!bp.remap) return i;
   const int g = (int)gridDim.x, x = i & 7;
   return x * (g >> 3) + min(x, g & 7) + (i >> 3);
 }
 
-// Estimated cost of each 256-slot block: candidates of its first particle
-// (the populations of the classes of its 27 cells) times its count of run
-// heads (the particles that compute).
-__global__ void k_block_cost(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
-                             const int* __restrict__ clsmap, const int* __restrict__ cstart,
-                             const int* __restrict__ cell, const int* __restrict__ boff, int nb,
-                             float* __restrict__ cost) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const int s = blk_scene[b];
-  float c = 0.0f;
-  if (s >= 0) {
-    const SceneC& S = scenes[s];
-    const int q0 = b * NPX_BLOCK;
-    const int na = min(NPX_BLOCK, S.seg_off + S.n - q0);
-    if (na > 0) {
-      const int* cm = clsmap + S.cls_base;
-      const int l = cell[q0];
-      const int wxy = S.wx * S.wy;
-      int cand = 0;
-      for (int r = 0; r < 9; ++r) {
-        const int base = l + (r % 3 - 1) * S.wx + (r / 3 - 1) * wxy;
-        for (int d = -1; d <= 1; ++d) {
-          const int k = cm[base + d];
-          cand += cstart[k + 1] - cstart[k];
-        }
-      }
-      c = (float)cand * (float)max(boff[b + 1] - boff[b], 1);  // run members do no work
-    }
-  }
-  cost[b] = c;
-}
-
-// Chooses the block order of the step's neighbour passes (one workgroup):
-// the cost order when the XCD order's eight contiguous chunks differ by more
-// than 15 % in estimated cost, else the XCD order.  The cost order is a
-// bucket sort by floor(log2(cost)), descending.  Also empties the step's
-// dense list (dn), which the build pass fills after it.
-__global__ void __launch_bounds__(1024) k_block_plan(const float* __restrict__ cost, int nb, int* __restrict__ order,
-                                                     int* __restrict__ mode, int* __restrict__ dn) {
-  __shared__ float chunk[8];
-  __shared__ int hist[64];
-  __shared__ int base[64];
-  __shared__ int lpt;
-  const int t = threadIdx.x;
-  if (t == 0) *dn = 0;
-  if (t < 8) chunk[t] = 0.0f;
-  if (t < 64) hist[t] = 0;
-  __syncthreads();
-  for (int b = t; b < nb; b += blockDim.x) atomicAdd(&chunk[(int)((long long)b * 8 / nb)], cost[b]);
-  __syncthreads();
-  if (t == 0) {
-    float tot = 0.0f, mx = 0.0f;
-    for (int x = 0; x < 8; ++x) {
-      tot += chunk[x];
-      mx = fmaxf(mx, chunk[x]);
-    }
-    lpt = (nb >= 16 && mx > 1.15f * (tot / 8.0f)) ? 1 : 0;
-    if (!lpt) *mode = 0;
-  }
-  __syncthreads();
-  if (!lpt) return;
-  auto bucket = [](float c) { return 63 - min(62, max(0, (int)log2f(c + 1.0f))); };
-  for (int b = t; b < nb; b += blockDim.x) atomicAdd(&hist[bucket(cost[b])], 1);
-  __syncthreads();
-  if (t == 0) {
-    int acc = 0;
-    for (int k = 0; k < 64; ++k) {
-      base[k] = acc;
-      acc += hist[k];
-    }
-  }
-  __syncthreads();
-  for (int b = t; b < nb; b += blockDim.x) order[atomicAdd(&base[bucket(cost[b])], 1)] = b;
-  __syncthreads();
-  if (t == 0) *mode = 1;
+// Empties the step's dense list, which the build pass fills after it.
+__global__ void k_dense_reset(int* __restrict__ dn) {
+  if (threadIdx.x == 0) *dn = 0;
 }
 
 // Neighbour-pass prologue.  `active` lanes own a particle; the others stay
