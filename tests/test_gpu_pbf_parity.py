@@ -321,6 +321,25 @@ def test_crowded_sheet_dense_path_bitexact(gpu, corr_k, dq):
     np.testing.assert_array_equal(gl, ol)
 
 
+def test_1m_configured_three_steps_bitexact(gpu):
+    """The full-size headline scene (1M particles, configured s_corr,
+    vorticity 1e-4) over its first three steps, where the blow-up presses
+    thousands of particles onto the walls (dense list, inline sweeps, list
+    recording all active), against the oracle bit for bit.  The oracle needs
+    about a minute here."""
+    sc = scenes.dam_break_1m()
+    _say("case: dam_break_1m configured, 3 steps")
+    c = _ctx(sc)
+    c.step(sc.dt, sc.gravity, sc.mass, 3)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
 def test_configured_collapse_20_steps_bitexact(gpu):
     """The configured s_corr (k = 0.1, dq = 0.2h) over 20 steps, long enough
     for the domain clamp to stack particles in the box corners (about 2800 on
