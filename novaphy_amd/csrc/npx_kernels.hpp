@@ -204,12 +204,18 @@ k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   }
   // One atomic per distinct class of the wave (slots arrive in the previous
   // step's class order, so a wave holds few classes, and a stack of
-  // particles on one point would otherwise serialise on one counter).  The
-  // offsets only place particles provisionally; k_rank fixes the order.
+  // particles on one point would otherwise serialise on one counter); after
+  // sixteen classes the rest of the wave (particles scattered over many cells)
+  // takes one atomic per lane.  The offsets only place particles
+  // provisionally; k_rank fixes the order.
   const int lane = (int)(threadIdx.x & 63);
   unsigned long long todo = __ballot(active);
   int off = 0;
-  while (todo) {
+  for (int round = 0; todo; ++round) {
+    if (round == 16) {
+      if ((todo >> lane) & 1ull) off = atomicAdd(&ccount[c], 1);
+      break;
+    }
     const int leader = (int)__builtin_ctzll(todo);
     const int cl = __shfl(c, leader, 64);
     const unsigned long long same = __ballot(active && c == cl) & todo;
