@@ -53,10 +53,12 @@ def pass_bytes(vort: bool):
 
 
 def fold_dense(kt):
-    """Kernel times with each pass's dense-particle kernel (name + "_dense",
-    one 64-lane block per particle whose skin list overflowed) added to the
-    pass's per-lane kernel: together they are one pass over every particle.
-    Launch counts stay those of the per-lane kernel."""
+    """Kernel times with the build pass's dense-particle kernel (name +
+    "_dense": the crowded particles whose skin list overflowed, one wave per
+    batch) added to the build's per-lane kernel: together they are one pass
+    over every particle.  The later passes sweep their crowded particles in
+    the first blocks of their own launch.  Launch counts stay those of the
+    per-lane kernel."""
     out = {k: list(v) for k, v in kt.items() if not (k.endswith("_dense") and k[:-6] in kt)}
     for k, v in kt.items():
         if k.endswith("_dense") and k[:-6] in kt:

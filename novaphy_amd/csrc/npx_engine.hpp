@@ -903,10 +903,17 @@ struct Ctx {
     DBuf<float4>* xs[2] = {&XS0, &XS1};
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
                  (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1};
-    // one 64-lane block per dense particle, at most this many at a time
+    // The build pass's dense list: one 64-lane block per batch, at most this
+    // many at a time.  The later passes sweep the list in their first nd
+    // launch blocks (a multiple of 8, for the XCD order of the rest), one
+    // wave per batch, beside their per-lane blocks.
     const int dense_blocks = std::max(1, std::min(8192, hw_blocks * (NPX_BLOCK / 64)));
+    const int wpb = nbr_block / 64;
+    const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
+    const size_t lds = (size_t)wpb * sizeof(float);
     if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 64, DN.p));
-    const BlockPlan rm{xcd_remap};
+    const BlockPlan rm{xcd_remap, 0};
+    const BlockPlan rd{xcd_remap, nd};
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
@@ -915,34 +922,32 @@ struct Ctx {
     for (int it = 0; it < npass; ++it) {
       auto dens = fast ? (it == 0 ? k_density_lambda<true, true> : k_density_lambda<true, false>)
                        : (it == 0 ? k_density_lambda<false, true> : k_density_lambda<false, false>);
-      NPX_TRY(launch(it == 0 ? "density_lambda_build" : "density_lambda", dens, blocks, nbr_block, sc, bs, cm, cst,
-                     cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
-      if (!fast)
-        NPX_TRY(launch(it == 0 ? "density_lambda_build_dense" : "density_lambda_dense", k_density_dense,
-                       dense_blocks, 64, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R));
+      if (it == 0 || fast) {
+        NPX_TRY(launch(it == 0 ? "density_lambda_build" : "density_lambda", dens, blocks, nbr_block, sc, bs, cm, cst,
+                       cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+      } else {
+        NPX_TRY(launch_shm("density_lambda", dens, blocks + nd, nbr_block, lds * npx_dense_lds_floats<5>(), sc, bs,
+                           cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rd));
+      }
+      if (!fast && it == 0)
+        NPX_TRY(launch_shm("density_lambda_build_dense", k_density_dense, dense_blocks, 64,
+                           sizeof(float) * npx_dense_lds_floats<5>(), sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p,
+                           it, R));
       if (fast) {
         NPX_TRY(launch("delta", k_delta<true>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
                        xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
       } else {
-        NPX_TRY(launch("delta", k_delta<false>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
-                       xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
-        NPX_TRY(launch("delta_dense", k_delta_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
-                       (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p,
-                       R));
+        NPX_TRY(launch_shm("delta", k_delta<false>, blocks + nd, nbr_block, lds * npx_dense_lds_floats<3>(), sc, bs,
+                           cm, cst, cel, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
+                           (const float*)RHO.p, VT.p, R, rd));
       }
       cur = 1 - cur;
     }
-    NPX_TRY(launch("omega_xsph", k_omega_xsph, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
-                   (const float4*)VT.p, OM.p, VX.p, R, rm));
-    if (!fast && npass > 0)
-      NPX_TRY(launch("omega_xsph_dense", k_omega_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
-                     (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, VX.p, R));
-    NPX_TRY(launch("vort_commit", k_vort_commit, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
-                   (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p, V.p, R, rm));
-    if (!fast && npass > 0)
-      NPX_TRY(launch("vort_commit_dense", k_commit_dense, dense_blocks, 64, sc, bs, cm, cst, cel,
-                     (const float4*)xs[cur]->p, (const float4*)VT.p, (const float4*)OM.p, (const float4*)VX.p, X.p,
-                     V.p, R));
+    NPX_TRY(launch_shm("omega_xsph", k_omega_xsph, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<6>() : 0,
+                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, VX.p, R, rd));
+    NPX_TRY(launch_shm("vort_commit", k_vort_commit, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<3>() : 0,
+                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, (const float4*)OM.p,
+                       (const float4*)VX.p, X.p, V.p, R, rd));
     NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, R, X.p, V.p, RHO.p, LAM.p));
     xs_final = cur;
     return NPX_OK;

@@ -535,12 +535,15 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 // the order: no pass reads what another block of the same launch writes.
 struct BlockPlan {
   int remap;  // 1: XCD order; 0: hardware order
+  int nd;     // leading blocks that sweep the dense list (a multiple of 8)
 };
 
+// The particle block of a neighbour-pass launch block (after the nd dense
+// blocks, which keep XCD i mod 8).
 __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
-  const int i = (int)blockIdx.x;
+  const int i = (int)blockIdx.x - bp.nd;
   if (!bp.remap) return i;
-  const int g = (int)gridDim.x, x = i & 7;
+  const int g = (int)gridDim.x - bp.nd, x = i & 7;
   return x * (g >> 3) + min(x, g & 7) + (i >> 3);
 }
 
@@ -604,7 +607,7 @@ struct Skin {
   const int* __restrict__ ps;               // [pieces] first slot of each run
   const unsigned* __restrict__ runb;       // [classes / 32] bit set: the class holds a run longer than one
   const int* __restrict__ runs_s;          // [scenes] 1: the scene holds a run longer than one
-  int* __restrict__ dl;                    // [slots] dense list: heads whose skin list overflowed
+  int* __restrict__ dl;                    // [slots] dense list: leaders of batches of crowded heads
   int* __restrict__ dn;                    // [1] dense list length
   int dense;                               // 1: dense particles go to the per-wave kernels
 };
@@ -1377,13 +1380,17 @@ __device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi,
 // crowded cells sweeps thousands of candidates in every pass.  A lane per particle would
 // leave those sweeps to the few waves that hold such particles, each walking
 // its candidates one dependent fetch after another.  Instead the build pass
-// puts them on a list (heads only, exact FP32 path only), the per-lane
-// kernels skip them, and one wave per listed particle sweeps its candidates
-// 64 at a time: lane t fetches candidate t of the chunk and evaluates its
-// pair terms (a[] from zero, k = 1), the terms go to LDS, and lanes 0..NT-1
-// add term t of the in-range candidates in candidate order, k times for a
-// run of k, as the per-lane sweep does.  Sums, order and results are those
-// of the per-lane sweep bit for bit.
+// marks them (heads only, exact FP32 path only) and groups them: the marked
+// particles of one 64-slot window that share a cell, and so share the
+// candidate sequence.  It lists one leader per batch of NPX_DENSE_BATCH
+// members of a group.  The per-lane kernels skip them, and one wave per
+// batch sweeps the candidates 64 at a time for all its members at once:
+// lane t fetches candidate t of the chunk
+// and evaluates its pair terms against each member (a[] from zero, k = 1);
+// the in-range terms go to LDS compacted in candidate order, one row per
+// (member, accumulator); and one lane per row adds its row in order, k
+// times for a run of k, as the per-lane sweep does.  Sums, order and results
+// are those of the per-lane sweep bit for bit.
 
 // Particles whose skin list overflows and whose 27 cells hold more than
 // this many candidates go on the dense list; sparser overflows keep the
@@ -1392,6 +1399,33 @@ __device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi,
 
 // Skin-list count of a listed dense particle (other overflows count -1).
 #define NPX_CNT_DENSE (-2)
+
+// Members of a dense group swept together by one wave.
+#ifndef NPX_DENSE_BATCH
+#define NPX_DENSE_BATCH 2
+#endif
+// Waves of a neighbour pass that sweep the dense list beside its per-lane
+// blocks (the build pass's separate dense launch uses up to 8192).
+#ifndef NPX_DENSE_WAVES
+#define NPX_DENSE_WAVES 8192
+#endif
+// LDS row stride of a dense sweep: rows start 4 banks apart.
+#define NPX_DENSE_RS 68
+
+// LDS floats per wave of a dense sweep with NT accumulators: the term rows,
+// their run counts and the per-member term counts.
+template <int NT>
+__host__ __device__ constexpr int npx_dense_lds_floats() {
+  return (NPX_DENSE_BATCH * NT * NPX_DENSE_RS + NPX_DENSE_BATCH * NPX_DENSE_RS + NPX_DENSE_BATCH + 3) & ~3;
+}
+
+// Orders one wave's LDS writes before its following reads (the dense
+// sweeps' waves work independently, so a block barrier does not fit).
+__device__ __forceinline__ void npx_wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // Whether slot q (a head of a scene with solver iterations, after this
 // step's build pass) is on the dense list.
@@ -1415,88 +1449,149 @@ __device__ __forceinline__ int candidate_total(int l, int wx, int wxy, const int
   return total;
 }
 
-// Appends q to the dense list (one atomic per wave).
-__device__ __forceinline__ void dense_push(const Skin& K, int q) {
-  const unsigned long long m = __ballot(1);
+// Lists q if it leads a batch: the calling lanes of the wave with window
+// cell l form a group, and every NPX_DENSE_BATCH-th of them in lane order
+// (from the lowest) leads the batch of itself and the next ones (one atomic
+// per wave).
+__device__ __forceinline__ void dense_push(const Skin& K, int q, int l) {
   const int lane = (int)(threadIdx.x & 63);
+  unsigned long long todo = __ballot(1);
+  bool lead = false;
+  while (todo) {
+    const int first = (int)__builtin_ctzll(todo);
+    const int lc = __shfl(l, first, 64);
+    const unsigned long long same = __ballot(l == lc) & todo;
+    if (l == lc) lead = ((int)__builtin_popcountll(same & ((1ull << lane) - 1ull)) % NPX_DENSE_BATCH) == 0;
+    todo &= ~same;
+  }
+  const unsigned long long m = __ballot(lead);
   const int leader = (int)__builtin_ctzll(m);
   const int rank = (int)__builtin_popcountll(m & ((1ull << lane) - 1ull));
   int base = 0;
   if (lane == leader) base = atomicAdd(K.dn, (int)__builtin_popcountll(m));
   base = __shfl(base, leader, 64);
-  K.dl[base + rank] = q;
+  if (lead) K.dl[base + rank] = q;
 }
 
-// The candidates of one particle (window cell l, position xi) in the
-// reference's order, as visit_all walks them, swept by the whole 64-lane
-// block; lane i < NT returns accumulator i.
-template <int NT, class L, class P>
-__device__ __forceinline__ float wave_sweep(const Skin& K, bool rs, int l, int wx, int wxy, const int* __restrict__ cm,
-                                            const int* __restrict__ cstart, int shift, float4 xi, float cut, L&& ld,
-                                            const P& pair) {
-  __shared__ float tl[64 * NT];
-  __shared__ int kl[64];
-  const int t = (int)threadIdx.x;
-  float acc = 0.0f;
-  // one chunk: m <= 64 slots from c, or m runs from piece c (runs: iend is
-  // the range's end piece and e its end slot)
-  auto chunk = [&](int c, int m, bool runs, int iend, int e) {
-    const bool valid = t < m;
-    int j, k = 1;
-    if (runs) {
-      const int i = c + (valid ? t : 0);
-      j = K.ps[i];
-      k = ((i + 1 < iend) ? K.ps[i + 1] : e) - j;
-    } else {
-      j = c + (valid ? t : 0);
+// The dense batch of leader q0 (window cell l): the marked heads of q0's
+// 64-slot window with cell l, from q0 on, at most NPX_DENSE_BATCH of them.
+// Their candidates are swept in the reference's order, as visit_all walks
+// them, by the whole 64-lane block; make(q) gives member q's pair functor
+// (its position in .xi), and fin(q, a) runs on member q's own lane with its
+// NT sums.  Each 64-candidate chunk writes, per member, the terms of its
+// in-range candidates compacted in candidate order into one padded LDS row
+// per accumulator; lane p * NT + u then adds row (p, u) in order.
+template <int NT, class L, class MK, class FIN>
+__device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int q0, bool rs, int l, int wxy,
+                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
+                                            const int* __restrict__ cell, float cut, float* __restrict__ lds, L&& ld,
+                                            MK&& make, FIN&& fin) {
+  constexpr int GB = NPX_DENSE_BATCH;
+  constexpr int RS = NPX_DENSE_RS;
+  static_assert(GB * NT <= 64, "one summing lane per row");
+  float* __restrict__ tl = lds;                                  // [GB * NT][RS]
+  int* __restrict__ kl = reinterpret_cast<int*>(lds + GB * NT * RS);  // [GB][RS]
+  int* __restrict__ cl = kl + GB * RS;                           // [GB]
+  const int t = (int)(threadIdx.x & 63);
+  const int wb = q0 & ~63;
+  const int qt = wb + t;
+  const bool member = qt < S.seg_off + S.n && (!rs || K.rstart[qt]) && K.cnt[qt] == NPX_CNT_DENSE && cell[qt] == l;
+  unsigned long long left = __ballot(member) & ~((1ull << (q0 & 63)) - 1ull);
+  const int ps = t / NT, us = t - ps * NT;  // the row this lane adds
+  {
+    int qs[GB];
+    int nb = 0;
+    unsigned long long bm = 0ull;
+#pragma unroll
+    for (int p = 0; p < GB; ++p) {
+      qs[p] = wb;
+      if (left) {
+        const int b = (int)__builtin_ctzll(left);
+        qs[p] = wb + b;
+        bm |= 1ull << b;
+        left &= left - 1ull;
+        ++nb;
+      }
     }
-    const auto d = ld(j);
-    const float4 xj = cand_pos(d);
-    const bool in = valid && dist2<false>(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) < cut;
+    float acc = 0.0f;
+    // one chunk: m <= 64 slots from c, or m runs from piece c (runs: iend is
+    // the range's end piece and e its end slot)
+    auto chunk = [&](int c, int m, bool runs, int iend, int e) {
+      const bool valid = t < m;
+      int j, k = 1;
+      if (runs) {
+        const int i = c + (valid ? t : 0);
+        j = K.ps[i];
+        k = ((i + 1 < iend) ? K.ps[i + 1] : e) - j;
+      } else {
+        j = c + (valid ? t : 0);
+      }
+      const auto d = ld(j);
+      const float4 xj = cand_pos(d);
+#pragma unroll
+      for (int p = 0; p < GB; ++p) {
+        if (p < nb) {
+          const auto P = make(qs[p]);
+          const bool in = valid && dist2<false>(P.xi.x - xj.x, P.xi.y - xj.y, P.xi.z - xj.z) < cut;
+          float a[NT];
+#pragma unroll
+          for (int u = 0; u < NT; ++u) a[u] = 0.0f;
+          if (in) P(j, d, 1, a);
+          const unsigned long long mask = __ballot(in);
+          if (in) {
+            const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+#pragma unroll
+            for (int u = 0; u < NT; ++u) tl[(p * NT + u) * RS + pos] = a[u];
+            kl[p * RS + pos] = k;
+          }
+          if (t == 0) cl[p] = (int)__builtin_popcountll(mask);
+        }
+      }
+      npx_wave_lds_sync();
+      if (ps < nb) {
+        const float* row = tl + (ps * NT + us) * RS;
+        const int* kr = kl + ps * RS;
+        const int n = cl[ps];
+        for (int i = 0; i < n; ++i) add_k(acc, row[i], kr[i]);
+      }
+      npx_wave_lds_sync();
+    };
+    auto range = [&](int b, int e, bool runs) {
+      if (b >= e) return;
+      if (runs) {
+        const int iend = K.pi[e - 1] + 1;
+        for (int c = K.pi[b]; c < iend; c += 64) chunk(c, min(64, iend - c), true, iend, e);
+      } else {
+        for (int c = b; c < e; c += 64) chunk(c, min(64, e - c), false, 0, e);
+      }
+    };
+    for (int r = 0; r < NPX_ROWS; ++r) {
+      const int base = l + (r % 3 - 1) * S.wx + (r / 3 - 1) * wxy;
+      const int c0 = cm[base - 1];
+      const int c1 = cm[base];
+      const int c2 = cm[base + 1];
+      if (c1 == c0 + 1 && c2 == c0 + 2) {
+        range(cstart[c0] - shift, cstart[c0 + 3] - shift, rs && cls_runs(K, c0, 3));
+      } else {
+        for (int u = 0; u < 3; ++u) {
+          const int c = (u == 0) ? c0 : (u == 1) ? c1 : c2;
+          range(cstart[c] - shift, cstart[c + 1] - shift, rs && cls_runs(K, c, 1));
+        }
+      }
+    }
+    // member slot qt has rank r in the batch: its sums are in lanes r*NT..
+    const bool mine = (bm >> t) & 1ull;
+    const int r = (int)__builtin_popcountll(bm & ((1ull << t) - 1ull));
+    const int src = mine ? r * NT : 0;
     float a[NT];
 #pragma unroll
-    for (int u = 0; u < NT; ++u) a[u] = 0.0f;
-    if (in) pair(j, d, 1, a);
-#pragma unroll
-    for (int u = 0; u < NT; ++u) tl[t * NT + u] = a[u];
-    kl[t] = k;
-    const unsigned long long mask = __ballot(in);
-    __syncthreads();
-    if (t < NT) {
-      for (unsigned long long mm = mask; mm; mm &= mm - 1ull) {
-        const int b = (int)__builtin_ctzll(mm);
-        add_k(acc, tl[b * NT + t], kl[b]);
-      }
-    }
-    __syncthreads();
-  };
-  auto range = [&](int b, int e, bool runs) {
-    if (b >= e) return;
-    if (runs) {
-      const int iend = K.pi[e - 1] + 1;
-      for (int c = K.pi[b]; c < iend; c += 64) chunk(c, min(64, iend - c), true, iend, e);
-    } else {
-      for (int c = b; c < e; c += 64) chunk(c, min(64, e - c), false, 0, e);
-    }
-  };
-  for (int r = 0; r < NPX_ROWS; ++r) {
-    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
-    const int c0 = cm[base - 1];
-    const int c1 = cm[base];
-    const int c2 = cm[base + 1];
-    if (c1 == c0 + 1 && c2 == c0 + 2) {
-      range(cstart[c0] - shift, cstart[c0 + 3] - shift, rs && cls_runs(K, c0, 3));
-    } else {
-      for (int u = 0; u < 3; ++u) {
-        const int c = (u == 0) ? c0 : (u == 1) ? c1 : c2;
-        range(cstart[c] - shift, cstart[c + 1] - shift, rs && cls_runs(K, c, 1));
-      }
-    }
+    for (int u = 0; u < NT; ++u) a[u] = __shfl(acc, src + u, 64);
+    if (mine) fin(qt, a);
   }
-  return acc;
 }
 
-// Scene data of dense-list particle q (uniform over the block).
+// Scene data of dense group leader q (uniform over the block).
 #define NPX_DENSE_PROLOGUE()                                 \
   const int q = K.dl[w];                                     \
   const int s = blk_scene[q / NPX_BLOCK];                    \
@@ -1507,12 +1602,32 @@ __device__ __forceinline__ float wave_sweep(const Skin& K, bool rs, int l, int w
   const int l = cell[q];                                     \
   const bool rs = K.runs_s[s] != 0;
 
-// Broadcasts the NT accumulators of a wave_sweep to every lane.
-template <int NT>
-__device__ __forceinline__ void wave_sums(float acc, float* a) {
-#pragma unroll
-  for (int u = 0; u < NT; ++u) a[u] = __shfl(acc, u, 64);
-}
+// The dense list's share of the neighbour passes (defined with each pass).
+// The passes after the build run it in their first bp.nd launch blocks,
+// beside the per-lane blocks, one wave per batch.
+__device__ __forceinline__ void dense_density(int, int, const SceneC* __restrict__, const int* __restrict__,
+                                              const int* __restrict__, const int* __restrict__,
+                                              const int* __restrict__, float4* __restrict__, float* __restrict__,
+                                              float* __restrict__, int, const Skin&);
+__device__ __forceinline__ void dense_delta(int, int, const SceneC* __restrict__, const int* __restrict__,
+                                            const int* __restrict__, const int* __restrict__, const int* __restrict__,
+                                            const float4* __restrict__, float4* __restrict__, int,
+                                            const float4* __restrict__, const float* __restrict__,
+                                            float4* __restrict__, const Skin&);
+__device__ __forceinline__ void dense_omega(int, int, const SceneC* __restrict__, const int* __restrict__,
+                                            const int* __restrict__, const int* __restrict__, const int* __restrict__,
+                                            const float4* __restrict__, const float4* __restrict__,
+                                            float4* __restrict__, float4* __restrict__, const Skin&);
+__device__ __forceinline__ void dense_commit(int, int, const SceneC* __restrict__, const int* __restrict__,
+                                             const int* __restrict__, const int* __restrict__, const int* __restrict__,
+                                             const float4* __restrict__, const float4* __restrict__,
+                                             const float4* __restrict__, const float4* __restrict__,
+                                             float4* __restrict__, float4* __restrict__, const Skin&);
+
+// Dense-list batch of wave (threadIdx.x / 64) of dense launch block blockIdx.x,
+// and the stride over the batches.
+#define NPX_DENSE_WAVE() ((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)))
+#define NPX_DENSE_STRIDE(bp) ((bp).nd * (int)(blockDim.x >> 6))
 
 // compute_density (pbf_solver.cpp:129-142) fused with compute_lambda
 // (:144-179): rho_i sums m W over all in-range candidates (self and
@@ -1525,6 +1640,11 @@ __global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                  float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
+  if (!FAST && !BUILD && (int)blockIdx.x < bp.nd) {
+    dense_density(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, RHO, LAM, iter,
+                  K);
+    return;
+  }
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
   if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
@@ -1547,7 +1667,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (n < 0 && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
       K.cnt[q] = NPX_CNT_DENSE;
       K.ecnt[q] = -1;
-      dense_push(K, q);
+      dense_push(K, q, l);
       return;
     }
     K.cnt[q] = n;
@@ -1563,21 +1683,29 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
   density_finish(S, q, a, XS, RHO, LAM);
 }
 
-// k_density_lambda for the dense list (iterations of the exact path).
+// k_density_lambda for the dense list, batches w0, w0 + ws, ... (iterations
+// of the exact path).
+__device__ __forceinline__ void dense_density(int w0, int ws, const SceneC* __restrict__ scenes,
+                                              const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                                              const int* __restrict__ cstart, const int* __restrict__ cell,
+                                              float4* __restrict__ XS, float* __restrict__ RHO,
+                                              float* __restrict__ LAM, int iter, const Skin& K) {
+  extern __shared__ float npx_dyn[];
+  for (int w = w0; w < *K.dn; w += ws) {
+    NPX_DENSE_PROLOGUE();
+    if (iter >= S.iters) continue;
+    dense_group<5>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<5>(), [&](int j) { return XS[j]; },
+                   [&](int qm) { return DensityPair<false>(S, qm, XS[qm]); },
+                   [&](int qm, const float* a) { density_finish(S, qm, a, XS, RHO, LAM); });
+  }
+}
+
+// The build pass's dense list (the first iteration), one wave per block.
 __global__ void __launch_bounds__(64)
 k_density_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                 const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K) {
-  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
-    NPX_DENSE_PROLOGUE();
-    if (iter >= S.iters) continue;
-    const float4 xi = XS[q];
-    const DensityPair<false> P(S, q, xi);
-    const float acc = wave_sweep<5>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut, [&](int j) { return XS[j]; }, P);
-    float a[5];
-    wave_sums<5>(acc, a);
-    if (threadIdx.x == 0) density_finish(S, q, a, XS, RHO, LAM);
-  }
+  dense_density((int)blockIdx.x, (int)gridDim.x, scenes, blk_scene, clsmap, cstart, cell, XS, RHO, LAM, iter, K);
 }
 
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
@@ -1586,11 +1714,16 @@ k_density_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_s
 // Adds the particle's |delta x*| to its summed displacement and keeps the
 // largest sum of each class for the skin-list test.
 template <bool FAST>
-__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(7)))
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
         float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
         float4* __restrict__ VT, Skin K, BlockPlan bp) {
+  if (!FAST && (int)blockIdx.x < bp.nd) {
+    dense_delta(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, XSn, iter, X, RHO,
+                VT, K);
+    return;
+  }
   NPX_NBR_PROLOGUE();
   const bool last = (iter == max(S.iters, 1) - 1);
   if (iter >= S.iters) {
@@ -1611,7 +1744,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     float a[3] = {0.0f, 0.0f, 0.0f};
     auto ld = [&](int j) { return XS[j]; };
     auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };  // k > 1 only without FAST
-    // one run / two slots at a time on the inline sweep: 64 VGPRs
+    // one run / two slots at a time on the inline sweep: within 72 VGPRs
     visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, P.cut, XS, ld, body);
     moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
   }
@@ -1631,25 +1764,24 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
   if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
 }
 
-// k_delta for the dense list.
-__global__ void __launch_bounds__(64)
-k_delta_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-              float4* __restrict__ XSn, int iter, const float4* __restrict__ X, const float* __restrict__ RHO,
-              float4* __restrict__ VT, Skin K) {
-  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+// k_delta for the dense list, batches w0, w0 + ws, ...
+__device__ __forceinline__ void dense_delta(int w0, int ws, const SceneC* __restrict__ scenes,
+                                            const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                                            const int* __restrict__ cstart, const int* __restrict__ cell,
+                                            const float4* __restrict__ XS, float4* __restrict__ XSn, int iter,
+                                            const float4* __restrict__ X, const float* __restrict__ RHO,
+                                            float4* __restrict__ VT, const Skin& K) {
+  extern __shared__ float npx_dyn[];
+  for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (iter >= S.iters) continue;  // the pass-through is the per-lane kernel's
     const bool last = (iter == max(S.iters, 1) - 1);
-    const float4 xi = XS[q];
-    const DeltaPair<false> P(S, q, xi);
-    const float acc = wave_sweep<3>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut, [&](int j) { return XS[j]; }, P);
-    float a[3];
-    wave_sums<3>(acc, a);
-    if (threadIdx.x == 0) {
-      const float moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
-      if (moved > 0.0f) atomicMax(K.cmax + cm[l], __float_as_uint(moved));
-    }
+    dense_group<3>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<3>(), [&](int j) { return XS[j]; },
+                   [&](int qm) { return DeltaPair<false>(S, qm, XS[qm]); },
+                   [&](int qm, const float* a) {
+                     const float moved = delta_finish(S, K, qm, XS[qm], a, XSn, iter, last, X, RHO, VT);
+                     if (moved > 0.0f) atomicMax(K.cmax + cm[l], __float_as_uint(moved));
+                   });
   }
 }
 
@@ -1663,6 +1795,10 @@ __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
              const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K, BlockPlan bp) {
+  if ((int)blockIdx.x < bp.nd) {
+    dense_omega(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, VT, OM, VX, K);
+    return;
+  }
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
   if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
@@ -1687,22 +1823,20 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
   omega_finish(S, q, vi, a, OM, VX);
 }
 
-// k_omega_xsph for the dense list.
-__global__ void __launch_bounds__(64)
-k_omega_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-              const float4* __restrict__ VT, float4* __restrict__ OM, float4* __restrict__ VX, Skin K) {
-  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+// k_omega_xsph for the dense list, batches w0, w0 + ws, ...
+__device__ __forceinline__ void dense_omega(int w0, int ws, const SceneC* __restrict__ scenes,
+                                            const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                                            const int* __restrict__ cstart, const int* __restrict__ cell,
+                                            const float4* __restrict__ XS, const float4* __restrict__ VT,
+                                            float4* __restrict__ OM, float4* __restrict__ VX, const Skin& K) {
+  extern __shared__ float npx_dyn[];
+  for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (!S.xsph_on && !S.vort_on) continue;  // no neighbour sum: the per-lane kernel's
-    const float4 xi = XS[q];
-    const float4 vi = VT[q];
-    const OmegaPair P(S, q, xi, vi);
-    const float acc = wave_sweep<6>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, P.cut,
-                                    [&](int j) { return XV{XS[j], VT[j]}; }, P);
-    float a[6];
-    wave_sums<6>(acc, a);
-    if (threadIdx.x == 0) omega_finish(S, q, vi, a, OM, VX);
+    dense_group<6>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<6>(),
+                   [&](int j) { return XV{XS[j], VT[j]}; },
+                   [&](int qm) { return OmegaPair(S, qm, XS[qm], VT[qm]); },
+                   [&](int qm, const float* a) { omega_finish(S, qm, VT[qm], a, OM, VX); });
   }
 }
 
@@ -1715,6 +1849,11 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
               float4* __restrict__ X, float4* __restrict__ V, Skin K, BlockPlan bp) {
+  if ((int)blockIdx.x < bp.nd) {
+    dense_commit(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, VT, OM, VX, X, V,
+                 K);
+    return;
+  }
   NPX_NBR_PROLOGUE();
   const bool rs = K.runs_s[s] != 0;
   if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
@@ -1732,23 +1871,22 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   commit_finish(S, q, xi, VX[q], oi, a, X, V);
 }
 
-// k_vort_commit for the dense list (scenes with vorticity).
-__global__ void __launch_bounds__(64)
-k_commit_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
-               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
-               const float4* __restrict__ VT, const float4* __restrict__ OM, const float4* __restrict__ VX,
-               float4* __restrict__ X, float4* __restrict__ V, Skin K) {
-  for (int w = (int)blockIdx.x; w < *K.dn; w += (int)gridDim.x) {
+// k_vort_commit for the dense list (scenes with vorticity), batches w0,
+// w0 + ws, ...
+__device__ __forceinline__ void dense_commit(int w0, int ws, const SceneC* __restrict__ scenes,
+                                             const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
+                                             const int* __restrict__ cstart, const int* __restrict__ cell,
+                                             const float4* __restrict__ XS, const float4* __restrict__ VT,
+                                             const float4* __restrict__ OM, const float4* __restrict__ VX,
+                                             float4* __restrict__ X, float4* __restrict__ V, const Skin& K) {
+  extern __shared__ float npx_dyn[];
+  for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (!S.vort_on) continue;  // commit only: the per-lane kernel's
-    const float4 xi = XS[q];
-    const float4 oi = OM[q];
-    const VortPair P(S, q, xi, oi.w);
-    const float acc = wave_sweep<3>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, S.cut2,
-                                    [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; }, P);
-    float a[3];
-    wave_sums<3>(acc, a);
-    if (threadIdx.x == 0) commit_finish(S, q, xi, VX[q], oi, a, X, V);
+    dense_group<3>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<3>(),
+                   [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; },
+                   [&](int qm) { return VortPair(S, qm, XS[qm], OM[qm].w); },
+                   [&](int qm, const float* a) { commit_finish(S, qm, XS[qm], VX[qm], OM[qm], a, X, V); });
   }
 }
 
