@@ -300,9 +300,9 @@ def _crowded_count(pos, h=0.1):
 
 @pytest.mark.parametrize("corr_k,dq", [(0.001, 0.3), (0.1, 0.2)])
 def test_crowded_sheet_dense_path_bitexact(gpu, corr_k, dq):
-    """A wall sheet of 3000 particles: the dense kernels (one wave per crowded
-    particle, terms added in candidate order from LDS) must give the oracle's
-    state bit for bit, with vorticity on so all four passes run dense."""
+    """A wall sheet of 3000 particles: the dense sweeps (one wave per batch of
+    crowded particles, terms added in candidate order from LDS) must give the
+    oracle's state bit for bit, with vorticity on so all four passes run dense."""
     pos, vel = _sheet(3000, 51)
     assert _crowded_count(pos) == pos.shape[0]  # every particle is crowded at the start
     prm = scenes.PBFParams(corr_k=corr_k, corr_delta_q=dq, vorticity_epsilon=1e-3, use_domain_bounds=True,
@@ -312,6 +312,31 @@ def test_crowded_sheet_dense_path_bitexact(gpu, corr_k, dq):
     c = capi.Context(prm, capacity=pos.shape[0])
     c.set_particles(pos, vel)
     c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
+def test_crowded_sheet_tunables_bitexact(gpu):
+    """The crowded sheet with block size and XCD order switched every step:
+    the passes after the build sweep the dense list in their first launch
+    blocks (one wave per batch, as many waves per block as the block size
+    gives), which must not change any result."""
+    pos, vel = _sheet(3000, 52)
+    assert _crowded_count(pos) == pos.shape[0]
+    prm = scenes.PBFParams(corr_k=0.1, corr_delta_q=0.2, vorticity_epsilon=1e-3, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.025)
+    _say("case: crowded wall sheet n=3000, block sizes 64/256/128, 3 steps")
+    c = capi.Context(prm, capacity=pos.shape[0])
+    c.set_particles(pos, vel)
+    for b, remap in ((64, 1), (256, 0), (128, 1)):
+        c.set_tuning(b, remap, 0.1)
+        c.step(dt, g, m, 1)
     gp, gv, gr, gl = c.get_particles()
     c.close()
     op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
