@@ -24,6 +24,9 @@ extern "C" {
 #define NPX_ENOMEM (-3)     /* capacity or device memory exhausted                    */
 #define NPX_EDOMAIN (-4)    /* a particle left the cell window (see npx_last_error)  */
 #define NPX_ESTATE (-5)     /* call out of order (e.g. no particles set)              */
+#define NPX_EINTERNAL (-6)  /* unexpected internal failure (an exception inside the
+                               library, which never crosses this interface; the
+                               reference would throw std::runtime_error)            */
 
 /* Creation flags. */
 #define NPX_FLAG_FAST_MATH 1u   /* FMA / rsqrt arithmetic instead of the reference's

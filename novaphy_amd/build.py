@@ -93,9 +93,15 @@ def build_arith_check(verbose: bool = True) -> str:
 
 
 def build(verbose: bool = True) -> None:
-    build_hip(verbose)
+    """Compiles every native part.  The two hipcc compiles run side by side;
+    the host module links libnpx_hip.so, so it comes after."""
+    import concurrent.futures as cf
+    jobs = [build_hip]
     if os.path.isfile(os.path.join(ROOT, "tests", "hip", "arith_check.hip")):
-        build_arith_check(verbose)
+        jobs.append(build_arith_check)
+    with cf.ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for f in [ex.submit(j, verbose) for j in jobs]:
+            f.result()
     if os.path.isdir(os.path.join(CSRC, "host")):
         build_core(verbose)
 

@@ -21,6 +21,7 @@ NPX_EDEVICE = -2
 NPX_ENOMEM = -3
 NPX_EDOMAIN = -4
 NPX_ESTATE = -5
+NPX_EINTERNAL = -6
 
 FLAG_FAST_MATH = 1
 FLAG_NO_GRAPH = 2

@@ -319,6 +319,37 @@ void destroy(Ctx* c) {
   delete c;
 }
 
+// Every entry point of the C ABI runs its body inside guard(): no C++
+// exception crosses the boundary.  std::bad_alloc (host allocation) becomes
+// NPX_ENOMEM, any other exception NPX_EINTERNAL; the message goes to the
+// context (or to the create-error slot when there is none).
+void set_guard_error(npx_ctx* ctx, const char* fn, const char* what) noexcept {
+  try {
+    std::string m = std::string(fn) + ": " + what;
+    if (ctx)
+      reinterpret_cast<Ctx*>(ctx)->err = std::move(m);
+    else
+      g_create_error = std::move(m);
+  } catch (...) {
+  }
+}
+
+template <class F>
+int guard(npx_ctx* ctx, const char* fn, F&& f) noexcept {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    set_guard_error(ctx, fn, "host memory allocation failed");
+    return NPX_ENOMEM;
+  } catch (const std::exception& e) {
+    set_guard_error(ctx, fn, e.what());
+    return NPX_EINTERNAL;
+  } catch (...) {
+    set_guard_error(ctx, fn, "unknown exception");
+    return NPX_EINTERNAL;
+  }
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -336,69 +367,87 @@ const char* npx_last_error(const npx_ctx* ctx) {
 }
 
 int npx_create(const npx_pbf_params* params, int32_t capacity, uint32_t flags, npx_ctx** out) {
-  if (!out) return NPX_EINVAL;
-  *out = nullptr;
-  Ctx* c = new Ctx();
-  int r = check_params(c, params);
-  if (r == NPX_OK) r = c->init(std::max(capacity, 1), 0, 1, flags, false);
-  if (r != NPX_OK) {
-    g_create_error = c->err;
-    destroy(c);
-    return r;
-  }
-  c->scenes[0].prm = *params;
-  *out = reinterpret_cast<npx_ctx*>(c);
-  return NPX_OK;
+  return guard(nullptr, __func__, [&]() -> int {
+    if (!out) return NPX_EINVAL;
+    *out = nullptr;
+    if (capacity > NPX_MAX_SLOTS) {
+      g_create_error = "npx_create: capacity beyond the slot limit (2^28)";
+      return NPX_EINVAL;
+    }
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) {
+      g_create_error = "host memory allocation failed";
+      return NPX_ENOMEM;
+    }
+    int r = check_params(c, params);
+    if (r == NPX_OK) r = c->init(std::max(capacity, 1), 0, 1, flags, false);
+    if (r != NPX_OK) {
+      g_create_error = c->err;
+      destroy(c);
+      return r;
+    }
+    c->scenes[0].prm = *params;
+    *out = reinterpret_cast<npx_ctx*>(c);
+    return NPX_OK;
+  });
 }
 
 int npx_destroy(npx_ctx* ctx) {
-  destroy(reinterpret_cast<Ctx*>(ctx));
-  return NPX_OK;
+  return guard(ctx, __func__, [&]() -> int {
+    destroy(reinterpret_cast<Ctx*>(ctx));
+    return NPX_OK;
+  });
 }
 
 int npx_set_params(npx_ctx* ctx, const npx_pbf_params* params) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  NPX_TRY(check_params(c, params));
-  SceneH& H = c->scenes[0];
-  const bool window_change = H.prm.kernel_radius != params->kernel_radius ||
-                             H.prm.use_domain_bounds != params->use_domain_bounds ||
-                             std::memcmp(H.prm.domain_lower, params->domain_lower, sizeof(float) * 3) != 0 ||
-                             std::memcmp(H.prm.domain_upper, params->domain_upper, sizeof(float) * 3) != 0;
-  H.prm = *params;
-  c->table_dirty = true;
-  if (window_change && H.used) {
-    H.window_valid = false;
-    NPX_TRY(c->scene_window(H));
-  }
-  return NPX_OK;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    NPX_TRY(check_params(c, params));
+    SceneH& H = c->scenes[0];
+    const bool window_change = H.prm.kernel_radius != params->kernel_radius ||
+                               H.prm.use_domain_bounds != params->use_domain_bounds ||
+                               std::memcmp(H.prm.domain_lower, params->domain_lower, sizeof(float) * 3) != 0 ||
+                               std::memcmp(H.prm.domain_upper, params->domain_upper, sizeof(float) * 3) != 0;
+    H.prm = *params;
+    c->table_dirty = true;
+    if (window_change && H.used) {
+      H.window_valid = false;
+      NPX_TRY(c->scene_window(H));
+    }
+    return NPX_OK;
+  });
 }
 
 int npx_set_particles(npx_ctx* ctx, const float* pos3, const float* vel3, int32_t n) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (c->batch) return c->fail(NPX_ESTATE, "use npx_scene_add on a batch context");
-  if (n < 0 || (n > 0 && !pos3)) return c->fail(NPX_EINVAL, "bad particle arrays");
-  NPX_TRY(c->reserve_slots(n));
-  SceneH& H = c->scenes[0];
-  H.used = true;
-  H.n = n;
-  H.seg_off = 0;
-  H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
-  std::fill(c->h_blk_scene.begin(), c->h_blk_scene.end(), -1);
-  for (int b = 0; b < cdiv(n, NPX_BLOCK); ++b) c->h_blk_scene[b] = 0;
-  c->table_dirty = true;
-  H.window_valid = false;
-  NPX_TRY(c->scene_window(H));
-  NPX_TRY(c->upload_particles(H, pos3, vel3));
-  NPX_TRY(c->upload_tables());
-  return NPX_OK;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (c->batch) return c->fail(NPX_ESTATE, "use npx_scene_add on a batch context");
+    if (n < 0 || (n > 0 && !pos3)) return c->fail(NPX_EINVAL, "bad particle arrays");
+    NPX_TRY(c->reserve_slots(n));
+    SceneH& H = c->scenes[0];
+    H.used = true;
+    H.n = n;
+    H.seg_off = 0;
+    H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
+    std::fill(c->h_blk_scene.begin(), c->h_blk_scene.end(), -1);
+    for (int b = 0; b < cdiv(n, NPX_BLOCK); ++b) c->h_blk_scene[b] = 0;
+    c->table_dirty = true;
+    H.window_valid = false;
+    NPX_TRY(c->scene_window(H));
+    NPX_TRY(c->upload_particles(H, pos3, vel3));
+    NPX_TRY(c->upload_tables());
+    return NPX_OK;
+  });
 }
 
 int npx_get_particles(npx_ctx* ctx, float* pos3, float* vel3, float* rho, float* lambda) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  return c->download(c->scenes[0], pos3, vel3, rho, lambda);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    return c->download(c->scenes[0], pos3, vel3, rho, lambda);
+  });
 }
 
 int32_t npx_num_particles(const npx_ctx* ctx) {
@@ -411,64 +460,80 @@ int32_t npx_num_particles(const npx_ctx* ctx) {
 }
 
 int npx_step_n(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass, int32_t nsteps) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (c->batch) return c->fail(NPX_ESTATE, "use npx_batch_step on a batch context");
-  if (!gravity) return c->fail(NPX_EINVAL, "gravity is NULL");
-  SceneH& H = c->scenes[0];
-  if (H.dt != dt || H.mass != particle_mass || std::memcmp(H.g, gravity, sizeof(float) * 3) != 0) {
-    H.dt = dt;
-    H.mass = particle_mass;
-    std::memcpy(H.g, gravity, sizeof(float) * 3);
-    c->table_dirty = true;
-  }
-  if (!H.used || H.n == 0) return NPX_OK;
-  return c->step_n(nsteps);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (c->batch) return c->fail(NPX_ESTATE, "use npx_batch_step on a batch context");
+    if (!gravity) return c->fail(NPX_EINVAL, "gravity is NULL");
+    SceneH& H = c->scenes[0];
+    if (H.dt != dt || H.mass != particle_mass || std::memcmp(H.g, gravity, sizeof(float) * 3) != 0) {
+      H.dt = dt;
+      H.mass = particle_mass;
+      std::memcpy(H.g, gravity, sizeof(float) * 3);
+      c->table_dirty = true;
+    }
+    if (!H.used || H.n == 0) return NPX_OK;
+    return c->step_n(nsteps);
+  });
 }
 
 int npx_step(npx_ctx* ctx, float dt, const float gravity[3], float particle_mass) {
-  return npx_step_n(ctx, dt, gravity, particle_mass, 1);
+  return guard(ctx, __func__, [&]() -> int {
+    return npx_step_n(ctx, dt, gravity, particle_mass, 1);
+  });
 }
 
 int npx_synchronize(npx_ctx* ctx) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  return c->check_flags();
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    return c->check_flags();
+  });
 }
 
 int npx_debug_grid(npx_ctx* ctx, uint64_t* keys, int32_t* cand_cnt, int32_t* nbr_cnt) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
-  return c->debug_grid(keys, cand_cnt, nbr_cnt);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
+    return c->debug_grid(keys, cand_cnt, nbr_cnt);
+  });
 }
 
 int npx_debug_candidates(npx_ctx* ctx, int32_t* lists, int64_t cap, int64_t* total) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c || !total) return NPX_EINVAL;
-  if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
-  return c->debug_candidates(lists, cap, total);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !total) return NPX_EINVAL;
+    if (c->batch) return c->fail(NPX_ESTATE, "grid diagnostics are for single-scene contexts");
+    return c->debug_candidates(lists, cap, total);
+  });
 }
 
 int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* query3, int32_t nq, int32_t* counts,
                    int32_t* lists, int64_t cap, int64_t* total) {
-  if (!total || n < 0 || nq < 0 || (n > 0 && !pos3) || (nq > 0 && !query3)) {
-    g_create_error = "npx_grid_query: bad arguments";
-    return NPX_EINVAL;
-  }
-  Ctx* c = new Ctx();
-  npx_pbf_params p{};
-  p.kernel_radius = cell_size;
-  p.rest_density = 1.0f;
-  int r = c->init(1, 0, 1, 0, false);
-  if (r == NPX_OK) {
-    c->scenes[0].prm = p;
-    r = grid_query(c, pos3, n, cell_size, query3, nq, counts, lists, cap, total);
-  }
-  if (r != NPX_OK) g_create_error = c->err;
-  destroy(c);
-  return r;
+  return guard(nullptr, __func__, [&]() -> int {
+    if (!total || n < 0 || nq < 0 || (n > 0 && !pos3) || (nq > 0 && !query3)) {
+      g_create_error = "npx_grid_query: bad arguments";
+      return NPX_EINVAL;
+    }
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) {
+      g_create_error = "host memory allocation failed";
+      return NPX_ENOMEM;
+    }
+    npx_pbf_params p{};
+    p.kernel_radius = cell_size;
+    p.rest_density = 1.0f;
+    int r = c->init(1, 0, 1, 0, false);
+    if (r == NPX_OK) {
+      c->scenes[0].prm = p;
+      r = grid_query(c, pos3, n, cell_size, query3, nq, counts, lists, cap, total);
+    }
+    if (r != NPX_OK) g_create_error = c->err;
+    destroy(c);
+    return r;
+  });
 }
 
 namespace {
@@ -551,258 +616,300 @@ int prepare_shapes(const npx_shape* shapes, int nshapes, float spacing, float ex
 
 int npx_sample_boundaries(const npx_shape* shapes, int32_t nshapes, float spacing, float plane_extent,
                           float* local3, int32_t* body, int64_t cap, int64_t* count) {
-  if (!count || nshapes < 0 || (nshapes > 0 && !shapes) || !(spacing > 0.0f)) {
-    g_create_error = "npx_sample_boundaries: bad arguments";
-    return NPX_EINVAL;
-  }
-  std::vector<ShapeS> hs;
-  long long total = 0;
-  if (prepare_shapes(shapes, nshapes, spacing, plane_extent, hs, &total) != NPX_OK) {
-    g_create_error = "npx_sample_boundaries: unknown shape type";
-    return NPX_EINVAL;
-  }
-  if (total > (1ll << 30)) {
-    g_create_error = "npx_sample_boundaries: too many boundary particles";
-    return NPX_EINVAL;
-  }
-  *count = total;
-  if (total == 0 || total > cap) return NPX_OK;
-  if (!local3 || !body) {
-    g_create_error = "npx_sample_boundaries: output is NULL";
-    return NPX_EINVAL;
-  }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-    g_create_error = "no HIP device available";
-    return NPX_EDEVICE;
-  }
-  ShapeS* ds = nullptr;
-  float* dl = nullptr;
-  int* db = nullptr;
-  hipError_t e = hipMalloc((void**)&ds, sizeof(ShapeS) * hs.size());
-  if (e == hipSuccess) e = hipMalloc((void**)&dl, sizeof(float) * 3 * (size_t)total);
-  if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof(int) * (size_t)total);
-  if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), sizeof(ShapeS) * hs.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    const int n = (int)total;
-    hipLaunchKernelGGL(k_sample_boundaries, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, 0, (const ShapeS*)ds,
-                       nshapes, n, spacing, dl, db);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpy(local3, dl, sizeof(float) * 3 * (size_t)total, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(body, db, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost);
-  if (ds) (void)hipFree(ds);
-  if (dl) (void)hipFree(dl);
-  if (db) (void)hipFree(db);
-  if (e != hipSuccess) {
-    g_create_error = std::string("npx_sample_boundaries: ") + hipGetErrorString(e);
-    return NPX_EDEVICE;
-  }
-  return NPX_OK;
+  return guard(nullptr, __func__, [&]() -> int {
+    if (!count || nshapes < 0 || (nshapes > 0 && !shapes) || !(spacing > 0.0f)) {
+      g_create_error = "npx_sample_boundaries: bad arguments";
+      return NPX_EINVAL;
+    }
+    std::vector<ShapeS> hs;
+    long long total = 0;
+    if (prepare_shapes(shapes, nshapes, spacing, plane_extent, hs, &total) != NPX_OK) {
+      g_create_error = "npx_sample_boundaries: unknown shape type";
+      return NPX_EINVAL;
+    }
+    if (total > (1ll << 30)) {
+      g_create_error = "npx_sample_boundaries: too many boundary particles";
+      return NPX_EINVAL;
+    }
+    *count = total;
+    if (total == 0 || total > cap) return NPX_OK;
+    if (!local3 || !body) {
+      g_create_error = "npx_sample_boundaries: output is NULL";
+      return NPX_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+      g_create_error = "no HIP device available";
+      return NPX_EDEVICE;
+    }
+    ShapeS* ds = nullptr;
+    float* dl = nullptr;
+    int* db = nullptr;
+    hipError_t e = hipMalloc((void**)&ds, sizeof(ShapeS) * hs.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&dl, sizeof(float) * 3 * (size_t)total);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof(int) * (size_t)total);
+    if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), sizeof(ShapeS) * hs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      const int n = (int)total;
+      hipLaunchKernelGGL(k_sample_boundaries, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, 0, (const ShapeS*)ds,
+                         nshapes, n, spacing, dl, db);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(local3, dl, sizeof(float) * 3 * (size_t)total, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(body, db, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost);
+    if (ds) (void)hipFree(ds);
+    if (dl) (void)hipFree(dl);
+    if (db) (void)hipFree(db);
+    if (e != hipSuccess) {
+      g_create_error = std::string("npx_sample_boundaries: ") + hipGetErrorString(e);
+      return NPX_EDEVICE;
+    }
+    return NPX_OK;
+  });
 }
 
 int npx_set_boundary(npx_ctx* ctx, const float* local3, const int32_t* body, const float* psi, int32_t nb,
                      int32_t nbodies) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (c->batch) return c->fail(NPX_ESTATE, "boundary coupling is for single-scene contexts");
-  return set_boundary(c, local3, body, psi, nb, nbodies);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (c->batch) return c->fail(NPX_ESTATE, "boundary coupling is for single-scene contexts");
+    return set_boundary(c, local3, body, psi, nb, nbodies);
+  });
 }
 
 int npx_compute_boundary_volumes(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* psi_out) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  return compute_volumes(c, pose7, nbodies, psi_out);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    return compute_volumes(c, pose7, nbodies, psi_out);
+  });
 }
 
 int npx_boundary_world_positions(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out3) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (c->nb == 0) return NPX_OK;
-  NPX_TRY(bpos(c, pose7, nbodies));
-  if (out3) {
-    std::vector<float4> bp(c->nb);
-    NPX_HIP_C(c, hipMemcpyAsync(bp.data(), c->BP.p, sizeof(float4) * c->nb, hipMemcpyDeviceToHost, c->stream));
-    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-    for (int b = 0; b < c->nb; ++b) {
-      out3[3 * b] = bp[b].x;
-      out3[3 * b + 1] = bp[b].y;
-      out3[3 * b + 2] = bp[b].z;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (c->nb == 0) return NPX_OK;
+    NPX_TRY(bpos(c, pose7, nbodies));
+    if (out3) {
+      std::vector<float4> bp(c->nb);
+      NPX_HIP_C(c, hipMemcpyAsync(bp.data(), c->BP.p, sizeof(float4) * c->nb, hipMemcpyDeviceToHost, c->stream));
+      NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+      for (int b = 0; b < c->nb; ++b) {
+        out3[3 * b] = bp[b].x;
+        out3[3 * b + 1] = bp[b].y;
+        out3[3 * b + 2] = bp[b].z;
+      }
     }
-  }
-  return NPX_OK;
+    return NPX_OK;
+  });
 }
 
 int npx_apply_boundary_density(npx_ctx* ctx) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  return boundary_density(c);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    return boundary_density(c);
+  });
 }
 
 int npx_coupling_forces(npx_ctx* ctx, const float* pose7, const uint8_t* is_static, int32_t nbodies, float* force3,
                         float* torque3) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  return coupling(c, pose7, is_static, nbodies, force3, torque3);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    return coupling(c, pose7, is_static, nbodies, force3, torque3);
+  });
 }
 
 // ------------------------------------------------------------- batch
 int npx_batch_create(int32_t slot_capacity, int32_t cell_capacity, int32_t max_scenes, uint32_t flags,
                      npx_ctx** out) {
-  if (!out || slot_capacity <= 0 || cell_capacity <= 0 || max_scenes <= 0) {
-    g_create_error = "npx_batch_create: capacities must be positive";
-    return NPX_EINVAL;
-  }
-  *out = nullptr;
-  Ctx* c = new Ctx();
-  int r = c->init(slot_capacity, cell_capacity, max_scenes, flags, true);
-  if (r != NPX_OK) {
-    g_create_error = c->err;
-    destroy(c);
-    return r;
-  }
-  *out = reinterpret_cast<npx_ctx*>(c);
-  return NPX_OK;
+  return guard(nullptr, __func__, [&]() -> int {
+    if (!out || slot_capacity <= 0 || cell_capacity <= 0 || max_scenes <= 0) {
+      g_create_error = "npx_batch_create: capacities must be positive";
+      return NPX_EINVAL;
+    }
+    if (slot_capacity > NPX_MAX_SLOTS || cell_capacity > (1 << 30) || max_scenes > (1 << 20)) {
+      g_create_error = "npx_batch_create: capacity beyond the limits (2^28 slots, 2^30 cells, 2^20 scenes)";
+      return NPX_EINVAL;
+    }
+    *out = nullptr;
+    Ctx* c = new (std::nothrow) Ctx();
+    if (!c) {
+      g_create_error = "host memory allocation failed";
+      return NPX_ENOMEM;
+    }
+    int r = c->init(slot_capacity, cell_capacity, max_scenes, flags, true);
+    if (r != NPX_OK) {
+      g_create_error = c->err;
+      destroy(c);
+      return r;
+    }
+    *out = reinterpret_cast<npx_ctx*>(c);
+    return NPX_OK;
+  });
 }
 
 int npx_scene_add(npx_ctx* ctx, const npx_pbf_params* params, float dt, const float gravity[3], float particle_mass,
                   const float* pos3, const float* vel3, int32_t n, int32_t* scene_id) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_add needs a batch context");
-  NPX_TRY(check_params(c, params));
-  if (!params->use_domain_bounds) return c->fail(NPX_EINVAL, "batched scenes need use_domain_bounds");
-  if (n < 0 || (n > 0 && !pos3) || !gravity) return c->fail(NPX_EINVAL, "bad scene arrays");
-  int sid = -1;
-  for (int s = 0; s < c->max_scenes; ++s)
-    if (!c->scenes[s].used) {
-      sid = s;
-      break;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_add needs a batch context");
+    NPX_TRY(check_params(c, params));
+    if (!params->use_domain_bounds) return c->fail(NPX_EINVAL, "batched scenes need use_domain_bounds");
+    if (n < 0 || (n > 0 && !pos3) || !gravity) return c->fail(NPX_EINVAL, "bad scene arrays");
+    int sid = -1;
+    for (int s = 0; s < c->max_scenes; ++s)
+      if (!c->scenes[s].used) {
+        sid = s;
+        break;
+      }
+    if (sid < 0) return c->fail(NPX_ENOMEM, "all %d scene slots are in use", c->max_scenes);
+    SceneH H;
+    H.prm = *params;
+    H.dt = dt;
+    H.mass = particle_mass;
+    std::memcpy(H.g, gravity, sizeof(float) * 3);
+    H.n = n;
+    H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
+    const int b0 = c->slot_alloc.alloc(H.seg_blocks);
+    if (b0 < 0) return c->fail(NPX_ENOMEM, "slot capacity exhausted (%d particles requested)", n);
+    H.seg_off = b0 * NPX_BLOCK;
+    int r = c->scene_window(H);
+    if (r != NPX_OK) {
+      c->slot_alloc.release(b0, H.seg_blocks);
+      return r;
     }
-  if (sid < 0) return c->fail(NPX_ENOMEM, "all %d scene slots are in use", c->max_scenes);
-  SceneH H;
-  H.prm = *params;
-  H.dt = dt;
-  H.mass = particle_mass;
-  std::memcpy(H.g, gravity, sizeof(float) * 3);
-  H.n = n;
-  H.seg_blocks = cdiv(std::max(n, 1), NPX_BLOCK);
-  const int b0 = c->slot_alloc.alloc(H.seg_blocks);
-  if (b0 < 0) return c->fail(NPX_ENOMEM, "slot capacity exhausted (%d particles requested)", n);
-  H.seg_off = b0 * NPX_BLOCK;
-  int r = c->scene_window(H);
-  if (r != NPX_OK) {
-    c->slot_alloc.release(b0, H.seg_blocks);
-    return r;
-  }
-  H.used = true;
-  c->scenes[sid] = H;
-  for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = (b < cdiv(n, NPX_BLOCK)) ? sid : -1;
-  c->table_dirty = true;
-  NPX_TRY(c->upload_particles(c->scenes[sid], pos3, vel3));
-  NPX_TRY(c->upload_tables());
-  if (scene_id) *scene_id = sid;
-  return NPX_OK;
+    H.used = true;
+    c->scenes[sid] = H;
+    for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = (b < cdiv(n, NPX_BLOCK)) ? sid : -1;
+    c->table_dirty = true;
+    NPX_TRY(c->upload_particles(c->scenes[sid], pos3, vel3));
+    NPX_TRY(c->upload_tables());
+    if (scene_id) *scene_id = sid;
+    return NPX_OK;
+  });
 }
 
 int npx_scene_remove(npx_ctx* ctx, int32_t scene_id) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
-    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
-  SceneH& H = c->scenes[scene_id];
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  const int b0 = H.seg_off / NPX_BLOCK;
-  for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = -1;
-  c->slot_alloc.release(b0, H.seg_blocks);
-  if (H.window_valid) c->cls_alloc.release(H.cls_base, H.ncell);
-  if (H.nbnd > 0) {
-    c->sbnd[scene_id] = Ctx::SceneBnd();
-    c->bnd_dirty = true;
-  }
-  H = SceneH();
-  c->table_dirty = true;
-  return c->upload_tables();
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+      return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+    SceneH& H = c->scenes[scene_id];
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    const int b0 = H.seg_off / NPX_BLOCK;
+    for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = -1;
+    c->slot_alloc.release(b0, H.seg_blocks);
+    if (H.window_valid) c->cls_alloc.release(H.cls_base, H.ncell);
+    if (H.nbnd > 0) {
+      c->sbnd[scene_id] = Ctx::SceneBnd();
+      c->bnd_dirty = true;
+    }
+    H = SceneH();
+    c->table_dirty = true;
+    return c->upload_tables();
+  });
 }
 
 int npx_scene_get(npx_ctx* ctx, int32_t scene_id, float* pos3, float* vel3, float* rho, float* lambda) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
-    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
-  return c->download(c->scenes[scene_id], pos3, vel3, rho, lambda);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+      return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+    return c->download(c->scenes[scene_id], pos3, vel3, rho, lambda);
+  });
 }
 
 int npx_scene_set_boundary(npx_ctx* ctx, int32_t scene_id, const float* local3, const int32_t* body,
                            const float* psi, int32_t nb, int32_t nbodies, const uint8_t* is_static) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_set_boundary needs a batch context");
-  if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
-    return c->fail(NPX_EINVAL, "no scene %d", scene_id);
-  if (nb < 0 || nbodies < 0) return c->fail(NPX_EINVAL, "negative boundary size");
-  if (nb > 0 && (!local3 || !body || !psi)) return c->fail(NPX_EINVAL, "boundary arrays are NULL");
-  if (nbodies > 0 && !is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  if (c->sbnd.size() < (size_t)c->max_scenes) c->sbnd.resize(c->max_scenes);
-  Ctx::SceneBnd& B = c->sbnd[scene_id];
-  B.local3.assign(local3, local3 + 3 * (size_t)nb);
-  B.body.assign(body, body + nb);
-  B.psi.assign(psi, psi + nb);
-  B.is_static.assign(is_static, is_static + nbodies);
-  B.nbodies = nbodies;
-  c->scenes[scene_id].nbnd = nb;
-  c->bnd_dirty = true;
-  c->table_dirty = true;
-  return c->upload_tables();
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (!c->batch) return c->fail(NPX_ESTATE, "npx_scene_set_boundary needs a batch context");
+    if (scene_id < 0 || scene_id >= c->max_scenes || !c->scenes[scene_id].used)
+      return c->fail(NPX_EINVAL, "no scene %d", scene_id);
+    if (nb < 0 || nbodies < 0) return c->fail(NPX_EINVAL, "negative boundary size");
+    if (nb > 0 && (!local3 || !body || !psi)) return c->fail(NPX_EINVAL, "boundary arrays are NULL");
+    if (nbodies > 0 && !is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    if (c->sbnd.size() < (size_t)c->max_scenes) c->sbnd.resize(c->max_scenes);
+    Ctx::SceneBnd& B = c->sbnd[scene_id];
+    B.local3.assign(local3, local3 + 3 * (size_t)nb);
+    B.body.assign(body, body + nb);
+    B.psi.assign(psi, psi + nb);
+    B.is_static.assign(is_static, is_static + nbodies);
+    B.nbodies = nbodies;
+    c->scenes[scene_id].nbnd = nb;
+    c->bnd_dirty = true;
+    c->table_dirty = true;
+    return c->upload_tables();
+  });
 }
 
 int32_t npx_batch_num_bodies(npx_ctx* ctx) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return -1;
-  if (c->bnd_dirty && c->rebuild_boundaries() != NPX_OK) return -1;
-  return c->bb_bodies;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return -1;
+    if (c->bnd_dirty && c->rebuild_boundaries() != NPX_OK) return -1;
+    return c->bb_bodies;
+  });
 }
 
 int npx_batch_coupling(npx_ctx* ctx, const float* pose7, int32_t nbodies, float* out6) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (!c->batch) return c->fail(NPX_ESTATE, "npx_batch_coupling needs a batch context");
-  return c->batch_coupling(pose7, nbodies, out6);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (!c->batch) return c->fail(NPX_ESTATE, "npx_batch_coupling needs a batch context");
+    return c->batch_coupling(pose7, nbodies, out6);
+  });
 }
 
 int npx_batch_step(npx_ctx* ctx, int32_t nsteps) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  return c->step_n(nsteps);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    return c->step_n(nsteps);
+  });
 }
 
 int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float skin_fraction) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  if (block_size != 64 && block_size != 128 && block_size != 256)
-    return c->fail(NPX_EINVAL, "block_size must be 64, 128 or 256");
-  if (xcd_remap != 0 && xcd_remap != 1) return c->fail(NPX_EINVAL, "xcd_remap must be 0 or 1");
-  if (!(skin_fraction >= 0.02f && skin_fraction <= 0.5f))
-    return c->fail(NPX_EINVAL, "skin_fraction must be in [0.02, 0.5]");
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  c->nbr_block = block_size;
-  c->xcd_remap = xcd_remap;
-  c->skin_frac = skin_fraction;
-  c->table_dirty = true;
-  return NPX_OK;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (block_size != 64 && block_size != 128 && block_size != 256)
+      return c->fail(NPX_EINVAL, "block_size must be 64, 128 or 256");
+    if (xcd_remap != 0 && xcd_remap != 1) return c->fail(NPX_EINVAL, "xcd_remap must be 0 or 1");
+    if (!(skin_fraction >= 0.02f && skin_fraction <= 0.5f))
+      return c->fail(NPX_EINVAL, "skin_fraction must be in [0.02, 0.5]");
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    c->nbr_block = block_size;
+    c->xcd_remap = xcd_remap;
+    c->skin_frac = skin_fraction;
+    c->table_dirty = true;
+    return NPX_OK;
+  });
 }
 
 int npx_batch_step_timed(npx_ctx* ctx, int32_t nsteps, float* step_ms) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c || nsteps < 0) return NPX_EINVAL;
-  return c->step_timed(nsteps, step_ms);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || nsteps < 0) return NPX_EINVAL;
+    return c->step_timed(nsteps, step_ms);
+  });
 }
 
 int npx_scene_status(npx_ctx* ctx, int32_t* status, int32_t max_scenes) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c || !status) return NPX_EINVAL;
-  return c->scene_status(status, max_scenes);
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !status) return NPX_EINVAL;
+    return c->scene_status(status, max_scenes);
+  });
 }
 
 int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
@@ -818,74 +925,80 @@ int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
 }
 
 int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c) return NPX_EINVAL;
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  for (auto& t : c->timers) {
-    c->ev_pool.push_back(t.a);
-    c->ev_pool.push_back(t.b);
-  }
-  c->timers.clear();
-  c->timing = enable != 0;
-  return NPX_OK;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    for (auto& t : c->timers) {
+      c->ev_pool.push_back(t.a);
+      c->ev_pool.push_back(t.b);
+    }
+    c->timers.clear();
+    c->timing = enable != 0;
+    return NPX_OK;
+  });
 }
 
 int npx_kernel_times(npx_ctx* ctx, char* names, int32_t names_cap, float* ms, int32_t* launches,
                      int32_t max_kernels, int32_t* count) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c || !count) return NPX_EINVAL;
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  std::vector<std::string> order;
-  std::map<std::string, std::pair<double, int>> acc;
-  for (auto& t : c->timers) {
-    float e = 0.0f;
-    NPX_HIP_C(c, hipEventElapsedTime(&e, t.a, t.b));
-    if (!acc.count(t.name)) order.push_back(t.name);
-    acc[t.name].first += e;
-    acc[t.name].second += 1;
-  }
-  *count = (int32_t)order.size();
-  std::string joined;
-  for (size_t k = 0; k < order.size(); ++k) {
-    if ((int)k < max_kernels) {
-      if (ms) ms[k] = (float)acc[order[k]].first;
-      if (launches) launches[k] = acc[order[k]].second;
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !count) return NPX_EINVAL;
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    for (auto& t : c->timers) {
+      float e = 0.0f;
+      NPX_HIP_C(c, hipEventElapsedTime(&e, t.a, t.b));
+      if (!acc.count(t.name)) order.push_back(t.name);
+      acc[t.name].first += e;
+      acc[t.name].second += 1;
     }
-    joined += order[k];
-    joined += '\n';
-  }
-  if (names && names_cap > 0) {
-    std::strncpy(names, joined.c_str(), names_cap - 1);
-    names[names_cap - 1] = '\0';
-  }
-  return NPX_OK;
+    *count = (int32_t)order.size();
+    std::string joined;
+    for (size_t k = 0; k < order.size(); ++k) {
+      if ((int)k < max_kernels) {
+        if (ms) ms[k] = (float)acc[order[k]].first;
+        if (launches) launches[k] = acc[order[k]].second;
+      }
+      joined += order[k];
+      joined += '\n';
+    }
+    if (names && names_cap > 0) {
+      std::strncpy(names, joined.c_str(), names_cap - 1);
+      names[names_cap - 1] = '\0';
+    }
+    return NPX_OK;
+  });
 }
 
 int npx_kernel_trace(npx_ctx* ctx, char* names, int32_t names_cap, float* start_ms, float* dur_ms,
                      int32_t max_launches, int32_t* count) {
-  Ctx* c = reinterpret_cast<Ctx*>(ctx);
-  if (!c || !count) return NPX_EINVAL;
-  NPX_HIP_C(c, hipStreamSynchronize(c->stream));
-  std::string joined;
-  int k = 0;
-  for (auto& t : c->timers) {
-    if (k < max_launches) {
-      float st = 0.0f, d = 0.0f;
-      NPX_HIP_C(c, hipEventElapsedTime(&st, c->timers.front().a, t.a));
-      NPX_HIP_C(c, hipEventElapsedTime(&d, t.a, t.b));
-      if (start_ms) start_ms[k] = st;
-      if (dur_ms) dur_ms[k] = d;
-      joined += t.name;
-      joined += '\n';
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !count) return NPX_EINVAL;
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    std::string joined;
+    int k = 0;
+    for (auto& t : c->timers) {
+      if (k < max_launches) {
+        float st = 0.0f, d = 0.0f;
+        NPX_HIP_C(c, hipEventElapsedTime(&st, c->timers.front().a, t.a));
+        NPX_HIP_C(c, hipEventElapsedTime(&d, t.a, t.b));
+        if (start_ms) start_ms[k] = st;
+        if (dur_ms) dur_ms[k] = d;
+        joined += t.name;
+        joined += '\n';
+      }
+      ++k;
     }
-    ++k;
-  }
-  *count = k;
-  if (names && names_cap > 0) {
-    std::strncpy(names, joined.c_str(), names_cap - 1);
-    names[names_cap - 1] = '\0';
-  }
-  return NPX_OK;
+    *count = k;
+    if (names && names_cap > 0) {
+      std::strncpy(names, joined.c_str(), names_cap - 1);
+      names[names_cap - 1] = '\0';
+    }
+    return NPX_OK;
+  });
 }
 
 }  // extern "C"

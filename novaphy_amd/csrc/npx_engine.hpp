@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -18,10 +20,14 @@ namespace npx {
 
 static const float kPI = 3.14159265358979323846f;  // math_utils.h:53
 
+// HIP error -> C ABI code: device memory exhaustion is NPX_ENOMEM, any other
+// runtime error NPX_EDEVICE.
+#define NPX_HIP_CODE(e) ((e) == hipErrorOutOfMemory ? NPX_ENOMEM : NPX_EDEVICE)
+
 #define NPX_HIP(expr)                                                                        \
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess) return fail(NPX_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+    if (e_ != hipSuccess) return fail(NPX_HIP_CODE(e_), "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                                       __FILE__, __LINE__);                                   \
   } while (0)
 
@@ -29,8 +35,13 @@ static const float kPI = 3.14159265358979323846f;  // math_utils.h:53
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
     if (e_ != hipSuccess)                                                                    \
-      return (c)->fail(NPX_EDEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return (c)->fail(NPX_HIP_CODE(e_), "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
   } while (0)
+
+// Largest particle-slot capacity of a context: slot indices, list words and
+// block counts stay inside int32 arithmetic (2^28 slots need about 200 GB of
+// state and lists, within one MI355X's 288 GB).
+#define NPX_MAX_SLOTS (1 << 28)
 
 #define NPX_TRY(expr)        \
   do {                       \
@@ -451,9 +462,44 @@ struct Ctx {
     return launch_shm(name, kern, blocks, threads, 0, args...);
   }
 
+  // Launch-shape limits: the device's LDS per workgroup (queried at init) and,
+  // per kernel, its static LDS and largest block size (its launch bounds),
+  // looked up once and cached.
+  size_t max_lds = 65536;
+  struct KernLimits {
+    size_t static_lds;
+    int max_threads;
+  };
+  std::map<const void*, KernLimits> klimits;
+
+  // Refuses a launch the device could not run as asked: a negative grid, a
+  // block size that is not a multiple of 64 in [64, 1024] or above the
+  // kernel's launch bounds, or static plus dynamic LDS above the device's
+  // per-workgroup limit.  Such a launch is an error of this library; it is
+  // reported instead of being handed to the queue.
+  int check_launch(const char* name, const void* kern, int blocks, int threads, size_t shmem) {
+    if (blocks < 0) return fail(NPX_EINVAL, "launch %s: negative grid (%d blocks)", name, blocks);
+    if (threads < 64 || threads > 1024 || (threads & 63) != 0)
+      return fail(NPX_EINVAL, "launch %s: block size %d is not a multiple of 64 in [64, 1024]", name, threads);
+    auto it = klimits.find(kern);
+    if (it == klimits.end()) {
+      hipFuncAttributes a;
+      NPX_HIP(hipFuncGetAttributes(&a, kern));
+      it = klimits.emplace(kern, KernLimits{(size_t)a.sharedSizeBytes, a.maxThreadsPerBlock}).first;
+    }
+    if (threads > it->second.max_threads)
+      return fail(NPX_EINVAL, "launch %s: block size %d exceeds the kernel's launch bounds (%d)", name, threads,
+                  it->second.max_threads);
+    if (it->second.static_lds + shmem > max_lds)
+      return fail(NPX_EINVAL, "launch %s: %zu B static + %zu B dynamic LDS exceed the %zu B per workgroup", name,
+                  it->second.static_lds, shmem, max_lds);
+    return NPX_OK;
+  }
+
   template <class K, class... A>
   int launch_shm(const char* name, K kern, int blocks, int threads, size_t shmem, A... args) {
-    if (blocks <= 0) return NPX_OK;
+    NPX_TRY(check_launch(name, (const void*)kern, blocks, threads, shmem));
+    if (blocks == 0) return NPX_OK;
     KTimer t;
     if (timing) {
       t.name = name;
@@ -533,6 +579,9 @@ struct Ctx {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NPX_EDEVICE, "no HIP device available");
     NPX_HIP(hipGetDevice(&device));
+    int lds = 0;
+    NPX_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    if (lds > 0) max_lds = (size_t)lds;
     NPX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     max_scenes = std::max(1, nscenes);
     scenes.assign(max_scenes, SceneH());
@@ -549,6 +598,7 @@ struct Ctx {
   }
 
   int reserve_slots(int slots) {
+    if (slots > NPX_MAX_SLOTS) return fail(NPX_EINVAL, "%d particle slots requested; at most %d", slots, NPX_MAX_SLOTS);
     int cap = cdiv(std::max(slots, 1), NPX_BLOCK) * NPX_BLOCK;
     if (cap <= slot_cap) return NPX_OK;
     if (slot_cap > 0 && batch) return fail(NPX_ENOMEM, "batch slot capacity is fixed at creation");
@@ -967,6 +1017,10 @@ struct Ctx {
   std::vector<const void*> graph_key;
   int graph_captures = 0;
 
+  // Position of the (vorticity * 2 + carry) entry in the graph key: a
+  // replay swaps density and lambda exactly when the captured step did.
+  static constexpr int kKeyCarry = 5;
+
   std::vector<const void*> current_graph_key() const {
     int imax = 0, vort = 0, carry = 0;
     for (const auto& H : scenes) {
@@ -1017,7 +1071,7 @@ struct Ctx {
     const int slot = (graphs[0].in == X.p) ? 0 : (graphs[1].in == X.p) ? 1 : -1;
     if (slot >= 0) {
       NPX_HIP(hipGraphLaunch(graphs[slot].exec, stream));
-      swap_committed(((intptr_t)key[4] & 1) != 0);
+      swap_committed(((intptr_t)key[kKeyCarry] & 1) != 0);
       xs_final = graph_xs_final;
       return NPX_OK;
     }

@@ -109,6 +109,9 @@ def lib():
         L.orc_coupling_f64.argtypes = [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p, i32p,
                                        C.c_int32, f64p, u8p, C.c_double, C.c_double, C.c_double,
                                        f64p, f64p]
+        L.orc_vorticity_confinement.restype = C.c_int32
+        L.orc_vorticity_confinement.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p, f32p, C.c_float,
+                                                C.c_float, f32p, f32p, f32p, f32p]
         L.orc_vorticity_omega.restype = C.c_int32
         L.orc_vorticity_omega.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p, f32p,
                                           C.c_float, f32p]
@@ -375,3 +378,16 @@ def vorticity_omega(prm, pos, vel, rho, mass):
     op = params(prm)
     lib().orc_vorticity_omega(C.byref(op), pos.shape[0], pos, vel, rho, np.float32(mass), om)
     return om
+
+
+def vorticity_confinement(prm, pos, vel, rho, mass, dt):
+    """omega, eta = grad|omega|, confinement force f and v + dt f per particle
+    (this project's vorticity confinement, see pbf_oracle.cpp)."""
+    op = params(prm)
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, np.float32).reshape(-1, 3)
+    rho = np.ascontiguousarray(rho, np.float32).reshape(-1)
+    n = pos.shape[0]
+    outs = [np.zeros((n, 3), np.float32) for _ in range(4)]
+    lib().orc_vorticity_confinement(C.byref(op), n, pos, vel, rho, np.float32(mass), np.float32(dt), *outs)
+    return tuple(outs)

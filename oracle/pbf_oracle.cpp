@@ -28,8 +28,11 @@
 // Vorticity confinement is NOT part of the reference (PBFSettings declares
 // vorticity_epsilon, pbf_solver.h:20, and never reads it).  The optional
 // vorticity pass below is this project's own definition, shared with the HIP
-// path; its parity against the reference is unpinned and it is checked by
-// closed forms (rigid rotation, uniform flow) and by the FP64 instantiation.
+// path.  There is no reference to compare it with; it is pinned by the
+// mathematics instead: uniform flow gives omega = 0, rigid rotation gives
+// omega = 2 Omega, and a Gaussian vortex gives the analytic curl, eta =
+// grad|omega| toward the axis and a confinement force eps (N x omega) along
+// the rotation with |f| = eps |omega| (tests/test_oracle_pins.py).
 
 #include <algorithm>
 #include <cmath>
@@ -352,26 +355,30 @@ struct Solver {
       omega[i] = w;
     }
   }
+  // Confinement force of particle i from the omega field: eta_i = grad|omega|
+  // (SPH gradient), N = eta / |eta|, f = eps_v (N x omega_i).  eta_out gets eta_i.
+  V3<T> vorticity_force(const State<T>& st, T m, int i, V3<T>* eta_out) const {
+    const T h = s.kernel_radius, rho0 = s.rest_density, ev = s.vorticity_epsilon;
+    T wi = norm(omega[i]);
+    V3<T> eta{T(0), T(0), T(0)};
+    for (int j : nbr[i]) {
+      if (j == i) continue;
+      V3<T> g = spiky_grad(st.xs[i] - st.xs[j], h);
+      T rj = (st.rho[j] > T(1e-6f)) ? st.rho[j] : rho0;
+      eta = eta + ((m / rj) * (norm(omega[j]) - wi)) * g;
+    }
+    if (eta_out) *eta_out = eta;
+    T el = norm(eta);
+    V3<T> f{T(0), T(0), T(0)};
+    if (el > T(1e-9f)) {
+      V3<T> N = eta * (T(1) / el);
+      f = ev * cross(N, omega[i]);
+    }
+    return f;
+  }
   void vorticity_apply(State<T>& st, T m, T dt, std::vector<V3<T>>& vnew) {
     const int n = (int)st.x.size();
-    const T h = s.kernel_radius, rho0 = s.rest_density, ev = s.vorticity_epsilon;
-    for (int i = 0; i < n; ++i) {
-      T wi = norm(omega[i]);
-      V3<T> eta{T(0), T(0), T(0)};
-      for (int j : nbr[i]) {
-        if (j == i) continue;
-        V3<T> g = spiky_grad(st.xs[i] - st.xs[j], h);
-        T rj = (st.rho[j] > T(1e-6f)) ? st.rho[j] : rho0;
-        eta = eta + ((m / rj) * (norm(omega[j]) - wi)) * g;
-      }
-      T el = norm(eta);
-      V3<T> f{T(0), T(0), T(0)};
-      if (el > T(1e-9f)) {
-        V3<T> N = eta * (T(1) / el);
-        f = ev * cross(N, omega[i]);
-      }
-      vnew[i] = st.v[i] + dt * f;
-    }
+    for (int i = 0; i < n; ++i) vnew[i] = st.v[i] + dt * vorticity_force(st, m, i, nullptr);
   }
 
   void step(State<T>& st, T dt, V3<T> g, T m, bool skip_grid = false) {
@@ -913,6 +920,40 @@ int32_t orc_vorticity_omega(const orc_params* prm, int32_t n, const float* pos3,
     omega3[3 * i] = sv.omega[i].x;
     omega3[3 * i + 1] = sv.omega[i].y;
     omega3[3 * i + 2] = sv.omega[i].z;
+  }
+  return 0;
+}
+
+// Vorticity confinement from a given state (for the closed-form checks):
+// omega, eta = grad|omega| and the force f = eps_v (N x omega) per particle,
+// with the full candidate lists of one grid build at x (no prediction), and
+// v + dt f as the step applies it.
+int32_t orc_vorticity_confinement(const orc_params* prm, int32_t n, const float* pos3, const float* vel3,
+                                  const float* rho, float mass, float dt, float* omega3, float* eta3,
+                                  float* force3, float* vnew3) {
+  State<float> st;
+  load_state<float>(st, n, pos3, vel3, rho, nullptr);
+  Solver<float> sv(Params<float>::from(*prm));
+  sv.grid.build(st.xs);
+  sv.nbr.resize(n);
+  std::vector<int> scratch;
+  for (int i = 0; i < n; ++i) {
+    sv.grid.query(st.xs[i], scratch);
+    sv.nbr[i] = scratch;
+  }
+  sv.vorticity_omega(st, mass);
+  std::vector<V3<float>> vnew(n);
+  sv.vorticity_apply(st, mass, dt, vnew);
+  for (int i = 0; i < n; ++i) {
+    V3<float> eta;
+    const V3<float> f = sv.vorticity_force(st, mass, i, &eta);
+    const V3<float>* src[4] = {&sv.omega[i], &eta, &f, &vnew[i]};
+    float* dst[4] = {omega3, eta3, force3, vnew3};
+    for (int k = 0; k < 4; ++k) {
+      dst[k][3 * i] = src[k]->x;
+      dst[k][3 * i + 1] = src[k]->y;
+      dst[k][3 * i + 2] = src[k]->z;
+    }
   }
   return 0;
 }

@@ -86,3 +86,50 @@ def test_boundary_sample_counts_match_host_sampler():
     bad = capi.NpxShape(type=7)
     assert capi.load().npx_sample_boundaries(C.byref(bad), 1, C.c_float(0.05), C.c_float(0.7), None, None, 0,
                                              C.byref(cnt)) == capi.NPX_EINVAL
+
+
+_ENOMEM_CHILD = r"""
+import ctypes as C, resource, sys
+sys.path.insert(0, sys.argv[1])
+from novaphy_amd import capi
+L = capi.load()
+vm = 0
+for line in open('/proc/self/status'):
+    if line.startswith('VmSize:'):
+        vm = int(line.split()[1]) * 1024
+# room for the interpreter to keep running, far too little for the request
+resource.setrlimit(resource.RLIMIT_AS, (vm + (1 << 30), resource.RLIM_INFINITY))
+shape = capi.NpxShape(type=capi.SHAPE_SPHERE, body=0, radius=0.1)
+cnt = C.c_int64()
+# 2^27 shapes: the host table of per-shape constants needs ~20 GB
+r = L.npx_sample_boundaries(C.byref(shape), 1 << 27, C.c_float(0.05), C.c_float(1.0), None, None, 0, C.byref(cnt))
+print(r)
+print(L.npx_last_error(None).decode())
+"""
+
+
+def test_host_allocation_failure_returns_enomem():
+    """A host allocation that fails inside an entry point comes back as
+    NPX_ENOMEM with a message; no C++ exception crosses the C ABI (without
+    the guard the child process would abort in std::terminate)."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _ENOMEM_CHILD, ROOT], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = r.stdout.strip().splitlines()
+    assert int(lines[0]) == capi.NPX_ENOMEM
+    assert "npx_sample_boundaries" in lines[1] and "allocation" in lines[1]
+
+
+def test_capacity_beyond_slot_limit_rejected():
+    """Capacities whose slot arithmetic would overflow int32 are refused with
+    NPX_EINVAL before any device work; the batch variant checks them at
+    creation too."""
+    import ctypes as C
+    L = capi.load()
+    h = C.c_void_p()
+    assert L.npx_batch_create(0x7FFFFF00, 1024, 4, 0, C.byref(h)) == capi.NPX_EINVAL
+    assert "slots" in L.npx_last_error(None).decode()
+    prm = capi.to_params(scenes.dam_break_8k().params)
+    assert L.npx_create(C.byref(prm), 0x7FFFFFF0, 0, C.byref(h)) == capi.NPX_EINVAL
+    assert not h.value

@@ -76,6 +76,18 @@ def test_spiky_and_cubic_reference_properties():
     assert O.spiky_grad(np.array([0.05, 0, 0], np.float32), H)[0] < 0
     assert O.cubic_spline(0.0, H) > 0
     assert O.cubic_spline(H, H) == 0.0
+    # closed form (sph_kernel.h:69-80): sigma = 8 / (pi h^3); q <= 1/2:
+    # sigma (6 (q^3 - q^2) + 1); else 2 sigma (1 - q)^3
+    sig = 8.0 / (math.pi * H ** 3)
+    assert O.cubic_spline(0.25 * H, H) == pytest.approx(sig * 46.0 / 64.0, rel=3e-6)
+    assert O.cubic_spline(0.5 * H, H) == pytest.approx(sig / 4.0, rel=3e-6)
+    assert O.cubic_spline(0.75 * H, H) == pytest.approx(sig / 32.0, rel=3e-6)
+    assert O.cubic_spline(0.0, H) == pytest.approx(sig, rel=3e-6)
+    # 3D normalisation: integral of W over the ball of radius h is 1
+    nq = 4000
+    dr = H / nq
+    integral = sum(O.cubic_spline((i + 0.5) * dr, H) * 4 * math.pi * ((i + 0.5) * dr) ** 2 * dr for i in range(nq))
+    assert abs(integral - 1.0) < 1e-3
     # 15/(pi h^6) (h - r)^3 at r = h/2
     assert O.spiky(0.05, H) == pytest.approx(15 / (math.pi * H ** 6) * 0.05 ** 3, rel=1e-5)
 
@@ -322,6 +334,62 @@ def test_vorticity_closed_forms():
     assert interior.sum() > 10
     np.testing.assert_allclose(om[interior].mean(0), 2 * Om, atol=0.3)
     assert np.abs(om[interior][:, :2]).max() < 1e-4
+
+
+def test_vorticity_confinement_differential_rotation():
+    """Vorticity confinement (this project's definition; the reference declares
+    vorticity_epsilon, pbf_solver.h:20, and never reads it) against the
+    mathematics of a Gaussian vortex about the y axis through c:
+    v = Omega(r) (y_hat x (x - c)), Omega(r) = Omega0 exp(-r^2 / s^2), so
+    curl v = (2 Omega + r Omega') y_hat, whose magnitude falls with r.  In the
+    interior, for 0.1 <= r <= 0.2 m: eta = grad|omega| and N point toward the
+    axis; f = eps (N x omega) is tangential with the rotation's sense (it
+    spins the vortex up, Macklin & Mueller 2013, sec. 5) and |f| = eps |omega|;
+    the step adds dt f to v.  A swapped cross product, a sign slip in eta or a
+    missing normalisation fails one of these."""
+    s_lat = 0.05
+    pos = O.generate_fluid_block([0, 0, 0], [1.2, 1.2, 1.2], s_lat)
+    n = pos.shape[0]
+    eps = 0.2
+    prm = scenes.PBFParams(kernel_radius=0.1, vorticity_epsilon=eps)
+    m = prm.particle_mass(s_lat)
+    rho = np.full(n, 1000.0, np.float32)
+    c = np.array([0.6, 0.6, 0.6])
+    om0, sig = 2.0, 0.3
+    d = pos.astype(np.float64) - c
+    rr = np.hypot(d[:, 0], d[:, 2])
+    Om = om0 * np.exp(-rr ** 2 / sig ** 2)
+    yhat = np.array([0.0, 1.0, 0.0])
+    vel = (Om[:, None] * np.cross(yhat, d)).astype(np.float32)
+    dt = 1.0 / 120.0
+    om, eta, f, vnew = O.vorticity_confinement(prm, pos, vel, rho, m, dt)
+    inner = np.all(np.abs(d) < 0.6 - 0.2 - 1e-6, axis=1) & (rr >= 0.1) & (rr <= 0.2)
+    assert inner.sum() > 100
+    rhat = np.zeros_like(d)
+    rhat[:, 0], rhat[:, 2] = d[:, 0] / np.maximum(rr, 1e-12), d[:, 2] / np.maximum(rr, 1e-12)
+    vhat = np.cross(yhat, rhat)  # direction of rotation (Omega > 0)
+    om_y = om0 * np.exp(-rr ** 2 / sig ** 2) * (2.0 - 2.0 * rr ** 2 / sig ** 2)
+    o, e, F = om[inner].astype(np.float64), eta[inner].astype(np.float64), f[inner].astype(np.float64)
+    # omega: axial, the analytic curl within the SPH quadrature error
+    np.testing.assert_allclose(o[:, 1], om_y[inner], rtol=0.12)
+    assert np.abs(o[:, [0, 2]]).max() < 0.02 * np.abs(o[:, 1]).min()
+    # eta = grad|omega| points toward the axis
+    N = e / np.linalg.norm(e, axis=1)[:, None]
+    assert np.min(np.sum(N * -rhat[inner], axis=1)) > 0.97
+    # f tangential, with the rotation's sense, |f| = eps |omega|
+    fn = np.linalg.norm(F, axis=1)
+    assert np.min(np.sum(F * vhat[inner], axis=1) / fn) > 0.97
+    np.testing.assert_allclose(fn, eps * np.linalg.norm(o, axis=1), rtol=0.03)
+    # the criterion has teeth: a swapped cross product (omega x N) points the other way
+    assert np.max(np.sum(np.cross(o, N) * vhat[inner], axis=1)) < 0.0
+    # v += dt f, one float32 multiply and add per component
+    np.testing.assert_array_equal(vnew, (vel + np.float32(dt) * f).astype(np.float32))
+    # no force where omega is uniform: rigid rotation gives |omega| constant in
+    # the interior, so eta vanishes up to rounding and |f| << eps |omega|
+    vel_r = np.cross(np.array([0.0, 1.5, 0.0]), d).astype(np.float32)
+    om_r, eta_r, f_r, _ = O.vorticity_confinement(prm, pos, vel_r, rho, m, dt)
+    core = np.all(np.abs(d) < 0.35, axis=1)
+    assert np.abs(eta_r[core]).max() < 1e-3 * np.abs(om_r[core]).max()
 
 
 # ------------------------------------------------------------ boundary
