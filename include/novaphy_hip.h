@@ -119,6 +119,18 @@ int npx_debug_candidates(npx_ctx* ctx, int32_t* lists, int64_t cap, int64_t* tot
 int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* query3,
                    int32_t nq, int32_t* counts, int32_t* lists, int64_t cap, int64_t* total);
 
+/* npx_grid_query without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as npx_grid_query, on the
+ * current device's null stream; every device buffer lives in one static
+ * per-GPU scratch array of about 22.5 MiB.  Limits: n <= 65536 points,
+ * nq <= 65536 queries, a window of <= 2^18 cells (NPX_EDOMAIN beyond),
+ * <= 2^21 list entries when lists are read (NPX_ENOMEM beyond).  Calls that
+ * overlap on one GPU share the scratch array and race. */
+int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const float* query3,
+                           int32_t nq, int32_t* counts, int32_t* lists, int64_t cap, int64_t* total);
+
 /* ------------------------------------------------------ Akinci boundary */
 
 /* Boundary shapes for npx_sample_boundaries: the fields of CollisionShape

@@ -39,7 +39,7 @@ SYMBOLS = (
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
     "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
     "npx_set_tuning", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
-    "npx_sample_boundaries",
+    "npx_sample_boundaries", "npx_grid_query_noalloc",
 )
 
 
@@ -123,6 +123,7 @@ def load() -> C.CDLL:
         "npx_debug_grid": (C.c_int, [vp, vp, vp, vp]),
         "npx_debug_candidates": (C.c_int, [vp, vp, i64, P(i64)]),
         "npx_grid_query": (C.c_int, [vp, i32, f32, vp, i32, vp, vp, i64, P(i64)]),
+        "npx_grid_query_noalloc": (C.c_int, [vp, i32, f32, vp, i32, vp, vp, i64, P(i64)]),
         "npx_set_boundary": (C.c_int, [vp, vp, vp, vp, i32, i32]),
         "npx_compute_boundary_volumes": (C.c_int, [vp, vp, i32, vp]),
         "npx_boundary_world_positions": (C.c_int, [vp, vp, i32, vp]),

@@ -5,6 +5,8 @@
 // (novaphy_amd/build.py).  The float operations of the default kernels are
 // the reference's, one IEEE operation at a time, so results are bitwise equal
 // to the FP32 CPU restatement of src/fluid/*.cpp on the same input.
+#include <mutex>
+
 #include "npx_engine.hpp"
 
 using npx::Ctx;
@@ -319,6 +321,83 @@ void destroy(Ctx* c) {
   delete c;
 }
 
+// ------------------------------------------------ allocation-free grid query
+// npx_grid_query_noalloc keeps every device buffer it needs in `scratch`, a
+// file-scope device array: each GPU the library runs on holds its own copy
+// (NPX_NA_SCRATCH_BYTES, about 22.5 MiB) from module load until the process
+// ends, and the call allocates nothing.  Host code reaches a GPU's copy only
+// through the address hipGetSymbolAddress returns for it on that device
+// (na_scratch_ptr); the array's name is not a device address on the host.
+// Every call on a GPU shares that GPU's copy: calls that overlap on one GPU
+// race.  What must start at zero is cleared by kernels on the call's stream.
+#define NPX_NA_MAX_POINTS (1 << 16)
+#define NPX_NA_MAX_QUERIES (1 << 16)
+#define NPX_NA_MAX_CELLS (1 << 18)
+#define NPX_NA_MAX_LIST (1 << 21)
+#define NPX_NA_MAX_DEVICES 64
+
+constexpr size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Byte offsets of the regions (each sized for the maxima, 256-aligned).
+struct NaLayout {
+  static constexpr size_t HT = 2 * (size_t)NPX_NA_MAX_CELLS;  // hash-table capacity bound (power of two)
+  static constexpr size_t htk = 0;
+  static constexpr size_t htv = htk + al(HT * 8);
+  static constexpr size_t clsmap = htv + al(HT * 4);
+  static constexpr size_t cst = clsmap + al((size_t)NPX_NA_MAX_CELLS * 4);
+  static constexpr size_t bsum = cst + al(((size_t)NPX_NA_MAX_CELLS + 1) * 4);
+  static constexpr size_t raw_p = bsum + al(((size_t)NPX_NA_MAX_CELLS / 1024 + 2) * 4);
+  static constexpr size_t P = raw_p + al((size_t)NPX_NA_MAX_POINTS * 12);
+  static constexpr size_t PS = P + al((size_t)NPX_NA_MAX_POINTS * 16);
+  static constexpr size_t raw_q = PS + al((size_t)NPX_NA_MAX_POINTS * 16);
+  static constexpr size_t Q = raw_q + al((size_t)NPX_NA_MAX_QUERIES * 12);
+  static constexpr size_t cls = Q + al((size_t)NPX_NA_MAX_QUERIES * 16);
+  static constexpr size_t off = cls + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t tperm = off + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t tkey = tperm + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t perm = tkey + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t cnt = perm + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t off64 = cnt + al(((size_t)NPX_NA_MAX_QUERIES + 1) * 4);
+  static constexpr size_t out = off64 + al((size_t)NPX_NA_MAX_QUERIES * 8);
+  static constexpr size_t total = out + al((size_t)NPX_NA_MAX_LIST * 4);
+};
+#define NPX_NA_SCRATCH_BYTES (NaLayout::total)
+
+static __device__ __align__(256) unsigned char scratch[NPX_NA_SCRATCH_BYTES];
+
+std::mutex g_na_mu;
+unsigned char* g_na_ptr[NPX_NA_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up once per device with
+// that device current.
+int na_scratch_ptr(unsigned char** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NPX_NA_MAX_DEVICES) {
+    g_create_error = "npx_grid_query_noalloc: no usable HIP device";
+    return NPX_EDEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_create_error = std::string("npx_grid_query_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return NPX_EDEVICE;
+    }
+    g_na_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_na_ptr[dev];
+  return NPX_OK;
+}
+
+// Launch on the null stream; an empty grid is no launch.
+template <class K, class... A>
+hipError_t na_launch(K kern, int blocks, int threads, A... args) {
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(threads), 0, 0, args...);
+  return hipGetLastError();
+}
+
 // Every entry point of the C ABI runs its body inside guard(): no C++
 // exception crosses the boundary.  std::bad_alloc (host allocation) becomes
 // NPX_ENOMEM, any other exception NPX_EINTERNAL; the message goes to the
@@ -533,6 +612,129 @@ int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* q
     if (r != NPX_OK) g_create_error = c->err;
     destroy(c);
     return r;
+  });
+}
+
+int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const float* query3, int32_t nq,
+                           int32_t* counts, int32_t* lists, int64_t cap, int64_t* total) {
+  return guard(nullptr, __func__, [&]() -> int {
+    auto bad = [](int code, const char* msg) {
+      g_create_error = std::string("npx_grid_query_noalloc: ") + msg;
+      return code;
+    };
+    if (!total || n < 0 || nq < 0 || (n > 0 && !pos3) || (nq > 0 && !query3)) return bad(NPX_EINVAL, "bad arguments");
+    *total = 0;
+    if (n > NPX_NA_MAX_POINTS) return bad(NPX_EINVAL, "more than 65536 points");
+    if (nq > NPX_NA_MAX_QUERIES) return bad(NPX_EINVAL, "more than 65536 queries");
+    if (nq == 0) return NPX_OK;
+    if (!(cell_size > 0.0f)) return bad(NPX_EINVAL, "cell_size must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bad(NPX_EDEVICE, "no HIP device available");
+    // window: bounding cells of points and queries, 2 cells of margin (as npx_grid_query)
+    const float inv = 1.0f / cell_size;
+    int lo[3] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF}, hi[3] = {-0x7FFFFFFF, -0x7FFFFFFF, -0x7FFFFFFF};
+    for (int k = 0; k < n + nq; ++k) {
+      const float* p = k < n ? pos3 + 3 * (size_t)k : query3 + 3 * (size_t)(k - n);
+      for (int d = 0; d < 3; ++d) {
+        const float f = std::floor(p[d] * inv);
+        if (!(std::fabs(f) < 1.0e8f)) return bad(NPX_EDOMAIN, "a point or query is not finite");
+        lo[d] = std::min(lo[d], (int)f);
+        hi[d] = std::max(hi[d], (int)f);
+      }
+    }
+    GridWin W;
+    long long ncell = 1;
+    int wdim[3], wlo[3];
+    for (int d = 0; d < 3; ++d) {
+      wlo[d] = lo[d] - 2;
+      wdim[d] = hi[d] + 2 - wlo[d] + 1;
+      ncell *= wdim[d];
+    }
+    if (ncell > NPX_NA_MAX_CELLS) return bad(NPX_EDOMAIN, "points span more than 2^18 cells");
+    W.inv_h = inv;
+    W.wlo0 = wlo[0];
+    W.wlo1 = wlo[1];
+    W.wlo2 = wlo[2];
+    W.wx = wdim[0];
+    W.wy = wdim[1];
+    W.wz = wdim[2];
+    W.ncell = (int)ncell;
+    const int nc = (int)ncell;
+    unsigned char* base = nullptr;
+    NPX_TRY(na_scratch_ptr(&base));
+    using L = NaLayout;
+    auto* htk = reinterpret_cast<unsigned long long*>(base + L::htk);
+    auto* htv = reinterpret_cast<int*>(base + L::htv);
+    auto* clsmap = reinterpret_cast<int*>(base + L::clsmap);
+    auto* cst = reinterpret_cast<int*>(base + L::cst);
+    auto* bsum = reinterpret_cast<int*>(base + L::bsum);
+    auto* raw_p = reinterpret_cast<float*>(base + L::raw_p);
+    auto* P = reinterpret_cast<float4*>(base + L::P);
+    auto* PS = reinterpret_cast<float4*>(base + L::PS);
+    auto* raw_q = reinterpret_cast<float*>(base + L::raw_q);
+    auto* Q = reinterpret_cast<float4*>(base + L::Q);
+    auto* cls = reinterpret_cast<int*>(base + L::cls);
+    auto* off = reinterpret_cast<int*>(base + L::off);
+    auto* tperm = reinterpret_cast<int*>(base + L::tperm);
+    auto* tkey = reinterpret_cast<int*>(base + L::tkey);
+    auto* perm = reinterpret_cast<int*>(base + L::perm);
+    auto* cnt = reinterpret_cast<int*>(base + L::cnt);
+    auto* off64 = reinterpret_cast<long long*>(base + L::off64);
+    auto* out = reinterpret_cast<int*>(base + L::out);
+    unsigned htcap = 1;
+    while ((long long)htcap < 2 * ncell) htcap <<= 1;  // <= 2^19 = L::HT
+    hipError_t e = hipSuccess;
+    auto scan = [&](int* data, int len) {
+      const int nb = cdiv(len, 1024);
+      hipError_t r = na_launch(k_scan_reduce, nb, 256, (const int*)data, len, bsum);
+      if (r == hipSuccess) r = na_launch(k_scan_bsum, 1, 1024, bsum, nb);
+      if (r == hipSuccess) r = na_launch(k_scan_down, nb, 256, data, len, (const int*)bsum);
+      return r;
+    };
+    // inputs to the device, repacked as float4
+    if (n > 0) e = hipMemcpy(raw_p, pos3, sizeof(float) * 3 * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(raw_q, query3, sizeof(float) * 3 * (size_t)nq, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = na_launch(k_pack_float4, cdiv(n, 256), 256, (const float*)raw_p, n, P);
+    if (e == hipSuccess) e = na_launch(k_pack_float4, cdiv(nq, 256), 256, (const float*)raw_q, nq, Q);
+    // alias-class table of the window (the reference's hash buckets)
+    if (e == hipSuccess) e = na_launch(k_ht_clear, cdiv(htcap, 256), 256, htk, htv, (int)htcap);
+    if (e == hipSuccess)
+      e = na_launch(k_ht_insert, cdiv(nc, 256), 256, htk, htv, htcap - 1, wlo[0], wlo[1], wlo[2], wdim[0], wdim[1],
+                    nc);
+    if (e == hipSuccess)
+      e = na_launch(k_ht_lookup, cdiv(nc, 256), 256, (const unsigned long long*)htk, (const int*)htv, htcap - 1,
+                    wlo[0], wlo[1], wlo[2], wdim[0], wdim[1], nc, 0, clsmap);
+    // points sorted into classes, ascending index inside a class (bucket order)
+    if (e == hipSuccess) e = na_launch(k_fill_i32, cdiv(nc + 1, 256), 256, cst, 0, nc + 1);
+    if (e == hipSuccess) e = na_launch(k_aux_bin, cdiv(n, 256), 256, W, (const float4*)P, n, (const int*)clsmap, cst, cls, off);
+    if (e == hipSuccess) e = scan(cst, nc + 1);
+    if (e == hipSuccess)
+      e = na_launch(k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)cls, (const int*)off,
+                    (const int*)nullptr, tperm, tkey);
+    if (e == hipSuccess)
+      e = na_launch(k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey);
+    if (e == hipSuccess)
+      e = na_launch(k_aux_rank, cdiv(n, 256), 256, n, nc, (const int*)cst, (const int*)cls, (const int*)tperm,
+                    (const int*)tkey, (const float4*)P, perm, PS);
+    // candidate counts per query, then their exclusive prefix (cnt[nq] = total)
+    if (e == hipSuccess) e = na_launch(k_fill_i32, cdiv(nq + 1, 256), 256, cnt, 0, nq + 1);
+    if (e == hipSuccess)
+      e = na_launch(k_query_count, cdiv(nq, 256), 256, W, (const float4*)Q, nq, (const int*)clsmap, (const int*)cst, cnt);
+    if (e == hipSuccess && counts) e = hipMemcpy(counts, cnt, sizeof(int) * (size_t)nq, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = scan(cnt, nq + 1);
+    int t = 0;
+    if (e == hipSuccess) e = hipMemcpy(&t, cnt + nq, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return bad(NPX_EDEVICE, hipGetErrorString(e));
+    *total = t;
+    if (!lists || cap < t || t == 0) return NPX_OK;
+    if (t > NPX_NA_MAX_LIST) return bad(NPX_ENOMEM, "candidate lists longer than 2^21 entries");
+    e = na_launch(k_widen_i32, cdiv(nq, 256), 256, (const int*)cnt, nq, off64);
+    if (e == hipSuccess)
+      e = na_launch(k_query_lists, cdiv(nq, 256), 256, W, (const float4*)Q, nq, (const int*)clsmap, (const int*)cst,
+                    (const int*)perm, (const long long*)off64, out);
+    if (e == hipSuccess) e = hipMemcpy(lists, out, sizeof(int) * (size_t)t, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return bad(NPX_EDEVICE, hipGetErrorString(e));
+    return NPX_OK;
   });
 }
 

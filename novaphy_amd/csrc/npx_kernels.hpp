@@ -2494,6 +2494,18 @@ __global__ void k_coupling_b(const SceneC* __restrict__ scenes, const float4* __
   FB[b] = make_float4(fx, fy, fz, 0.0f);
 }
 
+// AoS float triples -> float4 (w = 0), for points copied to the device raw.
+__global__ void k_pack_float4(const float* __restrict__ p3, int n, float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float4(p3[3 * (size_t)i], p3[3 * (size_t)i + 1], p3[3 * (size_t)i + 2], 0.0f);
+}
+
+// Widens int32 offsets to the int64 offsets the list writer takes.
+__global__ void k_widen_i32(const int* __restrict__ in, int n, long long* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (long long)in[i];
+}
+
 // query_neighbors for arbitrary points (standalone SpatialHashGrid API).
 __global__ void k_query_count(GridWin W, const float4* __restrict__ Q, int nq, const int* __restrict__ clsmap,
                               const int* __restrict__ cstart, int* __restrict__ counts) {
