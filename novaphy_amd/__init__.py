@@ -23,7 +23,8 @@ __all__ = [
     "ParticleState", "PBFSettings", "PBFSolver", "PerformanceMetric", "PerformanceMonitor",
     "PerformancePhaseStat", "RigidBody", "ShapeType", "SimState", "SolverSettings", "SPHKernels",
     "SpatialHashGrid", "Transform", "World", "generate_fluid_block", "sample_model_boundaries",
-    "device_count", "FLAG_FAST_MATH",
+    "device_count", "FLAG_FAST_MATH", "AABB", "ContactPoint", "BroadPhasePair", "SweepAndPrune",
+    "collide_shapes",
 ]
 
 

@@ -122,6 +122,15 @@ PYBIND11_MODULE(_core, m) {
             for (int i = 0; i < 9; ++i) b.inertia(i / 3, i % 3) = a.data()[i];
           })
       .def("inv_mass", &RigidBody::inv_mass)
+      .def("inv_inertia",
+           [](const RigidBody& b) {
+             const Mat3f I = b.inv_inertia();
+             py::array_t<float> a({3, 3});
+             auto mm = a.mutable_unchecked<2>();
+             for (int i = 0; i < 3; ++i)
+               for (int j = 0; j < 3; ++j) mm(i, j) = I(i, j);
+             return a;
+           })
       .def("is_static", &RigidBody::is_static)
       .def_static("from_box", &RigidBody::from_box, py::arg("mass"), py::arg("half_extents"))
       .def_static("from_sphere", &RigidBody::from_sphere, py::arg("mass"), py::arg("radius"))
@@ -143,6 +152,11 @@ PYBIND11_MODULE(_core, m) {
       .def_readwrite("radius", &CollisionShape::radius)
       .def_readwrite("normal", &CollisionShape::normal)
       .def_readwrite("offset", &CollisionShape::offset)
+      // the reference's names for the primitive payloads (bind_core.cpp:182-205)
+      .def_readwrite("box_half_extents", &CollisionShape::half_extents)
+      .def_readwrite("sphere_radius", &CollisionShape::radius)
+      .def_readwrite("plane_normal", &CollisionShape::normal)
+      .def_readwrite("plane_offset", &CollisionShape::offset)
       .def_static("make_box", &CollisionShape::make_box, py::arg("half_extents"), py::arg("body_index"),
                   py::arg("local") = Transform::identity(), py::arg("friction") = 0.5f,
                   py::arg("restitution") = 0.3f)
@@ -168,6 +182,47 @@ PYBIND11_MODULE(_core, m) {
       .def("build", &ModelBuilder::build)
       .def_property_readonly("num_bodies", &ModelBuilder::num_bodies)
       .def_property_readonly("num_shapes", &ModelBuilder::num_shapes);
+
+  // ---------------------------------------------------------- collision
+  // bind_core.cpp:94-120, 266-295; bind_collision.cpp:13-80
+  py::class_<AABB>(m, "AABB")
+      .def(py::init<>())
+      .def(py::init<const Vec3f&, const Vec3f&>(), py::arg("min"), py::arg("max"))
+      .def_readwrite("min", &AABB::min)
+      .def_readwrite("max", &AABB::max)
+      .def("overlaps", &AABB::overlaps, py::arg("other"))
+      .def_static("from_sphere", &AABB::from_sphere, py::arg("center"), py::arg("radius"))
+      .def_static("from_oriented_box", &AABB::from_oriented_box, py::arg("half_extents"), py::arg("transform"));
+  py::class_<ContactPoint>(m, "ContactPoint")
+      .def(py::init<>())
+      .def_readwrite("position", &ContactPoint::position)
+      .def_readwrite("normal", &ContactPoint::normal)
+      .def_readwrite("penetration", &ContactPoint::penetration)
+      .def_readwrite("body_a", &ContactPoint::body_a)
+      .def_readwrite("body_b", &ContactPoint::body_b)
+      .def_readwrite("shape_a", &ContactPoint::shape_a)
+      .def_readwrite("shape_b", &ContactPoint::shape_b)
+      .def_readwrite("friction", &ContactPoint::friction)
+      .def_readwrite("restitution", &ContactPoint::restitution)
+      .def_readwrite("accumulated_normal_impulse", &ContactPoint::accumulated_normal_impulse)
+      .def_readwrite("accumulated_tangent_impulse_1", &ContactPoint::accumulated_tangent_impulse_1)
+      .def_readwrite("accumulated_tangent_impulse_2", &ContactPoint::accumulated_tangent_impulse_2);
+  py::class_<BroadPhasePair>(m, "BroadPhasePair")
+      .def(py::init<>())
+      .def_readwrite("body_a", &BroadPhasePair::body_a)
+      .def_readwrite("body_b", &BroadPhasePair::body_b);
+  py::class_<SweepAndPrune>(m, "SweepAndPrune")
+      .def(py::init<>())
+      .def("update", &SweepAndPrune::update, py::arg("body_aabbs"), py::arg("static_mask"))
+      .def("get_pairs", &SweepAndPrune::get_pairs);
+  m.def(
+      "collide_shapes",
+      [](const CollisionShape& a, const Transform& ta, const CollisionShape& b, const Transform& tb) {
+        std::vector<ContactPoint> contacts;
+        const bool hit = collide_shapes(a, ta, b, tb, contacts);
+        return std::make_pair(hit, contacts);
+      },
+      py::arg("shape_a"), py::arg("transform_a"), py::arg("shape_b"), py::arg("transform_b"));
 
   // ---------------------------------------------------------------- sim
   py::class_<SolverSettings>(m, "SolverSettings")
@@ -231,6 +286,7 @@ PYBIND11_MODULE(_core, m) {
       .def_property_readonly("gravity", &World::gravity)
       .def_property_readonly("state", py::overload_cast<>(&World::state), py::return_value_policy::reference_internal)
       .def_property_readonly("model", &World::model, py::return_value_policy::reference_internal)
+      .def_property_readonly("contacts", &World::contacts)
       .def_property_readonly("performance_monitor", &World::performance_monitor,
                              py::return_value_policy::reference_internal)
       .def_property_readonly("solver_settings", &World::solver_settings,

@@ -8,6 +8,7 @@
 // (include/novaphy_hip.h); the rigid-body pipeline stays on the host.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -70,6 +71,14 @@ struct Vec3f {
     }
     return *this;
   }
+  Vec3f operator/(float s) const { return {v[0] / s, v[1] / s, v[2] / s}; }
+  Vec3f cwiseMin(const Vec3f& o) const {
+    return {std::min(v[0], o.v[0]), std::min(v[1], o.v[1]), std::min(v[2], o.v[2])};
+  }
+  Vec3f cwiseMax(const Vec3f& o) const {
+    return {std::max(v[0], o.v[0]), std::max(v[1], o.v[1]), std::max(v[2], o.v[2])};
+  }
+  static Vec3f Constant(float c) { return {c, c, c}; }
 };
 inline Vec3f operator*(float s, const Vec3f& a) { return {s * a.v[0], s * a.v[1], s * a.v[2]}; }
 
@@ -83,15 +92,17 @@ struct Mat3f {
   }
   float& operator()(int i, int j) { return m[i][j]; }
   float operator()(int i, int j) const { return m[i][j]; }
+  // Products sum their three terms as a0 + (a1 + a2), the order of Eigen's
+  // unrolled reduction for fixed size 3 (the same as dot / squaredNorm).
   Vec3f operator*(const Vec3f& a) const {
     Vec3f r;
-    for (int i = 0; i < 3; ++i) r[i] = m[i][0] * a[0] + m[i][1] * a[1] + m[i][2] * a[2];
+    for (int i = 0; i < 3; ++i) r[i] = m[i][0] * a[0] + (m[i][1] * a[1] + m[i][2] * a[2]);
     return r;
   }
   Mat3f operator*(const Mat3f& b) const {
     Mat3f r;
     for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][0] * b.m[0][j] + m[i][1] * b.m[1][j] + m[i][2] * b.m[2][j];
+      for (int j = 0; j < 3; ++j) r.m[i][j] = m[i][0] * b.m[0][j] + (m[i][1] * b.m[1][j] + m[i][2] * b.m[2][j]);
     return r;
   }
   Mat3f operator*(float s) const {
@@ -100,6 +111,7 @@ struct Mat3f {
       for (auto& e : row) e *= s;
     return r;
   }
+  Vec3f col(int j) const { return {m[0][j], m[1][j], m[2][j]}; }
   Mat3f transpose() const {
     Mat3f r;
     for (int i = 0; i < 3; ++i)
@@ -242,13 +254,78 @@ struct SolverSettings {  // free_body_solver.h:20-31
   float sleep_ema_alpha = 0.8f;
 };
 
-struct ContactPoint {  // contact.h
+struct ContactPoint {  // contact.h:17-40; normal points from body A toward body B
   Vec3f position, normal;
   float penetration = 0.0f;
   int body_a = -1, body_b = -1, shape_a = -1, shape_b = -1;
   float friction = 0.5f, restitution = 0.0f;
-  float acc_n = 0.0f, acc_t1 = 0.0f, acc_t2 = 0.0f;
+  float accumulated_normal_impulse = 0.0f;
+  float accumulated_tangent_impulse_1 = 0.0f;
+  float accumulated_tangent_impulse_2 = 0.0f;
 };
+
+inline float combine_friction(float a, float b) { return std::sqrt(a * b); }          // contact.h:55-57
+inline float combine_restitution(float a, float b) { return std::max(a, b); }        // contact.h:59-61
+
+// Axis-aligned box (aabb.h:16-140).
+struct AABB {
+  Vec3f min = Vec3f::Constant(3.40282347e38f);
+  Vec3f max = Vec3f::Constant(-3.40282347e38f);
+  AABB() = default;
+  AABB(const Vec3f& lo, const Vec3f& hi) : min(lo), max(hi) {}
+  bool overlaps(const AABB& o) const {
+    return (min.x() <= o.max.x() && max.x() >= o.min.x()) && (min.y() <= o.max.y() && max.y() >= o.min.y()) &&
+           (min.z() <= o.max.z() && max.z() >= o.min.z());
+  }
+  void expand(const Vec3f& p) {
+    min = min.cwiseMin(p);
+    max = max.cwiseMax(p);
+  }
+  static AABB from_sphere(const Vec3f& c, float r) {
+    const Vec3f e(r, r, r);
+    return AABB(c - e, c + e);
+  }
+  static AABB from_oriented_box(const Vec3f& he, const Transform& t) {  // the 8 transformed corners
+    AABB a;
+    for (int i = 0; i < 8; ++i)
+      a.expand(t.transform_point(Vec3f((i & 1) ? he[0] : -he[0], (i & 2) ? he[1] : -he[1], (i & 4) ? he[2] : -he[2])));
+    return a;
+  }
+};
+
+// Broadphase pair in canonical order (broadphase.h:21-30).
+struct BroadPhasePair {
+  int body_a = 0, body_b = 0;
+  bool operator==(const BroadPhasePair& o) const { return body_a == o.body_a && body_b == o.body_b; }
+};
+
+// Sweep and prune over shape AABBs (broadphase.h:37-96, broadphase.cpp:16-118):
+// endpoints sorted along x (std::sort on the first update, insertion sort
+// after, keeping the previous order), a sweep from each min endpoint to the
+// AABB's max, y / z overlap tests, static-static pairs skipped, then the pairs
+// sorted and deduplicated.  The result is every AABB-overlapping non-static
+// pair in ascending (a, b) order.
+class SweepAndPrune {
+ public:
+  void update(const std::vector<AABB>& aabbs, const std::vector<bool>& static_mask);
+  const std::vector<BroadPhasePair>& get_pairs() const { return pairs_; }
+
+ private:
+  struct Endpoint {
+    float value;
+    int body_index;
+    bool is_min;
+  };
+  std::vector<Endpoint> ex_, ey_, ez_;
+  std::vector<BroadPhasePair> pairs_;
+  bool initialized_ = false;
+};
+
+// Narrowphase dispatcher (narrowphase.cpp:384-432): appends the contacts of
+// one shape pair; true when they touch.
+bool collide_shapes(const CollisionShape& a, const Transform& ta, const CollisionShape& b, const Transform& tb,
+                    std::vector<ContactPoint>& contacts);
+AABB shape_aabb(const CollisionShape& s, const Transform& body_transform);  // CollisionShape::compute_aabb
 
 struct SimState {  // state.h
   std::vector<Transform> transforms;
@@ -373,13 +450,19 @@ class World {
   PerformanceMonitor monitor_;
 
  private:
-  void collide(std::vector<ContactPoint>& out);
   void solve_contacts(float dt);
   Model model_;
   SimState state_;
   SolverSettings settings_;
   Vec3f gravity_{0.0f, -9.81f, 0.0f};
   std::vector<ContactPoint> contacts_;
+  SweepAndPrune broadphase_;
+  struct ConstraintData {  // free_body_solver.h:110-120
+    Vec3f r_a, r_b, tangent1, tangent2;
+    float effective_mass_n = 0.0f, effective_mass_t1 = 0.0f, effective_mass_t2 = 0.0f, bias = 0.0f;
+    Mat3f inv_I_a_world, inv_I_b_world;
+  };
+  std::vector<ConstraintData> constraint_data_;
 };
 
 // ------------------------------------------------------------ fluid

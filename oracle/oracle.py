@@ -32,9 +32,33 @@ class OrcShape(C.Structure):
                 ("local_pos", C.c_float * 3), ("local_rot", C.c_float * 4)]
 
 
+class OrcRBody(C.Structure):
+    _fields_ = [("mass", C.c_float), ("inertia", C.c_float * 9)]
+
+
+class OrcRShape(C.Structure):
+    _fields_ = [("type", C.c_int32), ("body", C.c_int32), ("half", C.c_float * 3), ("radius", C.c_float),
+                ("normal", C.c_float * 3), ("offset", C.c_float), ("local_pos", C.c_float * 3),
+                ("local_rot", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float)]
+
+
+class OrcRSettings(C.Structure):
+    _fields_ = [("velocity_iterations", C.c_int32), ("baumgarte", C.c_float), ("slop", C.c_float),
+                ("warm_starting", C.c_int32)]
+
+
+class OrcContact(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("normal", C.c_float * 3), ("penetration", C.c_float),
+                ("body_a", C.c_int32), ("body_b", C.c_int32), ("friction", C.c_float), ("restitution", C.c_float),
+                ("acc_n", C.c_float), ("acc_t1", C.c_float), ("acc_t2", C.c_float)]
+
+
+_SOURCES = ("pbf_oracle.cpp", "rigid_oracle.cpp")
+
+
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH) or (
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "pbf_oracle.cpp"))):
+    newest = max(os.path.getmtime(os.path.join(HERE, f)) for f in _SOURCES)
+    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < newest:
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
     return LIB_PATH
 
@@ -109,6 +133,14 @@ def lib():
         L.orc_coupling_f64.argtypes = [C.c_int32, f64p, f64p, C.c_int32, f64p, f64p, i32p,
                                        C.c_int32, f64p, u8p, C.c_double, C.c_double, C.c_double,
                                        f64p, f64p]
+        L.orc_collide_shapes.restype = C.c_int32
+        L.orc_collide_shapes.argtypes = [C.POINTER(OrcRShape), f32p, C.POINTER(OrcRShape), f32p, vp, C.c_int32]
+        L.orc_sap_pairs.restype = C.c_int32
+        L.orc_sap_pairs.argtypes = [C.c_int32, f32p, f32p, u8p, i32p, C.c_int32]
+        L.orc_rigid_step.restype = C.c_int32
+        L.orc_rigid_step.argtypes = [vp, C.c_int32, vp, C.c_int32, C.POINTER(OrcRSettings), f32p, C.c_float, f32p,
+                                     f32p, f32p, f32p, f32p, vp, C.c_int32, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]
         L.orc_vorticity_confinement.restype = C.c_int32
         L.orc_vorticity_confinement.argtypes = [C.POINTER(OrcParams), C.c_int32, f32p, f32p, f32p, C.c_float,
                                                 C.c_float, f32p, f32p, f32p, f32p]
@@ -391,3 +423,99 @@ def vorticity_confinement(prm, pos, vel, rho, mass, dt):
     outs = [np.zeros((n, 3), np.float32) for _ in range(4)]
     lib().orc_vorticity_confinement(C.byref(op), n, pos, vel, rho, np.float32(mass), np.float32(dt), *outs)
     return tuple(outs)
+
+
+# ------------------------------------------------------------ rigid bodies
+SHAPE_R_BOX, SHAPE_R_SPHERE, SHAPE_R_PLANE = 0, 1, 2
+
+
+def rigid_shape(kind, body, half=(0.5, 0.5, 0.5), radius=0.5, normal=(0, 1, 0), offset=0.0,
+                local_pos=(0, 0, 0), local_rot=(1, 0, 0, 0), friction=0.5, restitution=0.3):
+    """CollisionShape fields (shape.h:43-60); local_rot is (w, x, y, z)."""
+    s = OrcRShape()
+    s.type, s.body = int(kind), int(body)
+    s.half[:] = [float(v) for v in half]
+    s.radius, s.offset = float(radius), float(offset)
+    s.normal[:] = [float(v) for v in normal]
+    s.local_pos[:] = [float(v) for v in local_pos]
+    s.local_rot[:] = [float(v) for v in local_rot]
+    s.friction, s.restitution = float(friction), float(restitution)
+    return s
+
+
+def rigid_body(mass, inertia):
+    b = OrcRBody()
+    b.mass = float(mass)
+    b.inertia[:] = [float(v) for v in np.asarray(inertia, np.float32).reshape(9)]
+    return b
+
+
+def _contacts(buf, n):
+    keys = ("position", "normal", "penetration", "body_a", "body_b", "friction", "restitution", "acc_n",
+            "acc_t1", "acc_t2")
+    out = []
+    for c in buf[:n]:
+        d = {k: getattr(c, k) for k in keys}
+        d["position"] = np.array(d["position"][:], np.float32)
+        d["normal"] = np.array(d["normal"][:], np.float32)
+        out.append(d)
+    return out
+
+
+def collide_shapes(a, pose_a7, b, pose_b7):
+    """Narrowphase of one shape pair (narrowphase.cpp:384-432): list of contact dicts."""
+    buf = (OrcContact * 16)()
+    n = lib().orc_collide_shapes(C.byref(a), np.ascontiguousarray(pose_a7, np.float32), C.byref(b),
+                                 np.ascontiguousarray(pose_b7, np.float32), buf, 16)
+    return _contacts(buf, n)
+
+
+def sap_pairs(lo, hi, static):
+    lo = np.ascontiguousarray(lo, np.float32).reshape(-1, 3)
+    hi = np.ascontiguousarray(hi, np.float32).reshape(-1, 3)
+    st = np.ascontiguousarray(static, np.uint8)
+    n = lo.shape[0]
+    cap = n * (n - 1) // 2 + 1
+    out = np.zeros(2 * cap, np.int32)
+    k = lib().orc_sap_pairs(n, lo, hi, st, out, cap)
+    return [tuple(int(v) for v in out[2 * i:2 * i + 2]) for i in range(k)]
+
+
+class RigidWorld:
+    """World::step_rigid_pipeline restated (rigid_oracle.cpp): bodies and
+    shapes as built by a ModelBuilder, state as float32 arrays."""
+
+    def __init__(self, bodies, shapes, poses7, velocity_iterations=30, baumgarte=0.3, slop=0.005,
+                 warm_starting=True, gravity=(0.0, -9.81, 0.0)):
+        self.bodies = (OrcRBody * max(len(bodies), 1))(*bodies)
+        self.nb = len(bodies)
+        self.shapes = (OrcRShape * max(len(shapes), 1))(*shapes)
+        self.ns = len(shapes)
+        self.settings = OrcRSettings(int(velocity_iterations), float(baumgarte), float(slop), int(warm_starting))
+        self.gravity = np.asarray(gravity, np.float32)
+        self.pose7 = np.ascontiguousarray(poses7, np.float32).reshape(-1, 7).copy()
+        z = np.zeros((self.nb, 3), np.float32)
+        self.lin, self.ang, self.force, self.torque = z.copy(), z.copy(), z.copy(), z.copy()
+        self.contacts = []
+        self.npairs = 0
+
+    def apply_force(self, i, f):
+        if 0 <= i < self.nb:
+            self.force[i] = (self.force[i] + np.asarray(f, np.float32)).astype(np.float32)
+
+    def apply_torque(self, i, t):
+        if 0 <= i < self.nb:
+            self.torque[i] = (self.torque[i] + np.asarray(t, np.float32)).astype(np.float32)
+
+    def step(self, dt):
+        cap = 64 * max(self.ns, 1)
+        buf = (OrcContact * cap)()
+        nc, npairs = C.c_int32(), C.c_int32()
+        for a in (self.pose7, self.lin, self.ang, self.force, self.torque):
+            assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float32
+        lib().orc_rigid_step(self.bodies, self.nb, self.shapes, self.ns, C.byref(self.settings), self.gravity,
+                             np.float32(dt), self.pose7.reshape(-1), self.lin.reshape(-1), self.ang.reshape(-1),
+                             self.force.reshape(-1), self.torque.reshape(-1), buf, cap, C.byref(nc),
+                             C.byref(npairs))
+        self.contacts = _contacts(buf, min(nc.value, cap))
+        self.npairs = npairs.value
