@@ -13,13 +13,19 @@ grows with the cluster size; the same scene with PBFSettings' default s_corr
 "default_s_corr" (--s-corr picks one or both).
 
 A step is one PBFSolver::step over all particles (predict, grid, 4 x
-(density+lambda, delta-p), velocity update, vorticity, XSPH, commit).  Inputs
-are resident on the device before the timed region.  The timed region is
-bracketed by device synchronisation on both sides; per-kernel durations come
-from HIP events recorded on the context's stream during the same region.
+(density+lambda, delta-p), velocity update, vorticity, XSPH, commit).  The
+timed region is the first K steps (default 500, the scene's specified run)
+from the initial state, after W untimed warm-up steps on the same context;
+inputs are resident on the device, the region is bracketed by device
+synchronisation, and per-kernel durations come from HIP events recorded on
+the context's stream over the same region (window_ms_per_step gives the
+step time across it).  The default run also reports, as extra keys, the
+batched workload (1024 scenes; the batch and the plain per-scene loop timed
+on the same step window) and a served sample (the first requests of the
+seeded Poisson stream).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch|coupled]
-                  [--s-corr both|configured|default] [--copy-calibration]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch|coupled|serve]
+                  [--s-corr both|configured|default] [--extras 0|1] [--no-graph] [--copy-calibration]
 """
 from __future__ import annotations
 
@@ -88,7 +94,39 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_counters(variant: str, kernel: str):
+    """VALU busy fraction and active lanes per VALU instruction of `kernel` in
+    the 1M workload `variant`, from the rocprofv3 SQ counter runs summarised in
+    profiles/pbf_1m_valu_counters.md (None when not profiled)."""
+    path = os.path.join(ROOT, "profiles", "pbf_1m_valu_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "valu" and f[1] == variant and f[2] == kernel:
+            return {"valu_busy": float(f[3]), "active_lanes_per_valu_instr": float(f[4])}
+    return None
+
+
+def window_ms(trace, steps, window):
+    """Mean ms per step over consecutive windows of `window` steps, from the
+    kernel trace of the timed region (each step starts with its first
+    kernel, "fill_counts")."""
+    starts = [st for name, st, _ in trace if name == "fill_counts"]
+    last_end = max(st + du for _, st, du in trace)
+    starts = starts[:steps] + [last_end]
+    out = []
+    for a in range(0, len(starts) - 1, window):
+        b = min(a + window, len(starts) - 1)
+        out.append({"steps": f"{a + 1}-{b}", "ms_per_step": round((starts[b] - starts[a]) / (b - a), 3)})
+    return out
+
+
 def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
+    """The timed region is the first `steps` steps of the workload from its
+    initial state (after `warmup` untimed steps on the same context, which is
+    then reset to the initial state).  Per-kernel HIP events are recorded on
+    the context's stream over the whole region."""
     from novaphy_amd import capi
     n = sc.n
     ctx = capi.Context(sc.params, capacity=n, flags=flags)
@@ -96,20 +134,21 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
     log(f"[bench] {sc.name}: n={n}, warmup {warmup} steps")
     ctx.step(sc.dt, sc.gravity, sc.mass, warmup)
     ctx.synchronize()
-    start_state = ctx.get_particles()
+    ctx.set_particles(sc.positions, sc.velocities)  # back to the initial state
+    start_state = (sc.positions, sc.velocities)
     ctx.set_kernel_timing(True)
     ctx.synchronize()
-    log(f"[bench] timed region: {steps} steps")
+    log(f"[bench] timed region: steps 1-{steps} from the initial state")
     t0 = time.perf_counter()
     ctx.step(sc.dt, sc.gravity, sc.mass, steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     kt_raw = ctx.kernel_times()
+    trace = ctx.kernel_trace()
     kt = fold_dense(kt_raw)
     ctx.set_kernel_timing(False)
     wall = t1 - t0
     value = n * steps / wall
-    # dominant kernel by total time
     iters = sc.params.solver_iterations
     vort = sc.params.vorticity_epsilon != 0
     PASS_BYTES = pass_bytes(vort)
@@ -118,16 +157,22 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
     avg_s = ms_tot / 1e3 / launches
     achieved = PASS_BYTES[main] * n / avg_s / 1e9
     B = step_bytes(iters, vort)
+    full = variant and n == 1000000 and not flags
     out = {
         "value": value,
         "ms_per_step": wall * 1e3 / steps,
+        "window_ms_per_step": window_ms(trace, steps, max(1, steps // 10)),
         "roofline": {"bound": "hbm", "kernel": main,
                      "kernels": [k for k in (main, main + "_dense") if k in kt_raw],
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
-                     "traffic": profiled_traffic(variant, main) if (variant and n == 1000000 and not flags) else None,
+                     "traffic": profiled_traffic(variant, main) if full else None,
                      "bytes_per_particle": PASS_BYTES[main], "particles_per_launch": n,
-                     "avg_launch_us": round(avg_s * 1e6, 2)},
+                     "avg_launch_us": round(avg_s * 1e6, 2),
+                     # no MFMA work: the HBM roof is the kernel's roofline; what limits it
+                     # is VALU issue and gather latency (SQ counters, profiles/)
+                     "limiter": "valu issue / gather latency",
+                     "counters": profiled_counters(variant, main) if full else None},
         "step_roofline": {"bytes_per_particle_step": B, "achieved": round(B * value / 1e9, 2),
                           "unit": "GB/s", "frac": round(B * value / 1e9 / HBM_PEAK_GBPS, 5)},
         "kernels": {k: {"total_ms": round(v[0], 3), "launches": v[1],
@@ -162,24 +207,32 @@ def copy_calibration(nbytes=256 << 20, reps=3):
             "copy_GBps_read_plus_write": round(2 * nbytes * reps / wall / 1e9, 1)}
 
 
-def run_batch(count, steps, warmup, flags, loop_steps):
-    """Batched scenes (one launch per pass over every scene, graph-replayed)
-    against the plain per-scene loop of single-scene contexts."""
-    from novaphy_amd import capi, scenes
-    scs = scenes.batched_1024(count=count, steps=steps)
+def run_batch(count, steps, warmup, flags, loop, s_corr="configured"):
+    """Batched scenes (one launch per pass over every scene, each step one
+    hipGraph replay) against the plain per-scene loop of single-scene
+    contexts, both timed on the same step window: steps warmup+1 ..
+    warmup+steps from the initial state.  The launch tunables are chosen by
+    the autotuner on a separate batch of the same scenes first."""
+    from novaphy_amd import autotune, capi, scenes
+    scs = scenes.batched_1024(count=count, s_corr=s_corr)
     total = sum(s.n for s in scs)
     slots = sum(((s.n + 255) // 256) * 256 for s in scs)
-    cells = sum(int(np.prod(scene_window_dims(s))) for s in scs)
-    log(f"[bench] batch: {count} scenes, {total} particles, {cells} cells")
-    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=count, flags=flags)
-    for s in scs:
-        b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities)
-    from novaphy_amd import autotune
-    # tuning steps are real steps of the workload (results are bitwise
-    # independent of the tunables), so they count towards the warm-up
+    cells = sum(scenes.window_cells(s) for s in scs)
+    log(f"[bench] batch: {count} scenes, {total} particles, {cells} cells, s_corr {s_corr}")
+
+    def make_batch():
+        b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=count, flags=flags)
+        for s in scs:
+            b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities)
+        return b
+
+    b = make_batch()
     best, tried = autotune.autotune(b, steps_per_candidate=1)
+    b.close()
     log(f"[bench] autotuner: block {best[0]}, xcd order {best[1]}, skin {best[2]}h "
         f"({', '.join(f'{k[0]}/{k[1]}/{k[2]}: {v:.1f} ms' for k, v in tried.items())})")
+    b = make_batch()
+    b.set_tuning(*best)
     b.step(warmup)
     b.synchronize()
     t0 = time.perf_counter()
@@ -188,16 +241,17 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     b.close()
     value = total * steps / wall
     out = {"value": value, "ms_per_step": wall * 1e3 / steps, "particles": total, "scenes": count,
+           "window": f"steps {warmup + 1}-{warmup + steps}",
            "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99)),
            "tuning": {"block": best[0], "xcd_order": best[1], "skin": best[2]}}
-    if loop_steps <= 0:
+    if not loop:
         out.update(loop_value=None, loop_ms_per_step=None, speedup_vs_loop=None)
         return out
-    # plain per-scene loop: one single-scene context per scene, stepped in turn
-    log(f"[bench] per-scene loop: {count} contexts, {loop_steps} steps")
+    log(f"[bench] per-scene loop: {count} contexts, steps {warmup + 1}-{warmup + steps}")
     ctxs = []
     for s in scs:
         c = capi.Context(s.params, capacity=s.n, flags=flags)
+        c.set_tuning(*best)
         c.set_particles(s.positions, s.velocities)
         ctxs.append((c, s))
     for c, s in ctxs:
@@ -205,7 +259,7 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     for c, _ in ctxs:
         c.synchronize()
     t0 = time.perf_counter()
-    for _ in range(loop_steps):
+    for _ in range(steps):
         for c, s in ctxs:
             c.step(s.dt, s.gravity, s.mass, 1)
     for c, _ in ctxs:
@@ -213,56 +267,16 @@ def run_batch(count, steps, warmup, flags, loop_steps):
     lwall = time.perf_counter() - t0
     for c, _ in ctxs:
         c.close()
-    out["loop_value"] = total * loop_steps / lwall
-    out["loop_ms_per_step"] = lwall * 1e3 / loop_steps
+    out["loop_value"] = total * steps / lwall
+    out["loop_ms_per_step"] = lwall * 1e3 / steps
     out["speedup_vs_loop"] = value / out["loop_value"]
     return out
 
 
 def coupled_world(seed=2):
-    """500k-particle block (100x50x100) in a 10x5x5 m box, a ground plane
-    sampled with boundary particles over the box, and 64 rigid bodies: 32
-    spheres (r ~ U(0.1, 0.2)) and 32 boxes (half extents ~ U(0.05, 0.15)^3),
-    density ~ U(300, 2000), seeded positions over the block footprint 0.3-1.0
-    m above its surface."""
-    import novaphy_amd as nv
+    """The rigid-fluid coupling workload (scenes.coupled_500k_world)."""
     from novaphy_amd import scenes
-    sc = scenes.coupled_500k_fluid()
-    rng = np.random.default_rng(seed)
-    b = nv.ModelBuilder()
-    b.add_ground_plane(y=0.0)
-    top = float(sc.positions[:, 1].max())
-    lo = sc.positions.min(0)
-    hi = sc.positions.max(0)
-    for k in range(64):
-        x = rng.uniform(lo[0] + 0.3, hi[0] - 0.3)
-        z = rng.uniform(lo[2] + 0.3, hi[2] - 0.3)
-        y = top + rng.uniform(0.3, 1.0)
-        dens = rng.uniform(300.0, 2000.0)
-        t = nv.Transform.from_translation(np.array([x, y, z], np.float32))
-        if k < 32:
-            r = rng.uniform(0.1, 0.2)
-            i = b.add_body(nv.RigidBody.from_sphere(float(dens * 4.0 / 3.0 * np.pi * r ** 3), float(r)), t)
-            b.add_shape(nv.CollisionShape.make_sphere(float(r), i))
-        else:
-            he = rng.uniform(0.05, 0.15, 3).astype(np.float32)
-            i = b.add_body(nv.RigidBody.from_box(float(dens * 8.0 * np.prod(he)), he), t)
-            b.add_shape(nv.CollisionShape.make_box(he, i))
-    model = b.build()
-    prm = nv.PBFSettings()
-    p = sc.params
-    for f in ("rest_density", "kernel_radius", "particle_radius", "solver_iterations", "epsilon",
-              "xsph_viscosity", "vorticity_epsilon", "corr_delta_q", "corr_k", "corr_n", "use_domain_bounds"):
-        setattr(prm, f, getattr(p, f))
-    prm.domain_lower = np.array(p.domain_lower, np.float32)
-    prm.domain_upper = np.array(p.domain_upper, np.float32)
-    blk = nv.FluidBlockDef()
-    blk.particle_spacing = 0.05
-    blk.lower = np.zeros(3, np.float32)
-    blk.upper = np.zeros(3, np.float32)
-    world = nv.FluidWorld(model, [blk], nv.SolverSettings(), prm, boundary_extent=10.0)
-    world.fluid_state.init_arrays(sc.positions, sc.velocities)
-    return world, sc
+    return scenes.coupled_500k_world(seed)
 
 
 def run_coupled(steps, warmup):
@@ -323,28 +337,24 @@ def run_serve(seconds, rate_hz, threads, backend="fused"):
             "mean_active": float(np.mean(active)) if active else 0.0, "rounds": srv.steps}
 
 
-def scene_window_dims(sc):
-    h = sc.params.kernel_radius
-    inv = np.float32(1.0) / np.float32(h)
-    lo = [int(np.floor(np.float32(v) * inv)) - 4 for v in sc.params.domain_lower]
-    hi = [int(np.floor(np.float32(v) * inv)) + 4 for v in sc.params.domain_upper]
-    return [b - a + 1 for a, b in zip(lo, hi)]
-
-
 def cpu_baseline(sc, state, budget_s):
     """The oracle (serial FP32 restatement of the reference's CPU solver, same
-    unordered_map grid and candidate vectors) on this host's cores, from the
-    GPU timed region's starting state."""
+    unordered_map grid and candidate vectors) on one host core, from the GPU
+    timed region's starting state: one untimed warm-up step, then as many
+    timed steps as fit in the budget (at least one)."""
     from oracle import oracle as O
     O.build()
     pos, vel = state[0], state[1]
     sim = O.Sim(sc.params, pos, vel)
+    tw = time.perf_counter()
+    sim.step(sc.dt, sc.gravity, sc.mass, 1)  # warm-up
+    warm = time.perf_counter() - tw
     t0 = time.perf_counter()
     steps = 0
     while True:
         sim.step(sc.dt, sc.gravity, sc.mass, 1)
         steps += 1
-        if time.perf_counter() - t0 >= budget_s * 0.5 or steps >= 20:
+        if time.perf_counter() - t0 >= budget_s or steps >= 10:
             break
     wall = time.perf_counter() - t0
     sim.close()
@@ -353,30 +363,60 @@ def cpu_baseline(sc, state, budget_s):
     except Exception:
         aff = os.cpu_count()
     return {"value": sc.n * steps / wall, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} serial step(s) of the same {sc.n}-particle scene from the GPU timed "
-                      f"region's starting state ({wall:.1f} s; host has {aff} cores available)"}
+            "sample": f"steps 2-{steps + 1} of the same {sc.n}-particle scene from its initial state, serial, "
+                      f"after 1 untimed warm-up step ({warm:.1f} s); {wall:.1f} s timed.  Fewer than 10 timed "
+                      f"steps when one step exceeds the {budget_s:.0f} s CPU budget of the run "
+                      f"(host: {aff} cores available, 1 used: the reference is serial)"}
+
+
+def batch_summary(r, s_corr):
+    return {"workload": f"batched_{r['scenes']}: {r['scenes']} dam breaks, N log-uniform 2k-64k "
+                        f"({r['particles']} particles), s_corr {s_corr}, one launch per pass over all scenes, "
+                        "each step one hipGraph replay",
+            "value": r["value"], "ms_per_step": r["ms_per_step"], "window": r["window"],
+            "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
+            "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"],
+                               "window": r["window"]},
+            "speedup_vs_per_scene_loop": None if r["speedup_vs_loop"] is None else round(r["speedup_vs_loop"], 2),
+            "tuning": r["tuning"]}
+
+
+def serve_summary(r, rate, backend):
+    return {"workload": f"served: the first {r['requests']} of the seeded Poisson request stream at {rate}/s "
+                        "(seed 4; N log-uniform 4k-256k, horizon 50-500, 0-8 spheres with Akinci coupling), "
+                        f"admission against 80% of 288 GB, {backend} coupling"
+                        + (", boundaries sampled on the device" if backend == "fused" else ""),
+            "value": r["value"], "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
+            "serve": {k: r[k] for k in ("requests", "done", "unfinished", "failed", "wall_s", "latency_samples",
+                                        "mean_active")}}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500,
+                    help="timed steps from the initial state (the 1M scene is specified for 500 steps)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="1m", choices=["1m", "8k", "batch", "coupled", "serve"])
-    ap.add_argument("--serve-seconds", type=float, default=40.0, help="arrival window of the served sample")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="1m workload: also measure the batched and served workloads (extra keys)")
+    ap.add_argument("--serve-seconds", type=float, default=15.0, help="arrival window of the served sample")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-threads", type=int, default=8, help="host threads stepping coupled requests (mixed)")
     ap.add_argument("--serve-backend", default="fused", choices=["fused", "mixed"],
                     help="coupling of requests with spheres: fused into the batch, or separate FluidWorlds")
     ap.add_argument("--scenes", type=int, default=1024, help="scene count of the batch workload")
-    ap.add_argument("--loop-steps", type=int, default=5,
-                    help="timed steps of the per-scene loop baseline (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="timed steps of the batch and of the per-scene loop (the same window)")
+    ap.add_argument("--loop", type=int, default=1, help="also time the per-scene loop baseline (0: skip)")
+    ap.add_argument("--batch-s-corr", default="configured", choices=["configured", "default"])
     ap.add_argument("--fast-math", action="store_true",
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of hipGraph replay")
     ap.add_argument("--corr-k", type=float, default=None,
                     help="override the s_corr k of the workload (the configured value is 0.1)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
-                    help="budget of the host CPU baseline samples (0 disables)")
+                    help="budget of the host CPU baseline sample of the headline scene (0 disables)")
     ap.add_argument("--s-corr", default="both", choices=["both", "configured", "default"],
                     help="single-scene workloads: the configured s_corr (k=0.1, dq=0.2h; the headline), "
                          "PBFSettings' default (k=0.001, dq=0.3h), or both")
@@ -384,8 +424,9 @@ def main():
                     help="only run 256 MiB device-to-device copies (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(independent scenes shard across GPUs without exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported: every GPU "
+              "box of this build has one GPU (how the path would shard is described in DESIGN.md)",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         print(json.dumps(copy_calibration()), flush=True)
@@ -394,54 +435,33 @@ def main():
     if capi.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
-    flags = capi.FLAG_FAST_MATH if args.fast_math else 0
+    flags = (capi.FLAG_FAST_MATH if args.fast_math else 0) | (capi.FLAG_NO_GRAPH if args.no_graph else 0)
+    common = {"metric": METRIC, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
+              "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic"}
     if args.workload == "batch":
-        r = run_batch(args.scenes, args.steps, args.warmup, flags, args.loop_steps)
-        line = {
-            "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic",
-            "config": {"workload": f"batched_{args.scenes}: {args.scenes} dam breaks, N log-uniform "
-                                   f"2k-64k ({r['particles']} particles), one launch per pass over all "
-                                   "scenes (stream launches at this size; steps of at most 512k slots are "
-                                   "hipGraph-replayed)", "particles": r["particles"]},
-            "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
-            "per_scene_loop": {"value": r["loop_value"], "ms_per_step": r["loop_ms_per_step"]},
-            "speedup_vs_per_scene_loop": None if r["speedup_vs_loop"] is None else round(r["speedup_vs_loop"], 2),
-            "tuning": r["tuning"],
-        }
+        r = run_batch(args.scenes, args.batch_steps, args.warmup, flags, args.loop, args.batch_s_corr)
+        bs = batch_summary(r, args.batch_s_corr)
+        line = {**common, "value": r["value"], "steps": args.batch_steps, "warmup": args.warmup,
+                "ms_per_step": r["ms_per_step"], "config": {"workload": bs.pop("workload"),
+                                                            "particles": r["particles"]}, **bs}
         print(json.dumps(line), flush=True)
         return
     if args.workload == "serve":
         r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend)
-        line = {
-            "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
-            "steps": r["rounds"], "warmup": 0, "ms_per_step": r["p50_step_ms"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic",
-            "config": {"workload": f"served: {r['requests']} Poisson requests at {args.serve_rate}/s (seed 4), "
-                                   "N log-uniform 4k-256k, horizon 50-500, 0-8 spheres with Akinci coupling, "
-                                   f"admission against 80% of 288 GB, {args.serve_backend} coupling"
-                                   + (", boundaries sampled on the device" if args.serve_backend == "fused" else "")},
-            "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
-            "serve": {k: r[k] for k in ("requests", "done", "unfinished", "failed", "wall_s", "latency_samples",
-                                        "mean_active")},
-        }
+        ss = serve_summary(r, args.serve_rate, args.serve_backend)
+        line = {**common, "value": r["value"], "steps": r["rounds"], "warmup": 0, "ms_per_step": r["p50_step_ms"],
+                "config": {"workload": ss.pop("workload")}, **ss}
         print(json.dumps(line), flush=True)
         return
     if args.workload == "coupled":
         r = run_coupled(args.steps, args.warmup)
-        line = {
-            "metric": METRIC, "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic",
-            "config": {"workload": f"coupled_500k: {r['particles']} fluid particles + 64 rigid bodies "
-                                   f"({r['boundary_particles']} Akinci boundary particles incl. ground plane), "
-                                   "FluidWorld.step with host rigid pipeline", "particles": r["particles"]},
-            "phases_avg_ms": r["phases_avg_ms"],
-        }
+        line = {**common, "value": r["value"], "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": r["ms_per_step"],
+                "config": {"workload": f"coupled_500k: {r['particles']} fluid particles + 64 rigid bodies "
+                                       f"({r['boundary_particles']} Akinci boundary particles incl. ground "
+                                       "plane), FluidWorld.step with host rigid pipeline",
+                           "particles": r["particles"]},
+                "phases_avg_ms": r["phases_avg_ms"]}
         print(json.dumps(line), flush=True)
         return
     import dataclasses
@@ -455,45 +475,34 @@ def main():
         dflt = scenes.PBFParams()
         variants.append(("default", dataclasses.replace(base, params=dataclasses.replace(
             base.params, corr_k=dflt.corr_k, corr_delta_q=dflt.corr_delta_q, corr_n=dflt.corr_n))))
-    cpu_s = args.cpu_seconds / len(variants)
-    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, cpu_s, name)) for name, sc in variants]
+    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds if i == 0 else 0, name))
+               for i, (name, sc) in enumerate(variants)]
 
     def describe(sc):
         p = sc.params
         return (f"{sc.name}: {sc.n} FP32 particles, box {p.domain_upper}, h={p.kernel_radius}, "
                 f"{p.solver_iterations} iterations, vorticity {p.vorticity_epsilon}, "
-                f"s_corr k={p.corr_k} n={p.corr_n} dq={p.corr_delta_q}h, dt=1/120, "
-                f"{'fast-math' if args.fast_math else 'reference operation order'}")
+                f"s_corr k={p.corr_k} n={p.corr_n} dq={p.corr_delta_q}h, dt=1/120, steps 1-{args.steps} "
+                f"from the initial state, {'fast-math' if args.fast_math else 'reference operation order'}")
 
     name, sc, r = results[0]
-    line = {
-        "metric": METRIC,
-        "value": r["value"],
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": describe(sc), "particles": sc.n, "s_corr": name},
-        "roofline": r["roofline"],
-        "step_roofline": r["step_roofline"],
-        "kernels": r["kernels"],
-    }
+    line = {**common, "value": r["value"], "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": r["ms_per_step"],
+            "config": {"workload": describe(sc), "particles": sc.n, "s_corr": name},
+            "window_ms_per_step": r["window_ms_per_step"],
+            "roofline": r["roofline"], "step_roofline": r["step_roofline"], "kernels": r["kernels"]}
     if "cpu_baseline" in r:
         line["cpu_baseline"] = r["cpu_baseline"]
         line["speedup_vs_cpu"] = round(r["value"] / r["cpu_baseline"]["value"], 1)
     for name2, sc2, r2 in results[1:]:
-        extra = {"workload": describe(sc2), "value": r2["value"], "ms_per_step": r2["ms_per_step"],
-                 "roofline": r2["roofline"], "step_roofline": r2["step_roofline"], "kernels": r2["kernels"]}
-        if "cpu_baseline" in r2:
-            extra["cpu_baseline"] = r2["cpu_baseline"]
-            extra["speedup_vs_cpu"] = round(r2["value"] / r2["cpu_baseline"]["value"], 1)
-        line[f"{name2}_s_corr"] = extra
+        line[f"{name2}_s_corr"] = {"workload": describe(sc2), "value": r2["value"], "ms_per_step": r2["ms_per_step"],
+                                   "window_ms_per_step": r2["window_ms_per_step"], "roofline": r2["roofline"],
+                                   "step_roofline": r2["step_roofline"], "kernels": r2["kernels"]}
+    if args.workload == "1m" and args.extras:
+        rb = run_batch(args.scenes, args.batch_steps, args.warmup, flags, args.loop, args.batch_s_corr)
+        line["batched"] = batch_summary(rb, args.batch_s_corr)
+        rs = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend)
+        line["served"] = serve_summary(rs, args.serve_rate, args.serve_backend)
     print(json.dumps(line), flush=True)
 
 

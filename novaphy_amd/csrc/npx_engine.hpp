@@ -1100,17 +1100,15 @@ struct Ctx {
   }
   int graph_xs_final = 0;
 
-  // Steps of at most kGraphMaxSlots slots are replayed as a hipGraph, where
-  // launch overhead matters; larger steps run the same kernels as stream
-  // launches.  At 1M particles and for a batch of 1024 scenes, replay and
-  // stream launches measure the same (3.16 vs 3.13 ms/step; 38.5 ms).
-  static constexpr int kGraphMaxSlots = 1 << 19;
-
+  // Every step is replayed as a hipGraph (one capture per buffer parity and
+  // launch shape; a step is about 30 launches whatever the scene count), unless
+  // the context was created with NPX_FLAG_NO_GRAPH or per-kernel timing is on
+  // (it records an event pair around each launch).
   int step_n(int nsteps) {
     for (int k = 0; k < nsteps; ++k) {
       if (!any_particles()) return NPX_OK;  // n == 0: PBFSolver::step returns (pbf_solver.cpp:21)
       NPX_TRY(prepare_step());
-      const bool use_graph = !(flags & NPX_FLAG_NO_GRAPH) && !timing && hw_blocks * NPX_BLOCK <= kGraphMaxSlots;
+      const bool use_graph = !(flags & NPX_FLAG_NO_GRAPH) && !timing;
       if (use_graph)
         NPX_TRY(step_graph());
       else

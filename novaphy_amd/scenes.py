@@ -168,27 +168,37 @@ def batched_scene_sizes(count: int = 1024, seed: int = 3, lo: int = 2000, hi: in
     return out
 
 
+def with_s_corr(sc: Scene, s_corr: str) -> Scene:
+    """s_corr "configured": the dam breaks' k = 0.1, n = 4, dq = 0.2h (the
+    parameters every workload shares); "default": PBFSettings' defaults
+    k = 0.001, dq = 0.3h (pbf_solver.h:21-23), under which the block stays a
+    liquid instead of collapsing into stacks."""
+    if s_corr == "default":
+        sc.params = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
+    elif s_corr != "configured":
+        raise ValueError(f"s_corr must be 'configured' or 'default', not {s_corr!r}")
+    return sc
+
+
 def batched_scene(nx: int, ny: int, nz: int, fill: float, seed: int, steps: int = 200,
-                  name: str = "batched") -> Scene:
+                  name: str = "batched", s_corr: str = "configured") -> Scene:
     """One scene of the batched / served workloads: a corner block whose box
     keeps the block's footprint in z, doubles height headroom and stretches x
-    so that the block volume is ``fill`` of the box.  These workloads fix r, h,
-    FP32 and 4 iterations only, so s_corr keeps PBFSettings' defaults
-    (k = 0.001, dq = 0.3h; pbf_solver.h:21-23)."""
+    so that the block volume is ``fill`` of the box.  Same parameters as the
+    other dam breaks (r, h, FP32, 4 iterations, XSPH, s_corr)."""
     s = 0.05
     bx, by, bz = nx * s, ny * s, nz * s
     box_z = bz + 0.05
     box_y = by * 1.5 + 0.05
     box_x = max(bx + 0.05, (bx * by * bz) / (fill * box_y * box_z))
     sc = dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=seed, steps=steps, name=name)
-    sc.params = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
-    return sc
+    return with_s_corr(sc, s_corr)
 
 
-def batched_1024(count: int = 1024, steps: int = 200) -> List[Scene]:
+def batched_1024(count: int = 1024, steps: int = 200, s_corr: str = "configured") -> List[Scene]:
     dims = batched_scene_sizes(count)
     return [batched_scene(nx, ny, nz, f, seed=3 * 100000 + k, steps=steps,
-                          name=f"batched_{k}") for k, (nx, ny, nz, f) in enumerate(dims)]
+                          name=f"batched_{k}", s_corr=s_corr) for k, (nx, ny, nz, f) in enumerate(dims)]
 
 
 def lattice_rest_density(spacing: float = 0.05, h: float = 0.1, rest_density: float = 1000.0):
@@ -230,7 +240,7 @@ class Sphere:
 
 
 def served_request(k: int, rng: np.random.Generator, lo: int = 4000, hi: int = 256000,
-                   max_spheres: int = 8) -> Scene:
+                   max_spheres: int = 8, s_corr: str = "configured") -> Scene:
     """One served request: a dam break (as in the batched workload, with a box
     tall enough for the drop) and 0..max_spheres rigid spheres above it.
     The scene carries ``spheres`` (list of Sphere); the box floor is a ground
@@ -241,8 +251,8 @@ def served_request(k: int, rng: np.random.Generator, lo: int = 4000, hi: int = 2
     box_z = bz + 0.05
     box_y = max(by * 1.5 + 0.05, by + 1.5)
     box_x = max(bx + 0.05, (bx * by * bz) / (fill * box_y * box_z))
-    sc = dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=4 * 1000003 + k, steps=0, name=f"served_{k}")
-    sc.params = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
+    sc = with_s_corr(dam_break(nx, ny, nz, (box_x, box_y, box_z), seed=4 * 1000003 + k, steps=0,
+                               name=f"served_{k}"), s_corr)
     spheres = []
     for _ in range(int(rng.integers(0, max_spheres + 1))):
         r = float(rng.uniform(0.1, 0.2))
@@ -252,3 +262,72 @@ def served_request(k: int, rng: np.random.Generator, lo: int = 4000, hi: int = 2
         spheres.append(Sphere(radius=r, density=float(rng.uniform(300.0, 2000.0)), position=(x, y, z)))
     sc.spheres = spheres
     return sc
+
+
+def coupled_500k_world(seed: int = 2):
+    """The rigid-fluid coupling workload as a FluidWorld: the 500k-particle
+    block (100x50x100) in the 10x5x5 m box with the dam-break parameters, a
+    ground plane sampled with boundary particles over the box, and 64 rigid
+    bodies: 32 spheres (r ~ U(0.1, 0.2)) and 32 boxes (half extents
+    ~ U(0.05, 0.15)^3), density ~ U(300, 2000), seeded positions over the
+    block footprint 0.3-1.0 m above its surface, identity orientation.
+    Returns (world, fluid scene)."""
+    import novaphy_amd as nv
+    sc = coupled_500k_fluid()
+    rng = np.random.default_rng(seed)
+    b = nv.ModelBuilder()
+    b.add_ground_plane(y=0.0)
+    top = float(sc.positions[:, 1].max())
+    lo = sc.positions.min(0)
+    hi = sc.positions.max(0)
+    for k in range(64):
+        x = rng.uniform(lo[0] + 0.3, hi[0] - 0.3)
+        z = rng.uniform(lo[2] + 0.3, hi[2] - 0.3)
+        y = top + rng.uniform(0.3, 1.0)
+        dens = rng.uniform(300.0, 2000.0)
+        t = nv.Transform.from_translation(np.array([x, y, z], np.float32))
+        if k < 32:
+            r = rng.uniform(0.1, 0.2)
+            i = b.add_body(nv.RigidBody.from_sphere(float(dens * 4.0 / 3.0 * np.pi * r ** 3), float(r)), t)
+            b.add_shape(nv.CollisionShape.make_sphere(float(r), i))
+        else:
+            he = rng.uniform(0.05, 0.15, 3).astype(np.float32)
+            i = b.add_body(nv.RigidBody.from_box(float(dens * 8.0 * np.prod(he)), he), t)
+            b.add_shape(nv.CollisionShape.make_box(he, i))
+    world = nv.FluidWorld(b.build(), [empty_block()], nv.SolverSettings(), pbf_settings(sc.params),
+                          boundary_extent=10.0)
+    world.fluid_state.init_arrays(sc.positions, sc.velocities)
+    return world, sc
+
+
+def pbf_settings(p: PBFParams):
+    """novaphy_amd.PBFSettings with the fields of a PBFParams."""
+    import novaphy_amd as nv
+    prm = nv.PBFSettings()
+    for f in ("rest_density", "kernel_radius", "particle_radius", "solver_iterations", "epsilon",
+              "xsph_viscosity", "vorticity_epsilon", "corr_delta_q", "corr_k", "corr_n", "use_domain_bounds"):
+        setattr(prm, f, getattr(p, f))
+    prm.domain_lower = np.array(p.domain_lower, np.float32)
+    prm.domain_upper = np.array(p.domain_upper, np.float32)
+    return prm
+
+
+def empty_block():
+    """A FluidBlockDef of spacing 0.05 whose particles are replaced with
+    ParticleState.init_arrays (it fixes the particle spacing and mass)."""
+    import novaphy_amd as nv
+    blk = nv.FluidBlockDef()
+    blk.particle_spacing = 0.05
+    blk.lower = np.zeros(3, np.float32)
+    blk.upper = np.zeros(3, np.float32)
+    return blk
+
+
+def window_cells(sc: Scene, margin: int = 4) -> int:
+    """Cells of a bounded scene's cell window (domain cells +- margin per
+    axis), the cell capacity a batch context needs for it."""
+    h = sc.params.kernel_radius
+    inv = np.float32(1.0) / np.float32(h)
+    lo = [int(np.floor(np.float32(v) * inv)) - margin for v in sc.params.domain_lower]
+    hi = [int(np.floor(np.float32(v) * inv)) + margin for v in sc.params.domain_upper]
+    return int(np.prod([b - a + 1 for a, b in zip(lo, hi)]))

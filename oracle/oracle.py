@@ -519,3 +519,45 @@ class RigidWorld:
                              C.byref(npairs))
         self.contacts = _contacts(buf, min(nc.value, cap))
         self.npairs = npairs.value
+
+
+def _sqn32(v):
+    v = np.asarray(v, np.float32)
+    return np.float32(v[0] * v[0] + np.float32(v[1] * v[1] + v[2] * v[2]))
+
+
+class FluidWorldRun:
+    """FluidWorld::step restated (fluid_world.cpp:46-174) on the oracle:
+    boundary world positions at the bodies' poses before the step, the PBF
+    step, boundary density on the committed positions, the per-body coupling
+    force and torque (reduced in boundary-particle order), handed to the
+    rigid world as World::apply_force / apply_torque (skipped for static
+    bodies and for a zero float32 force and torque), then the rigid step."""
+
+    def __init__(self, prm, pos, vel, mass, local, body, psi, rigid, is_static):
+        self.prm, self.mass = prm, float(np.float32(mass))
+        self.pos = np.ascontiguousarray(pos, np.float32).copy()
+        self.vel = np.ascontiguousarray(vel, np.float32).copy()
+        self.rho = np.zeros(self.pos.shape[0], np.float32)
+        self.lam = np.zeros(self.pos.shape[0], np.float32)
+        self.local = np.ascontiguousarray(local, np.float32).reshape(-1, 3)
+        self.body = np.ascontiguousarray(body, np.int32)
+        self.psi = np.ascontiguousarray(psi, np.float32)
+        self.rigid = rigid
+        self.is_static = np.ascontiguousarray(is_static, np.uint8)
+
+    def step(self, dt, gravity):
+        h, rho0 = float(np.float32(self.prm.kernel_radius)), float(np.float32(self.prm.rest_density))
+        poses = self.rigid.pose7.copy()
+        nb = self.local.shape[0]
+        bpos = boundary_world_positions(self.local, self.body, poses) if nb else None
+        self.pos, self.vel, self.rho, self.lam = pbf_run(self.prm, self.pos, self.vel, dt, gravity, self.mass, 1)
+        if nb:
+            self.rho = boundary_density(self.pos, self.rho, bpos, self.psi, h, rho0)
+            f, t = coupling(self.pos, self.rho, bpos, self.psi, self.body, poses, self.is_static, h, rho0, self.mass)
+            for k in range(poses.shape[0]):
+                if self.is_static[k] or (_sqn32(f[k]) == 0.0 and _sqn32(t[k]) == 0.0):
+                    continue
+                self.rigid.apply_force(k, f[k])
+                self.rigid.apply_torque(k, t[k])
+        self.rigid.step(dt)
