@@ -251,15 +251,18 @@ std::vector<int> SpatialHashGrid::query_neighbors(const Vec3f& p, float radius) 
 
 std::vector<std::pair<int, int>> SpatialHashGrid::query_all_pairs(std::span<const Vec3f> positions,
                                                                   float radius) const {
-  // neighbor_search.cpp:57-87: candidates of each point, j > i, squaredNorm < r^2
+  // neighbor_search.cpp:57-87: the built grid's candidates of each given
+  // point, j > i, with |positions[i] - positions[j]|^2 < r^2 (both from the
+  // given positions, as the reference; a candidate index past them, which the
+  // reference would read out of bounds, is skipped)
   std::vector<std::pair<int, int>> pairs;
   if (positions.empty()) return pairs;
   const float r2 = radius * radius;
   auto lists = query_neighbors_batch(positions);
   for (size_t i = 0; i < positions.size(); ++i)
     for (int j : lists[i]) {
-      if (j <= (int)i) continue;
-      if ((positions[i] - pts_[j]).squaredNorm() < r2) pairs.emplace_back((int)i, j);
+      if (j <= (int)i || j >= (int)positions.size()) continue;
+      if ((positions[i] - positions[(size_t)j]).squaredNorm() < r2) pairs.emplace_back((int)i, j);
     }
   return pairs;
 }

@@ -250,6 +250,8 @@ struct Ctx {
   DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM, VX;
   DBuf<float> RHO, RHO2, LAM, LAM2;
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
+  DBuf<int> MPERM, MKEY;  // the other slot / key buffers of the large-class merge levels
+  DBuf<uint8_t> BIGB;     // per 256-slot block: it holds part of a large class
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
@@ -625,6 +627,9 @@ struct Ctx {
     NPX_HIP(TPERM.ensure(n));
     NPX_HIP(TKEY.ensure(n));
     NPX_HIP(PERM.ensure(n));
+    NPX_HIP(MPERM.ensure(n));
+    NPX_HIP(MKEY.ensure(n));
+    NPX_HIP(BIGB.ensure(n / NPX_BLOCK + 4));
     NPX_HIP(SKL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(SKN.ensure(n));
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
@@ -899,9 +904,22 @@ struct Ctx {
     NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p));
     NPX_TRY(launch("rank_sort", k_rank_sort, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
-                   (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)CLS.p, TPERM.p, TKEY.p));
+                   (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)CLS.p, TPERM.p, TKEY.p, BIGB.p));
+    // OFF (the binning offsets, read only by k_scatter) then holds each sorted slot's class
     NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
-                   (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p));
+                   (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p, OFF.p));
+    // merge levels of the large classes spanning several blocks: enough for a
+    // class spanning every block (a fixed count, so a step's launches do not
+    // depend on the state and it can be replayed as a graph)
+    int levels = 0;
+    while ((1 << levels) < blocks + 1) ++levels;
+    for (int L = 0; L < levels; ++L) {
+      const bool even = (L & 1) == 0;
+      NPX_TRY(launch("rank_merge", k_rank_merge, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                     (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)OFF.p, (const unsigned char*)BIGB.p,
+                     L, (const int*)(even ? TPERM.p : MPERM.p), (const int*)(even ? TKEY.p : MKEY.p),
+                     even ? MPERM.p : TPERM.p, even ? MKEY.p : TKEY.p, PERM.p));
+    }
     // per-class displacement maxima and run flags, per-scene run flags: zero each step
     NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls + max_scenes, 256), 256, CMAX.p, 0, 2 * ncls + max_scenes));
     NPX_TRY(launch("gather", k_gather, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
@@ -1037,7 +1055,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p};
   }
 
   void drop_graphs() {

@@ -255,7 +255,8 @@ k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, 
 // sorted by index.  Bitonic sort in LDS.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
-            const int* __restrict__ cls_of, int* __restrict__ tperm, int* __restrict__ tkey) {
+            const int* __restrict__ cls_of, int* __restrict__ tperm, int* __restrict__ tkey,
+            unsigned char* __restrict__ bigb) {
   __shared__ unsigned long long sk[NPX_BLOCK];
   __shared__ int sp[NPX_BLOCK];
   const int s = blk_scene[blockIdx.x];
@@ -269,7 +270,9 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
     p = tperm[t];
     c = cls_of[p];
   }
-  if (!__syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT)) return;
+  const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
+  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge levels
+  if (!big) return;
   unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
   sk[i] = key;
@@ -300,80 +303,89 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
 // Pass 3b: final slot = class start + rank of the particle's original index
 // inside its class, so each bucket lists particles in ascending index like
 // the reference's push_back order (neighbor_search.cpp:18-21).  Indices are
-// unique within a scene.  Small classes count directly.  For a larger class
-// the rank is the slot's position in its own sorted block segment plus, for
-// every other block the class spans, the count of smaller indices in that
-// block's sorted segment.  The workgroup stages the blocks its large classes
-// span in LDS, eight at a time (coalesced loads), and each slot binary
-// searches its class's segment of each staged block there.
+// unique within a scene.  Small classes (at most NPX_RANK_DIRECT) count
+// directly.  A larger class lying inside one block is already sorted by
+// k_rank_sort; one spanning several blocks is a sequence of sorted pieces
+// (its parts of consecutive blocks) that the merge levels below combine.
+// The class of each sorted slot goes to tcls: the merge levels' slot / key
+// buffers hold valid entries only inside the ranges of the classes they
+// merge.
 #define NPX_RANK_STAGE 8
 
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
-       int* __restrict__ perm) {
-  __shared__ int sk[NPX_RANK_STAGE * NPX_BLOCK];
-  __shared__ int span_lo, span_hi;
+       int* __restrict__ perm, int* __restrict__ tcls) {
   const int s = blk_scene[blockIdx.x];
   if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
   const int t = npx_slot();
-  const bool active = t < S.seg_off + S.n;
+  if (t >= S.seg_off + S.n) return;
   const int shift = cstart[S.cls_base] - S.seg_off;
-  int p = 0, b = 0, e = 0, my = 0;
-  if (active) {
-    p = tperm[t];
-    const int c = cls_of[p];
-    b = cstart[c] - shift;
-    e = cstart[c + 1] - shift;
-    my = tkey[t];
-  }
-  const bool big = active && e - b > NPX_RANK_DIRECT;
-  if (active && !big) {
+  const int p = tperm[t];
+  const int c = cls_of[p];
+  tcls[t] = c;  // the class of every sorted slot, for the merge levels
+  const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+  if (e - b <= NPX_RANK_DIRECT) {
+    const int my = tkey[t];
     int r = 0;
     for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
     perm[b + r] = p;
+  } else if (b / NPX_BLOCK == (e - 1) / NPX_BLOCK) {
+    perm[t] = p;  // sorted by k_rank_sort
   }
-  if (threadIdx.x == 0) {
-    span_lo = INT_MAX;
-    span_hi = -1;
+}
+
+// Merge level L of the large classes spanning several blocks (a segmented
+// merge sort).  The class's pieces are its parts of consecutive 256-slot
+// blocks, each sorted by k_rank_sort; before level L, runs of 2^L
+// consecutive pieces are sorted, and level L merges each run with its partner
+// run (run index ^ 1): a slot moves to the pair's start + its offset in its
+// run + the number of smaller keys in the partner run (one binary search;
+// keys are unique within a class).  The level that leaves one run writes the
+// final permutation; earlier ones write the other key / slot buffers.  Blocks
+// without a large class (bigb, from k_rank_sort) exit at once, so idle levels
+// cost one byte per block.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
+             const int* __restrict__ tcls, const unsigned char* __restrict__ bigb, int level,
+             const int* __restrict__ pin, const int* __restrict__ kin, int* __restrict__ pout,
+             int* __restrict__ kout, int* __restrict__ perm) {
+  const int s = blk_scene[blockIdx.x];
+  if (s < 0 || !bigb[blockIdx.x]) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int t = npx_slot();
+  if (t >= S.seg_off + S.n) return;
+  const int shift = cstart[S.cls_base] - S.seg_off;
+  const int c = tcls[t];
+  const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+  if (e - b <= NPX_RANK_DIRECT) return;
+  const int B0 = b / NPX_BLOCK;
+  const int npieces = (e - 1) / NPX_BLOCK - B0 + 1;
+  if (npieces <= (1 << level)) return;  // merged at an earlier level (or a single piece)
+  const int p = pin[t];
+  auto run_lo = [&](int r) { return max(b, (B0 + (r << level)) * NPX_BLOCK); };
+  auto run_hi = [&](int r) { return min(e, (B0 + ((r + 1) << level)) * NPX_BLOCK); };
+  const int r = (t / NPX_BLOCK - B0) >> level;
+  const int q = r ^ 1;
+  const int my = kin[t];
+  int lo = run_lo(q), hi = run_hi(q);
+  const int q0 = lo;
+  while (lo < hi) {  // lower bound of my key in the partner run (empty when it starts past the class)
+    const int mid = (lo + hi) >> 1;
+    if (kin[mid] < my)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  if (!__syncthreads_or(big)) return;  // uniform per block
-  const int own = (int)blockIdx.x;
-  const int blo = big ? b / NPX_BLOCK : INT_MAX, bhi = big ? (e - 1) / NPX_BLOCK : -1;
-  if (big) {
-    atomicMin(&span_lo, blo);
-    atomicMax(&span_hi, bhi);
+  const int rank = (q0 < e) ? lo - q0 : 0;
+  const int dst = run_lo(r & ~1) + (t - run_lo(r)) + rank;
+  if (npieces <= (2 << level)) {
+    perm[dst] = p;  // the last level of this class
+  } else {
+    pout[dst] = p;
+    kout[dst] = my;
   }
-  __syncthreads();
-  const int lo = span_lo, hi = span_hi;
-  int r = big ? t - max(b, own * NPX_BLOCK) : 0;
-  for (int B0 = lo; B0 <= hi; B0 += NPX_RANK_STAGE) {
-#pragma unroll
-    for (int u = 0; u < NPX_RANK_STAGE; ++u)
-      if (B0 + u <= hi) sk[u * NPX_BLOCK + (int)threadIdx.x] = tkey[(B0 + u) * NPX_BLOCK + (int)threadIdx.x];
-    __syncthreads();
-    if (big) {
-      for (int u = 0; u < NPX_RANK_STAGE; ++u) {
-        const int B = B0 + u;
-        if (B > hi || B < blo || B > bhi || B == own) continue;
-        const int base = B * NPX_BLOCK;
-        const int l0 = max(b, base) - base;
-        int lb = l0, ub = min(e, base + NPX_BLOCK) - base;
-        const int* __restrict__ seg = sk + u * NPX_BLOCK;
-        while (lb < ub) {
-          const int mid = (lb + ub) >> 1;
-          if (seg[mid] < my)
-            lb = mid + 1;
-          else
-            ub = mid;
-        }
-        r += lb - l0;
-      }
-    }
-    __syncthreads();
-  }
-  if (big) perm[b + r] = p;
 }
 
 // Pass 4: reorder the state into the new slot order and form x*; also marks

@@ -290,11 +290,14 @@ def test_query_all_pairs_on_a_dam_break_matches_oracle(gpu):
     sc = scenes.dam_break_8k()
     pts = [p for p in sc.positions]
     grid = nv.SpatialHashGrid(0.1)
+    grid.build(pts)
     got = grid.query_all_pairs(pts, 0.1)
     want = O.grid_all_pairs(sc.positions, 0.1, 0.1)
     assert len(got) == want.shape[0] > 8000
     np.testing.assert_array_equal(np.array(got, np.int32), want)
     np.random.seed(42)
     pts = np.random.uniform(0, 1, (500, 3)).astype(F32)
-    got = nv.SpatialHashGrid(0.1).query_all_pairs(list(pts), 0.1)
+    grid = nv.SpatialHashGrid(0.1)
+    grid.build(list(pts))
+    got = grid.query_all_pairs(list(pts), 0.1)
     np.testing.assert_array_equal(np.array(got, np.int32).reshape(-1, 2), O.grid_all_pairs(pts, 0.1, 0.1))
