@@ -124,7 +124,7 @@ int npx_grid_query(const float* pos3, int32_t n, float cell_size, const float* q
  * under hipGraph capture, tight serving loops, harnesses that forbid
  * hipMalloc).  Same arguments, results and errors as npx_grid_query, on the
  * current device's null stream; every device buffer lives in one static
- * per-GPU scratch array of about 22.5 MiB.  Limits: n <= 65536 points,
+ * per-GPU scratch array of about 23.0 MiB.  Limits: n <= 65536 points,
  * nq <= 65536 queries, a window of <= 2^18 cells (NPX_EDOMAIN beyond),
  * <= 2^21 list entries when lists are read (NPX_ENOMEM beyond).  Calls that
  * overlap on one GPU share the scratch array and race. */

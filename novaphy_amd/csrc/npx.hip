@@ -289,10 +289,11 @@ void destroy(Ctx* c) {
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
                           &c->PERM, &c->MPERM, &c->MKEY, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
-                          &c->ATKEY, &c->APERM, &c->aux_clsmap};
+                          &c->ATKEY, &c->APERM, &c->AMPERM, &c->AMKEY, &c->aux_clsmap};
   for (auto* b : i1) b->release();
   c->htk.release();
   c->BIGB.release();
+  c->ABIGB.release();
   c->RSTART.release();
   c->PI.release();
   c->PS.release();
@@ -306,6 +307,9 @@ void destroy(Ctx* c) {
     ps->tperm.release();
     ps->tkey.release();
     ps->perm.release();
+    ps->mperm.release();
+    ps->mkey.release();
+    ps->bigb.release();
   }
   for (auto* b : {&c->BBL, &c->BBP, &c->BBPS, &c->BFP, &c->BFS, &c->BFB}) b->release();
   for (auto* b : {&c->BBODY, &c->BSCN, &c->BBPTR, &c->BBIDX, &c->FPSC}) b->release();
@@ -325,7 +329,7 @@ void destroy(Ctx* c) {
 // ------------------------------------------------ allocation-free grid query
 // npx_grid_query_noalloc keeps every device buffer it needs in `scratch`, a
 // file-scope device array: each GPU the library runs on holds its own copy
-// (NPX_NA_SCRATCH_BYTES, about 22.5 MiB) from module load until the process
+// (NPX_NA_SCRATCH_BYTES, about 23.0 MiB) from module load until the process
 // ends, and the call allocates nothing.  Host code reaches a GPU's copy only
 // through the address hipGetSymbolAddress returns for it on that device
 // (na_scratch_ptr); the array's name is not a device address on the host.
@@ -357,7 +361,10 @@ struct NaLayout {
   static constexpr size_t tperm = off + al((size_t)NPX_NA_MAX_POINTS * 4);
   static constexpr size_t tkey = tperm + al((size_t)NPX_NA_MAX_POINTS * 4);
   static constexpr size_t perm = tkey + al((size_t)NPX_NA_MAX_POINTS * 4);
-  static constexpr size_t cnt = perm + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t mperm = perm + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t mkey = mperm + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t bigb = mkey + al((size_t)NPX_NA_MAX_POINTS * 4);
+  static constexpr size_t cnt = bigb + al((size_t)NPX_NA_MAX_POINTS / 256 + 2);
   static constexpr size_t off64 = cnt + al(((size_t)NPX_NA_MAX_QUERIES + 1) * 4);
   static constexpr size_t out = off64 + al((size_t)NPX_NA_MAX_QUERIES * 8);
   static constexpr size_t total = out + al((size_t)NPX_NA_MAX_LIST * 4);
@@ -679,6 +686,9 @@ int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const 
     auto* tperm = reinterpret_cast<int*>(base + L::tperm);
     auto* tkey = reinterpret_cast<int*>(base + L::tkey);
     auto* perm = reinterpret_cast<int*>(base + L::perm);
+    auto* mperm = reinterpret_cast<int*>(base + L::mperm);
+    auto* mkey = reinterpret_cast<int*>(base + L::mkey);
+    auto* bigb = reinterpret_cast<unsigned char*>(base + L::bigb);
     auto* cnt = reinterpret_cast<int*>(base + L::cnt);
     auto* off64 = reinterpret_cast<long long*>(base + L::off64);
     auto* out = reinterpret_cast<int*>(base + L::out);
@@ -712,11 +722,20 @@ int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const 
     if (e == hipSuccess)
       e = na_launch(k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)cls, (const int*)off,
                     (const int*)nullptr, tperm, tkey);
+    // rank inside classes (off now holds each sorted slot's class), then the
+    // merge levels of classes spanning several 256-slot blocks
+    const int nblk = cdiv(n, NPX_BLOCK);
     if (e == hipSuccess)
-      e = na_launch(k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey);
+      e = na_launch(k_aux_rank_sort, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey, bigb);
     if (e == hipSuccess)
-      e = na_launch(k_aux_rank, cdiv(n, 256), 256, n, nc, (const int*)cst, (const int*)cls, (const int*)tperm,
-                    (const int*)tkey, (const float4*)P, perm, PS);
+      e = na_launch(k_aux_rank, nblk, NPX_BLOCK, n, nc, (const int*)cst, (const int*)cls, (const int*)tperm,
+                    (const int*)tkey, (const float4*)P, perm, PS, off);
+    for (int lv = 0; e == hipSuccess && (1 << lv) < nblk + 1; ++lv) {
+      const bool even = (lv & 1) == 0;
+      e = na_launch(k_aux_rank_merge, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)off,
+                    (const unsigned char*)bigb, lv, (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey),
+                    even ? mperm : tperm, even ? mkey : tkey, (const float4*)P, perm, PS);
+    }
     // candidate counts per query, then their exclusive prefix (cnt[nq] = total)
     if (e == hipSuccess) e = na_launch(k_fill_i32, cdiv(nq + 1, 256), 256, cnt, 0, nq + 1);
     if (e == hipSuccess)

@@ -289,7 +289,8 @@ struct Ctx {
   DBuf<float> POSE;
   DBuf<uint8_t> STATIC;
   DBuf<float> OUT6;
-  DBuf<int> Acstart, ACLS, AOFF, ATPERM, ATKEY, APERM;
+  DBuf<int> Acstart, ACLS, AOFF, ATPERM, ATKEY, APERM, AMPERM, AMKEY;
+  DBuf<uint8_t> ABIGB;
   DBuf<int> aux_clsmap;
 
   // batched coupling: per-scene boundaries gathered into batch-wide arrays
@@ -305,8 +306,30 @@ struct Ctx {
   bool bnd_dirty = false;
   int bb_n = 0, bb_bodies = 0;
   struct PointSort {
-    DBuf<int> cst, cls, off, tperm, tkey, perm;
+    DBuf<int> cst, cls, off, tperm, tkey, perm, mperm, mkey;
+    DBuf<uint8_t> bigb;
   };
+
+  // Rank inside classes of an auxiliary sort (k_aux_rank_sort, k_aux_rank and
+  // the merge levels of large classes): off is reused for each sorted slot's
+  // class, mperm / mkey are the merge levels' other buffers, bigb one byte per
+  // 256-slot block.
+  int aux_rank_all(const char* tag, int n, int ncls, const int* cst, const int* cls, int* off, int* tperm, int* tkey,
+                   int* mperm, int* mkey, uint8_t* bigb, const float4* P, int* perm, float4* PS) {
+    const int nblk = cdiv(n, NPX_BLOCK);
+    NPX_TRY(launch(tag, k_aux_rank_sort, nblk, NPX_BLOCK, ncls, cst, cls, tperm, tkey, bigb));
+    NPX_TRY(launch(tag, k_aux_rank, nblk, NPX_BLOCK, n, ncls, cst, cls, (const int*)tperm, (const int*)tkey, P, perm,
+                   PS, off));
+    int levels = 0;
+    while ((1 << levels) < nblk + 1) ++levels;
+    for (int L = 0; L < levels; ++L) {
+      const bool even = (L & 1) == 0;
+      NPX_TRY(launch(tag, k_aux_rank_merge, nblk, NPX_BLOCK, ncls, cst, (const int*)off, (const uint8_t*)bigb, L,
+                     (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey), even ? mperm : tperm,
+                     even ? mkey : tkey, P, perm, PS));
+    }
+    return NPX_OK;
+  }
   PointSort bsort, fsort;
   DBuf<float4> BBL, BBP, BBPS, BFP, BFS, BFB;
   DBuf<int> BBODY, BSCN, BBPTR, BBIDX, FPSC;
@@ -325,6 +348,9 @@ struct Ctx {
     NPX_HIP(ps.tperm.ensure(m));
     NPX_HIP(ps.tkey.ensure(m));
     NPX_HIP(ps.perm.ensure(m));
+    NPX_HIP(ps.mperm.ensure(m));
+    NPX_HIP(ps.mkey.ensure(m));
+    NPX_HIP(ps.bigb.ensure(m / NPX_BLOCK + 2));
     NPX_TRY(launch("bsort_fill", k_fill_i32, cdiv(ncls + 1, 256), 256, ps.cst.p, 0, ncls + 1));
     if (n == 0) return NPX_OK;
     NPX_TRY(launch("bsort_bin", k_bin_scene_points, cdiv(n, 256), 256, (const SceneC*)d_scenes.p, P, pscene, n,
@@ -332,11 +358,8 @@ struct Ctx {
     NPX_TRY(scan(ps.cst.p, ncls + 1));
     NPX_TRY(launch("bsort_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)ps.cst.p, (const int*)ps.cls.p,
                    (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p));
-    NPX_TRY(launch("bsort_rank_sort", k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, ncls, (const int*)ps.cst.p,
-                   (const int*)ps.cls.p, ps.tperm.p, ps.tkey.p));
-    NPX_TRY(launch("bsort_rank", k_aux_rank, cdiv(n, 256), 256, n, ncls, (const int*)ps.cst.p, (const int*)ps.cls.p,
-                   (const int*)ps.tperm.p, (const int*)ps.tkey.p, P, ps.perm.p, PS));
-    return NPX_OK;
+    return aux_rank_all("bsort_rank", n, ncls, ps.cst.p, ps.cls.p, ps.off.p, ps.tperm.p, ps.tkey.p, ps.mperm.p,
+                        ps.mkey.p, ps.bigb.p, P, ps.perm.p, PS);
   }
 
   // Gathers the scenes' boundaries into the batch-wide arrays.
@@ -1212,17 +1235,17 @@ struct Ctx {
     NPX_HIP(ATPERM.ensure(std::max(n, 1)));
     NPX_HIP(ATKEY.ensure(std::max(n, 1)));
     NPX_HIP(APERM.ensure(std::max(n, 1)));
+    NPX_HIP(AMPERM.ensure(std::max(n, 1)));
+    NPX_HIP(AMKEY.ensure(std::max(n, 1)));
+    NPX_HIP(ABIGB.ensure((size_t)cdiv(std::max(n, 1), NPX_BLOCK) + 1));
     int* cst = Acstart.p - cls_base;  // indexed by class id
     NPX_TRY(launch("aux_fill", k_fill_i32, cdiv(W.ncell + 1, 256), 256, Acstart.p, 0, W.ncell + 1));
     NPX_TRY(launch("aux_bin", k_aux_bin, cdiv(n, 256), 256, W, P, n, cm, cst, ACLS.p, AOFF.p));
     NPX_TRY(scan(Acstart.p, W.ncell + 1));
     NPX_TRY(launch("aux_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)ACLS.p,
                    (const int*)AOFF.p, key, ATPERM.p, ATKEY.p));
-    NPX_TRY(launch("aux_rank_sort", k_aux_rank_sort, cdiv(n, NPX_BLOCK), NPX_BLOCK, W.ncell + cls_base,
-                   (const int*)cst, (const int*)ACLS.p, ATPERM.p, ATKEY.p));
-    NPX_TRY(launch("aux_rank", k_aux_rank, cdiv(n, 256), 256, n, W.ncell + cls_base, (const int*)cst,
-                   (const int*)ACLS.p, (const int*)ATPERM.p, (const int*)ATKEY.p, P, APERM.p, PS));
-    return NPX_OK;
+    return aux_rank_all("aux_rank", n, W.ncell + cls_base, cst, ACLS.p, AOFF.p, ATPERM.p, ATKEY.p, AMPERM.p,
+                        AMKEY.p, ABIGB.p, P, APERM.p, PS);
   }
 
   GridWin scene_win(const SceneH& H) const {

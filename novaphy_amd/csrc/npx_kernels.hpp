@@ -2020,7 +2020,7 @@ __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* 
 // fluid sort (keys are unique within a class).
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of, int* __restrict__ tperm,
-                int* __restrict__ tkey) {
+                int* __restrict__ tkey, unsigned char* __restrict__ bigb) {
   __shared__ unsigned long long sk[NPX_BLOCK];
   __shared__ int sp[NPX_BLOCK];
   const int i = (int)threadIdx.x;
@@ -2031,7 +2031,9 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
     p = tperm[t];
     c = cls_of[p];
   }
-  if (!__syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT)) return;
+  const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
+  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge levels
+  if (!big) return;
   unsigned long long key = ~0ull;
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
   sk[i] = key;
@@ -2059,78 +2061,72 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
   }
 }
 
-// Rank inside the class by key (own block position plus, for a class larger
-// than NPX_RANK_DIRECT, binary searches in the class's other sorted blocks
-// staged in LDS, as k_rank) and write the sorted payload copy.  256-thread
+// Rank inside the class by key, as k_rank does for the fluid sort: small
+// classes count directly, a larger class inside one block is already sorted,
+// and one spanning several blocks is finished by k_aux_rank_merge.  Writes
+// the sorted payload copy and each sorted slot's class (tcls).  256-thread
 // blocks.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
            const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
-           int* __restrict__ perm, float4* __restrict__ PS) {
-  __shared__ int sk[NPX_RANK_STAGE * NPX_BLOCK];
-  __shared__ int span_lo, span_hi;
-  const int total = cstart[ncls];
+           int* __restrict__ perm, float4* __restrict__ PS, int* __restrict__ tcls) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const bool active = t < total;
-  int i = 0, b = 0, e = 0, my = 0;
-  if (active) {
-    i = tperm[t];
-    const int c = cls_of[i];
-    b = cstart[c];
-    e = cstart[c + 1];
-    my = tkey[t];
-  }
-  const bool big = active && e - b > NPX_RANK_DIRECT;
-  if (active && !big) {
+  if (t >= cstart[ncls]) return;
+  const int i = tperm[t];
+  const int c = cls_of[i];
+  tcls[t] = c;
+  const int b = cstart[c], e = cstart[c + 1];
+  if (e - b <= NPX_RANK_DIRECT) {
+    const int my = tkey[t];
     int r = 0;
     for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
     perm[b + r] = i;
     PS[b + r] = P[i];
+  } else if (b / NPX_BLOCK == (e - 1) / NPX_BLOCK) {
+    perm[t] = i;
+    PS[t] = P[i];
   }
-  if (threadIdx.x == 0) {
-    span_lo = INT_MAX;
-    span_hi = -1;
+}
+
+// Merge level of the auxiliary sort's large classes (k_rank_merge without
+// scenes); the class's last level writes the permutation and the payload.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict__ tcls,
+                 const unsigned char* __restrict__ bigb, int level, const int* __restrict__ pin,
+                 const int* __restrict__ kin, int* __restrict__ pout, int* __restrict__ kout,
+                 const float4* __restrict__ P, int* __restrict__ perm, float4* __restrict__ PS) {
+  if (!bigb[blockIdx.x]) return;  // uniform per block
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= cstart[ncls]) return;
+  const int c = tcls[t];
+  const int b = cstart[c], e = cstart[c + 1];
+  if (e - b <= NPX_RANK_DIRECT) return;
+  const int B0 = b / NPX_BLOCK;
+  const int npieces = (e - 1) / NPX_BLOCK - B0 + 1;
+  if (npieces <= (1 << level)) return;
+  const int p = pin[t];
+  auto run_lo = [&](int r) { return max(b, (B0 + (r << level)) * NPX_BLOCK); };
+  auto run_hi = [&](int r) { return min(e, (B0 + ((r + 1) << level)) * NPX_BLOCK); };
+  const int r = (t / NPX_BLOCK - B0) >> level;
+  const int q = r ^ 1;
+  const int my = kin[t];
+  int lo = run_lo(q), hi = run_hi(q);
+  const int q0 = lo;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (kin[mid] < my)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  if (!__syncthreads_or(big)) return;  // uniform per block
-  const int own = (int)blockIdx.x;
-  const int blo = big ? b / NPX_BLOCK : INT_MAX, bhi = big ? (e - 1) / NPX_BLOCK : -1;
-  if (big) {
-    atomicMin(&span_lo, blo);
-    atomicMax(&span_hi, bhi);
-  }
-  __syncthreads();
-  const int lo = span_lo, hi = span_hi;
-  int r = big ? t - max(b, own * NPX_BLOCK) : 0;
-  for (int B0 = lo; B0 <= hi; B0 += NPX_RANK_STAGE) {
-#pragma unroll
-    for (int u = 0; u < NPX_RANK_STAGE; ++u) {
-      const int x = (B0 + u) * NPX_BLOCK + (int)threadIdx.x;
-      if (B0 + u <= hi) sk[u * NPX_BLOCK + (int)threadIdx.x] = (x < total) ? tkey[x] : INT_MAX;
-    }
-    __syncthreads();
-    if (big) {
-      for (int u = 0; u < NPX_RANK_STAGE; ++u) {
-        const int B = B0 + u;
-        if (B > hi || B < blo || B > bhi || B == own) continue;
-        const int base = B * NPX_BLOCK;
-        const int l0 = max(b, base) - base;
-        int lb = l0, ub = min(e, base + NPX_BLOCK) - base;
-        const int* __restrict__ seg = sk + u * NPX_BLOCK;
-        while (lb < ub) {
-          const int mid = (lb + ub) >> 1;
-          if (seg[mid] < my)
-            lb = mid + 1;
-          else
-            ub = mid;
-        }
-        r += lb - l0;
-      }
-    }
-    __syncthreads();
-  }
-  if (big) {
-    perm[b + r] = i;
-    PS[b + r] = P[i];
+  const int rank = (q0 < e) ? lo - q0 : 0;
+  const int dst = run_lo(r & ~1) + (t - run_lo(r)) + rank;
+  if (npieces <= (2 << level)) {
+    perm[dst] = p;
+    PS[dst] = P[p];
+  } else {
+    pout[dst] = p;
+    kout[dst] = my;
   }
 }
 

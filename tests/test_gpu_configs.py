@@ -301,3 +301,56 @@ def test_query_all_pairs_on_a_dam_break_matches_oracle(gpu):
     grid.build(list(pts))
     got = grid.query_all_pairs(list(pts), 0.1)
     np.testing.assert_array_equal(np.array(got, np.int32).reshape(-1, 2), O.grid_all_pairs(pts, 0.1, 0.1))
+
+
+def test_aux_sorts_with_large_classes_match_oracle(gpu):
+    """The auxiliary grids' rank inside large classes (a stack of 3000
+    coincident points, 800 points in one cell, classes spanning many 256-slot
+    blocks, merged level by level): standalone queries return the oracle's
+    bucket lists in order, and coupling over a fluid holding a stack of 2000
+    coincident particles next to a body gives the oracle's forces."""
+    rng = np.random.default_rng(71)
+    pts = np.concatenate([np.full((3000, 3), 0.437, F32), rng.uniform(0.40, 0.49, (800, 3)),
+                          rng.uniform(-0.5, 1.5, (4000, 3))]).astype(F32)
+    pts = pts[rng.permutation(pts.shape[0])]
+    q = np.concatenate([pts[:50], rng.uniform(0.3, 0.6, (50, 3))]).astype(F32)
+    counts, lists = capi.grid_query(pts, 0.1, q)
+    oc, ol = O.grid_query(pts, 0.1, q)
+    np.testing.assert_array_equal(counts, oc)
+    np.testing.assert_array_equal(lists, ol)
+    # coupling: the committed-fluid grid sorts the fluid by original index
+    sc, local, body, poses, static = _stack_scene_with_body(rng)
+    h = sc.params.kernel_radius
+    c = capi.Context(sc.params, capacity=sc.n)
+    c.set_particles(sc.positions, sc.velocities)
+    c.set_boundary(local, body, None, nbodies=poses.shape[0])
+    psi = c.compute_boundary_volumes(poses)
+    c.step(sc.dt, sc.gravity, sc.mass, 1)
+    c.boundary_world_positions(poses)
+    c.apply_boundary_density()
+    f, t = c.coupling_forces(poses, static)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, _ = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 1)
+    wpos = O.boundary_world_positions(local, body, poses)
+    orho_b = O.boundary_density(op, orho, wpos, psi, h, sc.params.rest_density)
+    of, ot = O.coupling(op, orho_b, wpos, psi, body, poses, static, h, sc.params.rest_density, sc.mass)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gr, orho_b)
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(t, ot)
+    assert np.abs(of).max() > 0
+
+
+def _stack_scene_with_body(rng):
+    """A dam break whose first 2000 particles sit on one point beside a sphere
+    (a stack that stays together: equal positions and velocities), and the
+    sphere's boundary samples."""
+    sc = scenes.dam_break(12, 10, 12, (1.0, 1.0, 1.0), seed=72, steps=1)
+    pos = np.concatenate([np.tile(np.array([[0.31, 0.26, 0.33]], F32), (2000, 1)), sc.positions]).astype(F32)
+    sc.positions = pos
+    sc.velocities = np.zeros_like(pos)
+    shapes = [O.make_shape(O.SHAPE_SPHERE, 0, [0.12, 0, 0, 0])]
+    local, body = O.sample_boundaries(shapes, 0.05, 1.0)
+    poses = np.array([[0.42, 0.30, 0.36, 1, 0, 0, 0]], F32)
+    return sc, local, body, poses, np.array([0], np.uint8)
