@@ -122,7 +122,7 @@ def window_ms(trace, steps, window):
     return out
 
 
-def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
+def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None, kernel_timing=True):
     """The timed region is the first `steps` steps of the workload from its
     initial state (after `warmup` untimed steps on the same context, which is
     then reset to the initial state).  Per-kernel HIP events are recorded on
@@ -136,13 +136,17 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None):
     ctx.synchronize()
     ctx.set_particles(sc.positions, sc.velocities)  # back to the initial state
     start_state = (sc.positions, sc.velocities)
-    ctx.set_kernel_timing(True)
+    ctx.set_kernel_timing(kernel_timing)
     ctx.synchronize()
     log(f"[bench] timed region: steps 1-{steps} from the initial state")
     t0 = time.perf_counter()
     ctx.step(sc.dt, sc.gravity, sc.mass, steps)
     ctx.synchronize()
     t1 = time.perf_counter()
+    if not kernel_timing:  # counter-collection runs: no events around the launches
+        ctx.close()
+        return {"value": n * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
+                "window_ms_per_step": None, "roofline": None, "step_roofline": None, "kernels": None}
     kt_raw = ctx.kernel_times()
     trace = ctx.kernel_trace()
     kt = fold_dense(kt_raw)
@@ -413,6 +417,8 @@ def main():
     ap.add_argument("--fast-math", action="store_true",
                     help="FMA/rsqrt kernels (within tolerance) instead of the reference operation order")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels directly instead of hipGraph replay")
+    ap.add_argument("--kernel-timing", type=int, default=1,
+                    help="HIP events around every launch of the timed region (0: none; no roofline then)")
     ap.add_argument("--corr-k", type=float, default=None,
                     help="override the s_corr k of the workload (the configured value is 0.1)")
     ap.add_argument("--cpu-seconds", type=float, default=30.0,
@@ -475,7 +481,8 @@ def main():
         dflt = scenes.PBFParams()
         variants.append(("default", dataclasses.replace(base, params=dataclasses.replace(
             base.params, corr_k=dflt.corr_k, corr_delta_q=dflt.corr_delta_q, corr_n=dflt.corr_n))))
-    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds if i == 0 else 0, name))
+    results = [(name, sc, run_single(sc, args.steps, args.warmup, flags, args.cpu_seconds if i == 0 else 0, name,
+                                     kernel_timing=bool(args.kernel_timing)))
                for i, (name, sc) in enumerate(variants)]
 
     def describe(sc):
