@@ -723,15 +723,15 @@ int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const 
       e = na_launch(k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)cls, (const int*)off,
                     (const int*)nullptr, tperm, tkey);
     // rank inside classes (off now holds each sorted slot's class), then the
-    // merge levels of classes spanning several 256-slot blocks
+    // merge launches of classes spanning several 256-slot blocks
     const int nblk = cdiv(n, NPX_BLOCK);
     if (e == hipSuccess)
       e = na_launch(k_aux_rank_sort, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey, bigb);
     if (e == hipSuccess)
       e = na_launch(k_aux_rank, nblk, NPX_BLOCK, n, nc, (const int*)cst, (const int*)cls, (const int*)tperm,
                     (const int*)tkey, (const float4*)P, perm, PS, off);
-    for (int lv = 0; e == hipSuccess && (1 << lv) < nblk + 1; ++lv) {
-      const bool even = (lv & 1) == 0;
+    for (int lv = 0; e == hipSuccess && (1 << lv) < nblk + 1; lv += NPX_MERGE_LOG) {
+      const bool even = ((lv / NPX_MERGE_LOG) & 1) == 0;
       e = na_launch(k_aux_rank_merge, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)off,
                     (const unsigned char*)bigb, lv, (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey),
                     even ? mperm : tperm, even ? mkey : tkey, (const float4*)P, perm, PS);

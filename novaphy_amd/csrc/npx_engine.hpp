@@ -250,7 +250,7 @@ struct Ctx {
   DBuf<float4> X, X2, V, V2, XS0, XS1, VT, OM, VX;
   DBuf<float> RHO, RHO2, LAM, LAM2;
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
-  DBuf<int> MPERM, MKEY;  // the other slot / key buffers of the large-class merge levels
+  DBuf<int> MPERM, MKEY;  // the other slot / key buffers of the large-class merge launches
   DBuf<uint8_t> BIGB;     // per 256-slot block: it holds part of a large class
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
@@ -311,8 +311,8 @@ struct Ctx {
   };
 
   // Rank inside classes of an auxiliary sort (k_aux_rank_sort, k_aux_rank and
-  // the merge levels of large classes): off is reused for each sorted slot's
-  // class, mperm / mkey are the merge levels' other buffers, bigb one byte per
+  // the merge launches of large classes): off is reused for each sorted slot's
+  // class, mperm / mkey are the merge launches' other buffers, bigb one byte per
   // 256-slot block.
   int aux_rank_all(const char* tag, int n, int ncls, const int* cst, const int* cls, int* off, int* tperm, int* tkey,
                    int* mperm, int* mkey, uint8_t* bigb, const float4* P, int* perm, float4* PS) {
@@ -322,8 +322,8 @@ struct Ctx {
                    PS, off));
     int levels = 0;
     while ((1 << levels) < nblk + 1) ++levels;
-    for (int L = 0; L < levels; ++L) {
-      const bool even = (L & 1) == 0;
+    for (int L = 0; L < levels; L += NPX_MERGE_LOG) {
+      const bool even = ((L / NPX_MERGE_LOG) & 1) == 0;
       NPX_TRY(launch(tag, k_aux_rank_merge, nblk, NPX_BLOCK, ncls, cst, (const int*)off, (const uint8_t*)bigb, L,
                      (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey), even ? mperm : tperm,
                      even ? mkey : tkey, P, perm, PS));
@@ -920,8 +920,8 @@ struct Ctx {
     const int ncls = total_cls();
     const int blocks = hw_blocks;
     NPX_TRY(launch("fill_counts", k_fill_i32, cdiv(ncls + 1, 256), 256, cstart.p, 0, ncls + 1));
-    NPX_TRY(launch("predict_bin", k_predict_bin, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
-                   (const int*)d_blk_scene.p, (const float4*)X.p, (const float4*)V.p, (const int*)clsmap.p, cstart.p,
+    NPX_TRY(launch("predict_bin", k_predict_bin, cdiv(blocks, NPX_BIN_BLOCK / NPX_BLOCK), NPX_BIN_BLOCK,
+                   (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p, blocks, (const float4*)X.p, (const float4*)V.p, (const int*)clsmap.p, cstart.p,
                    CLS.p, OFF.p, CELL.p, d_flags.p));
     NPX_TRY(scan(cstart.p, ncls + 1));
     NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
@@ -931,13 +931,13 @@ struct Ctx {
     // OFF (the binning offsets, read only by k_scatter) then holds each sorted slot's class
     NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
                    (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p, OFF.p));
-    // merge levels of the large classes spanning several blocks: enough for a
-    // class spanning every block (a fixed count, so a step's launches do not
+    // merge launches of the large classes spanning several blocks: enough for
+    // a class spanning every block (a fixed count, so a step's launches do not
     // depend on the state and it can be replayed as a graph)
     int levels = 0;
     while ((1 << levels) < blocks + 1) ++levels;
-    for (int L = 0; L < levels; ++L) {
-      const bool even = (L & 1) == 0;
+    for (int L = 0; L < levels; L += NPX_MERGE_LOG) {
+      const bool even = ((L / NPX_MERGE_LOG) & 1) == 0;
       NPX_TRY(launch("rank_merge", k_rank_merge, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)OFF.p, (const unsigned char*)BIGB.p,
                      L, (const int*)(even ? TPERM.p : MPERM.p), (const int*)(even ? TKEY.p : MKEY.p),

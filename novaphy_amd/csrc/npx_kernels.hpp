@@ -171,61 +171,89 @@ __global__ void k_ht_lookup(const unsigned long long* __restrict__ tk, const int
 // :37-40): v' = v + g*dt, x* = x + v'*dt, cell of x*, class, and the slot
 // offset inside the class from an atomic counter.  Nothing is written back to
 // X / V: the gather pass recomputes v' and x* with the same operations.
-__global__ void __launch_bounds__(NPX_BLOCK)
-k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
-              const float4* __restrict__ V, const int* __restrict__ clsmap, int* __restrict__ ccount,
-              int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of, int* __restrict__ flags) {
-  const int s = blk_scene[npx_slot() / NPX_BLOCK];
-  if (s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
+//
+// Counting: slots arrive in the previous step's class order, so a wave holds
+// a few segments of one class each.  Each segment's first lane adds the
+// segment's length to its class's entry in a per-block LDS table (1024 slots
+// per block, an open-addressing table with room for every slot's class), and
+// after the block's table is complete each entry takes one global atomic.  A
+// stack of many particles in one cell thus costs one global atomic per 1024
+// of them instead of one per wave, and a wave's segments count in parallel
+// instead of one after the other.  The offsets only place particles
+// provisionally; k_rank fixes the order.
+#define NPX_BIN_BLOCK 1024
+#define NPX_BIN_TABLE 2048
+
+__global__ void __launch_bounds__(NPX_BIN_BLOCK)
+k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, int nblk,
+              const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ clsmap,
+              int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of,
+              int* __restrict__ flags) {
+  __shared__ int tk[NPX_BIN_TABLE];  // class, -1: empty
+  __shared__ int tc[NPX_BIN_TABLE];  // the block's count, then its global base
+  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += NPX_BIN_BLOCK) {
+    tk[i] = -1;
+    tc[i] = 0;
+  }
   const int q = npx_slot();
-  const bool active = q < S.seg_off + S.n;
+  const int sb = q / NPX_BLOCK;
+  const int s = (sb < nblk) ? blk_scene[sb] : -1;  // uniform per wave
+  bool active = false;
   int c = -1;
-  if (active) {
-    const float4 x = X[q];
-    const float4 v = V[q];
-    const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
-    const float sx = x.x + vx * S.dt, sy = x.y + vy * S.dt, sz = x.z + vz * S.dt;
-    int lx = cell_coord(sx, S.inv_h) - S.wlo0;
-    int ly = cell_coord(sy, S.inv_h) - S.wlo1;
-    int lz = cell_coord(sz, S.inv_h) - S.wlo2;
-    // The 27-cell neighbourhood must lie inside the window; outside it the
-    // class table is undefined, so the step is flagged as failed.
-    if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
-      raise_flag(flags, s, NPX_FLAG_WINDOW);
-      lx = min(max(lx, 1), S.wx - 2);
-      ly = min(max(ly, 1), S.wy - 2);
-      lz = min(max(lz, 1), S.wz - 2);
+  if (s >= 0) {
+    const SceneC& S = scenes[s];
+    active = q < S.seg_off + S.n;
+    if (active) {
+      const float4 x = X[q];
+      const float4 v = V[q];
+      const float vx = v.x + S.gdt0, vy = v.y + S.gdt1, vz = v.z + S.gdt2;
+      const float sx = x.x + vx * S.dt, sy = x.y + vy * S.dt, sz = x.z + vz * S.dt;
+      int lx = cell_coord(sx, S.inv_h) - S.wlo0;
+      int ly = cell_coord(sy, S.inv_h) - S.wlo1;
+      int lz = cell_coord(sz, S.inv_h) - S.wlo2;
+      // The 27-cell neighbourhood must lie inside the window; outside it the
+      // class table is undefined, so the step is flagged as failed.
+      if (lx < 1 || ly < 1 || lz < 1 || lx > S.wx - 2 || ly > S.wy - 2 || lz > S.wz - 2) {
+        raise_flag(flags, s, NPX_FLAG_WINDOW);
+        lx = min(max(lx, 1), S.wx - 2);
+        ly = min(max(ly, 1), S.wy - 2);
+        lz = min(max(lz, 1), S.wz - 2);
+      }
+      const int l = lx + S.wx * (ly + S.wy * lz);
+      c = clsmap[S.cls_base + l];
+      cls_of[q] = c;
+      cell_of[q] = l;
     }
-    const int l = lx + S.wx * (ly + S.wy * lz);
-    c = clsmap[S.cls_base + l];
-    cls_of[q] = c;
-    cell_of[q] = l;
   }
-  // One atomic per distinct class of the wave (slots arrive in the previous
-  // step's class order, so a wave holds few classes, and a stack of
-  // particles on one point would otherwise serialise on one counter); after
-  // sixteen classes the rest of the wave (particles scattered over many cells)
-  // takes one atomic per lane.  The offsets only place particles
-  // provisionally; k_rank fixes the order.
+  __syncthreads();  // the table is clear
   const int lane = (int)(threadIdx.x & 63);
-  unsigned long long todo = __ballot(active);
-  int off = 0;
-  for (int round = 0; todo; ++round) {
-    if (round == 16) {
-      if ((todo >> lane) & 1ull) off = atomicAdd(&ccount[c], 1);
-      break;
+  const int cprev = __shfl(c, max(lane - 1, 0), 64);
+  const bool head = active && (lane == 0 || cprev != c);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long ends = heads | ~__ballot(active);  // segment starts and inactive lanes
+  int slot = 0, base = 0;
+  if (head) {
+    const unsigned long long after = (lane == 63) ? 0ull : (ends & ~((2ull << lane) - 1ull));
+    const int len = (after ? (int)__builtin_ctzll(after) : 64) - lane;
+    unsigned h = ((unsigned)c * 2654435761u) >> 21;  // 11 bits: NPX_BIN_TABLE entries
+    for (;;) {  // at most NPX_BIN_BLOCK distinct classes in a table of twice that: ends
+      const int old = atomicCAS(&tk[h], -1, c);
+      if (old == -1 || old == c) break;
+      h = (h + 1u) & (NPX_BIN_TABLE - 1);
     }
-    const int leader = (int)__builtin_ctzll(todo);
-    const int cl = __shfl(c, leader, 64);
-    const unsigned long long same = __ballot(active && c == cl) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&ccount[cl], (int)__popcll(same));
-    base = __shfl(base, leader, 64);
-    if ((same >> lane) & 1ull) off = base + (int)__popcll(same & ((1ull << lane) - 1ull));
-    todo &= ~same;
+    slot = (int)h;
+    base = atomicAdd(&tc[h], len);
   }
-  if (active) off_of[q] = off;
+  const int hl = active ? 63 - (int)__builtin_clzll(heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull))) : lane;
+  slot = __shfl(slot, hl, 64);
+  base = __shfl(base, hl, 64);
+  __syncthreads();  // the block's counts are complete
+  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += NPX_BIN_BLOCK) {
+    const int k = tk[i];
+    if (k >= 0) tc[i] = atomicAdd(&ccount[k], tc[i]);
+  }
+  __syncthreads();
+  if (active) off_of[q] = tc[slot] + base + (lane - hl);
 }
 
 // Pass 2: provisional slot = class start + atomic offset.
@@ -271,7 +299,7 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
     c = cls_of[p];
   }
   const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
-  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge levels
+  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge launches
   if (!big) return;
   unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
@@ -306,8 +334,8 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
 // unique within a scene.  Small classes (at most NPX_RANK_DIRECT) count
 // directly.  A larger class lying inside one block is already sorted by
 // k_rank_sort; one spanning several blocks is a sequence of sorted pieces
-// (its parts of consecutive blocks) that the merge levels below combine.
-// The class of each sorted slot goes to tcls: the merge levels' slot / key
+// (its parts of consecutive blocks) that the merge launches below combine.
+// The class of each sorted slot goes to tcls: the merge launches' slot / key
 // buffers hold valid entries only inside the ranges of the classes they
 // merge.
 #define NPX_RANK_STAGE 8
@@ -324,7 +352,7 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
   const int shift = cstart[S.cls_base] - S.seg_off;
   const int p = tperm[t];
   const int c = cls_of[p];
-  tcls[t] = c;  // the class of every sorted slot, for the merge levels
+  tcls[t] = c;  // the class of every sorted slot, for the merge launches
   const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
   if (e - b <= NPX_RANK_DIRECT) {
     const int my = tkey[t];
@@ -336,16 +364,64 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
   }
 }
 
-// Merge level L of the large classes spanning several blocks (a segmented
+// Merge launches of the large classes spanning several blocks (a segmented
 // merge sort).  The class's pieces are its parts of consecutive 256-slot
-// blocks, each sorted by k_rank_sort; before level L, runs of 2^L
-// consecutive pieces are sorted, and level L merges each run with its partner
-// run (run index ^ 1): a slot moves to the pair's start + its offset in its
-// run + the number of smaller keys in the partner run (one binary search;
-// keys are unique within a class).  The level that leaves one run writes the
-// final permutation; earlier ones write the other key / slot buffers.  Blocks
-// without a large class (bigb, from k_rank_sort) exit at once, so idle levels
-// cost one byte per block.
+// blocks, each sorted by k_rank_sort.  Before the launch at `level`, runs of
+// 2^level consecutive pieces are sorted; the launch merges each group of
+// NPX_MERGE_FAN consecutive runs (two levels of a pairwise merge sort in one
+// launch): a slot moves to its group's start + its offset in its run + the
+// number of smaller keys in the group's other runs (one binary search per
+// run, the searches stepped together so their loads overlap; keys are unique
+// within a class).  A class spanning G blocks is sorted after
+// ceil(log4 G) launches.
+#define NPX_MERGE_LOG 2
+#define NPX_MERGE_FAN (1 << NPX_MERGE_LOG)
+
+__device__ __forceinline__ int merge_dst(int b, int e, int t, int level, int my, const int* __restrict__ kin) {
+  constexpr int F = NPX_MERGE_FAN - 1;
+  const int B0 = b / NPX_BLOCK;
+  auto run_lo = [&](int r) { return max(b, (B0 + (r << level)) * NPX_BLOCK); };
+  auto run_hi = [&](int r) { return min(e, (B0 + ((r + 1) << level)) * NPX_BLOCK); };
+  const int r = (t / NPX_BLOCK - B0) >> level;
+  const int g = r & ~(NPX_MERGE_FAN - 1);
+  int lo[F], hi[F], q0[F];
+#pragma unroll
+  for (int u = 0; u < F; ++u) {
+    const int q = g + u + ((g + u >= r) ? 1 : 0);  // the group's runs other than r
+    lo[u] = run_lo(q);
+    hi[u] = run_hi(q);  // below lo when the run starts past the class
+    q0[u] = lo[u];
+  }
+  for (;;) {  // lower bounds of my key
+    bool any = false;
+    int m[F], kv[F];
+#pragma unroll
+    for (int u = 0; u < F; ++u) {
+      m[u] = (lo[u] + hi[u]) >> 1;
+      kv[u] = (lo[u] < hi[u]) ? kin[m[u]] : my;
+    }
+#pragma unroll
+    for (int u = 0; u < F; ++u) {
+      if (lo[u] < hi[u]) {
+        any = true;
+        if (kv[u] < my)
+          lo[u] = m[u] + 1;
+        else
+          hi[u] = m[u];
+      }
+    }
+    if (!any) break;
+  }
+  int rank = 0;
+#pragma unroll
+  for (int u = 0; u < F; ++u) rank += (q0[u] < e) ? lo[u] - q0[u] : 0;
+  return run_lo(g) + (t - run_lo(r)) + rank;
+}
+
+// One merge launch over the fluid sort's slots.  The launch that leaves one
+// run writes the final permutation; earlier ones write the other key / slot
+// buffers.  Blocks without a large class (bigb, from k_rank_sort) exit at
+// once, so idle launches cost one byte per block.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
              const int* __restrict__ tcls, const unsigned char* __restrict__ bigb, int level,
@@ -360,28 +436,13 @@ k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
   const int c = tcls[t];
   const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
   if (e - b <= NPX_RANK_DIRECT) return;
-  const int B0 = b / NPX_BLOCK;
-  const int npieces = (e - 1) / NPX_BLOCK - B0 + 1;
-  if (npieces <= (1 << level)) return;  // merged at an earlier level (or a single piece)
+  const int npieces = (e - 1) / NPX_BLOCK - b / NPX_BLOCK + 1;
+  if (npieces <= (1 << level)) return;  // merged by an earlier launch (or a single piece)
   const int p = pin[t];
-  auto run_lo = [&](int r) { return max(b, (B0 + (r << level)) * NPX_BLOCK); };
-  auto run_hi = [&](int r) { return min(e, (B0 + ((r + 1) << level)) * NPX_BLOCK); };
-  const int r = (t / NPX_BLOCK - B0) >> level;
-  const int q = r ^ 1;
   const int my = kin[t];
-  int lo = run_lo(q), hi = run_hi(q);
-  const int q0 = lo;
-  while (lo < hi) {  // lower bound of my key in the partner run (empty when it starts past the class)
-    const int mid = (lo + hi) >> 1;
-    if (kin[mid] < my)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  const int rank = (q0 < e) ? lo - q0 : 0;
-  const int dst = run_lo(r & ~1) + (t - run_lo(r)) + rank;
-  if (npieces <= (2 << level)) {
-    perm[dst] = p;  // the last level of this class
+  const int dst = merge_dst(b, e, t, level, my, kin);
+  if (npieces <= (NPX_MERGE_FAN << level)) {
+    perm[dst] = p;  // the last launch of this class
   } else {
     pout[dst] = p;
     kout[dst] = my;
@@ -2032,7 +2093,7 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
     c = cls_of[p];
   }
   const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
-  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge levels
+  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge launches
   if (!big) return;
   unsigned long long key = ~0ull;
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
@@ -2088,8 +2149,8 @@ k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restric
   }
 }
 
-// Merge level of the auxiliary sort's large classes (k_rank_merge without
-// scenes); the class's last level writes the permutation and the payload.
+// Merge launch of the auxiliary sort's large classes (k_rank_merge without
+// scenes); the class's last launch writes the permutation and the payload.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict__ tcls,
                  const unsigned char* __restrict__ bigb, int level, const int* __restrict__ pin,
@@ -2101,27 +2162,12 @@ k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict
   const int c = tcls[t];
   const int b = cstart[c], e = cstart[c + 1];
   if (e - b <= NPX_RANK_DIRECT) return;
-  const int B0 = b / NPX_BLOCK;
-  const int npieces = (e - 1) / NPX_BLOCK - B0 + 1;
+  const int npieces = (e - 1) / NPX_BLOCK - b / NPX_BLOCK + 1;
   if (npieces <= (1 << level)) return;
   const int p = pin[t];
-  auto run_lo = [&](int r) { return max(b, (B0 + (r << level)) * NPX_BLOCK); };
-  auto run_hi = [&](int r) { return min(e, (B0 + ((r + 1) << level)) * NPX_BLOCK); };
-  const int r = (t / NPX_BLOCK - B0) >> level;
-  const int q = r ^ 1;
   const int my = kin[t];
-  int lo = run_lo(q), hi = run_hi(q);
-  const int q0 = lo;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (kin[mid] < my)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  const int rank = (q0 < e) ? lo - q0 : 0;
-  const int dst = run_lo(r & ~1) + (t - run_lo(r)) + rank;
-  if (npieces <= (2 << level)) {
+  const int dst = merge_dst(b, e, t, level, my, kin);
+  if (npieces <= (NPX_MERGE_FAN << level)) {
     perm[dst] = p;
     PS[dst] = P[p];
   } else {
