@@ -301,33 +301,59 @@ def run_coupled(steps, warmup):
             "boundary_particles": world.num_boundary_particles, "phases_avg_ms": phases}
 
 
-def run_serve(seconds, rate_hz, threads, backend="fused"):
+def run_serve(seconds, rate_hz, threads, backend="fused", n_requests=0, kernel_window=(0, 0)):
     """Served rollouts: Poisson arrivals (seed 4) of dam breaks with N
     log-uniform in [4k, 256k], horizons uniform in [50, 500] and 0-8 rigid
     spheres, admitted against 80 % of 288 GB.  fused: every request shares one
     batch context, with the coupling of all requests with spheres in one
     launch per pass and their boundary particles sampled on the device; mixed:
     requests with spheres run as FluidWorld instances stepped concurrently.
-    The sample is the requests arriving in the first `seconds` of server time;
-    the run ends when they have all finished or at 3x that time."""
+    The sample is the requests arriving in the first `seconds` of server time
+    (or the first `n_requests` of the stream); the run ends when they have
+    all finished or at 3x their arrival window.  kernel_window (a, b): per-kernel
+    HIP-event times of the batch over server rounds a..b-1 (none when a >= b)."""
     from novaphy_amd import capi, scenes, serve
-    n_req = max(1, int(round(seconds * rate_hz)))
-    reqs = serve.poisson_requests(n_req, rate_hz, seed=4, make_scene=lambda k, rng: scenes.served_request(k, rng))
-    log(f"[bench] serve: {n_req} requests over {seconds:.0f} s, N {min(r.scene.n for r in reqs)}-"
-        f"{max(r.scene.n for r in reqs)}, {sum(bool(r.scene.spheres) for r in reqs)} with spheres")
+    n_req = int(n_requests) if n_requests > 0 else max(1, int(round(seconds * rate_hz)))
+    # requests are drawn from the seeded stream as they arrive, so only the
+    # scenes that have arrived are held on the host
+    stream = serve.poisson_stream(n_req, rate_hz, seed=4, make_scene=lambda k, rng: scenes.served_request(k, rng))
+    log(f"[bench] serve: {n_req} requests at {rate_hz}/s")
     b = capi.Batch(slot_capacity=1 << 24, cell_capacity=1 << 26, max_scenes=256)
     be = (serve.FusedBackend(b, 256, device_sampling=True) if backend == "fused"
           else serve.MixedBackend(b, 256, threads=threads))
-    srv = serve.Server(be, hbm_budget_bytes=int(0.8 * 288e9), max_scenes=256, bytes_fn=capi.scene_bytes)
-    for r in reqs:
-        srv.submit(r)
+    srv = serve.Server(be, hbm_budget_bytes=int(0.8 * 288e9), max_scenes=256, bytes_fn=capi.scene_bytes,
+                       keep_results=False)
     t0 = time.perf_counter()
     srv.clock = lambda: time.perf_counter() - t0
+    nxt = next(stream, None)
+    ktimes = None
+    sizes, with_spheres, window = [], 0, 0.0
     active = []
-    while (srv.pending or srv.active) and time.perf_counter() - t0 < 3 * seconds:
+    next_log = 30.0
+    limit = 3.0 * max(n_req / rate_hz, 1.0)
+    while (nxt is not None or srv.pending or srv.active) and time.perf_counter() - t0 < limit:
+        now = srv.clock()
+        while nxt is not None and nxt.arrival <= now:
+            sizes.append(nxt.scene.n)
+            with_spheres += bool(nxt.scene.spheres)
+            window = nxt.arrival
+            srv.submit(nxt)
+            nxt = next(stream, None)
+        if srv.steps == kernel_window[0] and kernel_window[0] < kernel_window[1] and ktimes is None:
+            b.set_kernel_timing(True)
+            ktimes = {}
         if not srv.step():
             time.sleep(0.005)
+        if srv.steps == kernel_window[1] and ktimes == {}:
+            ktimes = {k: {"total_ms": round(v[0], 3), "launches": v[1]} for k, v in b.kernel_times().items()}
+            b.set_kernel_timing(False)
         active.append(len(srv.active))
+        el = time.perf_counter() - t0
+        if el >= next_log:
+            next_log += 30.0
+            log(f"[bench] serve t={el:.0f}s: {len(sizes)} arrived, {len(srv.finished)} finished, "
+                f"{len(srv.active)} active, {len(srv.pending)} waiting, "
+                f"{srv.particle_steps / el / 1e6:.1f} M particle-steps/s so far")
     wall = time.perf_counter() - t0
     if backend != "fused":
         be.close()
@@ -335,10 +361,15 @@ def run_serve(seconds, rate_hz, threads, backend="fused"):
     done = [r for r in srv.finished.values() if r.status == serve.DONE]
     lat = srv.latency_percentiles()
     return {"value": srv.particle_steps / wall, "wall_s": wall, "requests": n_req, "done": len(done),
-            "unfinished": len(srv.active) + len(srv.pending),
+            "unfinished": len(srv.active) + len(srv.pending) + (n_req - len(sizes)),
             "failed": sum(r.status == serve.FAILED for r in srv.finished.values()),
             "p50_step_ms": lat["p50_ms"], "p99_step_ms": lat["p99_ms"], "latency_samples": lat["samples"],
-            "mean_active": float(np.mean(active)) if active else 0.0, "rounds": srv.steps}
+            "mean_active": float(np.mean(active)) if active else 0.0, "rounds": srv.steps,
+            "arrival_window_s": window, "admission_s": srv.admit_s, "step_s": srv.busy_s,
+            "n_range": [min(sizes), max(sizes)] if sizes else None, "with_spheres": with_spheres,
+            "phases_s": {k: round(v, 3) for k, v in getattr(be, "phase_s", {}).items()},
+            "kernel_window": (f"rounds {kernel_window[0]}-{kernel_window[1] - 1}" if ktimes else None),
+            "kernels": ktimes or None}
 
 
 def cpu_baseline(sc, state, budget_s):
@@ -392,7 +423,8 @@ def serve_summary(r, rate, backend):
                         + (", boundaries sampled on the device" if backend == "fused" else ""),
             "value": r["value"], "p50_step_ms": r["p50_step_ms"], "p99_step_ms": r["p99_step_ms"],
             "serve": {k: r[k] for k in ("requests", "done", "unfinished", "failed", "wall_s", "latency_samples",
-                                        "mean_active")}}
+                                        "mean_active", "arrival_window_s", "admission_s", "step_s", "n_range",
+                                        "with_spheres", "phases_s", "kernel_window", "kernels")}}
 
 
 def main():
@@ -406,6 +438,10 @@ def main():
                     help="1m workload: also measure the batched and served workloads (extra keys)")
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="arrival window of the served sample")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-kernel-window", default="0,0",
+                    help="serve workload: per-kernel times over server rounds a,b (a >= b: off)")
+    ap.add_argument("--serve-requests", type=int, default=0,
+                    help="serve workload: the first N requests of the stream (0: those of --serve-seconds)")
     ap.add_argument("--serve-threads", type=int, default=8, help="host threads stepping coupled requests (mixed)")
     ap.add_argument("--serve-backend", default="fused", choices=["fused", "mixed"],
                     help="coupling of requests with spheres: fused into the batch, or separate FluidWorlds")
@@ -453,7 +489,8 @@ def main():
         print(json.dumps(line), flush=True)
         return
     if args.workload == "serve":
-        r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend)
+        r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend,
+                      args.serve_requests, tuple(int(x) for x in args.serve_kernel_window.split(",")))
         ss = serve_summary(r, args.serve_rate, args.serve_backend)
         line = {**common, "value": r["value"], "steps": r["rounds"], "warmup": 0, "ms_per_step": r["p50_step_ms"],
                 "config": {"workload": ss.pop("workload")}, **ss}

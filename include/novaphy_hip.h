@@ -154,10 +154,10 @@ typedef struct npx_shape {
    every shape, shape by shape in the reference's loop order, in body-local
    coordinates (world coordinates for planes), computed on the device.
    *count receives the particle count; nothing is written when it exceeds cap
-   (call with cap = 0 to size local3 (3 floats each) and body).  Box and
-   plane samples equal the reference's float operations bit for bit; sphere
-   samples evaluate acos / sin / cos in double and round to float, which can
-   differ from a host libm's float functions in the last bit. */
+   (call with cap = 0 to size local3 (3 floats each) and body).  Every
+   sample equals the reference's float operations bit for bit: the sphere
+   samples' acos / sin / cos are the host libm's float functions, evaluated
+   in this call before the launch (4 floats per sphere sample). */
 int npx_sample_boundaries(const npx_shape* shapes, int32_t nshapes, float spacing, float plane_extent,
                           float* local3, int32_t* body, int64_t cap, int64_t* count);
 

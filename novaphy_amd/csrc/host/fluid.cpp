@@ -514,6 +514,13 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
           auto it = kt.find(k);
           if (it != kt.end()) monitor_.record_phase_ms(ph, it->second.first);
         }
+        // the boundary and coupling kernels (no counterpart phase in the
+        // reference) under their own names
+        for (const auto& [k, v] : kt) {
+          bool mapped = false;
+          for (const auto& m : map) mapped = mapped || k == m.first;
+          if (!mapped) monitor_.record_phase_ms("fluid.gpu." + k, v.first);
+        }
       }
     }
     {

@@ -161,12 +161,16 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
     const GridWin W = c->scene_win(H);
     const int* cm = c->clsmap.p + H.cls_base;
     NPX_TRY(c->aux_sort(W, cm, H.cls_base, c->FP.p + H.seg_off, H.n, c->ID.p + H.seg_off, c->FS.p));
+    NPX_TRY(c->payload_runs(H.n, (const int*)(c->Acstart.p - H.cls_base), W.ncell + H.cls_base,
+                            (const int*)c->AOFF.p, (const float4*)c->FS.p));
     const float h = H.prm.kernel_radius;
     const float h6 = h * h * h * h * h * h;
     const float cs = 45.0f / (npx::kPI * h6);
-    NPX_TRY(c->launch("coupling", k_coupling_particles, cdiv(c->nb, 256), 256, W, h, h * h, cs, H.prm.rest_density,
+    // a wave per boundary particle
+    NPX_TRY(c->launch("coupling", k_coupling_particles, cdiv(c->nb, 4), 256, W, h, h * h, cs, H.prm.rest_density,
                       H.mass, (const float4*)c->BP.p, (const int*)c->BODY.p, c->nb, (const uint8_t*)c->STATIC.p,
-                      nbodies, cm, (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->FS.p, c->FB.p));
+                      nbodies, cm, (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->FS.p,
+                      (const int*)c->CRPI.p, (const int*)c->CRPS.p, c->FB.p));
     NPX_TRY(c->launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)c->BODY_PTR.p,
                       (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
                       (const float*)c->POSE.p, c->OUT6.p));
@@ -289,7 +293,8 @@ void destroy(Ctx* c) {
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
                           &c->PERM, &c->MPERM, &c->MKEY, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
-                          &c->ATKEY, &c->APERM, &c->AMPERM, &c->AMKEY, &c->aux_clsmap};
+                          &c->ATKEY, &c->APERM, &c->AMPERM, &c->AMKEY, &c->aux_clsmap, &c->CRF, &c->CRPI,
+                          &c->CRPS};
   for (auto* b : i1) b->release();
   c->htk.release();
   c->BIGB.release();
@@ -717,7 +722,7 @@ int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const 
                     wlo[0], wlo[1], wlo[2], wdim[0], wdim[1], nc, 0, clsmap);
     // points sorted into classes, ascending index inside a class (bucket order)
     if (e == hipSuccess) e = na_launch(k_fill_i32, cdiv(nc + 1, 256), 256, cst, 0, nc + 1);
-    if (e == hipSuccess) e = na_launch(k_aux_bin, cdiv(n, 256), 256, W, (const float4*)P, n, (const int*)clsmap, cst, cls, off);
+    if (e == hipSuccess) e = na_launch(k_aux_bin, cdiv(n, NPX_BIN_BLOCK), NPX_BIN_BLOCK, W, (const float4*)P, n, (const int*)clsmap, cst, cls, off);
     if (e == hipSuccess) e = scan(cst, nc + 1);
     if (e == hipSuccess)
       e = na_launch(k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)cls, (const int*)off,
@@ -864,22 +869,39 @@ int npx_sample_boundaries(const npx_shape* shapes, int32_t nshapes, float spacin
       g_create_error = "no HIP device available";
       return NPX_EDEVICE;
     }
+    // The sphere samples' trigonometry, with the reference's float calls
+    // (sample_sphere_boundary, boundary.cpp:55-91): the theta argument
+    // 1 - 2 (i + 0.5) / n and phi = golden_angle * i in float, then acos,
+    // sin and cos of float.
+    std::vector<float4> trig((size_t)total, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    for (const ShapeS& S : hs) {
+      if (S.type != NPX_SHAPE_SPHERE) continue;
+      for (int i = 0; i < S.n; ++i) {
+        const float theta = std::acos(1.0f - 2.0f * ((float)i + 0.5f) / (float)S.n);
+        const float phi = S.golden_angle * (float)i;
+        trig[(size_t)S.first + i] = make_float4(std::sin(theta), std::cos(theta), std::sin(phi), std::cos(phi));
+      }
+    }
     ShapeS* ds = nullptr;
+    float4* dt = nullptr;
     float* dl = nullptr;
     int* db = nullptr;
     hipError_t e = hipMalloc((void**)&ds, sizeof(ShapeS) * hs.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&dt, sizeof(float4) * (size_t)total);
     if (e == hipSuccess) e = hipMalloc((void**)&dl, sizeof(float) * 3 * (size_t)total);
     if (e == hipSuccess) e = hipMalloc((void**)&db, sizeof(int) * (size_t)total);
     if (e == hipSuccess) e = hipMemcpy(ds, hs.data(), sizeof(ShapeS) * hs.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, trig.data(), sizeof(float4) * (size_t)total, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
       const int n = (int)total;
       hipLaunchKernelGGL(k_sample_boundaries, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, 0, (const ShapeS*)ds,
-                         nshapes, n, spacing, dl, db);
+                         nshapes, n, spacing, (const float4*)dt, dl, db);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(local3, dl, sizeof(float) * 3 * (size_t)total, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(body, db, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost);
     if (ds) (void)hipFree(ds);
+    if (dt) (void)hipFree(dt);
     if (dl) (void)hipFree(dl);
     if (db) (void)hipFree(db);
     if (e != hipSuccess) {

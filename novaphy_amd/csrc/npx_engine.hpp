@@ -251,6 +251,7 @@ struct Ctx {
   DBuf<float> RHO, RHO2, LAM, LAM2;
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
   DBuf<int> MPERM, MKEY;  // the other slot / key buffers of the large-class merge launches
+  DBuf<int> CRF, CRPI, CRPS;  // runs of the coupling's fluid sort: flags / scan, run of each slot, run starts
   DBuf<uint8_t> BIGB;     // per 256-slot block: it holds part of a large class
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
@@ -353,13 +354,28 @@ struct Ctx {
     NPX_HIP(ps.bigb.ensure(m / NPX_BLOCK + 2));
     NPX_TRY(launch("bsort_fill", k_fill_i32, cdiv(ncls + 1, 256), 256, ps.cst.p, 0, ncls + 1));
     if (n == 0) return NPX_OK;
-    NPX_TRY(launch("bsort_bin", k_bin_scene_points, cdiv(n, 256), 256, (const SceneC*)d_scenes.p, P, pscene, n,
+    NPX_TRY(launch("bsort_bin", k_bin_scene_points, cdiv(n, NPX_BIN_BLOCK), NPX_BIN_BLOCK, (const SceneC*)d_scenes.p,
+                   P, pscene, n,
                    (const int*)clsmap.p, ps.cst.p, ps.cls.p, ps.off.p));
     NPX_TRY(scan(ps.cst.p, ncls + 1));
     NPX_TRY(launch("bsort_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)ps.cst.p, (const int*)ps.cls.p,
                    (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p));
     return aux_rank_all("bsort_rank", n, ncls, ps.cst.p, ps.cls.p, ps.off.p, ps.tperm.p, ps.tkey.p, ps.mperm.p,
                         ps.mkey.p, ps.bigb.p, P, ps.perm.p, PS);
+  }
+
+  // Runs of identical points of a sorted payload (k_payload_runs /
+  // k_payload_pieces) into CRPI / CRPS: n sorted slots at most, cst the class
+  // starts (ncls + 1 entries, by class id), tcls each sorted slot's class.
+  int payload_runs(int n, const int* cst, int ncls, const int* tcls, const float4* PS) {
+    NPX_HIP(CRF.ensure((size_t)n + 1));
+    NPX_HIP(CRPI.ensure((size_t)std::max(n, 1)));
+    NPX_HIP(CRPS.ensure((size_t)std::max(n, 1)));
+    NPX_TRY(launch("payload_runs", k_payload_runs, cdiv(n + 1, 256), 256, n, cst, ncls, tcls, PS, CRF.p));
+    NPX_TRY(scan(CRF.p, n + 1));
+    NPX_TRY(launch("payload_pieces", k_payload_pieces, cdiv(n, 256), 256, n, cst, ncls, (const int*)CRF.p, CRPI.p,
+                   CRPS.p));
+    return NPX_OK;
   }
 
   // Gathers the scenes' boundaries into the batch-wide arrays.
@@ -445,9 +461,11 @@ struct Ctx {
                    (const float*)RHO.p, BFP.p));
     NPX_TRY(launch("coupled_slots", k_coupled_slots, hw_blocks, NPX_BLOCK, sc, bs, FPSC.p));
     NPX_TRY(sort_scene_points(fsort, BFP.p, FPSC.p, hw_blocks * NPX_BLOCK, ID.p, BFS.p));
-    NPX_TRY(launch("coupling", k_coupling_b, cdiv(bb_n, 256), 256, sc, (const float4*)BBP.p, (const int*)BSCN.p,
+    NPX_TRY(payload_runs(hw_blocks * NPX_BLOCK, fsort.cst.p, total_cls(), fsort.off.p, BFS.p));
+    // a wave per boundary particle
+    NPX_TRY(launch("coupling", k_coupling_b, cdiv(bb_n, 4), 256, sc, (const float4*)BBP.p, (const int*)BSCN.p,
                    (const int*)BBODY.p, bb_n, (const uint8_t*)BSTAT.p, nbodies, (const int*)clsmap.p,
-                   (const int*)fsort.cst.p, (const float4*)BFS.p, BFB.p));
+                   (const int*)fsort.cst.p, (const float4*)BFS.p, (const int*)CRPI.p, (const int*)CRPS.p, BFB.p));
     if (nbodies > 0) {
       NPX_TRY(launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
                      nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p, BOUT6.p));
@@ -1240,7 +1258,7 @@ struct Ctx {
     NPX_HIP(ABIGB.ensure((size_t)cdiv(std::max(n, 1), NPX_BLOCK) + 1));
     int* cst = Acstart.p - cls_base;  // indexed by class id
     NPX_TRY(launch("aux_fill", k_fill_i32, cdiv(W.ncell + 1, 256), 256, Acstart.p, 0, W.ncell + 1));
-    NPX_TRY(launch("aux_bin", k_aux_bin, cdiv(n, 256), 256, W, P, n, cm, cst, ACLS.p, AOFF.p));
+    NPX_TRY(launch("aux_bin", k_aux_bin, cdiv(n, NPX_BIN_BLOCK), NPX_BIN_BLOCK, W, P, n, cm, cst, ACLS.p, AOFF.p));
     NPX_TRY(scan(Acstart.p, W.ncell + 1));
     NPX_TRY(launch("aux_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)ACLS.p,
                    (const int*)AOFF.p, key, ATPERM.p, ATKEY.p));

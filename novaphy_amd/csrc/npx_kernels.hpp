@@ -167,34 +167,79 @@ __global__ void k_ht_lookup(const unsigned long long* __restrict__ tk, const int
 }
 
 // ------------------------------------------------------ fluid sort
-// Pass 1 (PBFSolver::step predict, pbf_solver.cpp:28-34, and grid build
-// :37-40): v' = v + g*dt, x* = x + v'*dt, cell of x*, class, and the slot
-// offset inside the class from an atomic counter.  Nothing is written back to
-// X / V: the gather pass recomputes v' and x* with the same operations.
-//
-// Counting: slots arrive in the previous step's class order, so a wave holds
-// a few segments of one class each.  Each segment's first lane adds the
-// segment's length to its class's entry in a per-block LDS table (1024 slots
-// per block, an open-addressing table with room for every slot's class), and
-// after the block's table is complete each entry takes one global atomic.  A
-// stack of many particles in one cell thus costs one global atomic per 1024
-// of them instead of one per wave, and a wave's segments count in parallel
-// instead of one after the other.  The offsets only place particles
-// provisionally; k_rank fixes the order.
+// Class counting of the binning kernels (k_predict_bin, k_aux_bin,
+// k_bin_scene_points), blocks of NPX_BIN_BLOCK threads.  Points arrive in an
+// order where equal classes sit together (the previous step's class order),
+// so a wave holds a few segments of one class each.  Each segment's first
+// lane adds the segment's length to its class's entry in a per-block LDS table
+// (open addressing, room for every thread's class), and once the block's
+// table is complete each entry takes one global atomic.  A stack of many
+// particles in one cell thus costs one global atomic per block instead of one
+// per lane or per wave, and a wave's segments count in parallel.  The offsets
+// only place points provisionally; the rank kernels fix the order.
 #define NPX_BIN_BLOCK 1024
 #define NPX_BIN_TABLE 2048
 
+struct BinTable {
+  int k[NPX_BIN_TABLE];  // class, -1: empty
+  int c[NPX_BIN_TABLE];  // the block's count, then its global base
+};
+
+__device__ __forceinline__ void bin_clear(BinTable& T) {
+  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += (int)blockDim.x) {
+    T.k[i] = -1;
+    T.c[i] = 0;
+  }
+}
+
+// Offset of the calling point inside class c (c >= 0 when active).  Every
+// thread of the block calls it, after bin_clear.
+__device__ __forceinline__ int bin_offset(BinTable& T, int c, bool active, int* __restrict__ ccount) {
+  __syncthreads();  // the table is clear
+  const int lane = (int)(threadIdx.x & 63);
+  const int cprev = __shfl(c, max(lane - 1, 0), 64);
+  const bool head = active && (lane == 0 || cprev != c);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long ends = heads | ~__ballot(active);  // segment starts and idle lanes
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+  int slot = 0, base = 0;
+  if (head) {
+    const unsigned long long after = ends & ~upto;
+    const int len = (after ? (int)__builtin_ctzll(after) : 64) - lane;
+    unsigned h = ((unsigned)c * 2654435761u) >> 21;  // 11 bits: NPX_BIN_TABLE entries
+    for (;;) {  // at most blockDim distinct classes in a table of twice that: ends
+      const int old = atomicCAS(&T.k[h], -1, c);
+      if (old == -1 || old == c) break;
+      h = (h + 1u) & (NPX_BIN_TABLE - 1);
+    }
+    slot = (int)h;
+    base = atomicAdd(&T.c[h], len);
+  }
+  // an active lane's segment starts at the last head at or before it
+  const int hl = active ? 63 - (int)__builtin_clzll(heads & upto) : lane;
+  slot = __shfl(slot, hl, 64);
+  base = __shfl(base, hl, 64);
+  __syncthreads();  // the block's counts are complete
+  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += (int)blockDim.x) {
+    const int k = T.k[i];
+    if (k >= 0) T.c[i] = atomicAdd(&ccount[k], T.c[i]);
+  }
+  __syncthreads();
+  return active ? T.c[slot] + base + (lane - hl) : 0;
+}
+
+// Pass 1 (PBFSolver::step predict, pbf_solver.cpp:28-34, and grid build
+// :37-40): v' = v + g*dt, x* = x + v'*dt, cell of x*, class, and the slot
+// offset inside the class (bin_offset).  Nothing is written back to X / V:
+// the gather pass recomputes v' and x* with the same operations.  nblk: the
+// 256-slot blocks of the slot range.
 __global__ void __launch_bounds__(NPX_BIN_BLOCK)
 k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, int nblk,
               const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ clsmap,
               int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of,
               int* __restrict__ flags) {
-  __shared__ int tk[NPX_BIN_TABLE];  // class, -1: empty
-  __shared__ int tc[NPX_BIN_TABLE];  // the block's count, then its global base
-  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += NPX_BIN_BLOCK) {
-    tk[i] = -1;
-    tc[i] = 0;
-  }
+  __shared__ BinTable T;
+  bin_clear(T);
   const int q = npx_slot();
   const int sb = q / NPX_BLOCK;
   const int s = (sb < nblk) ? blk_scene[sb] : -1;  // uniform per wave
@@ -225,35 +270,8 @@ k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       cell_of[q] = l;
     }
   }
-  __syncthreads();  // the table is clear
-  const int lane = (int)(threadIdx.x & 63);
-  const int cprev = __shfl(c, max(lane - 1, 0), 64);
-  const bool head = active && (lane == 0 || cprev != c);
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long ends = heads | ~__ballot(active);  // segment starts and inactive lanes
-  int slot = 0, base = 0;
-  if (head) {
-    const unsigned long long after = (lane == 63) ? 0ull : (ends & ~((2ull << lane) - 1ull));
-    const int len = (after ? (int)__builtin_ctzll(after) : 64) - lane;
-    unsigned h = ((unsigned)c * 2654435761u) >> 21;  // 11 bits: NPX_BIN_TABLE entries
-    for (;;) {  // at most NPX_BIN_BLOCK distinct classes in a table of twice that: ends
-      const int old = atomicCAS(&tk[h], -1, c);
-      if (old == -1 || old == c) break;
-      h = (h + 1u) & (NPX_BIN_TABLE - 1);
-    }
-    slot = (int)h;
-    base = atomicAdd(&tc[h], len);
-  }
-  const int hl = active ? 63 - (int)__builtin_clzll(heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull))) : lane;
-  slot = __shfl(slot, hl, 64);
-  base = __shfl(base, hl, 64);
-  __syncthreads();  // the block's counts are complete
-  for (int i = (int)threadIdx.x; i < NPX_BIN_TABLE; i += NPX_BIN_BLOCK) {
-    const int k = tk[i];
-    if (k >= 0) tc[i] = atomicAdd(&ccount[k], tc[i]);
-  }
-  __syncthreads();
-  if (active) off_of[q] = tc[slot] + base + (lane - hl);
+  const int off = bin_offset(T, c, active, ccount);
+  if (active) off_of[q] = off;
 }
 
 // Pass 2: provisional slot = class start + atomic offset.
@@ -2031,11 +2049,14 @@ k_scene_health(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sc
 // SpatialHashGrid in the reference (fluid_world.cpp:106-107, 134-135,
 // boundary.cpp:162-163).  Points outside the window interior are left out
 // (cls = -1): no point inside the window can have them within h.
-// One atomic per distinct class of a wave, as in k_predict_bin: the fluid's
-// committed positions arrive in class order and stacks of particles on one
+// Offsets from bin_offset (blocks of NPX_BIN_BLOCK threads): the fluid's
+// committed positions arrive in class order, and stacks of particles on one
 // point would otherwise serialise on one counter.
-__global__ void k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const int* __restrict__ clsmap,
-                          int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
+__global__ void __launch_bounds__(NPX_BIN_BLOCK)
+k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const int* __restrict__ clsmap,
+          int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
+  __shared__ BinTable T;
+  bin_clear(T);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int c = -1;
   if (i < n) {
@@ -2048,19 +2069,7 @@ __global__ void k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const 
     cls_of[i] = c;
   }
   const bool active = c >= 0;
-  const int lane = (int)(threadIdx.x & 63);
-  unsigned long long todo = __ballot(active);
-  int off = 0;
-  while (todo) {
-    const int leader = (int)__builtin_ctzll(todo);
-    const int cl = __shfl(c, leader, 64);
-    const unsigned long long same = __ballot(active && c == cl) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&ccount[cl], (int)__popcll(same));
-    base = __shfl(base, leader, 64);
-    if ((same >> lane) & 1ull) off = base + (int)__popcll(same & ((1ull << lane) - 1ull));
-    todo &= ~same;
-  }
+  const int off = bin_offset(T, c, active, ccount);
   if (active) off_of[i] = off;
 }
 
@@ -2206,9 +2215,13 @@ __device__ __forceinline__ void transform_point3(const float* pos, const float* 
   for (int k = 0; k < 3; ++k) o[k] = pos[k] + ((v[k] + rot[0] * uv[k]) + c[k]);
 }
 
-// One thread per boundary particle.
+// One thread per boundary particle.  trig holds, per sphere sample, (sin
+// theta, cos theta, sin phi, cos phi) evaluated on the host with the float
+// functions of the host's libm, which the reference's sampler calls, so the
+// samples are the host samplers' bit for bit.
 __global__ void k_sample_boundaries(const ShapeS* __restrict__ shapes, int nshapes, int total, float spacing,
-                                    float* __restrict__ local3, int* __restrict__ body) {
+                                    const float4* __restrict__ trig, float* __restrict__ local3,
+                                    int* __restrict__ body) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= total) return;
   int s = 0;
@@ -2228,10 +2241,8 @@ __global__ void k_sample_boundaries(const ShapeS* __restrict__ shapes, int nshap
     lp[a2] = -S.he[a2] + (float)i2 * spacing;
     transform_point3(S.pos, S.rot, lp, o);
   } else if (S.type == NPX_SHAPE_SPHERE) {  // sample_sphere_boundary, boundary.cpp:55-91
-    const float theta = (float)acos((double)(1.0f - 2.0f * ((float)i + 0.5f) / (float)S.n));
-    const float phi = S.golden_angle * (float)i;
-    const float st = (float)sin((double)theta), ct = (float)cos((double)theta);
-    const float sp = (float)sin((double)phi), cp = (float)cos((double)phi);
+    const float4 tg = trig[p];
+    const float st = tg.x, ct = tg.y, sp = tg.z, cp = tg.w;
     lp[0] = S.r * st * cp;
     lp[1] = S.r * st * sp;
     lp[2] = S.r * ct;
@@ -2340,42 +2351,142 @@ k_fluid_payload(const SceneC* __restrict__ scenes, const int* __restrict__ blk_s
   P[q] = make_float4(x.x, x.y, x.z, RHO[q]);
 }
 
+// Runs of identical points in the coupling's fluid sort: slot j starts a run
+// unless it holds the same class and bitwise the same payload (committed
+// position and density) as slot j - 1.  F[j] = 1 at a start, 0 for the
+// slots past the sorted count cstart[ncls] and for F[n].
+__global__ void k_payload_runs(int n, const int* __restrict__ cstart, int ncls, const int* __restrict__ tcls,
+                               const float4* __restrict__ PS, int* __restrict__ F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > n) return;
+  int f = 0;
+  if (j < cstart[ncls]) {
+    f = 1;
+    if (j > 0 && tcls[j] == tcls[j - 1]) {
+      const float4 a = PS[j], b = PS[j - 1];
+      f = (__float_as_uint(a.x) != __float_as_uint(b.x) || __float_as_uint(a.y) != __float_as_uint(b.y) ||
+           __float_as_uint(a.z) != __float_as_uint(b.z) || __float_as_uint(a.w) != __float_as_uint(b.w))
+              ? 1
+              : 0;
+    }
+  }
+  F[j] = f;
+}
+
+// After the exclusive scan of F: pi[j] = the run of slot j, ps[r] = the
+// first slot of run r.
+__global__ void k_payload_pieces(int n, const int* __restrict__ cstart, int ncls, const int* __restrict__ E,
+                                 int* __restrict__ pi, int* __restrict__ ps) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || j >= cstart[ncls]) return;
+  pi[j] = E[j + 1] - 1;
+  if (E[j + 1] != E[j]) ps[E[j]] = j;
+}
+
+// The coupling force term of fluid point xf on boundary particle xb
+// (fluid_world.cpp:140-160): false when the pair adds nothing (beyond h, or
+// outside the kernel's support).
+__device__ __forceinline__ bool coupling_term(float4 xb, float4 xf, float h, float h2, float cs, float rho0,
+                                              float mass, float& tx, float& ty, float& tz) {
+  const float psi_b = xb.w;
+  const float rx = xb.x - xf.x, ry = xb.y - xf.y, rz = xb.z - xf.z;
+  const float r_sq = sqn3(rx, ry, rz);
+  if (r_sq >= h2) return false;
+  const float rho_f = (xf.w < 1.0f) ? 1.0f : xf.w;
+  const float pr = rho_f - rho0;
+  const float pressure = (0.0f < pr) ? pr : 0.0f;
+  const float pc = mass * pressure / (rho_f * rho_f);
+  bool in;
+  const float co = spiky_coeff(r_sq, h, cs, &in);
+  if (!in) return false;
+  const float kc = pc * rho0 * psi_b;
+  tx = kc * (co * rx);
+  ty = kc * (co * ry);
+  tz = kc * (co * rz);
+  return true;
+}
+
+__device__ __forceinline__ float lane_f(float v, int u) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u));
+}
+
+// F_b of one boundary particle in window cell l, computed by the calling
+// wave (every lane gets the sum).  The fluid candidates of b's 27 cells are
+// taken in the reference's order as runs of identical points (rpi / rps:
+// k_payload_pieces; a candidate range's first slot always starts a run), 64
+// runs at a time: each lane forms one run's term, and the terms are added in
+// run order, a run of k as k additions in a row (add_k), so the sum is the
+// reference's one-by-one sum bit for bit.  Crowded cells next to a body cost
+// a sweep per 64 candidates instead of one lane's serial walk.
+__device__ __forceinline__ void coupling_wave(float4 xb, int l, int wx, int wxy, const int* __restrict__ clsmap,
+                                              const int* __restrict__ cstart, const int* __restrict__ rpi,
+                                              const int* __restrict__ rps, const float4* __restrict__ FS, float h,
+                                              float h2, float cs, float rho0, float mass, float& fx, float& fy,
+                                              float& fz) {
+  const int lane = (int)(threadIdx.x & 63);
+  auto range = [&](int bgn, int e) {
+    if (bgn >= e) return;
+    const int r0 = rpi[bgn], r1 = rpi[e - 1] + 1;
+    for (int r = r0; r < r1; r += 64) {
+      const int ri = r + lane;
+      float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+      int k = 0;
+      bool has = false;
+      if (ri < r1) {
+        const int j = rps[ri];
+        k = ((ri + 1 < r1) ? rps[ri + 1] : e) - j;
+        has = coupling_term(xb, FS[j], h, h2, cs, rho0, mass, tx, ty, tz);
+      }
+      unsigned long long m = __ballot(has);
+      while (m) {
+        const int u = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        const int ku = __builtin_amdgcn_readlane(k, u);
+        add_k(fx, lane_f(tx, u), ku);
+        add_k(fy, lane_f(ty, u), ku);
+        add_k(fz, lane_f(tz, u), ku);
+      }
+    }
+  };
+#pragma unroll 1
+  for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int base = l + dy * wx + dz * wxy;
+      const int c0 = clsmap[base - 1];
+      const int c1 = clsmap[base];
+      const int c2 = clsmap[base + 1];
+      if (c1 == c0 + 1 && c2 == c0 + 2) {
+        range(cstart[c0], cstart[c0 + 3]);
+      } else {
+        range(cstart[c0], cstart[c0 + 1]);
+        range(cstart[c1], cstart[c1 + 1]);
+        range(cstart[c2], cstart[c2 + 1]);
+      }
+    }
+  }
+}
+
 // FluidWorld::apply_coupling_forces, per boundary particle
-// (fluid_world.cpp:140-160): F_b = sum_f ((m p_f / rho_f^2) rho0 psi_b) gradW.
-__global__ void k_coupling_particles(GridWin W, float h, float h2, float cs, float rho0, float mass,
-                                     const float4* __restrict__ BP, const int* __restrict__ body, int nb,
-                                     const uint8_t* __restrict__ is_static, int nbodies,
-                                     const int* __restrict__ fclsmap, const int* __restrict__ fcstart,
-                                     const float4* __restrict__ FS, float4* __restrict__ FB) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+// (fluid_world.cpp:140-160): F_b = sum_f ((m p_f / rho_f^2) rho0 psi_b) gradW,
+// one wave per boundary particle (coupling_wave).
+__global__ void __launch_bounds__(256)
+k_coupling_particles(GridWin W, float h, float h2, float cs, float rho0, float mass, const float4* __restrict__ BP,
+                     const int* __restrict__ body, int nb, const uint8_t* __restrict__ is_static, int nbodies,
+                     const int* __restrict__ fclsmap, const int* __restrict__ fcstart,
+                     const float4* __restrict__ FS, const int* __restrict__ rpi, const int* __restrict__ rps,
+                     float4* __restrict__ FB) {
+  const int b = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (b >= nb) return;  // uniform per wave
   const int bi = body[b];
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (bi >= 0 && bi < nbodies && !is_static[bi]) {
     const float4 xb = BP[b];
-    const float psi_b = xb.w;
     const int l = aux_cell(W, xb.x, xb.y, xb.z);
-    if (l >= 0) {
-      for_each_candidate(l, W.wx, W.wx * W.wy, fclsmap, fcstart, 0, [&](int j) {
-        const float4 xf = FS[j];
-        const float rx = xb.x - xf.x, ry = xb.y - xf.y, rz = xb.z - xf.z;
-        const float r_sq = sqn3(rx, ry, rz);
-        if (r_sq >= h2) return;
-        const float rho_f = (xf.w < 1.0f) ? 1.0f : xf.w;
-        const float pr = rho_f - rho0;
-        const float pressure = (0.0f < pr) ? pr : 0.0f;
-        const float pc = mass * pressure / (rho_f * rho_f);
-        bool in;
-        const float co = spiky_coeff(r_sq, h, cs, &in);
-        if (!in) return;
-        const float k = pc * rho0 * psi_b;
-        fx += k * (co * rx);
-        fy += k * (co * ry);
-        fz += k * (co * rz);
-      });
-    }
+    if (l >= 0)
+      coupling_wave(xb, l, W.wx, W.wx * W.wy, fclsmap, fcstart, rpi, rps, FS, h, h2, cs, rho0, mass, fx, fy, fz);
   }
-  FB[b] = make_float4(fx, fy, fz, 0.0f);
+  if ((threadIdx.x & 63) == 0) FB[b] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // Per-body sums in boundary-particle order (World::apply_force / apply_torque,
@@ -2451,21 +2562,26 @@ __device__ __forceinline__ int scene_cell(const SceneC& S, float x, float y, flo
 }
 
 // Bins points of known scene (pscene[i], -1: none) into global classes.
-__global__ void k_bin_scene_points(const SceneC* __restrict__ scenes, const float4* __restrict__ P,
-                                   const int* __restrict__ pscene, int n, const int* __restrict__ clsmap,
-                                   int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of) {
+__global__ void __launch_bounds__(NPX_BIN_BLOCK)
+k_bin_scene_points(const SceneC* __restrict__ scenes, const float4* __restrict__ P, const int* __restrict__ pscene,
+                   int n, const int* __restrict__ clsmap, int* __restrict__ ccount, int* __restrict__ cls_of,
+                   int* __restrict__ off_of) {
+  __shared__ BinTable T;
+  bin_clear(T);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int s = pscene[i];
   int c = -1;
-  if (s >= 0) {
-    const SceneC& S = scenes[s];
-    const float4 p = P[i];
-    const int l = scene_cell(S, p.x, p.y, p.z);
-    if (l >= 0) c = clsmap[S.cls_base + l];
+  if (i < n) {
+    const int s = pscene[i];
+    if (s >= 0) {
+      const SceneC& S = scenes[s];
+      const float4 p = P[i];
+      const int l = scene_cell(S, p.x, p.y, p.z);
+      if (l >= 0) c = clsmap[S.cls_base + l];
+    }
+    cls_of[i] = c;
   }
-  cls_of[i] = c;
-  if (c >= 0) off_of[i] = atomicAdd(&ccount[c], 1);
+  const int off = bin_offset(T, c, c >= 0, ccount);
+  if (c >= 0) off_of[i] = off;
 }
 
 // Scene of each fluid slot when its scene has a boundary, else -1.
@@ -2509,43 +2625,26 @@ k_boundary_density_b(const SceneC* __restrict__ scenes, const int* __restrict__ 
 
 // FluidWorld::apply_coupling_forces per boundary particle
 // (fluid_world.cpp:140-160), every scene: F_b = sum_f ((m p_f / rho_f^2)
-// rho0 psi_b) gradW over the committed-fluid buckets of b's scene.
-__global__ void k_coupling_b(const SceneC* __restrict__ scenes, const float4* __restrict__ BP,
-                             const int* __restrict__ bscene, const int* __restrict__ body, int nb,
-                             const uint8_t* __restrict__ is_static, int nbodies, const int* __restrict__ clsmap,
-                             const int* __restrict__ fcstart, const float4* __restrict__ FS,
-                             float4* __restrict__ FB) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+// rho0 psi_b) gradW over the committed-fluid buckets of b's scene, one wave
+// per boundary particle (coupling_wave).
+__global__ void __launch_bounds__(256)
+k_coupling_b(const SceneC* __restrict__ scenes, const float4* __restrict__ BP, const int* __restrict__ bscene,
+             const int* __restrict__ body, int nb, const uint8_t* __restrict__ is_static, int nbodies,
+             const int* __restrict__ clsmap, const int* __restrict__ fcstart, const float4* __restrict__ FS,
+             const int* __restrict__ rpi, const int* __restrict__ rps, float4* __restrict__ FB) {
+  const int b = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (b >= nb) return;  // uniform per wave
   const int bi = body[b];
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (bi >= 0 && bi < nbodies && !is_static[bi]) {
     const SceneC& S = scenes[bscene[b]];
     const float4 xb = BP[b];
-    const float psi_b = xb.w;
-    const float h = S.h, h2 = S.h2, cs = S.cs, rho0 = S.rho0, mass = S.mass;
     const int l = scene_cell(S, xb.x, xb.y, xb.z);
-    if (l >= 0) {
-      for_each_candidate(l, S.wx, S.wx * S.wy, clsmap + S.cls_base, fcstart, 0, [&](int j) {
-        const float4 xf = FS[j];
-        const float rx = xb.x - xf.x, ry = xb.y - xf.y, rz = xb.z - xf.z;
-        const float r_sq = sqn3(rx, ry, rz);
-        if (r_sq >= h2) return;
-        const float rho_f = (xf.w < 1.0f) ? 1.0f : xf.w;
-        const float pr = rho_f - rho0;
-        const float pressure = (0.0f < pr) ? pr : 0.0f;
-        const float pc = mass * pressure / (rho_f * rho_f);
-        bool in;
-        const float co = spiky_coeff(r_sq, h, cs, &in);
-        if (!in) return;
-        const float k = pc * rho0 * psi_b;
-        fx += k * (co * rx);
-        fy += k * (co * ry);
-        fz += k * (co * rz);
-      });
-    }
+    if (l >= 0)
+      coupling_wave(xb, l, S.wx, S.wx * S.wy, clsmap + S.cls_base, fcstart, rpi, rps, FS, S.h, S.h2, S.cs, S.rho0,
+                    S.mass, fx, fy, fz);
   }
-  FB[b] = make_float4(fx, fy, fz, 0.0f);
+  if ((threadIdx.x & 63) == 0) FB[b] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // AoS float triples -> float4 (w = 0), for points copied to the device raw.

@@ -34,7 +34,7 @@ import collections
 import concurrent.futures
 import dataclasses
 import time
-from typing import Callable, Dict, List, Optional
+from typing import Callable, Dict, Iterator, List, Optional
 
 import numpy as np
 
@@ -63,13 +63,14 @@ class Request:
 class Server:
     def __init__(self, backend, hbm_budget_bytes: int, max_scenes: int,
                  bytes_fn: Callable[[object, int], int], clock: Callable[[], float] = time.perf_counter,
-                 health_every: int = 10):
+                 health_every: int = 10, keep_results: bool = True):
         self.b = backend
         self.budget = int(hbm_budget_bytes)
         self.max_scenes = int(max_scenes)
         self.bytes_fn = bytes_fn
         self.clock = clock
         self.health_every = max(1, int(health_every))
+        self.keep_results = bool(keep_results)  # False: drop each result (and scene) once read back
         self.pending: "collections.deque[Request]" = collections.deque()
         self.active: Dict[int, Request] = {}
         self.finished: Dict[int, Request] = {}
@@ -79,6 +80,7 @@ class Server:
         self.step_ms: List[float] = []
         self.particle_steps = 0
         self.busy_s = 0.0
+        self.admit_s = 0.0       # host time spent admitting requests
 
     # ---------------------------------------------------------- intake
     def submit(self, req: Request) -> None:
@@ -146,6 +148,9 @@ class Server:
         r.finished_at = self.clock()
         del self.active[r.rid]
         self.finished[r.rid] = r
+        if not self.keep_results:
+            r.result = None
+            r.scene = None
 
     def _admit(self, now: float) -> None:
         while self.pending and self.pending[0].arrival <= now:
@@ -181,7 +186,9 @@ class Server:
         """One scheduling round + one device step.  Returns False when idle
         (nothing active and nothing that has arrived)."""
         now = self.clock()
+        ta = time.perf_counter()
         self._admit(now)
+        self.admit_s += time.perf_counter() - ta
         if not self.active:
             return False
         t0 = time.perf_counter()
@@ -323,12 +330,16 @@ class FusedBackend:
 
     device_sampling: sample the boundary particles on the device
     (npx_sample_boundaries) instead of with the host samplers FluidWorld
-    uses; sphere samples can then differ from FluidWorld's in the last bit."""
+    uses; the samples are the same bit for bit, and the arrays skip the
+    host's per-particle objects."""
 
     def __init__(self, batch, max_scenes: int, device_sampling: bool = False):
         self.batch = BatchBackend(batch, max_scenes)
         self.coupled: Dict[int, tuple] = {}   # sid -> (World, nbodies, is_static, dt)
         self.device_sampling = device_sampling
+        # host seconds per part of a round: the batch step (synchronised), the
+        # coupling call (with its readback), the host rigid steps
+        self.phase_s = {"fluid": 0.0, "coupling": 0.0, "rigid": 0.0}
 
     def _attach(self, sid, sc, snap=None):
         import novaphy_amd as nv
@@ -398,10 +409,14 @@ class FusedBackend:
     def step(self) -> float:
         t0 = time.perf_counter()
         self.batch.step()
+        t1 = time.perf_counter()
+        self.phase_s["fluid"] += t1 - t0
         if self.coupled:
             keys = sorted(self.coupled)  # the body table order of npx_batch_coupling
             poses = np.concatenate([_poses7(self.coupled[k][0].state.transforms) for k in keys])
             out = self.batch.b.coupling(poses)
+            t2 = time.perf_counter()
+            self.phase_s["coupling"] += t2 - t1
             off = 0
             for k in keys:
                 world, nb, static, dt = self.coupled[k]
@@ -418,6 +433,7 @@ class FusedBackend:
                     world.apply_torque(i, t)
                 world.step(dt)
                 off += nb
+            self.phase_s["rigid"] += time.perf_counter() - t2
         return (time.perf_counter() - t0) * 1e3
 
     def status(self) -> Dict[object, int]:
@@ -520,14 +536,19 @@ class MixedBackend:
         self.worlds.clear()
 
 
-def poisson_requests(n: int, rate_hz: float, seed: int, make_scene: Callable[[int, np.random.Generator], object],
-                     horizon_range=(50, 500)) -> List[Request]:
-    """n requests with exponential inter-arrival times at rate_hz (seeded)."""
+def poisson_stream(n: int, rate_hz: float, seed: int, make_scene: Callable[[int, np.random.Generator], object],
+                   horizon_range=(50, 500)) -> Iterator[Request]:
+    """n requests with exponential inter-arrival times at rate_hz (seeded),
+    each scene built when its request is drawn."""
     rng = np.random.default_rng(seed)
     t = 0.0
-    out = []
     for k in range(n):
         t += rng.exponential(1.0 / rate_hz)
         h = int(rng.integers(horizon_range[0], horizon_range[1] + 1))
-        out.append(Request(rid=k, scene=make_scene(k, rng), horizon=h, arrival=t))
-    return out
+        yield Request(rid=k, scene=make_scene(k, rng), horizon=h, arrival=t)
+
+
+def poisson_requests(n: int, rate_hz: float, seed: int, make_scene: Callable[[int, np.random.Generator], object],
+                     horizon_range=(50, 500)) -> List[Request]:
+    """The whole stream of poisson_stream as a list."""
+    return list(poisson_stream(n, rate_hz, seed, make_scene, horizon_range))

@@ -86,6 +86,55 @@ def test_boundary_density_and_coupling_bitexact(gpu):
     c.close()
 
 
+def test_coupling_with_stacked_fluid_bitexact(gpu):
+    """Coupling forces when runs of identical fluid particles touch the bodies
+    (two stacks of 1200 and 400 particles on one point each, under the two
+    spheres): the coupling walks the fluid candidates run by run and adds a
+    run's term k times over, which must equal the oracle's one-by-one sums bit
+    for bit, in a single context and in a batch."""
+    sc, local, body, poses, static = _scene_with_bodies()
+    pos = sc.positions.copy()
+    pos[:1200] = np.array([0.35, 0.28, 0.40], np.float32)
+    pos[1200:1600] = np.array([0.62, 0.18, 0.55], np.float32)
+    vel = np.zeros_like(pos)
+    h = sc.params.kernel_radius
+    _say("case: coupling with stacked fluid n=", sc.n, "nb=", local.shape[0])
+    wpos = O.boundary_world_positions(local, body, poses)
+    psi = O.boundary_volumes(wpos, h)
+    op, ov, orho, ol = O.pbf_run(sc.params, pos, vel, sc.dt, sc.gravity, sc.mass, 1)
+    orho_b = O.boundary_density(op, orho, wpos, psi, h, sc.params.rest_density)
+    of, ot = O.coupling(op, orho_b, wpos, psi, body, poses, static, h, sc.params.rest_density, sc.mass)
+    assert np.abs(of[0]).max() > 0 and np.abs(of[1]).max() > 0
+    assert np.unique(op[:1200], axis=0).shape[0] == 1  # the stack stays one point
+
+    c = capi.Context(sc.params, capacity=sc.n)
+    c.set_particles(pos, vel)
+    c.set_boundary(local, body, psi, nbodies=poses.shape[0])
+    c.boundary_world_positions(poses)
+    c.step(sc.dt, sc.gravity, sc.mass, 1)
+    c.apply_boundary_density()
+    f, t = c.coupling_forces(poses, static)
+    gp, _, gr, _ = c.get_particles()
+    c.close()
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gr, orho_b)
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(t, ot)
+
+    slots = ((sc.n + 255) // 256) * 256
+    b = capi.Batch(slot_capacity=slots, cell_capacity=capi.scene_bytes(sc.params, 1), max_scenes=2)
+    sid = b.add(sc.params, sc.dt, sc.gravity, sc.mass, pos, vel)
+    b.set_boundary(sid, local, body, psi, poses.shape[0], static)
+    b.step(1)
+    out = b.coupling(poses)
+    bp, _, br, _ = b.get(sid)
+    b.close()
+    np.testing.assert_array_equal(bp, op)
+    np.testing.assert_array_equal(br, orho_b)
+    np.testing.assert_array_equal(out[:, :3], of)
+    np.testing.assert_array_equal(out[:, 3:], ot)
+
+
 def test_batched_scenes_equal_alone(gpu):
     """Each scene advanced inside a batch matches the same scene alone,
     bitwise (per-scene arithmetic is independent of the batch)."""
@@ -129,9 +178,9 @@ def test_batched_scenes_equal_alone(gpu):
 
 def test_device_boundary_sampling_matches_host(gpu):
     """npx_sample_boundaries against the host samplers (boundary.cpp:15-150):
-    the same particles in the same order and bodies; box and plane samples
-    bit for bit; sphere samples (acos / sin / cos in double, rounded) within
-    two float ulps of the radius of the host libm's float functions."""
+    the same particles in the same order and bodies, every sample bit for
+    bit (the sphere samples' trigonometry is the host libm's float
+    functions, as in the reference's sampler)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from test_capi_cpu import _shape_model
     m = _shape_model()
@@ -149,12 +198,10 @@ def test_device_boundary_sampling_matches_host(gpu):
         cnt = len(nv.sample_model_boundaries(_single(sh), 0.05, 0.7))
         kinds += [k] * cnt
     kinds = np.array(kinds)
-    exact = kinds != "Sphere"
-    np.testing.assert_array_equal(local[exact], hl[exact])
-    d = np.abs(local[~exact] - hl[~exact]).max()
-    _say("  sphere samples: max |device - host| %.3e m, %d of %d differ" % (
-        d, int(np.any(local[~exact] != hl[~exact], axis=1).sum()), int((~exact).sum())))
-    assert d <= 2 * np.spacing(np.float32(1.0)) * 1.0
+    sph = kinds == "Sphere"
+    assert sph.sum() > 100 and (~sph).sum() > 100
+    _say("  sphere samples differing: %d of %d" % (int(np.any(local[sph] != hl[sph], axis=1).sum()), int(sph.sum())))
+    np.testing.assert_array_equal(local, hl)
 
 
 def _single(shape):

@@ -124,14 +124,15 @@ def test_served_coupled_requests_equal_alone(gpu, kind):
 def test_device_sampled_requests_equal_alone(gpu):
     """Fused serving with the boundary particles sampled on the device
     (npx_sample_boundaries): each coupled request run in the mix ends bitwise
-    equal to the same request served alone."""
+    equal to the same request served alone, and to the same request served
+    with the host samplers FluidWorld uses."""
     reqs = _requests(1, 3, seed=24)
     _say("case: device-sampled coupled requests", [(r.scene.n, len(r.scene.spheres)) for r in reqs])
 
-    def serve_all(rs):
+    def serve_all(rs, device=True):
         clock = Clock()
         b = _batch_for(rs)
-        srv = serve.Server(serve.FusedBackend(b, 8, device_sampling=True), hbm_budget_bytes=1 << 40, max_scenes=8,
+        srv = serve.Server(serve.FusedBackend(b, 8, device_sampling=device), hbm_budget_bytes=1 << 40, max_scenes=8,
                            bytes_fn=capi.scene_bytes, clock=clock, health_every=1)
         for r in rs:
             srv.submit(r)
@@ -141,15 +142,17 @@ def test_device_sampled_requests_equal_alone(gpu):
         return out
 
     mix = serve_all(reqs)
+    host = serve_all([serve.Request(rid=r.rid, scene=r.scene, horizon=r.horizon, arrival=r.arrival) for r in reqs],
+                     device=False)
     for r in reqs:
         alone = serve_all([serve.Request(rid=r.rid, scene=r.scene, horizon=r.horizon, arrival=0.0)])[r.rid]
-        got = mix[r.rid]
-        assert got.status == serve.DONE and alone.status == serve.DONE
-        for a, g in zip(alone.result[:3], got.result[:3]):
-            np.testing.assert_array_equal(g, a)
-        if r.scene.spheres:
-            for a, g in zip(alone.result[4], got.result[4]):
+        for got in (mix[r.rid], host[r.rid]):
+            assert got.status == serve.DONE and alone.status == serve.DONE
+            for a, g in zip(alone.result[:3], got.result[:3]):
                 np.testing.assert_array_equal(g, a)
+            if r.scene.spheres:
+                for a, g in zip(alone.result[4], got.result[4]):
+                    np.testing.assert_array_equal(g, a)
 
 
 @pytest.mark.parametrize("kind", ["fused", "mixed"])

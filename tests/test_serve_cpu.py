@@ -119,6 +119,39 @@ def test_poisson_requests_seeded():
     assert all(b.arrival > a_.arrival for a_, b in zip(a, a[1:]))
 
 
+def test_poisson_stream_is_the_list_lazily():
+    """The lazy stream draws the same requests (arrivals, horizons, scene
+    sizes) as the list, and builds each scene only when it is drawn."""
+    made = []
+
+    def mk(k, rng):
+        made.append(k)
+        return _scene(int(rng.integers(10, 20)))
+    a = serve.poisson_requests(30, 10.0, seed=7, make_scene=mk)
+    made.clear()
+    it = serve.poisson_stream(30, 10.0, seed=7, make_scene=mk)
+    first = next(it)
+    assert made == [0]
+    b = [first] + list(it)
+    assert [(r.rid, r.arrival, r.horizon, r.scene.n) for r in a] == [(r.rid, r.arrival, r.horizon, r.scene.n)
+                                                                     for r in b]
+
+
+def test_server_drops_results_when_asked():
+    """keep_results=False: a finished request keeps its status, step count and
+    latencies but not its result or scene."""
+    clk = Clock()
+    srv = serve.Server(FakeBatch(), hbm_budget_bytes=1000, max_scenes=8, bytes_fn=lambda p, n: n, clock=clk,
+                       keep_results=False)
+    srv.submit(serve.Request(rid=0, scene=_scene(10), horizon=2))
+    srv.step()
+    srv.step()
+    r = srv.finished[0]
+    assert r.status == serve.DONE and r.steps_done == 2 and len(r.step_ms) == 2
+    assert r.result is None and r.scene is None
+    assert srv.latency_percentiles()["samples"] == 2
+
+
 def test_preempt_and_resume_keep_progress():
     b = FakeBatch()
     clk = Clock()
