@@ -301,7 +301,7 @@ def run_coupled(steps, warmup):
             "boundary_particles": world.num_boundary_particles, "phases_avg_ms": phases}
 
 
-def run_serve(seconds, rate_hz, threads, backend="fused", n_requests=0, kernel_window=(0, 0)):
+def run_serve(seconds, rate_hz, threads, backend="fused", n_requests=0, kernel_window=(0, 0), max_seconds=0.0):
     """Served rollouts: Poisson arrivals (seed 4) of dam breaks with N
     log-uniform in [4k, 256k], horizons uniform in [50, 500] and 0-8 rigid
     spheres, admitted against 80 % of 288 GB.  fused: every request shares one
@@ -310,7 +310,8 @@ def run_serve(seconds, rate_hz, threads, backend="fused", n_requests=0, kernel_w
     requests with spheres run as FluidWorld instances stepped concurrently.
     The sample is the requests arriving in the first `seconds` of server time
     (or the first `n_requests` of the stream); the run ends when they have
-    all finished or at 3x their arrival window.  kernel_window (a, b): per-kernel
+    all finished, at 3x their arrival window or after max_seconds (when
+    positive), whichever comes first.  kernel_window (a, b): per-kernel
     HIP-event times of the batch over server rounds a..b-1 (none when a >= b)."""
     from novaphy_amd import capi, scenes, serve
     n_req = int(n_requests) if n_requests > 0 else max(1, int(round(seconds * rate_hz)))
@@ -331,6 +332,8 @@ def run_serve(seconds, rate_hz, threads, backend="fused", n_requests=0, kernel_w
     active = []
     next_log = 30.0
     limit = 3.0 * max(n_req / rate_hz, 1.0)
+    if max_seconds > 0:
+        limit = min(limit, max_seconds)
     while (nxt is not None or srv.pending or srv.active) and time.perf_counter() - t0 < limit:
         now = srv.clock()
         while nxt is not None and nxt.arrival <= now:
@@ -440,6 +443,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-kernel-window", default="0,0",
                     help="serve workload: per-kernel times over server rounds a,b (a >= b: off)")
+    ap.add_argument("--serve-max-seconds", type=float, default=0.0,
+                    help="serve workload: stop serving after this many seconds (0: 3x the arrival window)")
     ap.add_argument("--serve-requests", type=int, default=0,
                     help="serve workload: the first N requests of the stream (0: those of --serve-seconds)")
     ap.add_argument("--serve-threads", type=int, default=8, help="host threads stepping coupled requests (mixed)")
@@ -490,7 +495,8 @@ def main():
         return
     if args.workload == "serve":
         r = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend,
-                      args.serve_requests, tuple(int(x) for x in args.serve_kernel_window.split(",")))
+                      args.serve_requests, tuple(int(x) for x in args.serve_kernel_window.split(",")),
+                      args.serve_max_seconds)
         ss = serve_summary(r, args.serve_rate, args.serve_backend)
         line = {**common, "value": r["value"], "steps": r["rounds"], "warmup": 0, "ms_per_step": r["p50_step_ms"],
                 "config": {"workload": ss.pop("workload")}, **ss}

@@ -172,7 +172,7 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
                       nbodies, cm, (const int*)(c->Acstart.p - H.cls_base), (const float4*)c->FS.p,
                       (const int*)c->CRPI.p, (const int*)c->CRPS.p, c->FB.p));
     NPX_TRY(c->launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)c->BODY_PTR.p,
-                      (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
+                      (const int*)c->BODY_PTR.p + 1, (const int*)c->BODY_IDX.p, nbodies, (const float4*)c->FB.p, (const float4*)c->BP.p,
                       (const float*)c->POSE.p, c->OUT6.p));
     NPX_TRY(c->read_bodies(out.data(), c->OUT6.p, nbodies));
     NPX_TRY(c->check_flags());
@@ -317,7 +317,7 @@ void destroy(Ctx* c) {
     ps->bigb.release();
   }
   for (auto* b : {&c->BBL, &c->BBP, &c->BBPS, &c->BFP, &c->BFS, &c->BFB}) b->release();
-  for (auto* b : {&c->BBODY, &c->BSCN, &c->BBPTR, &c->BBIDX, &c->FPSC}) b->release();
+  for (auto* b : {&c->BBODY, &c->BSCN, &c->BBIDX, &c->FPSC, &c->BBSTART, &c->BBEND, &c->BSBASE}) b->release();
   for (auto* b : {&c->BPSI, &c->BPOSE, &c->BOUT6}) b->release();
   c->BSTAT.release();
   c->d_scenes.release();
@@ -1050,10 +1050,7 @@ int npx_scene_remove(npx_ctx* ctx, int32_t scene_id) {
     for (int b = 0; b < H.seg_blocks; ++b) c->h_blk_scene[b0 + b] = -1;
     c->slot_alloc.release(b0, H.seg_blocks);
     if (H.window_valid) c->cls_alloc.release(H.cls_base, H.ncell);
-    if (H.nbnd > 0) {
-      c->sbnd[scene_id] = Ctx::SceneBnd();
-      c->bnd_dirty = true;
-    }
+    NPX_TRY(c->release_boundary(scene_id));
     H = SceneH();
     c->table_dirty = true;
     return c->upload_tables();
@@ -1083,6 +1080,7 @@ int npx_scene_set_boundary(npx_ctx* ctx, int32_t scene_id, const float* local3, 
     if (nbodies > 0 && !is_static) return c->fail(NPX_EINVAL, "is_static is NULL");
     NPX_HIP_C(c, hipStreamSynchronize(c->stream));
     if (c->sbnd.size() < (size_t)c->max_scenes) c->sbnd.resize(c->max_scenes);
+    NPX_TRY(c->release_boundary(scene_id));
     Ctx::SceneBnd& B = c->sbnd[scene_id];
     B.local3.assign(local3, local3 + 3 * (size_t)nb);
     B.body.assign(body, body + nb);
@@ -1090,7 +1088,8 @@ int npx_scene_set_boundary(npx_ctx* ctx, int32_t scene_id, const float* local3, 
     B.is_static.assign(is_static, is_static + nbodies);
     B.nbodies = nbodies;
     c->scenes[scene_id].nbnd = nb;
-    c->bnd_dirty = true;
+    c->body_dirty = true;
+    NPX_TRY(c->place_boundary(scene_id));
     c->table_dirty = true;
     return c->upload_tables();
   });
@@ -1100,7 +1099,7 @@ int32_t npx_batch_num_bodies(npx_ctx* ctx) {
   return guard(ctx, __func__, [&]() -> int {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
     if (!c) return -1;
-    if (c->bnd_dirty && c->rebuild_boundaries() != NPX_OK) return -1;
+    if (c->body_dirty && c->rebuild_bodies() != NPX_OK) return -1;
     return c->bb_bodies;
   });
 }

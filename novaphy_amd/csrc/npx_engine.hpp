@@ -294,18 +294,27 @@ struct Ctx {
   DBuf<uint8_t> ABIGB;
   DBuf<int> aux_clsmap;
 
-  // batched coupling: per-scene boundaries gathered into batch-wide arrays
-  // (scenes with a boundary by ascending id, each scene's bodies in its own
-  // order); rebuilt when a boundary is set or a scene removed
+  // batched coupling: each scene's boundary particles occupy a segment of a
+  // boundary pool, placed when the boundary is set and released with the
+  // scene, so admitting or retiring a scene uploads only its own particles.
+  // The body table (the bodies of the scenes with boundary particles, by
+  // ascending scene id, each scene's bodies in its own order) is rebuilt
+  // from the per-scene records when that set changes: O(scenes + bodies).
   struct SceneBnd {
     std::vector<float> local3, psi;
     std::vector<int> body;
     std::vector<uint8_t> is_static;
     int nbodies = 0;
+    int off = -1;                   // pool segment [off, off + body.size())
+    std::vector<int> bstart, bend;  // per body: its particles' range in the segment's index list
   };
   std::vector<SceneBnd> sbnd;
-  bool bnd_dirty = false;
-  int bb_n = 0, bb_bodies = 0;
+  RangeAlloc bnd_alloc;
+  int bnd_cap = 0;
+  bool body_dirty = false;
+  int bb_n = 0;       // pool positions in use or released (the kernels' range)
+  int bb_live = 0;    // placed boundary particles
+  int bb_bodies = 0;  // bodies in the body table
   struct PointSort {
     DBuf<int> cst, cls, off, tperm, tkey, perm, mperm, mkey;
     DBuf<uint8_t> bigb;
@@ -333,7 +342,8 @@ struct Ctx {
   }
   PointSort bsort, fsort;
   DBuf<float4> BBL, BBP, BBPS, BFP, BFS, BFB;
-  DBuf<int> BBODY, BSCN, BBPTR, BBIDX, FPSC;
+  DBuf<int> BBODY, BSCN, BBIDX, FPSC;     // pool: local body, scene (-1: free), per-body index lists
+  DBuf<int> BBSTART, BBEND, BSBASE;       // body table: index-list ranges; per scene its first body
   DBuf<float> BPSI, BPOSE, BOUT6;
   DBuf<uint8_t> BSTAT;
 
@@ -378,61 +388,138 @@ struct Ctx {
     return NPX_OK;
   }
 
-  // Gathers the scenes' boundaries into the batch-wide arrays.
-  int rebuild_boundaries() {
-    if (sbnd.size() < (size_t)max_scenes) sbnd.resize(max_scenes);
-    std::vector<float4> bl;
-    std::vector<int> bd, sc, cnt(1, 0);
-    std::vector<float> ps;
-    std::vector<uint8_t> st;
-    int boff = 0;
-    for (int s = 0; s < max_scenes; ++s) {
-      const SceneBnd& B = sbnd[s];
-      const int nb = (int)B.body.size();
-      if (!scenes[s].used || nb == 0) continue;
-      for (int b = 0; b < nb; ++b) {
-        bl.push_back(make_float4(B.local3[3 * b], B.local3[3 * b + 1], B.local3[3 * b + 2], 0.0f));
-        const int k = B.body[b];
-        bd.push_back((k >= 0 && k < B.nbodies) ? boff + k : -1);
-        sc.push_back(s);
-        ps.push_back(B.psi[b]);
-      }
-      for (int k = 0; k < B.nbodies; ++k) st.push_back(B.is_static[k]);
-      boff += B.nbodies;
+  // Uploads scene s's boundary into its pool segment: local positions, local
+  // body (-1: world-owned), scene, psi, and the segment's index list (pool
+  // positions of the particles of each body, ascending, body by body).
+  int upload_boundary_segment(int s) {
+    SceneBnd& B = sbnd[s];
+    const int nb = (int)B.body.size(), off = B.off;
+    std::vector<float4> bl(nb);
+    std::vector<int> lb(nb), sc(nb, s), idx;
+    idx.reserve(nb);
+    for (int b = 0; b < nb; ++b) {
+      bl[b] = make_float4(B.local3[3 * b], B.local3[3 * b + 1], B.local3[3 * b + 2], 0.0f);
+      const int k = B.body[b];
+      lb[b] = (k >= 0 && k < B.nbodies) ? k : -1;
     }
-    bb_n = (int)bl.size();
-    bb_bodies = boff;
-    // per-body lists of boundary particles, ascending
-    cnt.assign(bb_bodies + 1, 0);
-    for (int b = 0; b < bb_n; ++b)
-      if (bd[b] >= 0) cnt[bd[b] + 1]++;
-    for (int k = 0; k < bb_bodies; ++k) cnt[k + 1] += cnt[k];
-    std::vector<int> idx(std::max(bb_n, 1), 0), fillp(cnt.begin(), cnt.end());
-    for (int b = 0; b < bb_n; ++b)
-      if (bd[b] >= 0) idx[fillp[bd[b]]++] = b;
-    const size_t n = (size_t)std::max(bb_n, 1), nbod = (size_t)std::max(bb_bodies, 1);
-    NPX_HIP(BBL.ensure(n));
-    NPX_HIP(BBP.ensure(n));
-    NPX_HIP(BBPS.ensure(n));
-    NPX_HIP(BFB.ensure(n));
-    NPX_HIP(BBODY.ensure(n));
-    NPX_HIP(BSCN.ensure(n));
+    B.bstart.assign(B.nbodies, 0);
+    B.bend.assign(B.nbodies, 0);
+    std::vector<int> cnt(B.nbodies + 1, 0);
+    for (int b = 0; b < nb; ++b)
+      if (lb[b] >= 0) cnt[lb[b] + 1]++;
+    for (int k = 0; k < B.nbodies; ++k) cnt[k + 1] += cnt[k];
+    idx.assign(std::max(cnt[B.nbodies], 1), 0);
+    std::vector<int> fillp(cnt.begin(), cnt.end());
+    for (int b = 0; b < nb; ++b)
+      if (lb[b] >= 0) idx[fillp[lb[b]]++] = off + b;
+    for (int k = 0; k < B.nbodies; ++k) {
+      B.bstart[k] = cnt[k];
+      B.bend[k] = cnt[k + 1];
+    }
+    NPX_TRY(copy_sync(BBL.p + off, bl.data(), sizeof(float4) * nb, hipMemcpyHostToDevice));
+    NPX_TRY(copy_sync(BBODY.p + off, lb.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    NPX_TRY(copy_sync(BSCN.p + off, sc.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    NPX_TRY(copy_sync(BPSI.p + off, B.psi.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+    if (cnt[B.nbodies] > 0)
+      NPX_TRY(copy_sync(BBIDX.p + off, idx.data(), sizeof(int) * cnt[B.nbodies], hipMemcpyHostToDevice));
+    bb_n = std::max(bb_n, off + nb);
+    return NPX_OK;
+  }
+
+  // Pool arrays of capacity cap, every position free (scene -1).
+  int alloc_boundary_pool(int cap) {
+    const size_t n = (size_t)std::max(cap, 1);
+    for (auto* b : {&BBL, &BBP, &BBPS, &BFB}) {
+      b->release();
+      NPX_HIP(b->ensure(n));
+    }
+    for (auto* b : {&BBODY, &BSCN, &BBIDX}) {
+      b->release();
+      NPX_HIP(b->ensure(n));
+    }
+    BPSI.release();
     NPX_HIP(BPSI.ensure(n));
-    NPX_HIP(BBIDX.ensure(n));
-    NPX_HIP(BBPTR.ensure(nbod + 1));
+    NPX_TRY(launch("bnd_free", k_fill_i32, cdiv((int)n, 256), 256, BSCN.p, -1, (int)n));
+    bnd_cap = cap;
+    bnd_alloc.init(cap);
+    bb_n = 0;
+    return NPX_OK;
+  }
+
+  // Places scene s's boundary in a pool segment and uploads it; when no
+  // segment fits, the pool grows (at least doubling) and every placed scene
+  // is uploaded again.
+  int place_boundary(int s) {
+    SceneBnd& B = sbnd[s];
+    const int nb = (int)B.body.size();
+    B.off = -1;
+    if (nb == 0) return NPX_OK;
+    int off = bnd_alloc.alloc(nb);
+    if (off < 0) {
+      const long long want = std::max<long long>({2ll * bnd_cap, (long long)bb_live + nb, 1 << 16});
+      if (want > (1ll << 30)) return fail(NPX_ENOMEM, "boundary pool beyond 2^30 particles");
+      NPX_TRY(alloc_boundary_pool((int)want));
+      for (int t = 0; t < (int)sbnd.size(); ++t) {
+        if (t == s || sbnd[t].off < 0) continue;
+        sbnd[t].off = bnd_alloc.alloc((int)sbnd[t].body.size());
+        NPX_TRY(upload_boundary_segment(t));
+      }
+      off = bnd_alloc.alloc(nb);
+      if (off < 0) return fail(NPX_ENOMEM, "boundary pool allocation failed");
+    }
+    B.off = off;
+    bb_live += nb;
+    body_dirty = true;
+    return upload_boundary_segment(s);
+  }
+
+  // Frees scene s's pool segment (its positions read as scene -1).
+  int release_boundary(int s) {
+    if (s >= (int)sbnd.size()) return NPX_OK;
+    SceneBnd& B = sbnd[s];
+    const int nb = (int)B.body.size();
+    if (B.off >= 0 && nb > 0) {
+      NPX_TRY(launch("bnd_free", k_fill_i32, cdiv(nb, 256), 256, BSCN.p + B.off, -1, nb));
+      bnd_alloc.release(B.off, nb);
+      bb_live -= nb;
+      body_dirty = true;
+    }
+    B = SceneBnd();
+    return NPX_OK;
+  }
+
+  // The body table: per scene its first body (BSBASE), per body its index
+  // list range (BBSTART / BBEND) and static flag.
+  int rebuild_bodies() {
+    std::vector<int> base(max_scenes, 0), bs, be;
+    std::vector<uint8_t> st;
+    int total = 0;
+    for (int s = 0; s < max_scenes && s < (int)sbnd.size(); ++s) {
+      const SceneBnd& B = sbnd[s];
+      if (!scenes[s].used || B.off < 0) continue;
+      base[s] = total;
+      for (int k = 0; k < B.nbodies; ++k) {
+        bs.push_back(B.off + B.bstart[k]);
+        be.push_back(B.off + B.bend[k]);
+        st.push_back(B.is_static[k]);
+      }
+      total += B.nbodies;
+    }
+    bb_bodies = total;
+    const size_t nbod = (size_t)std::max(total, 1);
+    NPX_HIP(BSBASE.ensure((size_t)max_scenes));
+    NPX_HIP(BBSTART.ensure(nbod));
+    NPX_HIP(BBEND.ensure(nbod));
     NPX_HIP(BPOSE.ensure(nbod * 7));
     NPX_HIP(BOUT6.ensure(nbod * 6));
     NPX_HIP(BSTAT.ensure(nbod));
-    if (bb_n > 0) {
-      NPX_TRY(copy_sync(BBL.p, bl.data(), sizeof(float4) * bb_n, hipMemcpyHostToDevice));
-      NPX_TRY(copy_sync(BBODY.p, bd.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
-      NPX_TRY(copy_sync(BSCN.p, sc.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
-      NPX_TRY(copy_sync(BPSI.p, ps.data(), sizeof(float) * bb_n, hipMemcpyHostToDevice));
-      NPX_TRY(copy_sync(BBIDX.p, idx.data(), sizeof(int) * bb_n, hipMemcpyHostToDevice));
+    NPX_TRY(copy_sync(BSBASE.p, base.data(), sizeof(int) * max_scenes, hipMemcpyHostToDevice));
+    if (total > 0) {
+      NPX_TRY(copy_sync(BBSTART.p, bs.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BBEND.p, be.data(), sizeof(int) * total, hipMemcpyHostToDevice));
+      NPX_TRY(copy_sync(BSTAT.p, st.data(), total, hipMemcpyHostToDevice));
     }
-    NPX_TRY(copy_sync(BBPTR.p, cnt.data(), sizeof(int) * (bb_bodies + 1), hipMemcpyHostToDevice));
-    if (bb_bodies > 0) NPX_TRY(copy_sync(BSTAT.p, st.data(), bb_bodies, hipMemcpyHostToDevice));
-    bnd_dirty = false;
+    body_dirty = false;
     return NPX_OK;
   }
 
@@ -440,15 +527,16 @@ struct Ctx {
   // with a boundary (FluidWorld::step after the solve, fluid_world.cpp:58-82),
   // out6: per body force and torque.
   int batch_coupling(const float* pose7, int nbodies, float* out6) {
-    if (bnd_dirty) NPX_TRY(rebuild_boundaries());
+    if (body_dirty) NPX_TRY(rebuild_bodies());
     if (nbodies != bb_bodies) return fail(NPX_EINVAL, "expected %d body poses, got %d", bb_bodies, nbodies);
-    if (bb_n == 0) return NPX_OK;
+    if (bb_live == 0) return NPX_OK;
     if (nbodies > 0 && (!pose7 || !out6)) return fail(NPX_EINVAL, "pose7 / out6 is NULL");
     for (const auto& H : scenes)
       if (H.used && H.nbnd > 0 && !H.window_valid) return fail(NPX_ESTATE, "no step has run yet");
     NPX_TRY(upload_bodies(BPOSE.p, pose7, nullptr, nullptr, nbodies));
-    NPX_TRY(launch("bpos", k_bpos, cdiv(bb_n, 256), 256, (const float4*)BBL.p, (const int*)BBODY.p,
-                   (const float*)BPSI.p, bb_n, (const float*)BPOSE.p, nbodies, BBP.p));
+    NPX_TRY(launch("bpos", k_bpos_b, cdiv(bb_n, 256), 256, (const float4*)BBL.p, (const int*)BBODY.p,
+                   (const int*)BSCN.p, (const int*)BSBASE.p, (const float*)BPSI.p, bb_n, (const float*)BPOSE.p,
+                   nbodies, BBP.p));
     NPX_TRY(sort_scene_points(bsort, BBP.p, BSCN.p, bb_n, nullptr, BBPS.p));
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
@@ -464,11 +552,13 @@ struct Ctx {
     NPX_TRY(payload_runs(hw_blocks * NPX_BLOCK, fsort.cst.p, total_cls(), fsort.off.p, BFS.p));
     // a wave per boundary particle
     NPX_TRY(launch("coupling", k_coupling_b, cdiv(bb_n, 4), 256, sc, (const float4*)BBP.p, (const int*)BSCN.p,
-                   (const int*)BBODY.p, bb_n, (const uint8_t*)BSTAT.p, nbodies, (const int*)clsmap.p,
-                   (const int*)fsort.cst.p, (const float4*)BFS.p, (const int*)CRPI.p, (const int*)CRPS.p, BFB.p));
+                   (const int*)BBODY.p, (const int*)BSBASE.p, bb_n, (const uint8_t*)BSTAT.p, nbodies,
+                   (const int*)clsmap.p, (const int*)fsort.cst.p, (const float4*)BFS.p, (const int*)CRPI.p,
+                   (const int*)CRPS.p, BFB.p));
     if (nbodies > 0) {
-      NPX_TRY(launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)BBPTR.p, (const int*)BBIDX.p,
-                     nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p, BOUT6.p));
+      NPX_TRY(launch("body_reduce", k_body_reduce, nbodies, 64, (const int*)BBSTART.p, (const int*)BBEND.p,
+                     (const int*)BBIDX.p, nbodies, (const float4*)BFB.p, (const float4*)BBP.p, (const float*)BPOSE.p,
+                     BOUT6.p));
       NPX_TRY(read_bodies(out6, BOUT6.p, nbodies));
     }
     return NPX_OK;  // a scene whose particles left its window is reported by npx_scene_status

@@ -2312,6 +2312,36 @@ __global__ void k_bpos(const float4* __restrict__ BL, const int* __restrict__ bo
   BP[b] = make_float4(ox, oy, oz, psi[b]);
 }
 
+// k_bpos over the batch's boundary pool: bscene[b] < 0 marks a free
+// position (its output is never read); a particle's body is its scene's first
+// body (sbase) + its local body.
+__global__ void k_bpos_b(const float4* __restrict__ BL, const int* __restrict__ lbody, const int* __restrict__ bscene,
+                         const int* __restrict__ sbase, const float* __restrict__ psi, int nb,
+                         const float* __restrict__ pose7, int nbodies, float4* __restrict__ BP) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int s = bscene[b];
+  if (s < 0) return;
+  const float4 p = BL[b];
+  const int lb = lbody[b];
+  const int bi = (lb >= 0) ? sbase[s] + lb : -1;
+  float ox = p.x, oy = p.y, oz = p.z;
+  if (bi >= 0 && bi < nbodies) {
+    const float* P = pose7 + 7 * bi;
+    const float qw = P[3], qx = P[4], qy = P[5], qz = P[6];
+    float ux = qy * p.z - qz * p.y, uy = qz * p.x - qx * p.z, uz = qx * p.y - qy * p.x;
+    ux = ux + ux;
+    uy = uy + uy;
+    uz = uz + uz;
+    const float cx = qy * uz - qz * uy, cy = qz * ux - qx * uz, cz = qx * uy - qy * ux;
+    const float rx = (p.x + qw * ux) + cx, ry = (p.y + qw * uy) + cy, rz = (p.z + qw * uz) + cz;
+    ox = P[0] + rx;
+    oy = P[1] + ry;
+    oz = P[2] + rz;
+  }
+  BP[b] = make_float4(ox, oy, oz, psi[b]);
+}
+
 // FluidWorld::apply_boundary_density (fluid_world.cpp:101-123) on the
 // committed positions: rho_i += sum_b (rho0 psi_b) W(x_i - x_b).
 __global__ void __launch_bounds__(NPX_BLOCK)
@@ -2494,9 +2524,10 @@ k_coupling_particles(GridWin W, float h, float h2, float cs, float rho0, float m
 // at a time, each lane forming its particle's force and torque terms, and the
 // terms are added in ascending particle order (broadcast lane by lane), so
 // the float accumulation order is the reference's and the result is
-// deterministic.
+// deterministic.  Body k's particles are body_idx[bstart[k] .. bend[k]).
 __global__ void __launch_bounds__(64)
-k_body_reduce(const int* __restrict__ body_ptr, const int* __restrict__ body_idx, int nbodies,
+k_body_reduce(const int* __restrict__ bstart, const int* __restrict__ bend, const int* __restrict__ body_idx,
+              int nbodies,
               const float4* __restrict__ FB, const float4* __restrict__ BP, const float* __restrict__ pose7,
               float* __restrict__ out6) {
   const int k = (int)blockIdx.x;
@@ -2504,8 +2535,8 @@ k_body_reduce(const int* __restrict__ body_ptr, const int* __restrict__ body_idx
   const int t = (int)threadIdx.x;
   const float cxm = pose7[7 * k], cym = pose7[7 * k + 1], czm = pose7[7 * k + 2];
   float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Tx = 0.0f, Ty = 0.0f, Tz = 0.0f;
-  const int end = body_ptr[k + 1];
-  for (int u0 = body_ptr[k]; u0 < end; u0 += 64) {
+  const int end = bend[k];
+  for (int u0 = bstart[k]; u0 < end; u0 += 64) {
     const int u = u0 + t;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f, tx = 0.0f, ty = 0.0f, tz = 0.0f;
     bool use = false;
@@ -2629,15 +2660,19 @@ k_boundary_density_b(const SceneC* __restrict__ scenes, const int* __restrict__ 
 // per boundary particle (coupling_wave).
 __global__ void __launch_bounds__(256)
 k_coupling_b(const SceneC* __restrict__ scenes, const float4* __restrict__ BP, const int* __restrict__ bscene,
-             const int* __restrict__ body, int nb, const uint8_t* __restrict__ is_static, int nbodies,
+             const int* __restrict__ lbody, const int* __restrict__ sbase, int nb,
+             const uint8_t* __restrict__ is_static, int nbodies,
              const int* __restrict__ clsmap, const int* __restrict__ fcstart, const float4* __restrict__ FS,
              const int* __restrict__ rpi, const int* __restrict__ rps, float4* __restrict__ FB) {
   const int b = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (b >= nb) return;  // uniform per wave
-  const int bi = body[b];
+  const int s = bscene[b];
+  if (s < 0) return;  // a free pool position
+  const int lb = lbody[b];
+  const int bi = (lb >= 0) ? sbase[s] + lb : -1;
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   if (bi >= 0 && bi < nbodies && !is_static[bi]) {
-    const SceneC& S = scenes[bscene[b]];
+    const SceneC& S = scenes[s];
     const float4 xb = BP[b];
     const int l = scene_cell(S, xb.x, xb.y, xb.z);
     if (l >= 0)

@@ -338,8 +338,9 @@ class FusedBackend:
         self.coupled: Dict[int, tuple] = {}   # sid -> (World, nbodies, is_static, dt)
         self.device_sampling = device_sampling
         # host seconds per part of a round: the batch step (synchronised), the
-        # coupling call (with its readback), the host rigid steps
-        self.phase_s = {"fluid": 0.0, "coupling": 0.0, "rigid": 0.0}
+        # gathering of the body poses, the coupling call (with its readback),
+        # the host rigid steps
+        self.phase_s = {"fluid": 0.0, "poses": 0.0, "coupling": 0.0, "rigid": 0.0}
 
     def _attach(self, sid, sc, snap=None):
         import novaphy_amd as nv
@@ -414,9 +415,11 @@ class FusedBackend:
         if self.coupled:
             keys = sorted(self.coupled)  # the body table order of npx_batch_coupling
             poses = np.concatenate([_poses7(self.coupled[k][0].state.transforms) for k in keys])
+            tp = time.perf_counter()
+            self.phase_s["poses"] += tp - t1
             out = self.batch.b.coupling(poses)
             t2 = time.perf_counter()
-            self.phase_s["coupling"] += t2 - t1
+            self.phase_s["coupling"] += t2 - tp
             off = 0
             for k in keys:
                 world, nb, static, dt = self.coupled[k]
