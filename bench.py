@@ -17,9 +17,12 @@ A step is one PBFSolver::step over all particles (predict, grid, 4 x
 timed region is the first K steps (default 500, the scene's specified run)
 from the initial state, after W untimed warm-up steps on the same context;
 inputs are resident on the device, the region is bracketed by device
-synchronisation, and per-kernel durations come from HIP events recorded on
-the context's stream over the same region (window_ms_per_step gives the
-step time across it).  The default run also reports, as extra keys, the
+synchronisation, and each step is one hipGraph replay.  Per-kernel
+durations come from HIP events recorded on the context's stream around
+every launch of a second run of the same K steps from the same initial
+state (events need stream launches; window_ms_per_step gives the step time
+across that run, and its final state is checked to be bitwise the timed
+run's).  The default run also reports, as extra keys, the
 batched workload (1024 scenes; the batch and the plain per-scene loop timed
 on the same step window) and a served sample (the first requests of the
 seeded Poisson stream).
@@ -125,8 +128,13 @@ def window_ms(trace, steps, window):
 def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None, kernel_timing=True):
     """The timed region is the first `steps` steps of the workload from its
     initial state (after `warmup` untimed steps on the same context, which is
-    then reset to the initial state).  Per-kernel HIP events are recorded on
-    the context's stream over the whole region."""
+    then reset to the initial state), each step one hipGraph replay as in
+    normal use.  The per-kernel durations of the roofline come from a second
+    run of the same steps from the same initial state, with a HIP event pair
+    recorded on the context's stream around every launch: events need stream
+    launches, which cost about a tenth more than graph replay, so that run is
+    not the timed one.  The steps are deterministic; the final states of the
+    two runs are compared bit for bit (`kernel_run_identical`)."""
     from novaphy_amd import capi
     n = sc.n
     ctx = capi.Context(sc.params, capacity=n, flags=flags)
@@ -136,23 +144,33 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None, kernel_timin
     ctx.synchronize()
     ctx.set_particles(sc.positions, sc.velocities)  # back to the initial state
     start_state = (sc.positions, sc.velocities)
-    ctx.set_kernel_timing(kernel_timing)
     ctx.synchronize()
     log(f"[bench] timed region: steps 1-{steps} from the initial state")
     t0 = time.perf_counter()
     ctx.step(sc.dt, sc.gravity, sc.mass, steps)
     ctx.synchronize()
     t1 = time.perf_counter()
+    wall = t1 - t0
+    value = n * steps / wall
     if not kernel_timing:  # counter-collection runs: no events around the launches
         ctx.close()
-        return {"value": n * steps / (t1 - t0), "ms_per_step": (t1 - t0) * 1e3 / steps,
+        return {"value": value, "ms_per_step": wall * 1e3 / steps,
                 "window_ms_per_step": None, "roofline": None, "step_roofline": None, "kernels": None}
+    final = ctx.get_particles()
+    log(f"[bench] kernel-timed run: steps 1-{steps} again from the initial state, HIP events around every launch")
+    ctx.set_particles(sc.positions, sc.velocities)
+    ctx.set_kernel_timing(True)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    ctx.step(sc.dt, sc.gravity, sc.mass, steps)
+    ctx.synchronize()
+    t3 = time.perf_counter()
+    final2 = ctx.get_particles()
+    identical = all(np.array_equal(a, b) for a, b in zip(final, final2))
     kt_raw = ctx.kernel_times()
     trace = ctx.kernel_trace()
     kt = fold_dense(kt_raw)
     ctx.set_kernel_timing(False)
-    wall = t1 - t0
-    value = n * steps / wall
     iters = sc.params.solver_iterations
     vort = sc.params.vorticity_epsilon != 0
     PASS_BYTES = pass_bytes(vort)
@@ -166,6 +184,8 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None, kernel_timin
         "value": value,
         "ms_per_step": wall * 1e3 / steps,
         "window_ms_per_step": window_ms(trace, steps, max(1, steps // 10)),
+        "kernel_run_ms_per_step": round((t3 - t2) * 1e3 / steps, 4),
+        "kernel_run_identical": identical,
         "roofline": {"bound": "hbm", "kernel": main,
                      "kernels": [k for k in (main, main + "_dense") if k in kt_raw],
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
@@ -540,13 +560,18 @@ def main():
             "ms_per_step": r["ms_per_step"],
             "config": {"workload": describe(sc), "particles": sc.n, "s_corr": name},
             "window_ms_per_step": r["window_ms_per_step"],
+            "kernel_run_ms_per_step": r.get("kernel_run_ms_per_step"),
+            "kernel_run_identical": r.get("kernel_run_identical"),
             "roofline": r["roofline"], "step_roofline": r["step_roofline"], "kernels": r["kernels"]}
     if "cpu_baseline" in r:
         line["cpu_baseline"] = r["cpu_baseline"]
         line["speedup_vs_cpu"] = round(r["value"] / r["cpu_baseline"]["value"], 1)
     for name2, sc2, r2 in results[1:]:
         line[f"{name2}_s_corr"] = {"workload": describe(sc2), "value": r2["value"], "ms_per_step": r2["ms_per_step"],
-                                   "window_ms_per_step": r2["window_ms_per_step"], "roofline": r2["roofline"],
+                                   "window_ms_per_step": r2["window_ms_per_step"],
+                                   "kernel_run_ms_per_step": r2.get("kernel_run_ms_per_step"),
+                                   "kernel_run_identical": r2.get("kernel_run_identical"),
+                                   "roofline": r2["roofline"],
                                    "step_roofline": r2["step_roofline"], "kernels": r2["kernels"]}
     if args.workload == "1m" and args.extras:
         rb = run_batch(args.scenes, args.batch_steps, args.warmup, flags, args.loop, args.batch_s_corr)

@@ -12,6 +12,7 @@
 
 #define NPX_BLOCK 256      // scene segment alignment and block size of the sort kernels
 #define NPX_MAX_BLOCK 256  // largest block size of the neighbour passes (tunable, <= NPX_BLOCK)
+#define NPX_SLOT_PAD 64     // elements of padding after the x* buffers (reads past a slot range are masked)
 #define NPX_WINDOW_MARGIN 4
 
 // Constants of one scene, computed on the host with the reference's float

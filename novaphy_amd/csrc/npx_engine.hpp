@@ -740,8 +740,8 @@ struct Ctx {
     NPX_HIP(X2.ensure(n));
     NPX_HIP(V.ensure(n));
     NPX_HIP(V2.ensure(n));
-    NPX_HIP(XS0.ensure(n));
-    NPX_HIP(XS1.ensure(n));
+    NPX_HIP(XS0.ensure(n + NPX_SLOT_PAD));  // test_slots reads up to three slots past a range
+    NPX_HIP(XS1.ensure(n + NPX_SLOT_PAD));
     NPX_HIP(VT.ensure(n));
     NPX_HIP(OM.ensure(n));
     NPX_HIP(VX.ensure(n));

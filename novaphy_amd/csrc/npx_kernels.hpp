@@ -778,16 +778,20 @@ __device__ __forceinline__ bool sweep_rows(Rows& w, const Skin& K, bool rs, floa
     unsigned long long mk = 0, ek = 0;
     if (ok) {
       for (int j = b; j < e; j += 4) {
-        const int j1 = min(j + 1, e - 1), j2 = min(j + 2, e - 1), j3 = min(j + 3, e - 1);
-        const float4 a0 = XS[j], a1 = XS[j1], a2 = XS[j2], a3 = XS[j3];
-        const float d0 = dist2<FAST>(xi.x - a0.x, xi.y - a0.y, xi.z - a0.z);
-        const float d1 = (j + 1 < e) ? dist2<FAST>(xi.x - a1.x, xi.y - a1.y, xi.z - a1.z) : INFINITY;
-        const float d2 = (j + 2 < e) ? dist2<FAST>(xi.x - a2.x, xi.y - a2.y, xi.z - a2.z) : INFINITY;
-        const float d3 = (j + 3 < e) ? dist2<FAST>(xi.x - a3.x, xi.y - a3.y, xi.z - a3.z) : INFINITY;
-        const unsigned sb = (unsigned)(d0 < skin_cut) | ((unsigned)(d1 < skin_cut) << 1) |
-                            ((unsigned)(d2 < skin_cut) << 2) | ((unsigned)(d3 < skin_cut) << 3);
-        const unsigned eb = (unsigned)(d0 < cut) | ((unsigned)(d1 < cut) << 1) | ((unsigned)(d2 < cut) << 2) |
-                            ((unsigned)(d3 < cut) << 3);
+        // four slots from one base address, up to three past e - 1 (the x*
+        // buffers carry NPX_SLOT_PAD elements of padding), masked
+        const float4* __restrict__ p = XS + j;
+        float4 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = p[u];
+        unsigned sb = 0, eb = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float d = dist2<FAST>(xi.x - a[u].x, xi.y - a[u].y, xi.z - a[u].z);
+          const bool v = j + u < e;
+          sb |= (unsigned)(v & (d < skin_cut)) << u;
+          eb |= (unsigned)(v & (d < cut)) << u;
+        }
         mk |= (unsigned long long)sb << (j - b);  // j - b <= 60
         ek |= (unsigned long long)eb << (j - b);
       }
@@ -1056,20 +1060,32 @@ __device__ __forceinline__ void visit_all(const Skin& K, bool rs, int l, int wx,
 // dense cells then evaluates each lane's in-range candidates side by side
 // instead of stepping every lane through every candidate.
 
-// Mask of the slots c .. c + m - 1 (1 <= m <= 64) with r^2 < cut at xi.
+// Mask of the slots c .. c + m - 1 (1 <= m <= 64) with r^2 < cut at xi, or
+// a superset of it: every caller re-tests r^2 < cut exactly before a slot
+// adds anything (the pair bodies return on r^2 >= cut, or on the spiky
+// support bound sp_hi <= cut for the vorticity force; the in-range recording
+// re-tests), so extra slots only cost a visit.  The exact test evaluates
+// x*x + (y*y + z*z) with three roundings; here r^2 is one FMA chain (also
+// within 3 ulp of the true value), compared against cut enlarged by 2^-20,
+// which admits every slot the exact test admits.  Slots are read four at a
+// time from one base address, up to three past c + m - 1 (the x* buffers
+// carry NPX_SLOT_PAD elements of padding), and masked.
 template <bool FAST>
 __device__ __forceinline__ unsigned long long test_slots(int c, int m, float4 xi, float cut,
                                                          const float4* __restrict__ XS) {
   unsigned long long rel = 0;
+  const float ct = FAST ? cut : cut * (1.0f + 0x1p-20f);
   for (int t = 0; t < m; t += 4) {
+    const float4* __restrict__ p = XS + c + t;
     float4 a[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = XS[c + min(t + u, m - 1)];
+    for (int u = 0; u < 4; ++u) a[u] = p[u];
     unsigned bits = 0;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const float d = dist2<FAST>(xi.x - a[u].x, xi.y - a[u].y, xi.z - a[u].z);
-      bits |= (unsigned)((d < cut) & (t + u < m)) << u;
+      const float rx = xi.x - a[u].x, ry = xi.y - a[u].y, rz = xi.z - a[u].z;
+      const float d = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+      bits |= (unsigned)((d < ct) & (t + u < m)) << u;
     }
     rel |= (unsigned long long)bits << t;  // t <= 60
   }
@@ -1673,7 +1689,9 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
     }
     // member slot qt has rank r in the batch: its sums are in lanes r*NT..
     const bool mine = (bm >> t) & 1ull;
-    const int r = (int)__builtin_popcountll(bm & ((1ull << t) - 1ull));
+    // (the count of bm's bits below lane t, without a 64-bit lane mask that
+    // would stay live across the sweep)
+    const int r = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(bm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bm, 0u));
     const int src = mine ? r * NT : 0;
     float a[NT];
 #pragma unroll
