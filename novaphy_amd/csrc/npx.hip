@@ -305,6 +305,7 @@ void destroy(Ctx* c) {
   c->BOFF.release();
   c->DL.release();
   c->DN.release();
+  c->LST.release();
   for (auto* ps : {&c->bsort, &c->fsort}) {
     ps->cst.release();
     ps->cls.release();
@@ -530,6 +531,7 @@ int npx_set_particles(npx_ctx* ctx, const float* pos3, const float* vel3, int32_
     NPX_TRY(c->scene_window(H));
     NPX_TRY(c->upload_particles(H, pos3, vel3));
     NPX_TRY(c->upload_tables());
+    NPX_TRY(c->reset_list_stats(0));
     return NPX_OK;
   });
 }
@@ -1033,6 +1035,7 @@ int npx_scene_add(npx_ctx* ctx, const npx_pbf_params* params, float dt, const fl
     c->table_dirty = true;
     NPX_TRY(c->upload_particles(c->scenes[sid], pos3, vel3));
     NPX_TRY(c->upload_tables());
+    NPX_TRY(c->reset_list_stats(sid));
     if (scene_id) *scene_id = sid;
     return NPX_OK;
   });

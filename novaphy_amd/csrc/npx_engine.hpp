@@ -261,6 +261,7 @@ struct Ctx {
   DBuf<int> PI, PS;      // per slot: its run; per run: its first slot
   DBuf<int> BOFF;        // per 256-slot block: run starts, then their exclusive prefix
   DBuf<int> DL, DN;      // dense list (heads whose skin list overflowed) and its length
+  DBuf<int> LST;         // per-scene skin-list statistics and build flags (Skin::lst)
   // class tables
   DBuf<int> clsmap, cstart, bsum;
   DBuf<unsigned long long> htk;
@@ -658,6 +659,12 @@ struct Ctx {
     return NPX_OK;
   }
 
+  // A scene that (re)starts has no list statistics: its first step builds lists.
+  int reset_list_stats(int sid) {
+    static const int zero[NPX_LST_STRIDE] = {};
+    return copy_sync(LST.p + NPX_LST_STRIDE * (size_t)sid, zero, sizeof(zero), hipMemcpyHostToDevice);
+  }
+
   int stage_bodies(int nbodies) {
     if (nbodies <= h_body_cap) return NPX_OK;
     NPX_HIP(hipStreamSynchronize(stream));
@@ -720,6 +727,8 @@ struct Ctx {
     scenes.assign(max_scenes, SceneH());
     NPX_HIP(d_scenes.ensure(max_scenes));
     NPX_HIP(d_flags.ensure(1 + max_scenes));
+    NPX_HIP(LST.ensure(NPX_LST_STRIDE * (size_t)max_scenes));
+    NPX_HIP(hipMemsetAsync(LST.p, 0, sizeof(int) * NPX_LST_STRIDE * (size_t)max_scenes, stream));  // first step: build lists
     NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
     NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));
@@ -1101,7 +1110,7 @@ struct Ctx {
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
-                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1};
+                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1, LST.p};
     // The build pass's dense list: one 64-lane block per batch, at most this
     // many at a time.  The later passes sweep the list in their first nd
     // launch blocks (a multiple of 8, for the XCD order of the rest), one
@@ -1110,7 +1119,7 @@ struct Ctx {
     const int wpb = nbr_block / 64;
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);
-    if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 64, DN.p));
+    if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes));
     const BlockPlan rm{xcd_remap, 0};
     const BlockPlan rd{xcd_remap, nd};
     const int* cm = clsmap.p;
@@ -1186,7 +1195,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p};
   }
 
   void drop_graphs() {

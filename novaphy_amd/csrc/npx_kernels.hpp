@@ -638,9 +638,36 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
   return x * (g >> 3) + min(x, g & 7) + (i >> 3);
 }
 
-// Empties the step's dense list, which the build pass fills after it.
-__global__ void k_dense_reset(int* __restrict__ dn) {
+// Per-scene list statistics (Skin::lst): the testing passes count the lanes
+// that test and those whose list bound (cd + C <= skin_allow) held, whether or
+// not a list was built; the next step builds skin lists only when at least a
+// quarter held (or nothing was counted yet).  Where the candidates move
+// farther than the skin per iteration, almost every particle sweeps inline
+// anyway, and building the lists would only cost their writes.  Both ways
+// give the same floats: a particle without a list sweeps inline.
+// The counts are a sample (every eighth wave of a testing pass) spread over
+// NPX_LST_COPIES counter pairs, 64 bytes apart, so the atomics of a pass do
+// not queue on one cache line.  Per scene: NPX_LST_COPIES x 16 ints of counter
+// pairs (tested, held), then the build flag.
+#define NPX_LST_COPIES 32
+#define NPX_LST_BUILD (16 * NPX_LST_COPIES)
+#define NPX_LST_STRIDE (NPX_LST_BUILD + 16)
+
+// Empties the step's dense list, which the build pass fills after it, and
+// turns the previous step's list statistics into this step's build flags.
+__global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int nscenes) {
   if (threadIdx.x == 0) *dn = 0;
+  for (int s = (int)threadIdx.x; s < nscenes; s += (int)blockDim.x) {
+    int* st = lst + (size_t)NPX_LST_STRIDE * s;
+    long long tested = 0, held = 0;
+    for (int k = 0; k < NPX_LST_COPIES; ++k) {
+      tested += st[16 * k];
+      held += st[16 * k + 1];
+      st[16 * k] = 0;
+      st[16 * k + 1] = 0;
+    }
+    st[NPX_LST_BUILD] = (tested == 0 || 4 * held >= tested) ? 1 : 0;
+  }
 }
 
 // Neighbour-pass prologue.  `active` lanes own a particle; the others stay
@@ -701,6 +728,8 @@ struct Skin {
   int* __restrict__ dl;                    // [slots] dense list: leaders of batches of crowded heads
   int* __restrict__ dn;                    // [1] dense list length
   int dense;                               // 1: dense particles go to the per-wave kernels
+  int* __restrict__ lst;                   // [scenes][NPX_LST_STRIDE]: sampled counts of testing lanes and of
+                                           // lanes whose list bound held, whether this step builds lists
 };
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
@@ -1200,9 +1229,19 @@ template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool rs, int q, float C, float4 xi, float cut,
                                            const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
-                                           bool record, L&& ld, F&& f) {
+                                           bool record, int* __restrict__ st, L&& ld, F&& f) {
   const int n = K.cnt[q];
-  if (n >= 0 && K.cd[q] + C <= S.skin_allow) {
+  const bool held = K.cd[q] + C <= S.skin_allow;
+  const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (!FAST && K.dense && (wv & 7) == 0) {  // list statistics for the next step, sampled
+    const unsigned long long tb = __ballot(1), hb = __ballot(held);
+    if ((int)(threadIdx.x & 63) == (int)__builtin_ctzll(tb)) {
+      int* c = st + 16 * ((wv >> 3) & (NPX_LST_COPIES - 1));
+      atomicAdd(c, (int)__builtin_popcountll(tb));
+      atomicAdd(c + 1, (int)__builtin_popcountll(hb));
+    }
+  }
+  if (n >= 0 && held) {
     const unsigned long long rel = test_list<FAST>(K, q, n, xi, cut, XS);
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
   } else if (record) {
@@ -1768,11 +1807,13 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     Rows w;
     int n = -1;
     unsigned long long rel = 0;
-    if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
-      n = store_skin(w, K, q, &rel);
-    else
-      n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
-    if (n > NPX_SKIN_CAP) n = -1;
+    if (FAST || K.lst[NPX_LST_STRIDE * s + NPX_LST_BUILD]) {  // uniform per scene: this step builds skin lists
+      if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
+        n = store_skin(w, K, q, &rel);
+      else
+        n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+      if (n > NPX_SKIN_CAP) n = -1;
+    }
     if (n < 0 && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
       K.cnt[q] = NPX_CNT_DENSE;
       K.ecnt[q] = -1;
@@ -1787,7 +1828,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, ld, body);
+    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, K.lst + NPX_LST_STRIDE * s, ld, body);
   }
   density_finish(S, q, a, XS, RHO, LAM);
 }
@@ -1923,7 +1964,8 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     if (S.iters > 0) {  // the skin lists exist once a density pass has run
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
       // the vorticity force pass reuses the in-range list
-      visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, ld, body);
+      visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
+                                 body);
     } else {
       K.ecnt[q] = -1;
       visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
