@@ -1190,30 +1190,98 @@ __device__ __forceinline__ float4 cand_pos(const D& d) {
     return d.x;
 }
 
-// visit_all for a testing pass whose particle has no valid skin list: the
-// in-range candidates it meets (dist2 < cut at xi, a run of k counting k
-// entries) are recorded, in order, as the particle's in-range list while
-// they fit in NPX_SKIN_CAP entries, so the reusing pass that follows walks
-// that list instead of sweeping again.  Returns the list length, or -1 when
-// it did not fit.
+// visit_all for a particle without a valid skin list, in two phases.  The
+// test phase sweeps the 27 cells in the reference's order and appends every
+// slot that passes the distance test (the superset test of test_slots; a run
+// of k identical candidates appends its head k times) to the particle's
+// in-range list; the visit phase then calls f over the list, G entries at a
+// time, exactly as the reusing passes walk it.  f sees the same candidates in
+// the same order as visit_all (k single calls where visit_all makes one call
+// for a run of k, which adds the same floats: add_k is k additions in a row),
+// minus calls it returns from at once.  The wave then evaluates its lanes'
+// pair terms side by side in one loop, instead of one loop per row in which
+// every lane waits for the lane with the most in-range candidates of that
+// row.  The list is left for the reusing pass that follows.  Returns its
+// length, or -1 when more than NPX_SKIN_CAP candidates pass: the test phase
+// stops there and nothing is visited (the caller sweeps with visit_all_c).
 template <bool FAST, int G, bool RUNS, class L, class F>
-__device__ __forceinline__ int visit_all_rec(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
+__device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
                                              int wxy, const int* __restrict__ cm, const int* __restrict__ cstart,
                                              int shift, const float4* __restrict__ XS, L&& ld, F&& f) {
+  int* __restrict__ out = K.ex + sk_at(q, 0);  // entry k at out[64 k]
+  const float ct = FAST ? cut : cut * (1.0f + 0x1p-20f);
   int e = 0;
-  auto rec = [&](int j, const auto& d, int k) {
-    const float4 xj = cand_pos(d);
-    if (e >= 0 && dist2<FAST>(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z) < cut) {
-      if (e + k <= NPX_SKIN_CAP) {
-        for (int t = 0; t < k; ++t) K.ex[sk_at(q, e + t)] = j;
-        e += k;
-      } else {
-        e = -1;
+  auto in = [&](const float4& a) {
+    const float rx = xi.x - a.x, ry = xi.y - a.y, rz = xi.z - a.z;
+    return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)) < ct;
+  };
+  // slots b .. en-1, four from one base address (up to three past en - 1:
+  // the x* buffers carry NPX_SLOT_PAD elements of padding), masked
+  auto range = [&](int b, int en) {
+    for (int t = b; t < en && e <= NPX_SKIN_CAP; t += 4) {
+      const float4* __restrict__ p = XS + t;
+      float4 a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = p[u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if ((t + u < en) && in(a[u])) {
+          if (e < NPX_SKIN_CAP) out[64 * e] = t + u;
+          ++e;
+        }
       }
     }
-    f(j, d, k);
   };
-  visit_all_c<FAST, G, RUNS>(K, rs, l, wx, wxy, cm, cstart, shift, xi, cut, XS, ld, rec);
+  // runs of identical state in slots b .. en-1, two run heads per fetch
+  auto runs = [&](int b, int en) {
+    const int iend = K.pi[en - 1] + 1;
+    for (int i = K.pi[b]; i < iend && e <= NPX_SKIN_CAP; i += 2) {
+      const int j0 = K.ps[i];
+      const int j1 = (i + 1 < iend) ? K.ps[i + 1] : en;
+      const int j2 = (i + 2 < iend) ? K.ps[i + 2] : en;
+      const float4 a0 = XS[j0], a1 = XS[(i + 1 < iend) ? j1 : j0];
+      const bool i0 = in(a0), i1 = (i + 1 < iend) && in(a1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = h ? j1 : j0, k = (h ? j2 : j1) - j;
+        if (h ? i1 : i0) {
+          if (e + k <= NPX_SKIN_CAP)
+            for (int u = 0; u < k; ++u) out[64 * (e + u)] = j;
+          e += k;
+        }
+      }
+    }
+  };
+#pragma unroll 1
+  for (int r = 0; r < NPX_ROWS && e <= NPX_SKIN_CAP; ++r) {
+    const int base = l + (r % 3 - 1) * wx + (r / 3 - 1) * wxy;
+    const int c0 = cm[base - 1];
+    const int c1 = cm[base];
+    const int c2 = cm[base + 1];
+    if (c1 == c0 + 1 && c2 == c0 + 2) {
+      const int b = cstart[c0] - shift, en = cstart[c0 + 3] - shift;
+      if (b < en) {
+        if (RUNS && rs && cls_runs(K, c0, 3))
+          runs(b, en);
+        else
+          range(b, en);
+      }
+    } else {
+#pragma unroll 1
+      for (int k = 0; k < 3; ++k) {
+        const int c = (k == 0) ? c0 : (k == 1) ? c1 : c2;
+        const int b = cstart[c] - shift, en = cstart[c + 1] - shift;
+        if (b < en) {
+          if (RUNS && rs && cls_runs(K, c, 1))
+            runs(b, en);
+          else
+            range(b, en);
+        }
+      }
+    }
+  }
+  if (e > NPX_SKIN_CAP) return -1;
+  visit_list<G>(K.ex, q, e, ld, f);
   return e;
 }
 
@@ -1244,11 +1312,10 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool 
   if (n >= 0 && held) {
     const unsigned long long rel = test_list<FAST>(K, q, n, xi, cut, XS);
     K.ecnt[q] = visit_rel<G>(K, q, rel, ld, f);
-  } else if (record) {
-    K.ecnt[q] = visit_all_rec<FAST, G, RUNS>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, f);
   } else {
-    K.ecnt[q] = -1;
-    visit_all_c<FAST, G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, f);
+    const int e = visit_all_two<FAST, G, RUNS>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, f);
+    if (e < 0) visit_all_c<FAST, G, RUNS>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, f);
+    K.ecnt[q] = record ? e : -1;
   }
 }
 
@@ -1541,7 +1608,7 @@ __device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi,
 // Particles whose skin list overflows and whose 27 cells hold more than
 // this many candidates go on the dense list; sparser overflows keep the
 // per-lane inline sweep.
-#define NPX_DENSE_MIN 1024
+#define NPX_DENSE_MIN 256
 
 // Skin-list count of a listed dense particle (other overflows count -1).
 #define NPX_CNT_DENSE (-2)
@@ -1814,7 +1881,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
         n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
       if (n > NPX_SKIN_CAP) n = -1;
     }
-    if (n < 0 && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
+    if (!FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
       K.cnt[q] = NPX_CNT_DENSE;
       K.ecnt[q] = -1;
       dense_push(K, q, l);
@@ -1824,7 +1891,9 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     if (n >= 0) {
       K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
     } else {
-      K.ecnt[q] = visit_all_rec<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
+      const int e = visit_all_two<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
+      if (e < 0) visit_all_c<FAST, 4, !FAST>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, body);
+      K.ecnt[q] = e;
     }
   } else {
     const float C = candidate_disp(K, l, S.wx, wxy, cm);
