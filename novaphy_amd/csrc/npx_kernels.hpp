@@ -1204,6 +1204,9 @@ __device__ __forceinline__ float4 cand_pos(const D& d) {
 // row.  The list is left for the reusing pass that follows.  Returns its
 // length, or -1 when more than NPX_SKIN_CAP candidates pass: the test phase
 // stops there and nothing is visited (the caller sweeps with visit_all_c).
+#ifndef NPX_TEST_W
+#define NPX_TEST_W 8  // candidates tested per round of the two-phase sweep (loads in flight per lane)
+#endif
 template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
                                              int wxy, const int* __restrict__ cm, const int* __restrict__ cstart,
@@ -1215,16 +1218,16 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
     const float rx = xi.x - a.x, ry = xi.y - a.y, rz = xi.z - a.z;
     return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)) < ct;
   };
-  // slots b .. en-1, four from one base address (up to three past en - 1:
-  // the x* buffers carry NPX_SLOT_PAD elements of padding), masked
+  // slots b .. en-1, NPX_TEST_W from one base address (up to NPX_TEST_W - 1
+  // past en - 1: the x* buffers carry NPX_SLOT_PAD elements of padding), masked
   auto range = [&](int b, int en) {
-    for (int t = b; t < en && e <= NPX_SKIN_CAP; t += 4) {
+    for (int t = b; t < en && e <= NPX_SKIN_CAP; t += NPX_TEST_W) {
       const float4* __restrict__ p = XS + t;
-      float4 a[4];
+      float4 a[NPX_TEST_W];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] = p[u];
+      for (int u = 0; u < NPX_TEST_W; ++u) a[u] = p[u];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < NPX_TEST_W; ++u) {
         if ((t + u < en) && in(a[u])) {
           if (e < NPX_SKIN_CAP) out[64 * e] = t + u;
           ++e;
