@@ -24,8 +24,10 @@ state (events need stream launches; window_ms_per_step gives the step time
 across that run, and its final state is checked to be bitwise the timed
 run's).  The default run also reports, as extra keys, the
 batched workload (1024 scenes; the batch and the plain per-scene loop timed
-on the same step window) and a served sample (the first requests of the
-seeded Poisson stream).
+on the same step window) and two served samples from the seeded Poisson
+stream: the requests of the first 15 s at 4/s (light load: p50/p99 step
+latency; the rate follows the arrivals) and 300 requests at 30/s, which
+arrive faster than they are served (the rate is the server's capacity).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 1m|8k|batch|coupled|serve]
                   [--s-corr both|configured|default] [--extras 0|1] [--no-graph] [--copy-calibration]
@@ -461,6 +463,10 @@ def main():
                     help="1m workload: also measure the batched and served workloads (extra keys)")
     ap.add_argument("--serve-seconds", type=float, default=15.0, help="arrival window of the served sample")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-saturated-rate", type=float, default=30.0,
+                    help="arrival rate of the default run's saturated served sample (requests/s)")
+    ap.add_argument("--serve-saturated-requests", type=int, default=300,
+                    help="requests of the default run's saturated served sample")
     ap.add_argument("--serve-kernel-window", default="0,0",
                     help="serve workload: per-kernel times over server rounds a,b (a >= b: off)")
     ap.add_argument("--serve-max-seconds", type=float, default=0.0,
@@ -578,6 +584,11 @@ def main():
         line["batched"] = batch_summary(rb, args.batch_s_corr)
         rs = run_serve(args.serve_seconds, args.serve_rate, args.serve_threads, args.serve_backend)
         line["served"] = serve_summary(rs, args.serve_rate, args.serve_backend)
+        # the light-load sample above is arrival-bound (its rate is the offered load); this one
+        # arrives faster than the GPU serves, so its rate is the server's capacity
+        rs2 = run_serve(0, args.serve_saturated_rate, args.serve_threads, args.serve_backend,
+                        n_requests=args.serve_saturated_requests)
+        line["served_saturated"] = serve_summary(rs2, args.serve_saturated_rate, args.serve_backend)
     print(json.dumps(line), flush=True)
 
 

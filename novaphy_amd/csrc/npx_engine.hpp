@@ -1115,7 +1115,7 @@ struct Ctx {
     // many at a time.  The later passes sweep the list in their first nd
     // launch blocks (a multiple of 8, for the XCD order of the rest), one
     // wave per batch, beside their per-lane blocks.
-    const int dense_blocks = std::max(1, std::min(8192, hw_blocks * (NPX_BLOCK / 64)));
+    const int dense_blocks = std::max(1, std::min(NPX_DENSE_WAVES, hw_blocks * (NPX_BLOCK / 64)));
     const int wpb = nbr_block / 64;
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);

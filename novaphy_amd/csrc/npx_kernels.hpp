@@ -1623,7 +1623,7 @@ __device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi,
 // Waves of a neighbour pass that sweep the dense list beside its per-lane
 // blocks (the build pass's separate dense launch uses up to 8192).
 #ifndef NPX_DENSE_WAVES
-#define NPX_DENSE_WAVES 8192
+#define NPX_DENSE_WAVES 32768
 #endif
 // LDS row stride of a dense sweep: rows start 4 banks apart.
 #define NPX_DENSE_RS 68
