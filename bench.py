@@ -64,16 +64,21 @@ def pass_bytes(vort: bool):
 
 
 def fold_dense(kt):
-    """Kernel times with the build pass's dense-particle kernel (name +
-    "_dense": the crowded particles whose skin list overflowed, one wave per
-    batch) added to the build's per-lane kernel: together they are one pass
-    over every particle.  The later passes sweep their crowded particles in
-    the first blocks of their own launch.  Launch counts stay those of the
-    per-lane kernel."""
-    out = {k: list(v) for k, v in kt.items() if not (k.endswith("_dense") and k[:-6] in kt)}
+    """Kernel times with the build pass's other launches added to its
+    list-free per-lane kernel ("density_lambda_build"): the dense-particle
+    kernel (name + "_dense": the crowded particles, one wave per batch) and
+    the list-building variant (name + "_lists", the scenes that build skin
+    lists this step; each block of either per-lane launch leaves at once when
+    its scene is the other's).  Together they are one pass over every
+    particle.  The later passes sweep their crowded particles in the first
+    blocks of their own launch.  Launch counts stay those of the per-lane
+    kernel."""
+    sub = ("_dense", "_lists")
+    out = {k: list(v) for k, v in kt.items() if not any(k.endswith(x) and k[:-len(x)] in kt for x in sub)}
     for k, v in kt.items():
-        if k.endswith("_dense") and k[:-6] in kt:
-            out[k[:-6]][0] += v[0]
+        for x in sub:
+            if k.endswith(x) and k[:-len(x)] in kt:
+                out[k[:-len(x)]][0] += v[0]
     return {k: tuple(v) for k, v in out.items()}
 
 
@@ -189,7 +194,7 @@ def run_single(sc, steps, warmup, flags, cpu_seconds, variant=None, kernel_timin
         "kernel_run_ms_per_step": round((t3 - t2) * 1e3 / steps, 4),
         "kernel_run_identical": identical,
         "roofline": {"bound": "hbm", "kernel": main,
-                     "kernels": [k for k in (main, main + "_dense") if k in kt_raw],
+                     "kernels": [k for k in (main, main + "_lists", main + "_dense") if k in kt_raw],
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 5),
                      "traffic": profiled_traffic(variant, main) if full else None,

@@ -485,6 +485,7 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"clear_cmax", "fluid.pbf.neighbor_query"},
             {"dense_reset", "fluid.pbf.neighbor_query"},
             {"density_lambda_build", "fluid.pbf.iteration.compute_density"},
+            {"density_lambda_build_lists", "fluid.pbf.iteration.compute_density"},
             {"density_lambda", "fluid.pbf.iteration.compute_density"},
             {"density_lambda_build_dense", "fluid.pbf.iteration.compute_density"},
             {"density_lambda_dense", "fluid.pbf.iteration.compute_density"},
@@ -497,10 +498,12 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"run_fill", "fluid.pbf.commit_positions"},
         };
         // algorithmic bytes per particle of each pass (FP32/int32, 12 B per
-        // 3-vector), for the GB/s counters of the trace's GPU track
+        // 3-vector), for the GB/s counters of the trace's GPU track.  The
+        // build pass is two launches of which one does the work (with or
+        // without skin lists), so it has no per-launch rate.
         const bool vort = solver_.settings().vorticity_epsilon != 0.0f;
         const std::map<std::string, double> bytes = {
-            {"predict_bin", 40}, {"gather", 56}, {"density_lambda_build", 20}, {"density_lambda", 20},
+            {"predict_bin", 40}, {"gather", 56}, {"density_lambda", 20},
             {"delta", 28}, {"omega_xsph", vort ? 36.0 : 24.0}, {"vort_commit", vort ? 64.0 : 28.0}};
         for (const auto& k : solver_.kernel_trace()) {
           auto b = bytes.find(k.name);

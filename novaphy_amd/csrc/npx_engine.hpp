@@ -1131,8 +1131,11 @@ struct Ctx {
       auto dens = fast ? (it == 0 ? k_density_lambda<true, true> : k_density_lambda<true, false>)
                        : (it == 0 ? k_density_lambda<false, true> : k_density_lambda<false, false>);
       if (it == 0 || fast) {
-        NPX_TRY(launch(it == 0 ? "density_lambda_build" : "density_lambda", dens, blocks, nbr_block, sc, bs, cm, cst,
-                       cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+        NPX_TRY(launch(it == 0 ? (fast ? "density_lambda_build" : "density_lambda_build_lists") : "density_lambda",
+                       dens, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+        if (it == 0 && !fast)  // the scenes that build no skin lists this step
+          NPX_TRY(launch("density_lambda_build", k_density_lambda<false, true, false>, blocks, nbr_block, sc, bs, cm,
+                         cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
       } else {
         NPX_TRY(launch_shm("density_lambda", dens, blocks + nd, nbr_block, lds * npx_dense_lds_floats<5>(), sc, bs,
                            cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rd));

@@ -1853,8 +1853,13 @@ __device__ __forceinline__ void dense_commit(int, int, const SceneC* __restrict_
 // when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
 // iteration) builds the step's skin lists, and lists the particles whose
 // lists overflow for the dense kernels; later iterations test the lists.
-template <bool FAST, bool BUILD>
-__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(BUILD ? 4 : 6)))
+// The build pass (BUILD) is two launches: with LISTS it serves the scenes
+// whose flag says this step builds skin lists, without LISTS the others
+// (every block belongs to one scene and leaves at once when its scene is the
+// other launch's).  The list-free variant carries no list-building code and
+// runs at the testing passes' occupancy.
+template <bool FAST, bool BUILD, bool LISTS = true>
+__global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu((BUILD && LISTS) ? 4 : 6)))
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                  float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K, BlockPlan bp) {
@@ -1864,9 +1869,17 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
     return;
   }
   NPX_NBR_PROLOGUE();
+  // uniform per block: the scene's flag picks the build launch (FAST always builds lists)
+  if (BUILD && !FAST && (K.lst[NPX_LST_STRIDE * s + NPX_LST_BUILD] != 0) != LISTS) return;
   const bool rs = K.runs_s[s] != 0;
   if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
   if (!BUILD && !FAST && is_dense(S, K, q)) return;                  // the dense kernel's particle
+  if (BUILD && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
+    K.cnt[q] = NPX_CNT_DENSE;  // crowded: the per-wave dense sweep takes it, no list needed
+    K.ecnt[q] = -1;
+    dense_push(K, q, l);
+    return;
+  }
   const float4 xi = XS[q];
   const DensityPair<FAST> P(S, q, xi);
   const float cut = P.cut;
@@ -1874,21 +1887,15 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
   auto ld = [&](int j) { return XS[j]; };
   auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };
   if (BUILD) {
-    Rows w;
     int n = -1;
     unsigned long long rel = 0;
-    if (FAST || K.lst[NPX_LST_STRIDE * s + NPX_LST_BUILD]) {  // uniform per scene: this step builds skin lists
+    if (LISTS) {  // this step builds skin lists
+      Rows w;
       if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
         n = store_skin(w, K, q, &rel);
       else
         n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
       if (n > NPX_SKIN_CAP) n = -1;
-    }
-    if (!FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
-      K.cnt[q] = NPX_CNT_DENSE;
-      K.ecnt[q] = -1;
-      dense_push(K, q, l);
-      return;
     }
     K.cnt[q] = n;
     if (n >= 0) {

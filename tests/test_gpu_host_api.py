@@ -129,7 +129,9 @@ def test_fluid_world_reference_behaviour(gpu):
 
 def test_fluid_world_first_step_matches_oracle(gpu):
     """One FluidWorld step = PBF + boundary density + coupling on the GPU; the
-    fluid state and the forces handed to the rigid step match the oracle."""
+    fluid positions and the boundary-corrected densities match the oracle.
+    (The forces and torques handed to the rigid step are compared in
+    test_gpu_configs.py::test_fluid_world_first_step_hands_oracle_forces_to_rigid_step.)"""
     world = _coupled_world(extent=0.5, iters=2)
     x0 = world.fluid_state.positions_numpy()
     bp = world.boundary_particles
