@@ -260,8 +260,8 @@ def run_batch(count, steps, warmup, flags, loop, s_corr="configured"):
     b = make_batch()
     best, tried = autotune.autotune(b, steps_per_candidate=1)
     b.close()
-    log(f"[bench] autotuner: block {best[0]}, xcd order {best[1]}, skin {best[2]}h "
-        f"({', '.join(f'{k[0]}/{k[1]}/{k[2]}: {v:.1f} ms' for k, v in tried.items())})")
+    log(f"[bench] autotuner: block {best[0]}, xcd order {best[1]}, skin {best[2]}h, persistent {best[3]} "
+        f"({', '.join(f'{k[0]}/{k[1]}/{k[2]}/{k[3]}: {v:.1f} ms' for k, v in tried.items())})")
     b = make_batch()
     b.set_tuning(*best)
     b.step(warmup)
@@ -274,7 +274,7 @@ def run_batch(count, steps, warmup, flags, loop, s_corr="configured"):
     out = {"value": value, "ms_per_step": wall * 1e3 / steps, "particles": total, "scenes": count,
            "window": f"steps {warmup + 1}-{warmup + steps}",
            "p50_step_ms": float(np.percentile(ms, 50)), "p99_step_ms": float(np.percentile(ms, 99)),
-           "tuning": {"block": best[0], "xcd_order": best[1], "skin": best[2]}}
+           "tuning": {"block": best[0], "xcd_order": best[1], "skin": best[2], "persistent": best[3]}}
     if not loop:
         out.update(loop_value=None, loop_ms_per_step=None, speedup_vs_loop=None)
         return out

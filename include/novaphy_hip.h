@@ -245,6 +245,13 @@ int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
  * only speed changes.  No reference counterpart (used by the autotuner). */
 int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float skin_fraction);
 
+/* Persistent neighbour passes: each pass launches at most `blocks` per-lane
+ * workgroups (a multiple of 8, at least 8), each looping over particle blocks
+ * in the same XCD order; 0 (the default) launches one workgroup per particle
+ * block.  Results are bitwise independent of it; only speed changes.  No
+ * reference counterpart (a launch tunable for the autotuner). */
+int npx_set_persistent(npx_ctx* ctx, int32_t blocks);
+
 /* npx_batch_step with a device timestamp after every step: step_ms[k] is the
  * duration of step k between consecutive HIP events on the context's stream
  * (may be NULL).  Blocks until the steps are done. */

@@ -3,9 +3,11 @@
 The tunables are the block size of the neighbour passes (64 / 128 / 256), the
 XCD-aware block order (each of the 8 XCDs sweeps one contiguous run of
 particles so the neighbour rows its blocks share stay in its own L2, or the
-hardware's round-robin order) and the skin of the per-step candidate lists
+hardware's round-robin order), the skin of the per-step candidate lists
 (a fraction of h: larger keeps the lists valid through more motion per step,
-smaller visits fewer candidates).  The cell size is not tunable: the reference's
+smaller visits fewer candidates) and persistent passes (at most that many
+per-lane workgroups per neighbour pass, each looping over particle blocks;
+0: one workgroup per particle block).  The cell size is not tunable: the reference's
 candidate order is defined on cells of size h, and the GPU reproduces it
 exactly.  The sort is a counting sort with a per-cell rank, so there is no
 radix to choose.
@@ -23,13 +25,14 @@ from typing import Dict, Iterable, Optional, Tuple
 
 import numpy as np
 
-Candidate = Tuple[int, int, float]
+Candidate = Tuple[int, int, float, int]
 DEFAULT_CANDIDATES: Tuple[Candidate, ...] = (
-    (256, 1, 0.1), (128, 1, 0.1), (256, 0, 0.1), (256, 1, 0.05), (256, 1, 0.2))
+    (256, 1, 0.1, 0), (128, 1, 0.1, 0), (256, 0, 0.1, 0), (256, 1, 0.05, 0), (256, 1, 0.2, 0),
+    (256, 1, 0.1, 4096), (256, 1, 0.1, 8192))
 
 
 def _norm(c) -> Candidate:
-    return (int(c[0]), int(c[1]), float(c[2]))
+    return (int(c[0]), int(c[1]), float(c[2]), int(c[3]) if len(c) > 3 else 0)
 
 
 def _timed_steps(target, nsteps: int, step_args) -> float:
@@ -48,11 +51,15 @@ def _timed_steps(target, nsteps: int, step_args) -> float:
 
 
 def autotune(target, candidates: Iterable[Candidate] = DEFAULT_CANDIDATES, steps_per_candidate: int = 2,
-             step_args=None, cache_path: Optional[str] = None, key: Optional[str] = None):
-    """Times each (block_size, xcd_remap, skin_fraction) candidate on ``target`` (a capi.Batch, or
+             step_args=None, cache_path: Optional[str] = None, key: Optional[str] = None, rounds: int = 3):
+    """Times each (block_size, xcd_remap, skin_fraction, persistent) candidate on ``target`` (a capi.Batch, or
     a capi.Context with ``step_args`` = (dt, gravity, mass)), applies the
-    fastest and returns (best, {candidate: ms_per_step}).  With ``cache_path``
-    and ``key`` the choice is stored / reused."""
+    fastest and returns (best, {candidate: ms_per_step}).  The simulation
+    moves on while it is timed, and the cost per step moves with it, so the
+    candidates take turns: ``rounds`` rounds, each timing every candidate for
+    ``steps_per_candidate`` steps (after one untimed step that captures its
+    launch shape), and a candidate's time is its median over the rounds.
+    With ``cache_path`` and ``key`` the choice is stored / reused."""
     if cache_path and key and os.path.exists(cache_path):
         with open(cache_path) as f:
             cache = json.load(f)
@@ -60,12 +67,14 @@ def autotune(target, candidates: Iterable[Candidate] = DEFAULT_CANDIDATES, steps
             best = _norm(cache[key]["best"])
             target.set_tuning(*best)
             return best, {}
-    results: Dict[Candidate, float] = {}
-    for c in candidates:
-        c = _norm(c)
-        target.set_tuning(*c)
-        _timed_steps(target, 1, step_args)           # graph capture / warm caches
-        results[c] = _timed_steps(target, steps_per_candidate, step_args)
+    cands = [_norm(c) for c in candidates]
+    times: Dict[Candidate, list] = {c: [] for c in cands}
+    for _ in range(max(1, rounds)):
+        for c in cands:
+            target.set_tuning(*c)
+            _timed_steps(target, 1, step_args)           # graph capture / warm caches
+            times[c].append(_timed_steps(target, steps_per_candidate, step_args))
+    results: Dict[Candidate, float] = {c: float(np.median(v)) for c, v in times.items()}
     best = min(results, key=results.get)
     target.set_tuning(*best)
     if cache_path and key:

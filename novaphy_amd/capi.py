@@ -38,7 +38,7 @@ SYMBOLS = (
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
     "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
-    "npx_set_tuning", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
+    "npx_set_tuning", "npx_set_persistent", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
     "npx_sample_boundaries", "npx_grid_query_noalloc",
 )
 
@@ -141,6 +141,7 @@ def load() -> C.CDLL:
         "npx_batch_step_timed": (C.c_int, [vp, i32, vp]),
         "npx_scene_status": (C.c_int, [vp, vp, i32]),
         "npx_set_tuning": (C.c_int, [vp, i32, i32, f32]),
+        "npx_set_persistent": (C.c_int, [vp, i32]),
         "npx_scene_set_boundary": (C.c_int, [vp, i32, vp, vp, vp, i32, i32, vp]),
         "npx_batch_num_bodies": (i32, [vp]),
         "npx_batch_coupling": (C.c_int, [vp, vp, i32, vp]),
@@ -305,8 +306,12 @@ class Context:
         keys = names.value.decode().split("\n")[:n]
         return [(k, float(st[i]), float(du[i])) for i, k in enumerate(keys)]
 
-    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1):
+    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1, persistent: int = 0):
+        """Launch tunables (results are bitwise independent of them); persistent > 0
+        caps each neighbour pass at that many per-lane workgroups (a multiple of 8),
+        each looping over particle blocks."""
         self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap), C.c_float(skin_fraction)))
+        self._c(self._L.npx_set_persistent(self._h, int(persistent)))
 
 
 def _kernel_times(L, h):
@@ -473,8 +478,12 @@ class Batch:
     def kernel_times(self):
         return _kernel_times(self._L, self._h)
 
-    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1):
+    def set_tuning(self, block_size: int, xcd_remap: int = 1, skin_fraction: float = 0.1, persistent: int = 0):
+        """Launch tunables (results are bitwise independent of them); persistent > 0
+        caps each neighbour pass at that many per-lane workgroups (a multiple of 8),
+        each looping over particle blocks."""
         self._c(self._L.npx_set_tuning(self._h, int(block_size), int(xcd_remap), C.c_float(skin_fraction)))
+        self._c(self._L.npx_set_persistent(self._h, int(persistent)))
 
 
 def scene_bytes(params, n: int) -> int:

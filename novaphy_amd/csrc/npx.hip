@@ -1142,6 +1142,19 @@ int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float sk
   });
 }
 
+int npx_set_persistent(npx_ctx* ctx, int32_t blocks) {
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (blocks != 0 && (blocks < 8 || blocks % 8 != 0 || blocks > (1 << 22)))
+      return c->fail(NPX_EINVAL, "persistent block count must be 0 or a multiple of 8 in [8, 2^22]");
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    c->persist_blocks = blocks;
+    c->table_dirty = true;
+    return NPX_OK;
+  });
+}
+
 int npx_batch_step_timed(npx_ctx* ctx, int32_t nsteps, float* step_ms) {
   return guard(ctx, __func__, [&]() -> int {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);

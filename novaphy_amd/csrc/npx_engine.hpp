@@ -242,6 +242,7 @@ struct Ctx {
   bool table_dirty = true;
   // tunables (results are bitwise independent of them)
   int nbr_block = 128;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
+  int persist_blocks = 0;  // > 0: at most this many per-lane blocks per neighbour pass, each looping
   int xcd_remap = 1;    // 1: XCD-aware block order in the neighbour passes
   float skin_frac = 0.1f;  // skin radius of the per-step candidate lists, as a fraction of h
   std::vector<int> h_blk_scene;
@@ -1105,7 +1106,11 @@ struct Ctx {
     }
     NPX_TRY(grid_pass(carry_rho));
     swap_committed(carry_rho);
-    const int blocks = cdiv((long long)hw_blocks * NPX_BLOCK, nbr_block);
+    const int nvb = cdiv((long long)hw_blocks * NPX_BLOCK, nbr_block);  // particle blocks of a neighbour pass
+    // per-lane launch blocks: one per particle block, or the persistent count
+    // (a multiple of 8) when that is smaller; each block then loops
+    const int persist = (persist_blocks > 0 && persist_blocks < nvb) ? 1 : 0;
+    const int blocks = persist ? persist_blocks : nvb;
     const SceneC* sc = d_scenes.p;
     const int* bs = d_blk_scene.p;
     DBuf<float4>* xs[2] = {&XS0, &XS1};
@@ -1120,8 +1125,8 @@ struct Ctx {
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);
     if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes));
-    const BlockPlan rm{xcd_remap, 0};
-    const BlockPlan rd{xcd_remap, nd};
+    const BlockPlan rm{xcd_remap, 0, nvb, persist};
+    const BlockPlan rd{xcd_remap, nd, nvb, persist};
     const int* cm = clsmap.p;
     const int* cst = cstart.p;
     const int* cel = CELL.p;
@@ -1198,7 +1203,8 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
+            (const void*)(intptr_t)persist_blocks};
   }
 
   void drop_graphs() {

@@ -625,8 +625,10 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 // candidate rows its blocks share stay in its L2.  Results do not depend on
 // the order: no pass reads what another block of the same launch writes.
 struct BlockPlan {
-  int remap;  // 1: XCD order; 0: hardware order
-  int nd;     // leading blocks that sweep the dense list (a multiple of 8)
+  int remap;    // 1: XCD order; 0: hardware order
+  int nd;       // leading blocks that sweep the dense list (a multiple of 8)
+  int nvb;      // particle blocks of the pass
+  int persist;  // 1: fewer launch blocks (a multiple of 8) than particle blocks, each looping
 };
 
 // The particle block of a neighbour-pass launch block (after the nd dense
@@ -672,8 +674,32 @@ __global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int n
 
 // Neighbour-pass prologue.  `active` lanes own a particle; the others stay
 // alive to the end (the delta pass reduces over whole waves).
-#define NPX_NBR_PROLOGUE()                                   \
-  const int q = npx_lblock(bp) * (int)blockDim.x + (int)threadIdx.x; \
+// Runs f(lb) for the particle blocks of a neighbour-pass launch (the launch
+// blocks after the nd dense ones): one launch block per particle block, in
+// XCD order (npx_lblock).  With bp.persist the launch has fewer blocks (a
+// multiple of 8, set by npx_set_persistent) and each loops over particle
+// blocks; in XCD order the blocks of XCD x take the particle blocks of the
+// x-th contiguous chunk in turn, so each XCD still sweeps one run of
+// particles.
+template <class F>
+__device__ __forceinline__ void npx_for_blocks(const BlockPlan& bp, F&& f) {
+  if (!bp.persist) {
+    f(npx_lblock(bp));
+    return;
+  }
+  const int i = (int)blockIdx.x - bp.nd, g = (int)gridDim.x - bp.nd, V = bp.nvb;
+  if (!bp.remap) {
+    for (int lb = i; lb < V; lb += g) f(lb);
+    return;
+  }
+  const int x = i & 7, per = g >> 3;
+  const int cs = (V >> 3) + ((x < (V & 7)) ? 1 : 0), c0 = x * (V >> 3) + min(x, V & 7);
+  for (int k = i >> 3; k < cs; k += per) f(c0 + k);
+}
+
+// Neighbour-pass prologue for particle block lb.
+#define NPX_NBR_PROLOGUE(lb)                                 \
+  const int q = (lb) * (int)blockDim.x + (int)threadIdx.x;   \
   const int s = blk_scene[q / NPX_BLOCK];                    \
   if (s < 0) return; /* uniform per block */                 \
   const SceneC& S = scenes[s];                               \
@@ -1868,48 +1894,55 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
                   K);
     return;
   }
-  NPX_NBR_PROLOGUE();
-  // uniform per block: the scene's flag picks the build launch (FAST always builds lists)
-  if (BUILD && !FAST && (K.lst[NPX_LST_STRIDE * s + NPX_LST_BUILD] != 0) != LISTS) return;
-  const bool rs = K.runs_s[s] != 0;
-  if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
-  if (!BUILD && !FAST && is_dense(S, K, q)) return;                  // the dense kernel's particle
-  if (BUILD && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
-    K.cnt[q] = NPX_CNT_DENSE;  // crowded: the per-wave dense sweep takes it, no list needed
-    K.ecnt[q] = -1;
-    dense_push(K, q, l);
-    return;
-  }
-  const float4 xi = XS[q];
-  const DensityPair<FAST> P(S, q, xi);
-  const float cut = P.cut;
-  float a[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  auto ld = [&](int j) { return XS[j]; };
-  auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };
-  if (BUILD) {
-    int n = -1;
-    unsigned long long rel = 0;
-    if (LISTS) {  // this step builds skin lists
-      Rows w;
-      if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
-        n = store_skin(w, K, q, &rel);
-      else
-        n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
-      if (n > NPX_SKIN_CAP) n = -1;
+  npx_for_blocks(bp, [&](const int lb) {
+    if (BUILD && !FAST) {
+      // the scene's flag picks the build launch (FAST always builds lists); tested
+      // from the block's first slot (one scene per block) before the prologue's loads
+      const int s0 = blk_scene[lb * (int)blockDim.x / NPX_BLOCK];
+      if (s0 < 0 || (K.lst[NPX_LST_STRIDE * s0 + NPX_LST_BUILD] != 0) != LISTS) return;
     }
-    K.cnt[q] = n;
-    if (n >= 0) {
-      K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
+    NPX_NBR_PROLOGUE(lb);
+    const bool rs = K.runs_s[s] != 0;
+    if (iter >= S.iters || !active || (rs && !K.rstart[q])) return;  // run members take their head's values
+    if (!BUILD && !FAST && is_dense(S, K, q)) return;                  // the dense kernel's particle
+    if (BUILD && !FAST && K.dense && candidate_total(l, S.wx, wxy, cm, cstart) > NPX_DENSE_MIN) {
+      K.cnt[q] = NPX_CNT_DENSE;  // crowded: the per-wave dense sweep takes it, no list needed
+      K.ecnt[q] = -1;
+      dense_push(K, q, l);
+      return;
+    }
+    const float4 xi = XS[q];
+    const DensityPair<FAST> P(S, q, xi);
+    const float cut = P.cut;
+    float a[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    auto ld = [&](int j) { return XS[j]; };
+    auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };
+    if (BUILD) {
+      int n = -1;
+      unsigned long long rel = 0;
+      if (LISTS) {  // this step builds skin lists
+        Rows w;
+        if (sweep_rows<FAST>(w, K, rs, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift))
+          n = store_skin(w, K, q, &rel);
+        else
+          n = sweep_direct<FAST, !FAST>(K, rs, q, xi, cut, S.skin_cut2, XS, l, S.wx, wxy, cm, cstart, shift, &rel);
+        if (n > NPX_SKIN_CAP) n = -1;
+      }
+      K.cnt[q] = n;
+      if (n >= 0) {
+        K.ecnt[q] = visit_rel<4>(K, q, rel, ld, body);
+      } else {
+        const int e = visit_all_two<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
+        if (e < 0) visit_all_c<FAST, 4, !FAST>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, body);
+        K.ecnt[q] = e;
+      }
     } else {
-      const int e = visit_all_two<FAST, 4, !FAST>(K, rs, q, xi, cut, l, S.wx, wxy, cm, cstart, shift, XS, ld, body);
-      if (e < 0) visit_all_c<FAST, 4, !FAST>(K, rs, l, S.wx, wxy, cm, cstart, shift, xi, cut, XS, ld, body);
-      K.ecnt[q] = e;
+      const float C = candidate_disp(K, l, S.wx, wxy, cm);
+      visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, K.lst + NPX_LST_STRIDE * s, ld, body);
     }
-  } else {
-    const float C = candidate_disp(K, l, S.wx, wxy, cm);
-    visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, K.lst + NPX_LST_STRIDE * s, ld, body);
-  }
-  density_finish(S, q, a, XS, RHO, LAM);
+    density_finish(S, q, a, XS, RHO, LAM);
+
+  });
 }
 
 // k_density_lambda for the dense list, batches w0, w0 + ws, ... (iterations
@@ -1953,44 +1986,47 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
                 VT, K);
     return;
   }
-  NPX_NBR_PROLOGUE();
-  const bool last = (iter == max(S.iters, 1) - 1);
-  if (iter >= S.iters) {
-    // scenes with fewer iterations than the batch maximum pass x* through so
-    // every scene ends in the same ping-pong buffer
-    if (!active) return;
-    const float4 xi = XS[q];
-    XSn[q] = xi;
-    if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
-    return;
-  }
-  float moved = 0.0f;
-  const bool rs = K.runs_s[s] != 0;
-  // run members take their head's values; dense-list particles are the dense kernel's
-  if (active && !(rs && !K.rstart[q]) && !(!FAST && is_dense(S, K, q))) {
-    const float4 xi = XS[q];
-    const DeltaPair<FAST> P(S, q, xi);
-    float a[3] = {0.0f, 0.0f, 0.0f};
-    auto ld = [&](int j) { return XS[j]; };
-    auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };  // k > 1 only without FAST
-    // one run / two slots at a time on the inline sweep: within 72 VGPRs
-    visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, P.cut, XS, ld, body);
-    moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
-  }
-  // Per-class maxima: slots are sorted by class, so the lanes of a class form
-  // one run in the wave; a segmented max leaves the run's maximum in its first
-  // lane, which alone does the atomic.  (The own class of a particle is the
-  // class of its cell, cm[l].)
-  const int cls = active ? cm[l] : -1 - (int)(threadIdx.x & 63);
-  const int lane = (int)(threadIdx.x & 63);
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const float mo = __shfl_down(moved, o, 64);
-    const int co = __shfl_down(cls, o, 64);
-    if (lane + o < 64 && co == cls) moved = fmaxf(moved, mo);
-  }
-  const int prev = __shfl_up(cls, 1, 64);
-  if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
+  npx_for_blocks(bp, [&](const int lb) {
+    NPX_NBR_PROLOGUE(lb);
+    const bool last = (iter == max(S.iters, 1) - 1);
+    if (iter >= S.iters) {
+      // scenes with fewer iterations than the batch maximum pass x* through so
+      // every scene ends in the same ping-pong buffer
+      if (!active) return;
+      const float4 xi = XS[q];
+      XSn[q] = xi;
+      if (last) velocity_update(S, q, xi.x, xi.y, xi.z, X, RHO, VT);
+      return;
+    }
+    float moved = 0.0f;
+    const bool rs = K.runs_s[s] != 0;
+    // run members take their head's values; dense-list particles are the dense kernel's
+    if (active && !(rs && !K.rstart[q]) && !(!FAST && is_dense(S, K, q))) {
+      const float4 xi = XS[q];
+      const DeltaPair<FAST> P(S, q, xi);
+      float a[3] = {0.0f, 0.0f, 0.0f};
+      auto ld = [&](int j) { return XS[j]; };
+      auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };  // k > 1 only without FAST
+      // one run / two slots at a time on the inline sweep: within 72 VGPRs
+      visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, P.cut, XS, ld, body);
+      moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
+    }
+    // Per-class maxima: slots are sorted by class, so the lanes of a class form
+    // one run in the wave; a segmented max leaves the run's maximum in its first
+    // lane, which alone does the atomic.  (The own class of a particle is the
+    // class of its cell, cm[l].)
+    const int cls = active ? cm[l] : -1 - (int)(threadIdx.x & 63);
+    const int lane = (int)(threadIdx.x & 63);
+  #pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const float mo = __shfl_down(moved, o, 64);
+      const int co = __shfl_down(cls, o, 64);
+      if (lane + o < 64 && co == cls) moved = fmaxf(moved, mo);
+    }
+    const int prev = __shfl_up(cls, 1, 64);
+    if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
+
+  });
 }
 
 // k_delta for the dense list, batches w0, w0 + ws, ...
@@ -2028,29 +2064,32 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     dense_omega(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, VT, OM, VX, K);
     return;
   }
-  NPX_NBR_PROLOGUE();
-  const bool rs = K.runs_s[s] != 0;
-  if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
-  const float4 vi = VT[q];
-  const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
-  float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  if (xs || vort) {
-    if (is_dense(S, K, q)) return;  // the dense kernel's particle
-    const float4 xi = XS[q];
-    const OmegaPair P(S, q, xi, vi);
-    auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
-    auto body = [&](int j, const XV& d, int k) { P(j, d, k, a); };
-    if (S.iters > 0) {  // the skin lists exist once a density pass has run
-      const float C = candidate_disp(K, l, S.wx, wxy, cm);
-      // the vorticity force pass reuses the in-range list
-      visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
-                                 body);
-    } else {
-      K.ecnt[q] = -1;
-      visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
+  npx_for_blocks(bp, [&](const int lb) {
+    NPX_NBR_PROLOGUE(lb);
+    const bool rs = K.runs_s[s] != 0;
+    if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
+    const float4 vi = VT[q];
+    const bool xs = S.xsph_on != 0, vort = S.vort_on != 0;
+    float a[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (xs || vort) {
+      if (is_dense(S, K, q)) return;  // the dense kernel's particle
+      const float4 xi = XS[q];
+      const OmegaPair P(S, q, xi, vi);
+      auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
+      auto body = [&](int j, const XV& d, int k) { P(j, d, k, a); };
+      if (S.iters > 0) {  // the skin lists exist once a density pass has run
+        const float C = candidate_disp(K, l, S.wx, wxy, cm);
+        // the vorticity force pass reuses the in-range list
+        visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
+                                   body);
+      } else {
+        K.ecnt[q] = -1;
+        visit_all<2, false>(K, false, l, S.wx, wxy, cm, cstart, shift, ld, body);
+      }
     }
-  }
-  omega_finish(S, q, vi, a, OM, VX);
+    omega_finish(S, q, vi, a, OM, VX);
+
+  });
 }
 
 // k_omega_xsph for the dense list, batches w0, w0 + ws, ...
@@ -2084,21 +2123,24 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
                  K);
     return;
   }
-  NPX_NBR_PROLOGUE();
-  const bool rs = K.runs_s[s] != 0;
-  if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
-  const float4 xi = XS[q];
-  float a[3] = {0.0f, 0.0f, 0.0f};
-  float4 oi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  if (S.vort_on) {
-    if (is_dense(S, K, q)) return;  // the dense kernel's particle
-    oi = OM[q];
-    const VortPair P(S, q, xi, oi.w);
-    auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
-    auto body = [&](int j, const XOW& d, int k) { P(j, d, k, a); };
-    visit_reuse<false, 4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, S.cut2, XS, ld, body);  // list of the XSPH / curl pass
-  }
-  commit_finish(S, q, xi, VX[q], oi, a, X, V);
+  npx_for_blocks(bp, [&](const int lb) {
+    NPX_NBR_PROLOGUE(lb);
+    const bool rs = K.runs_s[s] != 0;
+    if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
+    const float4 xi = XS[q];
+    float a[3] = {0.0f, 0.0f, 0.0f};
+    float4 oi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (S.vort_on) {
+      if (is_dense(S, K, q)) return;  // the dense kernel's particle
+      oi = OM[q];
+      const VortPair P(S, q, xi, oi.w);
+      auto ld = [&](int j) { return XOW{XS[j], VT[j].w, OM[j].w}; };
+      auto body = [&](int j, const XOW& d, int k) { P(j, d, k, a); };
+      visit_reuse<false, 4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, S.cut2, XS, ld, body);  // list of the XSPH / curl pass
+    }
+    commit_finish(S, q, xi, VX[q], oi, a, X, V);
+
+  });
 }
 
 // k_vort_commit for the dense list (scenes with vorticity), batches w0,

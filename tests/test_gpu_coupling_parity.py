@@ -135,6 +135,32 @@ def test_coupling_with_stacked_fluid_bitexact(gpu):
     np.testing.assert_array_equal(out[:, 3:], ot)
 
 
+def test_batch_persistent_passes_equal_alone(gpu):
+    """A batch stepped with persistent neighbour passes (16 workgroups per
+    pass, each looping over particle blocks of several scenes) matches each
+    scene alone at the default launch shape, bitwise."""
+    dims = [(6, 9, 5), (12, 7, 10), (9, 9, 9), (20, 6, 8)]
+    scs = [scenes.batched_scene(nx, ny, nz, 0.4, seed=200 + k, steps=6) for k, (nx, ny, nz) in enumerate(dims)]
+    _say("case: persistent batch of", len(scs), "scenes, sizes", [s.n for s in scs])
+    slots = sum(((s.n + 255) // 256) * 256 for s in scs)
+    cells = sum(capi.scene_bytes(s.params, 1) for s in scs)  # generous upper bound on cells
+    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=8)
+    b.set_tuning(128, 1, 0.1, 16)
+    ids = [b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities) for s in scs]
+    b.step(6)
+    for s, sid in zip(scs, ids):
+        bp, bv, br, _ = b.get(sid)
+        c = capi.Context(s.params, capacity=s.n)
+        c.set_particles(s.positions, s.velocities)
+        c.step(s.dt, s.gravity, s.mass, 6)
+        ap, av, ar, _ = c.get_particles()
+        c.close()
+        np.testing.assert_array_equal(bp, ap)
+        np.testing.assert_array_equal(bv, av)
+        np.testing.assert_array_equal(br, ar)
+    b.close()
+
+
 def test_batched_scenes_equal_alone(gpu):
     """Each scene advanced inside a batch matches the same scene alone,
     bitwise (per-scene arithmetic is independent of the batch)."""

@@ -166,16 +166,20 @@ def test_grid_query_matches_oracle(gpu):
 
 
 def test_tunables_do_not_change_results(gpu):
-    """Block size, XCD block order and skin change speed only: a rollout that
-    switches tunables every step is bitwise equal to the oracle."""
+    """Block size, XCD block order, skin and persistent passes change speed
+    only: a rollout that switches tunables every step is bitwise equal to the
+    oracle.  The 8000 particles (32 segments of 256 slots) make 128 / 64 / 32
+    particle blocks at block sizes 64 / 128 / 256, so persistent counts of
+    8-24 workgroups loop over several particle blocks each (in XCD order and
+    in hardware order)."""
     sc = scenes.dam_break_8k()
     prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
     sc = dataclasses.replace(sc, params=prm)
-    _say("case: tunables (block size / XCD order / skin) rollout")
+    _say("case: tunables (block size / XCD order / skin / persistent) rollout")
     c = _ctx(sc)
-    for k, (b, remap, skin) in enumerate([(64, 0, 0.02), (128, 1, 0.1), (256, 1, 0.5), (64, 1, 0.05),
-                                          (128, 0, 0.3), (256, 0, 0.1)] * 2):
-        c.set_tuning(b, remap, skin)
+    for k, (b, remap, skin, pers) in enumerate([(64, 0, 0.02, 8), (128, 1, 0.1, 0), (256, 1, 0.5, 16),
+                                                (64, 1, 0.05, 24), (128, 0, 0.3, 8), (256, 0, 0.1, 0)] * 2):
+        c.set_tuning(b, remap, skin, pers)
         c.step(sc.dt, sc.gravity, sc.mass, 1)
     gp, gv, gr, gl = c.get_particles()
     op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 12)
@@ -187,9 +191,12 @@ def test_tunables_do_not_change_results(gpu):
 
 def test_dense_cluster_list_overflow(gpu):
     """A cluster whose particles all lie within h of each other has more
-    in-range candidates than an in-range list holds, so every pass takes its
-    inline-sweep path; the state stays bitwise equal to the oracle (with
-    vorticity on, so the curl and XSPH passes are covered too)."""
+    in-range candidates than an in-range list holds (160 > 64) but fewer
+    candidates than a crowded particle (160 <= 256), so every pass takes its
+    per-lane inline path: the two-phase sweep's test phase overflows and the
+    particle sweeps row by row, and the reusing passes find no list.  The
+    state stays bitwise equal to the oracle (with vorticity on, so the curl
+    and XSPH passes are covered too)."""
     rng = np.random.default_rng(11)
     pos = (np.array([0.5, 0.5, 0.5], np.float32) + rng.uniform(0.0, 0.04, (160, 3))).astype(np.float32)
     vel = rng.uniform(-0.2, 0.2, (160, 3)).astype(np.float32)
