@@ -1095,6 +1095,51 @@ struct Ctx {
     return 0;
   }
 
+  // The neighbour passes of one step (persistent instantiations when PERSIST).
+  template <bool PERSIST>
+  int enqueue_passes(int npass, int blocks, int nd, int dense_blocks, size_t lds, const Skin& R, const BlockPlan& rm,
+                     const BlockPlan& rd, int& cur) {
+    const SceneC* sc = d_scenes.p;
+    const int* bs = d_blk_scene.p;
+    DBuf<float4>* xs[2] = {&XS0, &XS1};
+    const int* cm = clsmap.p;
+    const int* cst = cstart.p;
+    const int* cel = CELL.p;
+    for (int it = 0; it < npass; ++it) {
+      auto dens = fast ? (it == 0 ? k_density_lambda<true, true, true, PERSIST> : k_density_lambda<true, false, true, PERSIST>)
+                       : (it == 0 ? k_density_lambda<false, true, true, PERSIST> : k_density_lambda<false, false, true, PERSIST>);
+      if (it == 0 || fast) {
+        NPX_TRY(launch(it == 0 ? (fast ? "density_lambda_build" : "density_lambda_build_lists") : "density_lambda",
+                       dens, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+        if (it == 0 && !fast)  // the scenes that build no skin lists this step
+          NPX_TRY(launch("density_lambda_build", k_density_lambda<false, true, false, PERSIST>, blocks, nbr_block, sc, bs, cm,
+                         cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+      } else {
+        NPX_TRY(launch_shm("density_lambda", dens, blocks + nd, nbr_block, lds * npx_dense_lds_floats<5>(), sc, bs,
+                           cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rd));
+      }
+      if (!fast && it == 0)
+        NPX_TRY(launch_shm("density_lambda_build_dense", k_density_dense, dense_blocks, 64,
+                           sizeof(float) * npx_dense_lds_floats<5>(), sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p,
+                           it, R));
+      if (fast) {
+        NPX_TRY(launch("delta", k_delta<true, PERSIST>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
+                       xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
+      } else {
+        NPX_TRY(launch_shm("delta", k_delta<false, PERSIST>, blocks + nd, nbr_block, lds * npx_dense_lds_floats<3>(), sc, bs,
+                           cm, cst, cel, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
+                           (const float*)RHO.p, VT.p, R, rd));
+      }
+      cur = 1 - cur;
+    }
+    NPX_TRY(launch_shm("omega_xsph", k_omega_xsph<PERSIST>, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<6>() : 0,
+                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, VX.p, R, rd));
+    NPX_TRY(launch_shm("vort_commit", k_vort_commit<PERSIST>, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<3>() : 0,
+                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, (const float4*)OM.p,
+                       (const float4*)VX.p, X.p, V.p, R, rd));
+    return NPX_OK;
+  }
+
   // Enqueues one step (no host synchronisation, so it can be captured).
   int enqueue_step() {
     int imax = 0;
@@ -1111,9 +1156,6 @@ struct Ctx {
     // (a multiple of 8) when that is smaller; each block then loops
     const int persist = (persist_blocks > 0 && persist_blocks < nvb) ? 1 : 0;
     const int blocks = persist ? persist_blocks : nvb;
-    const SceneC* sc = d_scenes.p;
-    const int* bs = d_blk_scene.p;
-    DBuf<float4>* xs[2] = {&XS0, &XS1};
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
                  (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1, LST.p};
     // The build pass's dense list: one 64-lane block per batch, at most this
@@ -1125,46 +1167,14 @@ struct Ctx {
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);
     if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes));
-    const BlockPlan rm{xcd_remap, 0, nvb, persist};
-    const BlockPlan rd{xcd_remap, nd, nvb, persist};
-    const int* cm = clsmap.p;
-    const int* cst = cstart.p;
-    const int* cel = CELL.p;
+    const BlockPlan rm{xcd_remap, 0, nvb};
+    const BlockPlan rd{xcd_remap, nd, nvb};
     int cur = 0;
     const int npass = std::max(imax, 1);
-    for (int it = 0; it < npass; ++it) {
-      auto dens = fast ? (it == 0 ? k_density_lambda<true, true> : k_density_lambda<true, false>)
-                       : (it == 0 ? k_density_lambda<false, true> : k_density_lambda<false, false>);
-      if (it == 0 || fast) {
-        NPX_TRY(launch(it == 0 ? (fast ? "density_lambda_build" : "density_lambda_build_lists") : "density_lambda",
-                       dens, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
-        if (it == 0 && !fast)  // the scenes that build no skin lists this step
-          NPX_TRY(launch("density_lambda_build", k_density_lambda<false, true, false>, blocks, nbr_block, sc, bs, cm,
-                         cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
-      } else {
-        NPX_TRY(launch_shm("density_lambda", dens, blocks + nd, nbr_block, lds * npx_dense_lds_floats<5>(), sc, bs,
-                           cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rd));
-      }
-      if (!fast && it == 0)
-        NPX_TRY(launch_shm("density_lambda_build_dense", k_density_dense, dense_blocks, 64,
-                           sizeof(float) * npx_dense_lds_floats<5>(), sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p,
-                           it, R));
-      if (fast) {
-        NPX_TRY(launch("delta", k_delta<true>, blocks, nbr_block, sc, bs, cm, cst, cel, (const float4*)xs[cur]->p,
-                       xs[1 - cur]->p, it, (const float4*)X.p, (const float*)RHO.p, VT.p, R, rm));
-      } else {
-        NPX_TRY(launch_shm("delta", k_delta<false>, blocks + nd, nbr_block, lds * npx_dense_lds_floats<3>(), sc, bs,
-                           cm, cst, cel, (const float4*)xs[cur]->p, xs[1 - cur]->p, it, (const float4*)X.p,
-                           (const float*)RHO.p, VT.p, R, rd));
-      }
-      cur = 1 - cur;
-    }
-    NPX_TRY(launch_shm("omega_xsph", k_omega_xsph, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<6>() : 0,
-                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, OM.p, VX.p, R, rd));
-    NPX_TRY(launch_shm("vort_commit", k_vort_commit, blocks + nd, nbr_block, nd ? lds * npx_dense_lds_floats<3>() : 0,
-                       sc, bs, cm, cst, cel, (const float4*)xs[cur]->p, (const float4*)VT.p, (const float4*)OM.p,
-                       (const float4*)VX.p, X.p, V.p, R, rd));
-    NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, sc, bs, R, X.p, V.p, RHO.p, LAM.p));
+    NPX_TRY(persist ? enqueue_passes<true>(npass, blocks, nd, dense_blocks, lds, R, rm, rd, cur)
+                    : enqueue_passes<false>(npass, blocks, nd, dense_blocks, lds, R, rm, rd, cur));
+    NPX_TRY(launch("run_fill", k_run_fill, hw_blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, R, X.p, V.p, RHO.p, LAM.p));
     xs_final = cur;
     return NPX_OK;
   }

@@ -625,10 +625,9 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 // candidate rows its blocks share stay in its L2.  Results do not depend on
 // the order: no pass reads what another block of the same launch writes.
 struct BlockPlan {
-  int remap;    // 1: XCD order; 0: hardware order
-  int nd;       // leading blocks that sweep the dense list (a multiple of 8)
-  int nvb;      // particle blocks of the pass
-  int persist;  // 1: fewer launch blocks (a multiple of 8) than particle blocks, each looping
+  int remap;  // 1: XCD order; 0: hardware order
+  int nd;     // leading blocks that sweep the dense list (a multiple of 8)
+  int nvb;    // particle blocks of the pass
 };
 
 // The particle block of a neighbour-pass launch block (after the nd dense
@@ -676,14 +675,16 @@ __global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int n
 // alive to the end (the delta pass reduces over whole waves).
 // Runs f(lb) for the particle blocks of a neighbour-pass launch (the launch
 // blocks after the nd dense ones): one launch block per particle block, in
-// XCD order (npx_lblock).  With bp.persist the launch has fewer blocks (a
-// multiple of 8, set by npx_set_persistent) and each loops over particle
-// blocks; in XCD order the blocks of XCD x take the particle blocks of the
-// x-th contiguous chunk in turn, so each XCD still sweeps one run of
-// particles.
-template <class F>
+// XCD order (npx_lblock).  The persistent instantiations (PERSIST) are launched
+// with fewer blocks (a multiple of 8, set by npx_set_persistent), and each
+// loops over particle blocks; in XCD order the blocks of XCD x take the
+// particle blocks of the x-th contiguous chunk in turn, so each XCD still
+// sweeps one run of particles.  (A separate instantiation: the loop would
+// otherwise keep values live across particle blocks and make the one-shot
+// kernels spill.)
+template <bool PERSIST, class F>
 __device__ __forceinline__ void npx_for_blocks(const BlockPlan& bp, F&& f) {
-  if (!bp.persist) {
+  if constexpr (!PERSIST) {
     f(npx_lblock(bp));
     return;
   }
@@ -1246,18 +1247,23 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
   };
   // slots b .. en-1, NPX_TEST_W from one base address (up to NPX_TEST_W - 1
   // past en - 1: the x* buffers carry NPX_SLOT_PAD elements of padding), masked
+  // (the tests of a round go to a bit mask without branches; only the
+  // passing slots, about one in seven, take the append loop)
   auto range = [&](int b, int en) {
     for (int t = b; t < en && e <= NPX_SKIN_CAP; t += NPX_TEST_W) {
       const float4* __restrict__ p = XS + t;
       float4 a[NPX_TEST_W];
 #pragma unroll
       for (int u = 0; u < NPX_TEST_W; ++u) a[u] = p[u];
+      unsigned m = 0u;
 #pragma unroll
-      for (int u = 0; u < NPX_TEST_W; ++u) {
-        if ((t + u < en) && in(a[u])) {
-          if (e < NPX_SKIN_CAP) out[64 * e] = t + u;
-          ++e;
-        }
+      for (int u = 0; u < NPX_TEST_W; ++u) m |= (unsigned)in(a[u]) << u;
+      if (en - t < NPX_TEST_W) m &= (1u << (en - t)) - 1u;
+      while (m) {
+        const int u = (int)__builtin_ctz(m);
+        m &= m - 1u;
+        if (e < NPX_SKIN_CAP) out[64 * e] = t + u;
+        ++e;
       }
     }
   };
@@ -1884,7 +1890,7 @@ __device__ __forceinline__ void dense_commit(int, int, const SceneC* __restrict_
 // (every block belongs to one scene and leaves at once when its scene is the
 // other launch's).  The list-free variant carries no list-building code and
 // runs at the testing passes' occupancy.
-template <bool FAST, bool BUILD, bool LISTS = true>
+template <bool FAST, bool BUILD, bool LISTS = true, bool PERSIST = false>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu((BUILD && LISTS) ? 4 : 6)))
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                  const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
@@ -1894,7 +1900,7 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
                   K);
     return;
   }
-  npx_for_blocks(bp, [&](const int lb) {
+  npx_for_blocks<PERSIST>(bp, [&](const int lb) {
     if (BUILD && !FAST) {
       // the scene's flag picks the build launch (FAST always builds lists); tested
       // from the block's first slot (one scene per block) before the prologue's loads
@@ -1940,8 +1946,11 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
       const float C = candidate_disp(K, l, S.wx, wxy, cm);
       visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, K.lst + NPX_LST_STRIDE * s, ld, body);
     }
-    density_finish(S, q, a, XS, RHO, LAM);
-
+    // the slot's addresses are formed again here rather than kept live across
+    // the sweep (an empty asm hides q's value from the compiler)
+    int qf = q;
+    __asm__ volatile("" : "+v"(qf));
+    density_finish(S, qf, a, XS, RHO, LAM);
   });
 }
 
@@ -1975,7 +1984,7 @@ k_density_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_s
 // also performs the velocity update with the h/dt speed clamp (:81-93).
 // Adds the particle's |delta x*| to its summed displacement and keeps the
 // largest sum of each class for the skin-list test.
-template <bool FAST>
+template <bool FAST, bool PERSIST = false>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(7)))
 k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
         const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
@@ -1986,7 +1995,7 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
                 VT, K);
     return;
   }
-  npx_for_blocks(bp, [&](const int lb) {
+  npx_for_blocks<PERSIST>(bp, [&](const int lb) {
     NPX_NBR_PROLOGUE(lb);
     const bool last = (iter == max(S.iters, 1) - 1);
     if (iter >= S.iters) {
@@ -2025,7 +2034,6 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
     }
     const int prev = __shfl_up(cls, 1, 64);
     if (active && (lane == 0 || prev != cls) && moved > 0.0f) atomicMax(K.cmax + cls, __float_as_uint(moved));
-
   });
 }
 
@@ -2056,6 +2064,7 @@ __device__ __forceinline__ void dense_delta(int w0, int ws, const SceneC* __rest
 //   c_i     = sum_{j!=i, r^2<h^2} (m/rho~_j) (v_j - v_i) W(x*_i - x*_j)
 //   omega_i = sum_{j!=i} (m/rho~_j) (v_j - v_i) x gradW(x*_j - x*_i)
 // VX = v_i + xsph_c * c_i (v_i when XSPH is off); OM = (omega_i, |omega_i|).
+template <bool PERSIST = false>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
              const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
@@ -2064,7 +2073,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     dense_omega(NPX_DENSE_WAVE(), NPX_DENSE_STRIDE(bp), scenes, blk_scene, clsmap, cstart, cell, XS, VT, OM, VX, K);
     return;
   }
-  npx_for_blocks(bp, [&](const int lb) {
+  npx_for_blocks<PERSIST>(bp, [&](const int lb) {
     NPX_NBR_PROLOGUE(lb);
     const bool rs = K.runs_s[s] != 0;
     if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
@@ -2088,7 +2097,6 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
       }
     }
     omega_finish(S, q, vi, a, OM, VX);
-
   });
 }
 
@@ -2113,6 +2121,7 @@ __device__ __forceinline__ void dense_omega(int w0, int ws, const SceneC* __rest
 // (|omega_j| - |omega_i|) gradW(x*_i - x*_j); v += dt * eps (N x omega_i))
 // after XSPH, then the commit with the domain clamp (pbf_solver.cpp:102-126)
 // into the new committed X / V.
+template <bool PERSIST = false>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK)
 k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
               const int* __restrict__ cstart, const int* __restrict__ cell, const float4* __restrict__ XS,
@@ -2123,7 +2132,7 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
                  K);
     return;
   }
-  npx_for_blocks(bp, [&](const int lb) {
+  npx_for_blocks<PERSIST>(bp, [&](const int lb) {
     NPX_NBR_PROLOGUE(lb);
     const bool rs = K.runs_s[s] != 0;
     if (!active || (rs && !K.rstart[q])) return;  // run members take their head's values
@@ -2139,7 +2148,6 @@ k_vort_commit(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
       visit_reuse<false, 4, true>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, S.cut2, XS, ld, body);  // list of the XSPH / curl pass
     }
     commit_finish(S, q, xi, VX[q], oi, a, X, V);
-
   });
 }
 
