@@ -2089,7 +2089,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
       if (S.iters > 0) {  // the skin lists exist once a density pass has run
         const float C = candidate_disp(K, l, S.wx, wxy, cm);
         // the vorticity force pass reuses the in-range list
-        visit_test<false, 2, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
+        visit_test<false, 4, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
                                    body);
       } else {
         K.ecnt[q] = -1;
