@@ -731,11 +731,10 @@ __device__ __forceinline__ void npx_for_blocks(const BlockPlan& bp, F&& f) {
 // particle with more than NPX_SKIN_CAP skin candidates, or one whose bound
 // fails, sweeps its candidates inline (an inline sweep of a testing pass
 // still records the in-range list when it fits), and a crowded one with an
-// overflowing list goes to the per-wave dense kernels.  Lists are stored per
-// 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64, so entry k
-// of all lanes is one coalesced access (the compacted list is written entry
-// by entry in lockstep across the wave, so its stores are coalesced too).
-// Visitors fetch the data of G candidates before evaluating any of them, so
+// overflowing list goes to the per-wave dense kernels.  Skin lists are stored
+// per 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64, so entry
+// k of all lanes is one coalesced access; in-range lists are one row per slot
+// (ex_at).  Visitors fetch the data of G candidates before evaluating any of them, so
 // G loads are in flight per wave.
 #define NPX_ROWS 9
 #define NPX_SKIN_CAP 64
@@ -761,6 +760,15 @@ struct Skin {
 
 __device__ __forceinline__ size_t sk_at(int q, int k) {
   return ((size_t)(q >> 6) * NPX_SKIN_CAP + k) * 64 + (q & 63);
+}
+
+// Entry k of slot q's in-range list.  Entries go in chunks of four per slot,
+// and chunk c of the 64 slots of a wave forms one 1 KB row: a lane writes
+// and reads four entries with one 16-byte access, and lanes at the same chunk
+// share a row (the testing passes append four entries per store; the reusing
+// passes, walking their lists in lockstep, read one row per load).
+__device__ __forceinline__ size_t ex_at(int q, int k) {
+  return ((((size_t)(q >> 6) * (NPX_SKIN_CAP / 4) + (k >> 2)) * 64 + (q & 63)) << 2) + (k & 3);
 }
 
 // Largest summed displacement among the candidates of the particle in window
@@ -1009,10 +1017,14 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
     for (int u = 0; u < G; ++u) v[u] = ld(js[u]);
 #pragma unroll
     for (int u = 0; u < G; ++u) keep(v[u]);
+    // every group but the last appends G entries, so with G = 4 a group
+    // starts at a chunk boundary and its entries go out as one chunk (entries
+    // of a partial last group past the list's end are never read)
+    if constexpr (G == 4) *reinterpret_cast<int4*>(K.ex + ex_at(q, e)) = make_int4(js[0], js[1], js[2], js[3]);
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (ks[u] >= 0) {
-        K.ex[sk_at(q, e)] = js[u];  // e = G * group + u for every lane still in the loop
+        if constexpr (G != 4) K.ex[ex_at(q, e)] = js[u];
         ++e;
         f(js[u], v[u], 1);
       }
@@ -1021,13 +1033,24 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
   return e;
 }
 
-// f(j, ld(j)) over the n entries of a list (skin or in-range layout).
+// f(j, ld(j)) over the n entries of slot q's in-range list (ex_at layout).
+// With G = 4 each round reads one chunk with a 16-byte load (k is a multiple
+// of 4); entries past n belong to the partial last chunk or to earlier steps
+// and are replaced by entry k before any use.
 template <int G, class L, class F>
 __device__ __forceinline__ void visit_list(const int* __restrict__ list, int q, int n, L&& ld, F&& f) {
   for (int k = 0; k < n; k += G) {
     int js[G];
+    if constexpr (G == 4) {
+      const int4 w = *reinterpret_cast<const int4*>(list + ex_at(q, k));
+      js[0] = w.x;
+      js[1] = (k + 1 < n) ? w.y : w.x;
+      js[2] = (k + 2 < n) ? w.z : w.x;
+      js[3] = (k + 3 < n) ? w.w : w.x;
+    } else {
 #pragma unroll
-    for (int u = 0; u < G; ++u) js[u] = list[sk_at(q, min(k + u, n - 1))];
+      for (int u = 0; u < G; ++u) js[u] = list[ex_at(q, min(k + u, n - 1))];
+    }
     decltype(ld(0)) v[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) v[u] = ld(js[u]);
@@ -1238,9 +1261,20 @@ template <bool FAST, int G, bool RUNS, class L, class F>
 __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, float4 xi, float cut, int l, int wx,
                                              int wxy, const int* __restrict__ cm, const int* __restrict__ cstart,
                                              int shift, const float4* __restrict__ XS, L&& ld, F&& f) {
-  int* __restrict__ out = K.ex + sk_at(q, 0);  // entry k at out[64 k]
   const float ct = FAST ? cut : cut * (1.0f + 0x1p-20f);
   int e = 0;
+  // appends collect four entries in registers and store them as one chunk;
+  // past NPX_SKIN_CAP entries nothing is stored (the list has overflowed)
+  int4 buf = make_int4(0, 0, 0, 0);
+  auto push = [&](int j) {
+    const int c = e & 3;
+    buf.x = (c == 0) ? j : buf.x;
+    buf.y = (c == 1) ? j : buf.y;
+    buf.z = (c == 2) ? j : buf.z;
+    buf.w = (c == 3) ? j : buf.w;
+    if (c == 3 && e < NPX_SKIN_CAP) *reinterpret_cast<int4*>(K.ex + ex_at(q, e - 3)) = buf;
+    ++e;
+  };
   auto in = [&](const float4& a) {
     const float rx = xi.x - a.x, ry = xi.y - a.y, rz = xi.z - a.z;
     return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz)) < ct;
@@ -1262,8 +1296,7 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
       while (m) {
         const int u = (int)__builtin_ctz(m);
         m &= m - 1u;
-        if (e < NPX_SKIN_CAP) out[64 * e] = t + u;
-        ++e;
+        push(t + u);
       }
     }
   };
@@ -1281,8 +1314,9 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
         const int j = h ? j1 : j0, k = (h ? j2 : j1) - j;
         if (h ? i1 : i0) {
           if (e + k <= NPX_SKIN_CAP)
-            for (int u = 0; u < k; ++u) out[64 * (e + u)] = j;
-          e += k;
+            for (int u = 0; u < k; ++u) push(j);
+          else
+            e += k;
         }
       }
     }
@@ -1316,6 +1350,7 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
     }
   }
   if (e > NPX_SKIN_CAP) return -1;
+  if (e & 3) *reinterpret_cast<int4*>(K.ex + ex_at(q, e & ~3)) = buf;  // the last, partial chunk
   visit_list<G>(K.ex, q, e, ld, f);
   return e;
 }
