@@ -731,11 +731,10 @@ __device__ __forceinline__ void npx_for_blocks(const BlockPlan& bp, F&& f) {
 // particle with more than NPX_SKIN_CAP skin candidates, or one whose bound
 // fails, sweeps its candidates inline (an inline sweep of a testing pass
 // still records the in-range list when it fits), and a crowded one with an
-// overflowing list goes to the per-wave dense kernels.  Skin lists are stored
-// per 64-slot wave, word ((q / 64) * NPX_SKIN_CAP + k) * 64 + q % 64, so entry
-// k of all lanes is one coalesced access; in-range lists are one row per slot
-// (ex_at).  Visitors fetch the data of G candidates before evaluating any of them, so
-// G loads are in flight per wave.
+// overflowing list goes to the per-wave dense kernels.  Both kinds of list
+// are stored in chunks of four entries per slot (list_at).  Visitors fetch
+// the data of G candidates before evaluating any of them, so G loads are in
+// flight per wave.
 #define NPX_ROWS 9
 #define NPX_SKIN_CAP 64
 
@@ -758,17 +757,47 @@ struct Skin {
                                            // lanes whose list bound held, whether this step builds lists
 };
 
-__device__ __forceinline__ size_t sk_at(int q, int k) {
-  return ((size_t)(q >> 6) * NPX_SKIN_CAP + k) * 64 + (q & 63);
+// Entry k of slot q's skin or in-range list.  Entries go in chunks of four
+// per slot, and chunk c of the 64 slots of a wave forms one 1 KB row: a lane
+// writes and reads four entries with one 16-byte access, and lanes at the
+// same chunk share a row (the passes that build a list collect four entries
+// per store; the passes that walk one, in lockstep, read one row per load).
+__device__ __forceinline__ size_t list_at(int q, int k) {
+  return ((((size_t)(q >> 6) * (NPX_SKIN_CAP / 4) + (k >> 2)) * 64 + (q & 63)) << 2) + (k & 3);
 }
 
-// Entry k of slot q's in-range list.  Entries go in chunks of four per slot,
-// and chunk c of the 64 slots of a wave forms one 1 KB row: a lane writes
-// and reads four entries with one 16-byte access, and lanes at the same chunk
-// share a row (the testing passes append four entries per store; the reusing
-// passes, walking their lists in lockstep, read one row per load).
-__device__ __forceinline__ size_t ex_at(int q, int k) {
-  return ((((size_t)(q >> 6) * (NPX_SKIN_CAP / 4) + (k >> 2)) * 64 + (q & 63)) << 2) + (k & 3);
+// Appends of one lane to a list in list_at layout, four entries per store;
+// past NPX_SKIN_CAP entries nothing is stored (the list has overflowed).
+struct ListWriter {
+  int* __restrict__ list;
+  int q;
+  int n = 0;
+  int4 buf = make_int4(0, 0, 0, 0);
+  __device__ __forceinline__ ListWriter(int* __restrict__ list_, int q_) : list(list_), q(q_) {}
+  __device__ __forceinline__ void push(int j) {
+    const int c = n & 3;
+    buf.x = (c == 0) ? j : buf.x;
+    buf.y = (c == 1) ? j : buf.y;
+    buf.z = (c == 2) ? j : buf.z;
+    buf.w = (c == 3) ? j : buf.w;
+    if (c == 3 && n < NPX_SKIN_CAP) *reinterpret_cast<int4*>(list + list_at(q, n - 3)) = buf;
+    ++n;
+  }
+  // stores the last, partial chunk (entries past n in it are never read)
+  __device__ __forceinline__ void flush() {
+    if ((n & 3) && n <= NPX_SKIN_CAP) *reinterpret_cast<int4*>(list + list_at(q, n & ~3)) = buf;
+  }
+};
+
+// Entries k .. k+3 of slot q's list (k a multiple of 4) with one 16-byte load;
+// those at or past n (n > k) are replaced by entry k, so every index is one
+// written this step.
+__device__ __forceinline__ void list_read4(const int* __restrict__ list, int q, int k, int n, int* js) {
+  const int4 w = *reinterpret_cast<const int4*>(list + list_at(q, k));
+  js[0] = w.x;
+  js[1] = (k + 1 < n) ? w.y : w.x;
+  js[2] = (k + 2 < n) ? w.z : w.x;
+  js[3] = (k + 3 < n) ? w.w : w.x;
 }
 
 // Largest summed displacement among the candidates of the particle in window
@@ -876,7 +905,7 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
   int b = w.b[0];
   unsigned rf = w.rf;  // bit 0: the current row holds runs (its entries map to run heads)
   int left = NPX_ROWS - 1;
-  int k = 0;
+  ListWriter out(K.idx, q);
   for (;;) {
     while (mk == 0 && left > 0) {
       mk = w.m[1];
@@ -894,14 +923,12 @@ __device__ __forceinline__ int store_skin(Rows& w, const Skin& K, int q, unsigne
     if (mk == 0) break;
     const int t = (int)__builtin_ctzll(mk);
     mk &= mk - 1;
-    if (k < NPX_SKIN_CAP) {
-      K.idx[sk_at(q, k)] = (rf & 1u) ? K.ps[K.pi[b + t]] : b + t;  // the run's head holds the data
-      rel |= ((ek >> t) & 1ull) << k;
-    }
-    ++k;
+    if (out.n < NPX_SKIN_CAP) rel |= ((ek >> t) & 1ull) << out.n;
+    out.push((rf & 1u) ? K.ps[K.pi[b + t]] : b + t);  // the run's head holds the data
   }
+  out.flush();
   *rel_out = rel;
-  return k;
+  return out.n;
 }
 
 // Builds the skin list of slot q by sweeping every candidate in order (any
@@ -946,7 +973,7 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, bool rs, int q, float
                 k = NPX_SKIN_CAP + 1;  // overflow
               } else {
                 for (int u = 0; u < kk; ++u, ++k) {
-                  K.idx[sk_at(q, k)] = j;  // the run's head
+                  K.idx[list_at(q, k)] = j;  // the run's head
                   rel |= (unsigned long long)(d < cut) << k;
                 }
               }
@@ -959,7 +986,7 @@ __device__ __forceinline__ int sweep_direct(const Skin& K, bool rs, int q, float
           const float d = dist2<FAST>(xi.x - a.x, xi.y - a.y, xi.z - a.z);
           if (d < skin_cut) {
             if (k < NPX_SKIN_CAP) {
-              K.idx[sk_at(q, k)] = j;  // no runs here: every slot is a head
+              K.idx[list_at(q, k)] = j;  // no runs here: every slot is a head
               rel |= (unsigned long long)(d < cut) << k;
             }
             ++k;
@@ -979,8 +1006,7 @@ __device__ __forceinline__ unsigned long long test_list(const Skin& K, int q, in
   unsigned long long rel = 0;
   for (int k = 0; k < n; k += 4) {
     int js[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) js[u] = K.idx[sk_at(q, min(k + u, n - 1))];
+    list_read4(K.idx, q, k, n, js);
     float4 a[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) a[u] = XS[js[u]];
@@ -1011,7 +1037,7 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
     if (ks[0] < 0) break;
     int js[G];
 #pragma unroll
-    for (int u = 0; u < G; ++u) js[u] = K.idx[sk_at(q, ks[u] < 0 ? ks[0] : ks[u])];
+    for (int u = 0; u < G; ++u) js[u] = K.idx[list_at(q, ks[u] < 0 ? ks[0] : ks[u])];
     decltype(ld(0)) v[G];
 #pragma unroll
     for (int u = 0; u < G; ++u) v[u] = ld(js[u]);
@@ -1020,11 +1046,11 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
     // every group but the last appends G entries, so with G = 4 a group
     // starts at a chunk boundary and its entries go out as one chunk (entries
     // of a partial last group past the list's end are never read)
-    if constexpr (G == 4) *reinterpret_cast<int4*>(K.ex + ex_at(q, e)) = make_int4(js[0], js[1], js[2], js[3]);
+    if constexpr (G == 4) *reinterpret_cast<int4*>(K.ex + list_at(q, e)) = make_int4(js[0], js[1], js[2], js[3]);
 #pragma unroll
     for (int u = 0; u < G; ++u) {
       if (ks[u] >= 0) {
-        if constexpr (G != 4) K.ex[ex_at(q, e)] = js[u];
+        if constexpr (G != 4) K.ex[list_at(q, e)] = js[u];
         ++e;
         f(js[u], v[u], 1);
       }
@@ -1033,7 +1059,7 @@ __device__ __forceinline__ int visit_rel(const Skin& K, int q, unsigned long lon
   return e;
 }
 
-// f(j, ld(j)) over the n entries of slot q's in-range list (ex_at layout).
+// f(j, ld(j)) over the n entries of slot q's in-range list (list_at layout).
 // With G = 4 each round reads one chunk with a 16-byte load (k is a multiple
 // of 4); entries past n belong to the partial last chunk or to earlier steps
 // and are replaced by entry k before any use.
@@ -1042,14 +1068,10 @@ __device__ __forceinline__ void visit_list(const int* __restrict__ list, int q, 
   for (int k = 0; k < n; k += G) {
     int js[G];
     if constexpr (G == 4) {
-      const int4 w = *reinterpret_cast<const int4*>(list + ex_at(q, k));
-      js[0] = w.x;
-      js[1] = (k + 1 < n) ? w.y : w.x;
-      js[2] = (k + 2 < n) ? w.z : w.x;
-      js[3] = (k + 3 < n) ? w.w : w.x;
+      list_read4(list, q, k, n, js);
     } else {
 #pragma unroll
-      for (int u = 0; u < G; ++u) js[u] = list[ex_at(q, min(k + u, n - 1))];
+      for (int u = 0; u < G; ++u) js[u] = list[list_at(q, min(k + u, n - 1))];
     }
     decltype(ld(0)) v[G];
 #pragma unroll
@@ -1263,16 +1285,9 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
                                              int shift, const float4* __restrict__ XS, L&& ld, F&& f) {
   const float ct = FAST ? cut : cut * (1.0f + 0x1p-20f);
   int e = 0;
-  // appends collect four entries in registers and store them as one chunk;
-  // past NPX_SKIN_CAP entries nothing is stored (the list has overflowed)
-  int4 buf = make_int4(0, 0, 0, 0);
+  ListWriter out(K.ex, q);
   auto push = [&](int j) {
-    const int c = e & 3;
-    buf.x = (c == 0) ? j : buf.x;
-    buf.y = (c == 1) ? j : buf.y;
-    buf.z = (c == 2) ? j : buf.z;
-    buf.w = (c == 3) ? j : buf.w;
-    if (c == 3 && e < NPX_SKIN_CAP) *reinterpret_cast<int4*>(K.ex + ex_at(q, e - 3)) = buf;
+    out.push(j);
     ++e;
   };
   auto in = [&](const float4& a) {
@@ -1313,10 +1328,12 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
       for (int h = 0; h < 2; ++h) {
         const int j = h ? j1 : j0, k = (h ? j2 : j1) - j;
         if (h ? i1 : i0) {
-          if (e + k <= NPX_SKIN_CAP)
+          if (e + k <= NPX_SKIN_CAP) {
             for (int u = 0; u < k; ++u) push(j);
-          else
-            e += k;
+          } else {
+            e += k;  // overflow: the list is dropped
+            out.n += k;
+          }
         }
       }
     }
@@ -1350,7 +1367,7 @@ __device__ __forceinline__ int visit_all_two(const Skin& K, bool rs, int q, floa
     }
   }
   if (e > NPX_SKIN_CAP) return -1;
-  if (e & 3) *reinterpret_cast<int4*>(K.ex + ex_at(q, e & ~3)) = buf;  // the last, partial chunk
+  out.flush();
   visit_list<G>(K.ex, q, e, ld, f);
   return e;
 }
