@@ -2677,24 +2677,30 @@ __device__ __forceinline__ bool coupling_term(float4 xb, float4 xf, float h, flo
   return true;
 }
 
-__device__ __forceinline__ float lane_f(float v, int u) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), u));
-}
-
 // F_b of one boundary particle in window cell l, computed by the calling
 // wave (every lane gets the sum).  The fluid candidates of b's 27 cells are
 // taken in the reference's order as runs of identical points (rpi / rps:
 // k_payload_pieces; a candidate range's first slot always starts a run), 64
-// runs at a time: each lane forms one run's term, and the terms are added in
-// run order, a run of k as k additions in a row (add_k), so the sum is the
-// reference's one-by-one sum bit for bit.  Crowded cells next to a body cost
-// a sweep per 64 candidates instead of one lane's serial walk.
+// runs at a time: each lane forms one run's term; the in-range terms go to
+// LDS compacted in run order (mbcnt of the ballot), and lanes 0 / 1 / 2 add
+// the x / y / z row in order, a run of k as k additions in a row (add_k), so
+// each component is the reference's one-by-one sum bit for bit.  The three
+// sums advance side by side at one LDS read and one addition per term, which
+// matters next to piles of fluid: a boundary particle there has thousands of
+// in-range terms in one serial sum.  Callers run 256-thread blocks (at most
+// four waves; each wave uses its own LDS rows).
 __device__ __forceinline__ void coupling_wave(float4 xb, int l, int wx, int wxy, const int* __restrict__ clsmap,
                                               const int* __restrict__ cstart, const int* __restrict__ rpi,
                                               const int* __restrict__ rps, const float4* __restrict__ FS, float h,
                                               float h2, float cs, float rho0, float mass, float& fx, float& fy,
                                               float& fz) {
+  __shared__ float cw_t[4][3][64];  // per wave: the chunk's in-range terms, one row per component
+  __shared__ int cw_k[4][64];       // per wave: their run lengths
+  const int w = (int)(threadIdx.x >> 6);
+  float* __restrict__ rt = &cw_t[w][0][0];
+  int* __restrict__ rk = &cw_k[w][0];
   const int lane = (int)(threadIdx.x & 63);
+  float acc = 0.0f;  // lane u < 3: component u of F_b
   auto range = [&](int bgn, int e) {
     if (bgn >= e) return;
     const int r0 = rpi[bgn], r1 = rpi[e - 1] + 1;
@@ -2708,15 +2714,21 @@ __device__ __forceinline__ void coupling_wave(float4 xb, int l, int wx, int wxy,
         k = ((ri + 1 < r1) ? rps[ri + 1] : e) - j;
         has = coupling_term(xb, FS[j], h, h2, cs, rho0, mass, tx, ty, tz);
       }
-      unsigned long long m = __ballot(has);
-      while (m) {
-        const int u = (int)__builtin_ctzll(m);
-        m &= m - 1;
-        const int ku = __builtin_amdgcn_readlane(k, u);
-        add_k(fx, lane_f(tx, u), ku);
-        add_k(fy, lane_f(ty, u), ku);
-        add_k(fz, lane_f(tz, u), ku);
+      const unsigned long long m = __ballot(has);
+      if (has) {
+        const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        rt[pos] = tx;
+        rt[64 + pos] = ty;
+        rt[128 + pos] = tz;
+        rk[pos] = k;
       }
+      npx_wave_lds_sync();
+      const int n = (int)__builtin_popcountll(m);
+      if (lane < 3) {
+        const float* __restrict__ row = rt + 64 * lane;
+        for (int i = 0; i < n; ++i) add_k(acc, row[i], rk[i]);
+      }
+      npx_wave_lds_sync();
     }
   };
 #pragma unroll 1
@@ -2736,6 +2748,9 @@ __device__ __forceinline__ void coupling_wave(float4 xb, int l, int wx, int wxy,
       }
     }
   }
+  fx = __shfl(acc, 0, 64);
+  fy = __shfl(acc, 1, 64);
+  fz = __shfl(acc, 2, 64);
 }
 
 // FluidWorld::apply_coupling_forces, per boundary particle
