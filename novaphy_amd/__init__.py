@@ -24,7 +24,7 @@ __all__ = [
     "PerformancePhaseStat", "RigidBody", "ShapeType", "SimState", "SolverSettings", "SPHKernels",
     "SpatialHashGrid", "Transform", "World", "generate_fluid_block", "sample_model_boundaries",
     "device_count", "FLAG_FAST_MATH", "AABB", "ContactPoint", "BroadPhasePair", "SweepAndPrune",
-    "collide_shapes",
+    "collide_shapes", "gather_poses7", "apply_forces_and_step",
 ]
 
 

@@ -294,6 +294,60 @@ PYBIND11_MODULE(_core, m) {
       .def("apply_force", &World::apply_force, py::arg("body_index"), py::arg("force"))
       .def("apply_torque", &World::apply_torque, py::arg("body_index"), py::arg("torque"));
 
+  // A coupling round over many rigid worlds in one call each way, for a
+  // server stepping many coupled scenes: the poses of every body of every
+  // world (world by world, rows of position, w, x, y, z: the pose layout of
+  // npx_batch_coupling), and the coupling forces handed to each world followed
+  // by its rigid step, with FluidWorld::step's rules (static bodies, and
+  // bodies whose force and torque both have a zero float32 squared norm, get
+  // nothing).
+  m.def(
+      "gather_poses7",
+      [](const std::vector<World*>& worlds) {
+        size_t nb = 0;
+        for (const World* w : worlds) nb += w->state().transforms.size();
+        py::array_t<float> out({(py::ssize_t)nb, (py::ssize_t)7});
+        float* p = out.mutable_data();
+        for (const World* w : worlds)
+          for (const Transform& t : w->state().transforms) {
+            p[0] = t.position[0];
+            p[1] = t.position[1];
+            p[2] = t.position[2];
+            p[3] = t.rotation.w();
+            p[4] = t.rotation.x();
+            p[5] = t.rotation.y();
+            p[6] = t.rotation.z();
+            p += 7;
+          }
+        return out;
+      },
+      py::arg("worlds"));
+  m.def(
+      "apply_forces_and_step",
+      [](const std::vector<World*>& worlds, py::array_t<float, py::array::c_style | py::array::forcecast> out6,
+         const std::vector<float>& dts) {
+        if (dts.size() != worlds.size()) throw std::invalid_argument("one dt per world expected");
+        size_t nb = 0;
+        for (const World* w : worlds) nb += w->state().transforms.size();
+        if (out6.ndim() != 2 || out6.shape(1) != 6 || (size_t)out6.shape(0) != nb)
+          throw std::invalid_argument("forces must be (" + std::to_string(nb) + ", 6): force and torque per body");
+        const float* f = out6.data();
+        py::gil_scoped_release nogil;
+        for (size_t w = 0; w < worlds.size(); ++w) {
+          World& W = *worlds[w];
+          const int n = (int)W.state().transforms.size();
+          for (int k = 0; k < n; ++k, f += 6) {
+            if (W.model().bodies[k].is_static()) continue;
+            const Vec3f F(f[0], f[1], f[2]), T(f[3], f[4], f[5]);
+            if (F.squaredNorm() == 0.0f && T.squaredNorm() == 0.0f) continue;
+            W.apply_force(k, F);
+            W.apply_torque(k, T);
+          }
+          W.step(dts[w]);
+        }
+      },
+      py::arg("worlds"), py::arg("forces"), py::arg("dts"));
+
   // ---------------------------------------------------------------- fluid
   py::class_<SPHKernels>(m, "SPHKernels")
       .def_static("poly6", &SPHKernels::poly6, py::arg("r_sq"), py::arg("h"))

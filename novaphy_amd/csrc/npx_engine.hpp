@@ -1030,7 +1030,17 @@ struct Ctx {
   }
 
   // --------------------------------------------------------- the step
-  int total_cls() const { return batch ? cls_cap : (scenes[0].used ? scenes[0].ncell : 0); }
+  // Classes the counting sorts and the per-class arrays cover: the single
+  // scene's window, or in a batch the end of the highest class range a live
+  // scene holds.  Work over classes (fills, scans, the displacement maxima)
+  // then scales with the classes in use, not with the batch's cell capacity.
+  int total_cls() const {
+    if (!batch) return scenes[0].used ? scenes[0].ncell : 0;
+    int top = 0;
+    for (const auto& H : scenes)
+      if (H.used && H.window_valid) top = std::max(top, H.cls_base + H.ncell);
+    return top;
+  }
 
   // predict + grid: writes the cell-sorted state into the *2 buffers and x*
   // into XS0; the caller swaps the buffers to commit the reorder.

@@ -307,11 +307,6 @@ def _model_for(sc):
     return b.build()
 
 
-def _sqn32(v) -> np.float32:
-    v = np.asarray(v, np.float32)
-    return v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])
-
-
 def _poses7(transforms):
     out = np.zeros((len(transforms), 7), np.float32)
     for k, t in enumerate(transforms):
@@ -413,29 +408,19 @@ class FusedBackend:
         t1 = time.perf_counter()
         self.phase_s["fluid"] += t1 - t0
         if self.coupled:
+            import novaphy_amd as nv
             keys = sorted(self.coupled)  # the body table order of npx_batch_coupling
-            poses = np.concatenate([_poses7(self.coupled[k][0].state.transforms) for k in keys])
+            worlds = [self.coupled[k][0] for k in keys]
+            poses = nv.gather_poses7(worlds)
             tp = time.perf_counter()
             self.phase_s["poses"] += tp - t1
             out = self.batch.b.coupling(poses)
             t2 = time.perf_counter()
             self.phase_s["coupling"] += t2 - tp
-            off = 0
-            for k in keys:
-                world, nb, static, dt = self.coupled[k]
-                for i in range(nb):
-                    if static[i]:
-                        continue
-                    f, t = out[off + i, :3], out[off + i, 3:]
-                    # FluidWorld skips a body when both squared norms are 0 in
-                    # float32 (Eigen order x*x + (y*y + z*z)); tiny forces
-                    # whose squares underflow are skipped too
-                    if _sqn32(f) == 0.0 and _sqn32(t) == 0.0:
-                        continue
-                    world.apply_force(i, f)
-                    world.apply_torque(i, t)
-                world.step(dt)
-                off += nb
+            # FluidWorld::step's rules in one host call: static bodies and
+            # bodies whose force and torque both have a zero float32 squared
+            # norm get nothing, then each world takes its rigid step
+            nv.apply_forces_and_step(worlds, out, [self.coupled[k][3] for k in keys])
             self.phase_s["rigid"] += time.perf_counter() - t2
         return (time.perf_counter() - t0) * 1e3
 

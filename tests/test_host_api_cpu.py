@@ -201,3 +201,67 @@ def test_world_performance_monitor_cpu(tmp_path):
     mon.write_trace_json(str(out))
     import json
     assert json.loads(out.read_text())["traceEvents"] == []
+
+
+def _coupling_worlds():
+    """Three rigid worlds of different body counts: spheres and boxes over a
+    ground plane, a static body in the second."""
+    rng = np.random.default_rng(7)
+    worlds = []
+    for nb, with_static in ((2, False), (3, True), (1, False)):
+        b = novaphy.ModelBuilder()
+        b.add_ground_plane()
+        for k in range(nb):
+            pos = np.array([rng.uniform(-1, 1), rng.uniform(0.5, 2.0), rng.uniform(-1, 1)], np.float32)
+            body = novaphy.RigidBody.make_static() if (with_static and k == 1) else \
+                novaphy.RigidBody.from_sphere(float(rng.uniform(0.5, 2.0)), 0.2)
+            i = b.add_body(body, novaphy.Transform.from_translation(pos))
+            b.add_shape(novaphy.CollisionShape.make_sphere(0.2, i))
+        worlds.append(novaphy.World(b.build()))
+    return worlds
+
+
+def _state(worlds):
+    return [np.array([np.r_[t.position, t.rotation] for t in w.state.transforms] +
+                     [np.array(v) for v in w.state.linear_velocities] +
+                     [np.array(v) for v in w.state.angular_velocities], dtype=object) for w in worlds]
+
+
+def test_batched_coupling_round_equals_per_body_loop():
+    """gather_poses7 / apply_forces_and_step (one host call each per served
+    coupling round) equal the per-body loop of FluidWorld::step's rules:
+    poses world by world as (position, w, x, y, z); static bodies get no force;
+    a body whose force and torque both have a zero float32 squared norm
+    (x*x + (y*y + z*z)), tiny values whose squares underflow included, gets
+    none; then each world steps with its own dt."""
+    a, b = _coupling_worlds(), _coupling_worlds()
+    dts = [1.0 / 60.0, 1.0 / 120.0, 1.0 / 90.0]
+    rng = np.random.default_rng(3)
+    for rnd in range(4):
+        poses = novaphy.gather_poses7(a)
+        want = np.concatenate([np.array([[*t.position, t.rotation[3], *t.rotation[:3]] for t in w.state.transforms],
+                                        np.float32) for w in b])
+        npt.assert_array_equal(poses, want)
+        out = rng.normal(0.0, 5.0, (poses.shape[0], 6)).astype(np.float32)
+        out[1] = 0.0                      # zero force and torque
+        out[3, :3] = 1e-30                # squares underflow: skipped
+        out[3, 3:] = 0.0
+        out[4, :3] = 0.0                  # torque only: applied
+        novaphy.apply_forces_and_step(a, out, dts)
+        off = 0
+        for w, dt in zip(b, dts):
+            for i, body in enumerate(w.model.bodies):
+                f, t = out[off + i, :3], out[off + i, 3:]
+                sq = lambda v: v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])
+                if not body.is_static() and not (sq(f) == 0.0 and sq(t) == 0.0):
+                    w.apply_force(i, f)
+                    w.apply_torque(i, t)
+            w.step(dt)
+            off += len(w.model.bodies)
+        for sa, sb in zip(_state(a), _state(b)):
+            for x, y in zip(sa, sb):
+                npt.assert_array_equal(np.asarray(x, np.float32), np.asarray(y, np.float32))
+    with pytest.raises(ValueError):
+        novaphy.apply_forces_and_step(a, np.zeros((2, 6), np.float32), dts)
+    with pytest.raises(ValueError):
+        novaphy.apply_forces_and_step(a, np.zeros((poses.shape[0], 6), np.float32), dts[:2])
