@@ -252,6 +252,14 @@ int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float sk
  * reference counterpart (a launch tunable for the autotuner). */
 int npx_set_persistent(npx_ctx* ctx, int32_t blocks);
 
+/* Cell classes spanning more than 2^huge_levels 256-slot blocks (default 6;
+ * 0..24) are ranked by original index in index space (a histogram, scan and
+ * placement over the scene's indices) instead of by merge launches, up to 64
+ * such classes per step.  Results are bitwise independent of it; only speed
+ * changes.  No reference counterpart (a launch tunable, set low by tests to
+ * exercise the index-space rank on small scenes). */
+int npx_set_rank_split(npx_ctx* ctx, int32_t huge_levels);
+
 /* npx_batch_step with a device timestamp after every step: step_ms[k] is the
  * duration of step k between consecutive HIP events on the context's stream
  * (may be NULL).  Blocks until the steps are done. */

@@ -480,6 +480,10 @@ void FluidWorld::step(float dt) {  // fluid_world.cpp:46-99
             {"scatter", "fluid.pbf.grid_build"},
             {"rank_sort", "fluid.pbf.neighbor_query"},
             {"rank", "fluid.pbf.neighbor_query"},
+            {"rank_merge", "fluid.pbf.neighbor_query"},
+            {"huge_hist", "fluid.pbf.neighbor_query"},
+            {"huge_scan", "fluid.pbf.neighbor_query"},
+            {"huge_place", "fluid.pbf.neighbor_query"},
             {"gather", "fluid.pbf.neighbor_query"},
             {"pieces", "fluid.pbf.neighbor_query"},
             {"clear_cmax", "fluid.pbf.neighbor_query"},

@@ -291,13 +291,14 @@ void destroy(Ctx* c) {
   npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->CDB, &c->PSI, &c->POSE, &c->OUT6};
   for (auto* b : f1) b->release();
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
-                          &c->PERM, &c->MPERM, &c->MKEY, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->PERM, &c->MPERM, &c->MKEY, &c->HIDP, &c->HCNT, &c->HUGE, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
                           &c->ATKEY, &c->APERM, &c->AMPERM, &c->AMKEY, &c->aux_clsmap, &c->CRF, &c->CRPI,
                           &c->CRPS};
   for (auto* b : i1) b->release();
   c->htk.release();
   c->BIGB.release();
+  c->HMARK.release();
   c->ABIGB.release();
   c->RSTART.release();
   c->PI.release();
@@ -1155,6 +1156,17 @@ int npx_set_persistent(npx_ctx* ctx, int32_t blocks) {
   });
 }
 
+int npx_set_rank_split(npx_ctx* ctx, int32_t huge_levels) {
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (huge_levels < 0 || huge_levels > 24) return c->fail(NPX_EINVAL, "huge_levels must be in [0, 24]");
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    c->huge_levels = huge_levels;
+    return NPX_OK;
+  });
+}
+
 int npx_batch_step_timed(npx_ctx* ctx, int32_t nsteps, float* step_ms) {
   return guard(ctx, __func__, [&]() -> int {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);
@@ -1178,9 +1190,10 @@ int64_t npx_scene_bytes(const npx_pbf_params* params, int32_t n) {
                           NPX_WINDOW_MARGIN, wlo, wdim);
   const long long cells = (long long)wdim[0] * wdim[1] * wdim[2];
   const long long slots = (long long)cdiv(std::max(n, 1), NPX_BLOCK) * NPX_BLOCK;
-  // 9 float4 + 5 float + 16 int + 1 byte arrays and the skin and in-range
-  // lists per slot; class map, start, displacement maximum and run flag per cell
-  return slots * (9 * 16 + 5 * 4 + 16 * 4 + 1 + 8 * NPX_SKIN_CAP) + cells * 16;
+  // 9 float4 + 5 float + 17 int + 2 byte arrays, a huge-class count row
+  // entry per slot and the skin and in-range lists per slot; class map,
+  // start, displacement maximum and run flag per cell
+  return slots * (9 * 16 + 5 * 4 + 17 * 4 + 2 + NPX_HUGE_MAX * 4 / NPX_BLOCK + 8 * NPX_SKIN_CAP) + cells * 16;
 }
 
 int npx_set_kernel_timing(npx_ctx* ctx, int32_t enable) {

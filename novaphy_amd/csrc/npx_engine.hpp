@@ -243,6 +243,7 @@ struct Ctx {
   // tunables (results are bitwise independent of them)
   int nbr_block = 128;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
   int persist_blocks = 0;  // > 0: at most this many per-lane blocks per neighbour pass, each looping
+  int huge_levels = NPX_HUGE_LEVELS;  // classes spanning more than 2^this blocks rank in index space
   int xcd_remap = 1;    // 1: XCD-aware block order in the neighbour passes
   float skin_frac = 0.1f;  // skin radius of the per-step candidate lists, as a fraction of h
   std::vector<int> h_blk_scene;
@@ -253,7 +254,12 @@ struct Ctx {
   DBuf<int> ID, ID2, CELL, CELL2, CLS, OFF, TPERM, TKEY, PERM;
   DBuf<int> MPERM, MKEY;  // the other slot / key buffers of the large-class merge launches
   DBuf<int> CRF, CRPI, CRPS;  // runs of the coupling's fluid sort: flags / scan, run of each slot, run starts
-  DBuf<uint8_t> BIGB;     // per 256-slot block: it holds part of a large class
+  DBuf<uint8_t> BIGB;     // per 256-slot block: merge launches its large classes need
+  // index-space rank of the huge classes: per original index (seg_off +
+  // index) its huge class and its unsorted slot; per huge class and tile its
+  // member count (then offset); the step's huge-class count, scenes and slot ranges
+  DBuf<uint8_t> HMARK;
+  DBuf<int> HIDP, HCNT, HUGE;
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
@@ -771,6 +777,10 @@ struct Ctx {
     NPX_HIP(MPERM.ensure(n));
     NPX_HIP(MKEY.ensure(n));
     NPX_HIP(BIGB.ensure(n / NPX_BLOCK + 4));
+    NPX_HIP(HMARK.ensure(n));
+    NPX_HIP(HIDP.ensure(n));
+    NPX_HIP(HCNT.ensure((size_t)NPX_HUGE_MAX * (n / NPX_BLOCK)));
+    NPX_HIP(HUGE.ensure(1 + 3 * NPX_HUGE_MAX));
     NPX_HIP(SKL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(SKN.ensure(n));
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
@@ -1050,15 +1060,28 @@ struct Ctx {
     NPX_TRY(launch("fill_counts", k_fill_i32, cdiv(ncls + 1, 256), 256, cstart.p, 0, ncls + 1));
     NPX_TRY(launch("predict_bin", k_predict_bin, cdiv(blocks, NPX_BIN_BLOCK / NPX_BLOCK), NPX_BIN_BLOCK,
                    (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p, blocks, (const float4*)X.p, (const float4*)V.p, (const int*)clsmap.p, cstart.p,
-                   CLS.p, OFF.p, CELL.p, d_flags.p));
+                   CLS.p, OFF.p, CELL.p, d_flags.p, HUGE.p));
     NPX_TRY(scan(cstart.p, ncls + 1));
+    // the huge classes' index h per class (read only for huge classes) lives
+    // in CMAX's per-class displacement maxima until clear_cmax below
+    int* hid = CMAX.p;
     NPX_TRY(launch("scatter", k_scatter, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
-                   (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p));
+                   (const int*)cstart.p, (const int*)CLS.p, (const int*)OFF.p, (const int*)ID.p, TPERM.p, TKEY.p,
+                   hid, HUGE.p, huge_levels));
     NPX_TRY(launch("rank_sort", k_rank_sort, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
-                   (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)CLS.p, TPERM.p, TKEY.p, BIGB.p));
+                   (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)CLS.p, TPERM.p, TKEY.p, BIGB.p,
+                   (const int*)hid, huge_levels));
     // OFF (the binning offsets, read only by k_scatter) then holds each sorted slot's class
     NPX_TRY(launch("rank", k_rank, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
-                   (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p, OFF.p));
+                   (const int*)cstart.p, (const int*)CLS.p, (const int*)TPERM.p, (const int*)TKEY.p, PERM.p, OFF.p,
+                   (const int*)hid, HMARK.p, HIDP.p, huge_levels));
+    NPX_TRY(launch("huge_hist", k_huge_hist, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
+                   (const int*)HUGE.p, (const uint8_t*)HMARK.p, blocks, HCNT.p));
+    NPX_TRY(launch("huge_scan", k_huge_scan, NPX_HUGE_MAX, 1024, (const SceneC*)d_scenes.p, (const int*)HUGE.p,
+                   blocks, HCNT.p));
+    NPX_TRY(launch("huge_place", k_huge_place, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
+                   (const int*)d_blk_scene.p, (const int*)HUGE.p, (const uint8_t*)HMARK.p, (const int*)HIDP.p, blocks,
+                   (const int*)HCNT.p, PERM.p));
     // merge launches of the large classes spanning several blocks: enough for
     // a class spanning every block (a fixed count, so a step's launches do not
     // depend on the state and it can be replayed as a graph)
@@ -1069,7 +1092,7 @@ struct Ctx {
       NPX_TRY(launch("rank_merge", k_rank_merge, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)cstart.p, (const int*)OFF.p, (const unsigned char*)BIGB.p,
                      L, (const int*)(even ? TPERM.p : MPERM.p), (const int*)(even ? TKEY.p : MKEY.p),
-                     even ? MPERM.p : TPERM.p, even ? MKEY.p : TKEY.p, PERM.p));
+                     even ? MPERM.p : TPERM.p, even ? MKEY.p : TKEY.p, PERM.p, (const int*)hid, huge_levels));
     }
     // per-class displacement maxima and run flags, per-scene run flags: zero each step
     NPX_TRY(launch("clear_cmax", k_fill_i32, cdiv(2 * ncls + max_scenes, 256), 256, CMAX.p, 0, 2 * ncls + max_scenes));
@@ -1223,8 +1246,8 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
-            (const void*)(intptr_t)persist_blocks};
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, HMARK.p, HIDP.p, HCNT.p, HUGE.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
+            (const void*)(intptr_t)persist_blocks, (const void*)(intptr_t)huge_levels};
   }
 
   void drop_graphs() {

@@ -237,8 +237,9 @@ __global__ void __launch_bounds__(NPX_BIN_BLOCK)
 k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, int nblk,
               const float4* __restrict__ X, const float4* __restrict__ V, const int* __restrict__ clsmap,
               int* __restrict__ ccount, int* __restrict__ cls_of, int* __restrict__ off_of, int* __restrict__ cell_of,
-              int* __restrict__ flags) {
+              int* __restrict__ flags, int* __restrict__ huge) {
   __shared__ BinTable T;
+  if (blockIdx.x == 0 && threadIdx.x == 0) huge[0] = 0;  // the step's huge-class count (k_scatter)
   bin_clear(T);
   const int q = npx_slot();
   const int sb = q / NPX_BLOCK;
@@ -274,25 +275,62 @@ k_predict_bin(const SceneC* __restrict__ scenes, const int* __restrict__ blk_sce
   if (active) off_of[q] = off;
 }
 
-// Pass 2: provisional slot = class start + atomic offset.
+// Classes with more particles than this are ranked through sorted blocks;
+// smaller ones by direct counting.
+#define NPX_RANK_DIRECT 64
+
+// Merge launches a large class in slot range [b, e) has work in: ceil(log2)
+// of the number of 256-slot blocks it spans (0 inside one block).  The merge
+// launch at `level` merges the class only while this exceeds `level`.  The
+// sort passes store each block's maximum (bigb) so that a merge launch's
+// blocks whose classes are done exit before any per-slot load.
+__device__ __forceinline__ int merge_levels(int b, int e) {
+  const int np = (e - 1) / NPX_BLOCK - b / NPX_BLOCK + 1;
+  return np <= 1 ? 0 : 32 - __clz(np - 1);
+}
+
+// Huge classes: those spanning more than 2^hl blocks (hl: a launch argument,
+// NPX_HUGE_LEVELS by default; a scene collapsing into one cell puts most of
+// its particles in one class).  Any class ranks the same either way.  Up to
+// NPX_HUGE_MAX of them per step are ranked in original-index space instead
+// of by merge launches (k_huge_hist / k_huge_scan / k_huge_place); further
+// ones take the merge launches.  k_scatter gives each its index h (hid[c],
+// -1 past the cap) and records its scene, first slot and end slot in
+// huge[1 + 3h .. 3 + 3h]; huge[0] counts them.
+#define NPX_HUGE_LEVELS 6
+#define NPX_HUGE_MAX 64
+#define NPX_HUGE_NONE 0xFF
+
+__device__ __forceinline__ bool is_huge(int b, int e, int hl) { return merge_levels(b, e) > hl; }
+
+// Pass 2: provisional slot = class start + atomic offset.  The point at
+// offset 0 of a huge class registers it (hid, huge).
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
           const int* __restrict__ cls_of, const int* __restrict__ off_of, const int* __restrict__ id,
-          int* __restrict__ tperm, int* __restrict__ tkey) {
+          int* __restrict__ tperm, int* __restrict__ tkey, int* __restrict__ hid, int* __restrict__ huge, int hl) {
   const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
   const int q = npx_slot();
   if (q >= S.seg_off + S.n) return;
   const int shift = cstart[S.cls_base] - S.seg_off;
-  const int t = cstart[cls_of[q]] - shift + off_of[q];
+  const int c = cls_of[q];
+  const int o = off_of[q];
+  const int t = cstart[c] - shift + o;
   tperm[t] = q;
   tkey[t] = id[q];
+  if (o == 0 && is_huge(t, cstart[c + 1] - shift, hl)) {
+    const int h = atomicAdd(huge, 1);
+    hid[c] = h < NPX_HUGE_MAX ? h : -1;
+    if (h < NPX_HUGE_MAX) {
+      huge[1 + 3 * h] = s;
+      huge[2 + 3 * h] = t;
+      huge[3 + 3 * h] = cstart[c + 1] - shift;
+    }
+  }
 }
 
-// Classes with more particles than this are ranked through sorted blocks;
-// smaller ones by direct counting.
-#define NPX_RANK_DIRECT 64
 
 // Pass 3a: inside each 256-slot block holding part of a class larger than
 // NPX_RANK_DIRECT, sorts the provisional slots by (class, original index).
@@ -302,22 +340,31 @@ k_scatter(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, 
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
             const int* __restrict__ cls_of, int* __restrict__ tperm, int* __restrict__ tkey,
-            unsigned char* __restrict__ bigb) {
+            unsigned char* __restrict__ bigb, const int* __restrict__ hid, int hl) {
   __shared__ unsigned long long sk[NPX_BLOCK];
   __shared__ int sp[NPX_BLOCK];
   const int s = blk_scene[blockIdx.x];
   if (s < 0) return;  // uniform per block
+  __shared__ int slv;
   const SceneC& S = scenes[s];
   const int i = (int)threadIdx.x;
   const int t = npx_slot();
   const bool active = t < S.seg_off + S.n;
-  int p = 0, c = 0;
+  if (i == 0) slv = 0;
+  int p = 0, c = 0, lv = 0;
+  bool large = false;
   if (active) {
     p = tperm[t];
     c = cls_of[p];
+    const int shift = cstart[S.cls_base] - S.seg_off;
+    const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
+    large = e - b > NPX_RANK_DIRECT && !(is_huge(b, e, hl) && hid[c] >= 0);  // ranked in index space
+    if (large) lv = merge_levels(b, e);
   }
-  const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
-  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge launches
+  const bool big = __syncthreads_or(large) != 0;
+  if (lv > 0) atomicMax(&slv, lv);
+  __syncthreads();
+  if (i == 0) bigb[blockIdx.x] = (unsigned char)slv;  // read by the merge launches
   if (!big) return;
   unsigned long long key = ~0ull;  // inactive slots (a suffix of the block) sort last
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
@@ -361,7 +408,8 @@ k_rank_sort(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
        const int* __restrict__ cls_of, const int* __restrict__ tperm, const int* __restrict__ tkey,
-       int* __restrict__ perm, int* __restrict__ tcls) {
+       int* __restrict__ perm, int* __restrict__ tcls, const int* __restrict__ hid,
+       unsigned char* __restrict__ hmark, int* __restrict__ hidp, int hl) {
   const int s = blk_scene[blockIdx.x];
   if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
@@ -372,7 +420,14 @@ k_rank(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, con
   const int c = cls_of[p];
   tcls[t] = c;  // the class of every sorted slot, for the merge launches
   const int b = cstart[c] - shift, e = cstart[c + 1] - shift;
-  if (e - b <= NPX_RANK_DIRECT) {
+  // every original index of the scene gets its mark: the huge class's index
+  // h, or none
+  const int h = is_huge(b, e, hl) ? hid[c] : -1;
+  const int ix = S.seg_off + tkey[t];
+  hmark[ix] = h >= 0 ? (unsigned char)h : (unsigned char)NPX_HUGE_NONE;
+  if (h >= 0) {
+    hidp[ix] = p;  // placed by k_huge_place
+  } else if (e - b <= NPX_RANK_DIRECT) {
     const int my = tkey[t];
     int r = 0;
     for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
@@ -438,15 +493,15 @@ __device__ __forceinline__ int merge_dst(int b, int e, int t, int level, int my,
 
 // One merge launch over the fluid sort's slots.  The launch that leaves one
 // run writes the final permutation; earlier ones write the other key / slot
-// buffers.  Blocks without a large class (bigb, from k_rank_sort) exit at
-// once, so idle launches cost one byte per block.
+// buffers.  Blocks whose classes need no merge at this level (bigb, from
+// k_rank_sort) exit at once, so idle launches cost one byte per block.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ cstart,
              const int* __restrict__ tcls, const unsigned char* __restrict__ bigb, int level,
              const int* __restrict__ pin, const int* __restrict__ kin, int* __restrict__ pout,
-             int* __restrict__ kout, int* __restrict__ perm) {
+             int* __restrict__ kout, int* __restrict__ perm, const int* __restrict__ hid, int hl) {
   const int s = blk_scene[blockIdx.x];
-  if (s < 0 || !bigb[blockIdx.x]) return;  // uniform per block
+  if (s < 0 || bigb[blockIdx.x] <= level) return;  // uniform per block
   const SceneC& S = scenes[s];
   const int t = npx_slot();
   if (t >= S.seg_off + S.n) return;
@@ -456,6 +511,7 @@ k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
   if (e - b <= NPX_RANK_DIRECT) return;
   const int npieces = (e - 1) / NPX_BLOCK - b / NPX_BLOCK + 1;
   if (npieces <= (1 << level)) return;  // merged by an earlier launch (or a single piece)
+  if (is_huge(b, e, hl) && hid[c] >= 0) return;  // ranked in index space
   const int p = pin[t];
   const int my = kin[t];
   const int dst = merge_dst(b, e, t, level, my, kin);
@@ -465,6 +521,108 @@ k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     pout[dst] = p;
     kout[dst] = my;
   }
+}
+
+// Index-space rank of the huge classes.  Original indices are unique in
+// [0, n) within a scene, and a class lists its particles in ascending index,
+// so a huge class's particle goes to the class's first slot + the number of
+// its members with a smaller index.  k_rank marked each index of a scene
+// with its huge class (hmark[seg_off + index], NPX_HUGE_NONE otherwise) and
+// the particle's unsorted slot (hidp).  Tiles are the 256 indices of one
+// slot block.
+//
+// k_huge_hist: members of each huge class per tile, into hcnt[h * ntiles +
+// tile] (every tile of a scene writes every row; a row's scan reads only the
+// tiles of its own scene).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_huge_hist(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
+            const unsigned char* __restrict__ hmark, int ntiles, int* __restrict__ hcnt) {
+  __shared__ int cnt[NPX_HUGE_MAX];
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  const int s = blk_scene[blockIdx.x];
+  if (nh == 0 || s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int i = (int)threadIdx.x;
+  if (i < NPX_HUGE_MAX) cnt[i] = 0;
+  __syncthreads();
+  const int ix = npx_slot();
+  if (ix < S.seg_off + S.n) {
+    const int m = hmark[ix];
+    if (m < nh) atomicAdd(&cnt[m], 1);
+  }
+  __syncthreads();
+  if (i < nh) hcnt[(size_t)i * ntiles + blockIdx.x] = cnt[i];
+}
+
+// k_huge_scan: one block per huge class, an exclusive scan of its row over
+// the tiles of its scene.
+__global__ void __launch_bounds__(1024)
+k_huge_scan(const SceneC* __restrict__ scenes, const int* __restrict__ huge, int ntiles, int* __restrict__ hcnt) {
+  __shared__ int wsum[16];
+  const int h = (int)blockIdx.x;
+  if (h >= min(huge[0], NPX_HUGE_MAX)) return;  // uniform per block
+  const SceneC& S = scenes[huge[1 + 3 * h]];
+  const int t0 = S.seg_off / NPX_BLOCK, t1 = t0 + (S.n + NPX_BLOCK - 1) / NPX_BLOCK;
+  int* row = hcnt + (size_t)h * ntiles;
+  const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+  int carry = 0;
+  for (int base = t0; base < t1; base += 1024) {
+    const int k = base + (int)threadIdx.x;
+    const int v = k < t1 ? row[k] : 0;
+    int incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int pre = carry, tot = 0;
+    for (int w = 0; w < 16; ++w) {
+      const int x = wsum[w];
+      pre += (w < wv) ? x : 0;
+      tot += x;
+    }
+    if (k < t1) row[k] = pre + incl - v;
+    carry += tot;
+    __syncthreads();  // wsum is rewritten by the next chunk
+  }
+}
+
+// k_huge_place: final slot of each huge class member = the class's first
+// slot + members in earlier tiles + members at lower indices of this tile
+// (earlier waves through LDS, earlier lanes by ballot).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_huge_place(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
+             const unsigned char* __restrict__ hmark, const int* __restrict__ hidp, int ntiles,
+             const int* __restrict__ hcnt, int* __restrict__ perm) {
+  __shared__ int wc[NPX_BLOCK / 64][NPX_HUGE_MAX];
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  const int s = blk_scene[blockIdx.x];
+  if (nh == 0 || s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int i = (int)threadIdx.x;
+  for (int k = i; k < (NPX_BLOCK / 64) * NPX_HUGE_MAX; k += NPX_BLOCK) (&wc[0][0])[k] = 0;
+  __syncthreads();
+  const int ix = npx_slot();
+  int m = (ix < S.seg_off + S.n) ? (int)hmark[ix] : NPX_HUGE_NONE;
+  if (m >= nh) m = NPX_HUGE_NONE;  // marks are this step's (k_rank writes every index of the scene)
+  const int lane = i & 63, wv = i >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned long long rest = __ballot(m != NPX_HUGE_NONE);
+  int r = 0;
+  while (rest) {  // one round per distinct huge class among the wave's indices
+    const int v = __shfl(m, (int)__builtin_ctzll(rest), 64);
+    const unsigned long long same = __ballot(m == v);
+    if (m == v) r = __popcll(same & below);
+    if (lane == 0) wc[wv][v] = __popcll(same);
+    rest &= ~same;
+  }
+  __syncthreads();
+  if (m == NPX_HUGE_NONE) return;
+  for (int w = 0; w < wv; ++w) r += wc[w][m];
+  const int dst = huge[2 + 3 * m] + hcnt[(size_t)m * ntiles + blockIdx.x] + r;
+  if (dst < huge[3 + 3 * m]) perm[dst] = hidp[ix];
 }
 
 // Pass 4: reorder the state into the new slot order and form x*; also marks
@@ -2336,14 +2494,21 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
   __shared__ int sp[NPX_BLOCK];
   const int i = (int)threadIdx.x;
   const int t = (int)blockIdx.x * NPX_BLOCK + i;
+  __shared__ int slv;
   const bool active = t < cstart[ncls];
-  int p = 0, c = 0;
+  if (i == 0) slv = 0;
+  int p = 0, c = 0, lv = 0;
+  bool large = false;
   if (active) {
     p = tperm[t];
     c = cls_of[p];
+    large = cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT;
+    if (large) lv = merge_levels(cstart[c], cstart[c + 1]);
   }
-  const bool big = __syncthreads_or(active && cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT) != 0;
-  if (i == 0) bigb[blockIdx.x] = big ? 1 : 0;  // read by the merge launches
+  const bool big = __syncthreads_or(large) != 0;
+  if (lv > 0) atomicMax(&slv, lv);
+  __syncthreads();
+  if (i == 0) bigb[blockIdx.x] = (unsigned char)slv;  // read by the merge launches
   if (!big) return;
   unsigned long long key = ~0ull;
   if (active) key = ((unsigned long long)(unsigned)c << 32) | (unsigned)tkey[t];
@@ -2406,7 +2571,7 @@ k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict
                  const unsigned char* __restrict__ bigb, int level, const int* __restrict__ pin,
                  const int* __restrict__ kin, int* __restrict__ pout, int* __restrict__ kout,
                  const float4* __restrict__ P, int* __restrict__ perm, float4* __restrict__ PS) {
-  if (!bigb[blockIdx.x]) return;  // uniform per block
+  if (bigb[blockIdx.x] <= level) return;  // uniform per block
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= cstart[ncls]) return;
   const int c = tcls[t];

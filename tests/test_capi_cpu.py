@@ -44,10 +44,10 @@ def test_invalid_params_rejected_before_device():
 def test_scene_bytes_admission_estimate():
     sc = scenes.dam_break_8k()
     b = capi.scene_bytes(sc.params, sc.n)
-    # 8000 particles -> 8192 slots * (229 B of state + 64-entry skin and in-range lists),
-    # plus the cell window (cells -4..24, -4..14, -4..14 for the 2x1x1 m domain
-    # at h = 0.1) * 16 B
-    assert b == 8192 * (229 + 512) + 29 * 19 * 19 * 16
+    # 8000 particles -> 8192 slots * (235 B of state and sort scratch + 64-entry
+    # skin and in-range lists), plus the cell window (cells -4..24, -4..14, -4..14
+    # for the 2x1x1 m domain at h = 0.1) * 16 B
+    assert b == 8192 * (235 + 512) + 29 * 19 * 19 * 16
     assert capi.scene_bytes(sc.params, 2 * sc.n) > b
 
 

@@ -188,6 +188,36 @@ def test_config3_1024_scene_batch(gpu):
     b.close()
 
 
+def test_batch_index_space_rank_equals_alone(gpu):
+    """Six collapsing scenes (configured parameters, 6k-40k particles) in one
+    batch with huge_levels 0 (every class spanning two or more blocks ranks in
+    index space, each class's row scanned over its own scene's tiles, up to 64
+    classes per step and the rest by merge launches) over 40 steps: bitwise
+    equal to each scene stepped alone with the default split (the single
+    contexts are pinned to the oracle by the single-scene tests)."""
+    scs = [scenes.batched_scene(nx, ny, nz, f, seed=700 + k, s_corr="configured")
+           for k, (nx, ny, nz, f) in enumerate([(20, 20, 15, 0.3), (40, 25, 40, 0.25), (30, 30, 30, 0.4),
+                                                (25, 20, 25, 0.2), (16, 24, 16, 0.3), (34, 20, 30, 0.35)])]
+    slots = sum(((s.n + 255) // 256) * 256 for s in scs)
+    cells = sum(scenes.window_cells(s) for s in scs)
+    _say("case: batch with index-space rank,", len(scs), "scenes,", sum(s.n for s in scs), "particles, 40 steps")
+    b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=len(scs))
+    b.set_rank_split(0)
+    ids = [b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities) for s in scs]
+    b.step(40)
+    assert not b.status(len(scs)).any()
+    for k, s in enumerate(scs):
+        c = capi.Context(s.params, capacity=s.n)
+        c.set_particles(s.positions, s.velocities)
+        c.step(s.dt, s.gravity, s.mass, 40)
+        want = c.get_particles()
+        c.close()
+        got = b.get(ids[k])
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g, w, err_msg=f"scene {k}")
+    b.close()
+
+
 class _Clock:
     t = 0.0
 
@@ -256,18 +286,23 @@ def _state_hash(pos, vel, rho, lam):
     return h.hexdigest()
 
 
-def test_config0_configured_200_steps_against_oracle_rollout(gpu):
+@pytest.mark.parametrize("huge_levels", [6, 2])
+def test_config0_configured_200_steps_against_oracle_rollout(gpu, huge_levels):
     """The 8000-particle dam break with the configured s_corr (k = 0.1,
     dq = 0.2h), the scene the reference's compute_delta_position drives into
     collapse: 200 GPU steps against the oracle's committed rollout
     (tests/golden/make_config0_rollout.py).  Mean density, mean density error
     and kinetic energy every 10 steps within 1 %, the tolerance of a 200-step
     rollout, and the final state's hash equal (the default kernels are
-    bitwise)."""
+    bitwise).  huge_levels 2: the corner stack (most of the 8000 particles in
+    one class, up to 31 blocks) ranks in index space instead of by merge
+    launches, with the same hash."""
     with open(os.path.join(HERE, "golden", "config0_configured_200.json")) as f:
         gold = json.load(f)
     sc = scenes.dam_break_8k()
+    _say("case: config 0 configured 200 steps, huge_levels", huge_levels)
     c = capi.Context(sc.params, capacity=sc.n)
+    c.set_rank_split(huge_levels)
     c.set_particles(sc.positions, sc.velocities)
     m = float(F32(sc.mass))
     for row in gold["rows"]:

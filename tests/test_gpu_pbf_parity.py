@@ -353,6 +353,52 @@ def test_crowded_sheet_tunables_bitexact(gpu):
     np.testing.assert_array_equal(gl, ol)
 
 
+@pytest.mark.parametrize("huge_levels", [0, 2])
+def test_index_space_rank_bitexact(gpu, huge_levels):
+    """Cell classes spanning more than 2**huge_levels 256-slot blocks rank by
+    original index in index space (per-tile histogram, per-class scan,
+    placement) instead of by merge launches.  huge_levels 0 sends every class
+    spanning two or more blocks there, up to 64 per step (the rest take the
+    merge launches beside it).  The stacked-particle scene (a 3100-particle
+    stack in one class, 13 blocks) over 3 steps and the configured 8k collapse
+    over 20 steps stay bitwise equal to the oracle."""
+    rng = np.random.default_rng(31)
+    block = (np.array([0.03, 0.03, 0.03], np.float32) + rng.uniform(0.0, 0.35, (3000, 3))).astype(np.float32)
+    vblock = rng.uniform(-0.5, 0.5, block.shape).astype(np.float32)
+    pa = np.array([0.21, 0.13, 0.17], np.float32)
+    pos = np.concatenate([block, np.repeat(pa[None], 3100, 0)]).astype(np.float32)
+    vel = np.concatenate([vblock, np.repeat(np.array([[0.3, -0.2, 0.1]], np.float32), 3100, 0)]).astype(np.float32)
+    order = rng.permutation(pos.shape[0])
+    pos, vel = pos[order], vel[order]
+    prm = scenes.PBFParams(corr_k=0.1, corr_delta_q=0.2, vorticity_epsilon=1e-4, use_domain_bounds=True,
+                           domain_lower=(0.0, 0.0, 0.0), domain_upper=(1.0, 1.0, 1.0))
+    dt, g, m = float(np.float32(1.0 / 120.0)), (0.0, -9.81, 0.0), prm.particle_mass(0.025)
+    _say("case: index-space rank, stacked scene n=", pos.shape[0], "huge_levels", huge_levels)
+    c = capi.Context(prm, capacity=pos.shape[0])
+    c.set_rank_split(huge_levels)
+    c.set_particles(pos, vel)
+    c.step(dt, g, m, 3)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(prm, pos, vel, dt, g, m, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+    sc = scenes.dam_break_8k()
+    _say("case: index-space rank, configured 8k collapse 20 steps, huge_levels", huge_levels)
+    c = _ctx(sc)
+    c.set_rank_split(huge_levels)
+    c.step(sc.dt, sc.gravity, sc.mass, 20)
+    gp, gv, gr, gl = c.get_particles()
+    c.close()
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 20)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gr, orho)
+    np.testing.assert_array_equal(gl, ol)
+
+
 def test_1m_configured_three_steps_bitexact(gpu):
     """The full-size headline scene (1M particles, configured s_corr,
     vorticity 1e-4) over its first three steps, where the blow-up presses
