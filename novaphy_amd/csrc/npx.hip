@@ -156,8 +156,10 @@ int coupling(Ctx* c, const float* pose7, const uint8_t* is_static, int nbodies, 
     NPX_TRY(c->upload_bodies(c->POSE.p, pose7, c->STATIC.p, is_static, nbodies));
     NPX_HIP_C(c, c->FP.ensure(c->slot_cap));
     NPX_HIP_C(c, c->FS.ensure(c->slot_cap));
+    NPX_HIP_C(c, c->BFK.ensure(c->slot_cap));
     NPX_TRY(c->launch("fluid_payload", k_fluid_payload, c->hw_blocks, NPX_BLOCK, (const SceneC*)c->d_scenes.p,
-                      (const int*)c->d_blk_scene.p, (const float4*)c->X.p, (const float*)c->RHO.p, c->FP.p));
+                      (const int*)c->d_blk_scene.p, (const float4*)c->X.p, (const float*)c->RHO.p, c->FP.p,
+                      (const int*)c->ID.p, c->BFK.p));
     const GridWin W = c->scene_win(H);
     const int* cm = c->clsmap.p + H.cls_base;
     NPX_TRY(c->aux_sort(W, cm, H.cls_base, c->FP.p + H.seg_off, H.n, c->ID.p + H.seg_off, c->FS.p));
@@ -317,9 +319,16 @@ void destroy(Ctx* c) {
     ps->mperm.release();
     ps->mkey.release();
     ps->bigb.release();
+    ps->hid.release();
+    ps->hidp.release();
+    ps->hcnt.release();
+    ps->huge.release();
+    ps->hmark.release();
   }
   for (auto* b : {&c->BBL, &c->BBP, &c->BBPS, &c->BFP, &c->BFS, &c->BFB}) b->release();
-  for (auto* b : {&c->BBODY, &c->BSCN, &c->BBIDX, &c->FPSC, &c->BBSTART, &c->BBEND, &c->BSBASE}) b->release();
+  for (auto* b : {&c->BBODY, &c->BSCN, &c->BBIDX, &c->FPSC, &c->BFK, &c->BBSTART, &c->BBEND, &c->BSBASE}) b->release();
+  for (auto* b : {&c->AHID, &c->AHIDP, &c->AHCNT, &c->AHUGE}) b->release();
+  c->AHMARK.release();
   for (auto* b : {&c->BPSI, &c->BPOSE, &c->BOUT6}) b->release();
   c->BSTAT.release();
   c->d_scenes.release();
@@ -729,20 +738,22 @@ int npx_grid_query_noalloc(const float* pos3, int32_t n, float cell_size, const 
     if (e == hipSuccess) e = scan(cst, nc + 1);
     if (e == hipSuccess)
       e = na_launch(k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)cls, (const int*)off,
-                    (const int*)nullptr, tperm, tkey);
+                    (const int*)nullptr, tperm, tkey, (int*)nullptr, (int*)nullptr, 31);
     // rank inside classes (off now holds each sorted slot's class), then the
     // merge launches of classes spanning several 256-slot blocks
     const int nblk = cdiv(n, NPX_BLOCK);
     if (e == hipSuccess)
-      e = na_launch(k_aux_rank_sort, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey, bigb);
+      e = na_launch(k_aux_rank_sort, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)cls, tperm, tkey, bigb,
+                    (const int*)nullptr, 31);
     if (e == hipSuccess)
       e = na_launch(k_aux_rank, nblk, NPX_BLOCK, n, nc, (const int*)cst, (const int*)cls, (const int*)tperm,
-                    (const int*)tkey, (const float4*)P, perm, PS, off);
+                    (const int*)tkey, (const float4*)P, perm, PS, off, (const int*)nullptr, 31,
+                    (unsigned char*)nullptr, (int*)nullptr);
     for (int lv = 0; e == hipSuccess && (1 << lv) < nblk + 1; lv += NPX_MERGE_LOG) {
       const bool even = ((lv / NPX_MERGE_LOG) & 1) == 0;
       e = na_launch(k_aux_rank_merge, nblk, NPX_BLOCK, nc, (const int*)cst, (const int*)off,
                     (const unsigned char*)bigb, lv, (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey),
-                    even ? mperm : tperm, even ? mkey : tkey, (const float4*)P, perm, PS);
+                    even ? mperm : tperm, even ? mkey : tkey, (const float4*)P, perm, PS, (const int*)nullptr, 31);
     }
     // candidate counts per query, then their exclusive prefix (cnt[nq] = total)
     if (e == hipSuccess) e = na_launch(k_fill_i32, cdiv(nq + 1, 256), 256, cnt, 0, nq + 1);

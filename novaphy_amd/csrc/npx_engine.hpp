@@ -300,6 +300,8 @@ struct Ctx {
   DBuf<float> OUT6;
   DBuf<int> Acstart, ACLS, AOFF, ATPERM, ATKEY, APERM, AMPERM, AMKEY;
   DBuf<uint8_t> ABIGB;
+  DBuf<int> AHID, AHIDP, AHCNT, AHUGE;  // index-space rank of the huge classes (per class, per key,
+  DBuf<uint8_t> AHMARK;                 // per class and key tile, the count and ranges)
   DBuf<int> aux_clsmap;
 
   // batched coupling: each scene's boundary particles occupy a segment of a
@@ -326,31 +328,68 @@ struct Ctx {
   struct PointSort {
     DBuf<int> cst, cls, off, tperm, tkey, perm, mperm, mkey;
     DBuf<uint8_t> bigb;
+    DBuf<int> hid, hidp, hcnt, huge;  // index-space rank of the huge classes
+    DBuf<uint8_t> hmark;
   };
 
-  // Rank inside classes of an auxiliary sort (k_aux_rank_sort, k_aux_rank and
-  // the merge launches of large classes): off is reused for each sorted slot's
-  // class, mperm / mkey are the merge launches' other buffers, bigb one byte per
-  // 256-slot block.
+  // Index-space rank of an auxiliary sort's huge classes: the buffers for
+  // ncls classes and keys unique in [0, nkeys), cleared before the scatter.
+  struct HugeBufs {
+    int* hid;  // indexed by class id (offset like the class starts)
+    int *hidp, *hcnt, *huge;
+    uint8_t* hmark;
+    int nkeys;
+  };
+  int huge_prepare(DBuf<int>& hid, DBuf<int>& hidp, DBuf<int>& hcnt, DBuf<int>& huge, DBuf<uint8_t>& hmark, int ncls,
+                   int nkeys, int cls_base, HugeBufs& hb) {
+    const int words = cdiv(std::max(nkeys, 1), 4);
+    NPX_HIP(hid.ensure((size_t)std::max(ncls, 1)));
+    NPX_HIP(hidp.ensure((size_t)std::max(nkeys, 1)));
+    NPX_HIP(hcnt.ensure((size_t)NPX_HUGE_MAX * cdiv(std::max(nkeys, 1), NPX_BLOCK)));
+    NPX_HIP(huge.ensure(1 + 3 * NPX_HUGE_MAX));
+    NPX_HIP(hmark.ensure((size_t)words * 4));
+    NPX_TRY(launch("huge_reset", k_huge_reset, cdiv(words, 256), 256, (unsigned*)hmark.p, words, huge.p));
+    hb = HugeBufs{hid.p - cls_base, hidp.p, hcnt.p, huge.p, hmark.p, nkeys};
+    return NPX_OK;
+  }
+
+  // Rank inside classes of an auxiliary sort (k_aux_rank_sort, k_aux_rank,
+  // the index-space rank of huge classes and the merge launches of the other
+  // large classes): off is reused for each sorted slot's class, mperm / mkey
+  // are the merge launches' other buffers, bigb one byte per 256-slot block.
+  // hb == nullptr: no index-space rank (every large class merges).
   int aux_rank_all(const char* tag, int n, int ncls, const int* cst, const int* cls, int* off, int* tperm, int* tkey,
-                   int* mperm, int* mkey, uint8_t* bigb, const float4* P, int* perm, float4* PS) {
+                   int* mperm, int* mkey, uint8_t* bigb, const float4* P, int* perm, float4* PS,
+                   const HugeBufs* hb) {
     const int nblk = cdiv(n, NPX_BLOCK);
-    NPX_TRY(launch(tag, k_aux_rank_sort, nblk, NPX_BLOCK, ncls, cst, cls, tperm, tkey, bigb));
+    const int hl = hb ? huge_levels : 31;
+    const int* hid = hb ? hb->hid : nullptr;
+    NPX_TRY(launch(tag, k_aux_rank_sort, nblk, NPX_BLOCK, ncls, cst, cls, tperm, tkey, bigb, hid, hl));
     NPX_TRY(launch(tag, k_aux_rank, nblk, NPX_BLOCK, n, ncls, cst, cls, (const int*)tperm, (const int*)tkey, P, perm,
-                   PS, off));
+                   PS, off, hid, hl, hb ? hb->hmark : nullptr, hb ? hb->hidp : nullptr));
+    if (hb) {
+      const int ntiles = cdiv(std::max(hb->nkeys, 1), NPX_BLOCK);
+      NPX_TRY(launch("huge_hist", k_aux_huge_hist, ntiles, NPX_BLOCK, hb->nkeys, (const int*)hb->huge,
+                     (const uint8_t*)hb->hmark, ntiles, hb->hcnt));
+      NPX_TRY(launch("huge_scan", k_huge_scan, NPX_HUGE_MAX, 1024, (const SceneC*)nullptr, (const int*)hb->huge,
+                     ntiles, hb->hcnt));
+      NPX_TRY(launch("huge_place", k_aux_huge_place, ntiles, NPX_BLOCK, hb->nkeys, (const int*)hb->huge,
+                     (const uint8_t*)hb->hmark, (const int*)hb->hidp, ntiles, (const int*)hb->hcnt, P, perm, PS));
+    }
     int levels = 0;
     while ((1 << levels) < nblk + 1) ++levels;
     for (int L = 0; L < levels; L += NPX_MERGE_LOG) {
       const bool even = ((L / NPX_MERGE_LOG) & 1) == 0;
       NPX_TRY(launch(tag, k_aux_rank_merge, nblk, NPX_BLOCK, ncls, cst, (const int*)off, (const uint8_t*)bigb, L,
                      (const int*)(even ? tperm : mperm), (const int*)(even ? tkey : mkey), even ? mperm : tperm,
-                     even ? mkey : tkey, P, perm, PS));
+                     even ? mkey : tkey, P, perm, PS, hid, hl));
     }
     return NPX_OK;
   }
   PointSort bsort, fsort;
   DBuf<float4> BBL, BBP, BBPS, BFP, BFS, BFB;
   DBuf<int> BBODY, BSCN, BBIDX, FPSC;     // pool: local body, scene (-1: free), per-body index lists
+  DBuf<int> BFK;  // per slot: the fluid sort's key (segment offset + original index)
   DBuf<int> BBSTART, BBEND, BSBASE;       // body table: index-list ranges; per scene its first body
   DBuf<float> BPSI, BPOSE, BOUT6;
   DBuf<uint8_t> BSTAT;
@@ -358,6 +397,7 @@ struct Ctx {
   // Sorts points of known scene (pscene, -1: none) into the global classes:
   // ps.cst holds the class starts, PS the payload in class order, ascending
   // key inside a class (key = point index when NULL).
+  // key: NULL (the point index) or keys unique in [0, n) over all points.
   int sort_scene_points(PointSort& ps, const float4* P, const int* pscene, int n, const int* key, float4* PS) {
     const int ncls = total_cls();
     NPX_HIP(ps.cst.ensure((size_t)ncls + 1));
@@ -376,10 +416,12 @@ struct Ctx {
                    P, pscene, n,
                    (const int*)clsmap.p, ps.cst.p, ps.cls.p, ps.off.p));
     NPX_TRY(scan(ps.cst.p, ncls + 1));
+    HugeBufs hb;
+    NPX_TRY(huge_prepare(ps.hid, ps.hidp, ps.hcnt, ps.huge, ps.hmark, ncls, n, 0, hb));
     NPX_TRY(launch("bsort_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)ps.cst.p, (const int*)ps.cls.p,
-                   (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p));
+                   (const int*)ps.off.p, key, ps.tperm.p, ps.tkey.p, hb.hid, hb.huge, huge_levels));
     return aux_rank_all("bsort_rank", n, ncls, ps.cst.p, ps.cls.p, ps.off.p, ps.tperm.p, ps.tkey.p, ps.mperm.p,
-                        ps.mkey.p, ps.bigb.p, P, ps.perm.p, PS);
+                        ps.mkey.p, ps.bigb.p, P, ps.perm.p, PS, &hb);
   }
 
   // Runs of identical points of a sorted payload (k_payload_runs /
@@ -553,10 +595,12 @@ struct Ctx {
     NPX_HIP(BFP.ensure(slot_cap));
     NPX_HIP(BFS.ensure(slot_cap));
     NPX_HIP(FPSC.ensure(slot_cap));
+    NPX_HIP(BFK.ensure(slot_cap));
     NPX_TRY(launch("fluid_payload", k_fluid_payload, hw_blocks, NPX_BLOCK, sc, bs, (const float4*)X.p,
-                   (const float*)RHO.p, BFP.p));
+                   (const float*)RHO.p, BFP.p, (const int*)ID.p, BFK.p));
     NPX_TRY(launch("coupled_slots", k_coupled_slots, hw_blocks, NPX_BLOCK, sc, bs, FPSC.p));
-    NPX_TRY(sort_scene_points(fsort, BFP.p, FPSC.p, hw_blocks * NPX_BLOCK, ID.p, BFS.p));
+    // keys: segment offset + index, unique over the batch
+    NPX_TRY(sort_scene_points(fsort, BFP.p, FPSC.p, hw_blocks * NPX_BLOCK, BFK.p, BFS.p));
     NPX_TRY(payload_runs(hw_blocks * NPX_BLOCK, fsort.cst.p, total_cls(), fsort.off.p, BFS.p));
     // a wave per boundary particle
     NPX_TRY(launch("coupling", k_coupling_b, cdiv(bb_n, 4), 256, sc, (const float4*)BBP.p, (const int*)BSCN.p,
@@ -1396,7 +1440,8 @@ struct Ctx {
   // --------------------------------------------------------- aux grids
   // Sorts n float4 points (payload in w) into the classes of window W
   // (clsmap values are class ids relative to cls_base), ordering each class
-  // by key (NULL = point index).  Results: Acstart (ncls+1), APERM, PS.
+  // by key (NULL = point index; keys are unique in [0, n)).  Results:
+  // Acstart (ncls+1), APERM, PS.
   int aux_sort(const GridWin& W, const int* cm, int cls_base, const float4* P, int n, const int* key, float4* PS) {
     NPX_HIP(Acstart.ensure((size_t)W.ncell + 1));
     NPX_HIP(ACLS.ensure(std::max(n, 1)));
@@ -1411,10 +1456,12 @@ struct Ctx {
     NPX_TRY(launch("aux_fill", k_fill_i32, cdiv(W.ncell + 1, 256), 256, Acstart.p, 0, W.ncell + 1));
     NPX_TRY(launch("aux_bin", k_aux_bin, cdiv(n, NPX_BIN_BLOCK), NPX_BIN_BLOCK, W, P, n, cm, cst, ACLS.p, AOFF.p));
     NPX_TRY(scan(Acstart.p, W.ncell + 1));
+    HugeBufs hb;
+    NPX_TRY(huge_prepare(AHID, AHIDP, AHCNT, AHUGE, AHMARK, W.ncell, n, cls_base, hb));
     NPX_TRY(launch("aux_scatter", k_aux_scatter, cdiv(n, 256), 256, n, (const int*)cst, (const int*)ACLS.p,
-                   (const int*)AOFF.p, key, ATPERM.p, ATKEY.p));
+                   (const int*)AOFF.p, key, ATPERM.p, ATKEY.p, hb.hid, hb.huge, huge_levels));
     return aux_rank_all("aux_rank", n, W.ncell + cls_base, cst, ACLS.p, AOFF.p, ATPERM.p, ATKEY.p, AMPERM.p,
-                        AMKEY.p, ABIGB.p, P, APERM.p, PS);
+                        AMKEY.p, ABIGB.p, P, APERM.p, PS, &hb);
   }
 
   GridWin scene_win(const SceneH& H) const {

@@ -531,22 +531,15 @@ k_rank_merge(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
 // the particle's unsorted slot (hidp).  Tiles are the 256 indices of one
 // slot block.
 //
-// k_huge_hist: members of each huge class per tile, into hcnt[h * ntiles +
-// tile] (every tile of a scene writes every row; a row's scan reads only the
-// tiles of its own scene).
-__global__ void __launch_bounds__(NPX_BLOCK)
-k_huge_hist(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
-            const unsigned char* __restrict__ hmark, int ntiles, int* __restrict__ hcnt) {
+// Histogram of one tile (block-wide): members of each huge class among the
+// tile's indices, into hcnt[h * ntiles + tile] for every h < nh.
+__device__ __forceinline__ void huge_hist_tile(int nh, bool valid, int ix, const unsigned char* __restrict__ hmark,
+                                               int ntiles, int* __restrict__ hcnt) {
   __shared__ int cnt[NPX_HUGE_MAX];
-  const int nh = min(huge[0], NPX_HUGE_MAX);
-  const int s = blk_scene[blockIdx.x];
-  if (nh == 0 || s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
   const int i = (int)threadIdx.x;
   if (i < NPX_HUGE_MAX) cnt[i] = 0;
   __syncthreads();
-  const int ix = npx_slot();
-  if (ix < S.seg_off + S.n) {
+  if (valid) {
     const int m = hmark[ix];
     if (m < nh) atomicAdd(&cnt[m], 1);
   }
@@ -554,15 +547,33 @@ k_huge_hist(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene
   if (i < nh) hcnt[(size_t)i * ntiles + blockIdx.x] = cnt[i];
 }
 
+// k_huge_hist: members of each huge class per tile (every tile of a scene
+// writes every row; a row's scan reads only the tiles of its own scene).
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_huge_hist(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
+            const unsigned char* __restrict__ hmark, int ntiles, int* __restrict__ hcnt) {
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  const int s = blk_scene[blockIdx.x];
+  if (nh == 0 || s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int ix = npx_slot();
+  huge_hist_tile(nh, ix < S.seg_off + S.n, ix, hmark, ntiles, hcnt);
+}
+
 // k_huge_scan: one block per huge class, an exclusive scan of its row over
-// the tiles of its scene.
+// the tiles of its scene (all ntiles tiles for an auxiliary sort's class,
+// whose scene entry is -1).
 __global__ void __launch_bounds__(1024)
 k_huge_scan(const SceneC* __restrict__ scenes, const int* __restrict__ huge, int ntiles, int* __restrict__ hcnt) {
   __shared__ int wsum[16];
   const int h = (int)blockIdx.x;
   if (h >= min(huge[0], NPX_HUGE_MAX)) return;  // uniform per block
-  const SceneC& S = scenes[huge[1 + 3 * h]];
-  const int t0 = S.seg_off / NPX_BLOCK, t1 = t0 + (S.n + NPX_BLOCK - 1) / NPX_BLOCK;
+  const int s = huge[1 + 3 * h];
+  int t0 = 0, t1 = ntiles;
+  if (s >= 0) {
+    t0 = scenes[s].seg_off / NPX_BLOCK;
+    t1 = t0 + (scenes[s].n + NPX_BLOCK - 1) / NPX_BLOCK;
+  }
   int* row = hcnt + (size_t)h * ntiles;
   const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
   int carry = 0;
@@ -589,24 +600,19 @@ k_huge_scan(const SceneC* __restrict__ scenes, const int* __restrict__ huge, int
   }
 }
 
-// k_huge_place: final slot of each huge class member = the class's first
-// slot + members in earlier tiles + members at lower indices of this tile
-// (earlier waves through LDS, earlier lanes by ballot).
-__global__ void __launch_bounds__(NPX_BLOCK)
-k_huge_place(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
-             const unsigned char* __restrict__ hmark, const int* __restrict__ hidp, int ntiles,
-             const int* __restrict__ hcnt, int* __restrict__ perm) {
+// Final slot of a huge class member (block-wide, one index per thread) =
+// the class's first slot + members in earlier tiles + members at lower
+// indices of this tile (earlier waves through LDS, earlier lanes by ballot);
+// -1 for an index of no huge class.
+__device__ __forceinline__ int huge_place_tile(int nh, bool valid, int ix, const int* __restrict__ huge,
+                                               const unsigned char* __restrict__ hmark, int ntiles,
+                                               const int* __restrict__ hcnt) {
   __shared__ int wc[NPX_BLOCK / 64][NPX_HUGE_MAX];
-  const int nh = min(huge[0], NPX_HUGE_MAX);
-  const int s = blk_scene[blockIdx.x];
-  if (nh == 0 || s < 0) return;  // uniform per block
-  const SceneC& S = scenes[s];
   const int i = (int)threadIdx.x;
   for (int k = i; k < (NPX_BLOCK / 64) * NPX_HUGE_MAX; k += NPX_BLOCK) (&wc[0][0])[k] = 0;
   __syncthreads();
-  const int ix = npx_slot();
-  int m = (ix < S.seg_off + S.n) ? (int)hmark[ix] : NPX_HUGE_NONE;
-  if (m >= nh) m = NPX_HUGE_NONE;  // marks are this step's (k_rank writes every index of the scene)
+  int m = valid ? (int)hmark[ix] : NPX_HUGE_NONE;
+  if (m >= nh) m = NPX_HUGE_NONE;
   const int lane = i & 63, wv = i >> 6;
   const unsigned long long below = (1ull << lane) - 1ull;
   unsigned long long rest = __ballot(m != NPX_HUGE_NONE);
@@ -619,10 +625,24 @@ k_huge_place(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
     rest &= ~same;
   }
   __syncthreads();
-  if (m == NPX_HUGE_NONE) return;
+  if (m == NPX_HUGE_NONE) return -1;
   for (int w = 0; w < wv; ++w) r += wc[w][m];
   const int dst = huge[2 + 3 * m] + hcnt[(size_t)m * ntiles + blockIdx.x] + r;
-  if (dst < huge[3 + 3 * m]) perm[dst] = hidp[ix];
+  return dst < huge[3 + 3 * m] ? dst : -1;
+}
+
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_huge_place(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ huge,
+             const unsigned char* __restrict__ hmark, const int* __restrict__ hidp, int ntiles,
+             const int* __restrict__ hcnt, int* __restrict__ perm) {
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  const int s = blk_scene[blockIdx.x];
+  if (nh == 0 || s < 0) return;  // uniform per block
+  const SceneC& S = scenes[s];
+  const int ix = npx_slot();
+  // marks are this step's: k_rank writes every index of the scene
+  const int dst = huge_place_tile(nh, ix < S.seg_off + S.n, ix, huge, hmark, ntiles, hcnt);
+  if (dst >= 0) perm[dst] = hidp[ix];
 }
 
 // Pass 4: reorder the state into the new slot order and form x*; also marks
@@ -2472,16 +2492,62 @@ k_aux_bin(GridWin W, const float4* __restrict__ P, int n, const int* __restrict_
   if (active) off_of[i] = off;
 }
 
+// Provisional slot = class start + atomic offset; the point at offset 0 of
+// a huge class registers it (hid, huge; its scene entry is -1) when hl < 31.
 __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* __restrict__ cls_of,
                               const int* __restrict__ off_of, const int* __restrict__ key, int* __restrict__ tperm,
-                              int* __restrict__ tkey) {
+                              int* __restrict__ tkey, int* __restrict__ hid, int* __restrict__ huge, int hl) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int c = cls_of[i];
   if (c < 0) return;
-  const int t = cstart[c] + off_of[i];
+  const int o = off_of[i];
+  const int t = cstart[c] + o;
   tperm[t] = i;
   tkey[t] = key ? key[i] : i;
+  if (o == 0 && is_huge(t, cstart[c + 1], hl)) {
+    const int h = atomicAdd(huge, 1);
+    hid[c] = h < NPX_HUGE_MAX ? h : -1;
+    if (h < NPX_HUGE_MAX) {
+      huge[1 + 3 * h] = -1;
+      huge[2 + 3 * h] = t;
+      huge[3 + 3 * h] = cstart[c + 1];
+    }
+  }
+}
+
+// Before an auxiliary sort with huge classes: no index is marked, no class
+// registered.  hmark is cleared as nwords 32-bit words.
+__global__ void k_huge_reset(unsigned* __restrict__ hmark_words, int nwords, int* __restrict__ huge) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i == 0) huge[0] = 0;
+  if (i < nwords) hmark_words[i] = 0xFFFFFFFFu;
+}
+
+// Index-space rank of an auxiliary sort's huge classes: keys are unique in
+// [0, nkeys) across the whole sort, one tile per 256 keys.
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_aux_huge_hist(int nkeys, const int* __restrict__ huge, const unsigned char* __restrict__ hmark, int ntiles,
+                int* __restrict__ hcnt) {
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  if (nh == 0) return;  // uniform
+  const int ix = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  huge_hist_tile(nh, ix < nkeys, ix, hmark, ntiles, hcnt);
+}
+
+__global__ void __launch_bounds__(NPX_BLOCK)
+k_aux_huge_place(int nkeys, const int* __restrict__ huge, const unsigned char* __restrict__ hmark,
+                 const int* __restrict__ hidp, int ntiles, const int* __restrict__ hcnt, const float4* __restrict__ P,
+                 int* __restrict__ perm, float4* __restrict__ PS) {
+  const int nh = min(huge[0], NPX_HUGE_MAX);
+  if (nh == 0) return;  // uniform
+  const int ix = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int dst = huge_place_tile(nh, ix < nkeys, ix, huge, hmark, ntiles, hcnt);
+  if (dst >= 0) {
+    const int i = hidp[ix];
+    perm[dst] = i;
+    PS[dst] = P[i];
+  }
 }
 
 // Sorts each 256-slot block of the binned slots that holds part of a class
@@ -2489,7 +2555,7 @@ __global__ void k_aux_scatter(int n, const int* __restrict__ cstart, const int* 
 // fluid sort (keys are unique within a class).
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of, int* __restrict__ tperm,
-                int* __restrict__ tkey, unsigned char* __restrict__ bigb) {
+                int* __restrict__ tkey, unsigned char* __restrict__ bigb, const int* __restrict__ hid, int hl) {
   __shared__ unsigned long long sk[NPX_BLOCK];
   __shared__ int sp[NPX_BLOCK];
   const int i = (int)threadIdx.x;
@@ -2502,8 +2568,9 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
   if (active) {
     p = tperm[t];
     c = cls_of[p];
-    large = cstart[c + 1] - cstart[c] > NPX_RANK_DIRECT;
-    if (large) lv = merge_levels(cstart[c], cstart[c + 1]);
+    const int b = cstart[c], e = cstart[c + 1];
+    large = e - b > NPX_RANK_DIRECT && !(is_huge(b, e, hl) && hid[c] >= 0);  // ranked in index space
+    if (large) lv = merge_levels(b, e);
   }
   const bool big = __syncthreads_or(large) != 0;
   if (lv > 0) atomicMax(&slv, lv);
@@ -2545,14 +2612,19 @@ k_aux_rank_sort(int ncls, const int* __restrict__ cstart, const int* __restrict_
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank(int n, int ncls, const int* __restrict__ cstart, const int* __restrict__ cls_of,
            const int* __restrict__ tperm, const int* __restrict__ tkey, const float4* __restrict__ P,
-           int* __restrict__ perm, float4* __restrict__ PS, int* __restrict__ tcls) {
+           int* __restrict__ perm, float4* __restrict__ PS, int* __restrict__ tcls, const int* __restrict__ hid,
+           int hl, unsigned char* __restrict__ hmark, int* __restrict__ hidp) {
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= cstart[ncls]) return;
   const int i = tperm[t];
   const int c = cls_of[i];
   tcls[t] = c;
   const int b = cstart[c], e = cstart[c + 1];
-  if (e - b <= NPX_RANK_DIRECT) {
+  const int h = is_huge(b, e, hl) ? hid[c] : -1;
+  if (h >= 0) {  // placed by k_aux_huge_place (the marks were cleared by k_huge_reset)
+    hmark[tkey[t]] = (unsigned char)h;
+    hidp[tkey[t]] = i;
+  } else if (e - b <= NPX_RANK_DIRECT) {
     const int my = tkey[t];
     int r = 0;
     for (int u = b; u < e; ++u) r += (tkey[u] < my) ? 1 : 0;
@@ -2570,7 +2642,8 @@ __global__ void __launch_bounds__(NPX_BLOCK)
 k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict__ tcls,
                  const unsigned char* __restrict__ bigb, int level, const int* __restrict__ pin,
                  const int* __restrict__ kin, int* __restrict__ pout, int* __restrict__ kout,
-                 const float4* __restrict__ P, int* __restrict__ perm, float4* __restrict__ PS) {
+                 const float4* __restrict__ P, int* __restrict__ perm, float4* __restrict__ PS,
+                 const int* __restrict__ hid, int hl) {
   if (bigb[blockIdx.x] <= level) return;  // uniform per block
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= cstart[ncls]) return;
@@ -2579,6 +2652,7 @@ k_aux_rank_merge(int ncls, const int* __restrict__ cstart, const int* __restrict
   if (e - b <= NPX_RANK_DIRECT) return;
   const int npieces = (e - 1) / NPX_BLOCK - b / NPX_BLOCK + 1;
   if (npieces <= (1 << level)) return;
+  if (is_huge(b, e, hl) && hid[c] >= 0) return;  // ranked in index space
   const int p = pin[t];
   const int my = kin[t];
   const int dst = merge_dst(b, e, t, level, my, kin);
@@ -2777,7 +2851,8 @@ k_boundary_density(const SceneC* __restrict__ scenes, const int* __restrict__ bl
 // Payload for the committed-fluid grid: (x, rho) with key = original index.
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_fluid_payload(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const float4* __restrict__ X,
-                const float* __restrict__ RHO, float4* __restrict__ P) {
+                const float* __restrict__ RHO, float4* __restrict__ P, const int* __restrict__ id,
+                int* __restrict__ gkey) {
   const int s = blk_scene[npx_slot() / NPX_BLOCK];
   if (s < 0) return;
   const SceneC& S = scenes[s];
@@ -2785,6 +2860,7 @@ k_fluid_payload(const SceneC* __restrict__ scenes, const int* __restrict__ blk_s
   if (q >= S.seg_off + S.n) return;
   const float4 x = X[q];
   P[q] = make_float4(x.x, x.y, x.z, RHO[q]);
+  gkey[q] = S.seg_off + id[q];  // unique over the batch, ordered like the index within a scene
 }
 
 // Runs of identical points in the coupling's fluid sort: slot j starts a run

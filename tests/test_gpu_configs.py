@@ -338,12 +338,15 @@ def test_query_all_pairs_on_a_dam_break_matches_oracle(gpu):
     np.testing.assert_array_equal(np.array(got, np.int32).reshape(-1, 2), O.grid_all_pairs(pts, 0.1, 0.1))
 
 
-def test_aux_sorts_with_large_classes_match_oracle(gpu):
+@pytest.mark.parametrize("huge_levels", [6, 1])
+def test_aux_sorts_with_large_classes_match_oracle(gpu, huge_levels):
     """The auxiliary grids' rank inside large classes (a stack of 3000
     coincident points, 800 points in one cell, classes spanning many 256-slot
     blocks, merged level by level): standalone queries return the oracle's
     bucket lists in order, and coupling over a fluid holding a stack of 2000
-    coincident particles next to a body gives the oracle's forces."""
+    coincident particles next to a body gives the oracle's forces.  With
+    huge_levels 1 the context ranks the stack's class (8 blocks) in index
+    space in its step and its coupling grid."""
     rng = np.random.default_rng(71)
     pts = np.concatenate([np.full((3000, 3), 0.437, F32), rng.uniform(0.40, 0.49, (800, 3)),
                           rng.uniform(-0.5, 1.5, (4000, 3))]).astype(F32)
@@ -357,6 +360,7 @@ def test_aux_sorts_with_large_classes_match_oracle(gpu):
     sc, local, body, poses, static = _stack_scene_with_body(rng)
     h = sc.params.kernel_radius
     c = capi.Context(sc.params, capacity=sc.n)
+    c.set_rank_split(huge_levels)
     c.set_particles(sc.positions, sc.velocities)
     c.set_boundary(local, body, None, nbodies=poses.shape[0])
     psi = c.compute_boundary_volumes(poses)

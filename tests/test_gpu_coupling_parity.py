@@ -86,19 +86,23 @@ def test_boundary_density_and_coupling_bitexact(gpu):
     c.close()
 
 
-def test_coupling_with_stacked_fluid_bitexact(gpu):
+@pytest.mark.parametrize("huge_levels", [6, 0])
+def test_coupling_with_stacked_fluid_bitexact(gpu, huge_levels):
     """Coupling forces when runs of identical fluid particles touch the bodies
     (two stacks of 1200 and 400 particles on one point each, under the two
     spheres): the coupling walks the fluid candidates run by run and adds a
     run's term k times over, which must equal the oracle's one-by-one sums bit
-    for bit, in a single context and in a batch."""
+    for bit, in a single context and in a batch.  huge_levels 0: every class
+    spanning two or more blocks (the 1200-particle stack's among them) ranks
+    in index space in the step's sort and in the coupling's fluid and
+    boundary sorts."""
     sc, local, body, poses, static = _scene_with_bodies()
     pos = sc.positions.copy()
     pos[:1200] = np.array([0.35, 0.28, 0.40], np.float32)
     pos[1200:1600] = np.array([0.62, 0.18, 0.55], np.float32)
     vel = np.zeros_like(pos)
     h = sc.params.kernel_radius
-    _say("case: coupling with stacked fluid n=", sc.n, "nb=", local.shape[0])
+    _say("case: coupling with stacked fluid n=", sc.n, "nb=", local.shape[0], "huge_levels", huge_levels)
     wpos = O.boundary_world_positions(local, body, poses)
     psi = O.boundary_volumes(wpos, h)
     op, ov, orho, ol = O.pbf_run(sc.params, pos, vel, sc.dt, sc.gravity, sc.mass, 1)
@@ -108,6 +112,7 @@ def test_coupling_with_stacked_fluid_bitexact(gpu):
     assert np.unique(op[:1200], axis=0).shape[0] == 1  # the stack stays one point
 
     c = capi.Context(sc.params, capacity=sc.n)
+    c.set_rank_split(huge_levels)
     c.set_particles(pos, vel)
     c.set_boundary(local, body, psi, nbodies=poses.shape[0])
     c.boundary_world_positions(poses)
@@ -123,6 +128,7 @@ def test_coupling_with_stacked_fluid_bitexact(gpu):
 
     slots = ((sc.n + 255) // 256) * 256
     b = capi.Batch(slot_capacity=slots, cell_capacity=capi.scene_bytes(sc.params, 1), max_scenes=2)
+    b.set_rank_split(huge_levels)
     sid = b.add(sc.params, sc.dt, sc.gravity, sc.mass, pos, vel)
     b.set_boundary(sid, local, body, psi, poses.shape[0], static)
     b.step(1)
