@@ -1889,6 +1889,7 @@ __device__ __forceinline__ void commit_finish(const SceneC& S, int q, float4 xi,
 #endif
 // LDS row stride of a dense sweep: rows start 4 banks apart.
 #define NPX_DENSE_RS 68
+static_assert(NPX_DENSE_RS % 4 == 0 && NPX_DENSE_RS >= 64 + 4, "16-byte rows with a four-entry tail");
 
 // LDS floats per wave of a dense sweep with NT accumulators: the term rows,
 // their run counts and the per-member term counts.
@@ -1958,7 +1959,10 @@ __device__ __forceinline__ void dense_push(const Skin& K, int q, int l) {
 // (its position in .xi), and fin(q, a) runs on member q's own lane with its
 // NT sums.  Each 64-candidate chunk writes, per member, the terms of its
 // in-range candidates compacted in candidate order into one padded LDS row
-// per accumulator; lane p * NT + u then adds row (p, u) in order.
+// per accumulator; lane p * NT + u then adds row (p, u) in order.  In a
+// range without runs it reads four entries per 16-byte LDS read: each row's
+// tail after its last term holds four -0.0f, and x + -0.0f is x for every
+// float x.
 template <int NT, class L, class MK, class FIN>
 __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int q0, bool rs, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
@@ -1992,19 +1996,21 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
       }
     }
     float acc = 0.0f;
-    // one chunk: m <= 64 slots from c, or m runs from piece c (runs: iend is
-    // the range's end piece and e its end slot)
-    auto chunk = [&](int c, int m, bool runs, int iend, int e) {
+    using D = decltype(ld(0));
+    // lane t's candidate of a chunk: m <= 64 slots from c, or m runs from
+    // piece c (runs: iend is the range's end piece and e its end slot; k the
+    // run's length)
+    auto cand = [&](int c, int m, bool runs, int iend, int e, int& k) -> int {
+      const int i = c + (t < m ? t : 0);
+      k = 1;
+      if (!runs) return i;
+      const int j = K.ps[i];
+      k = ((i + 1 < iend) ? K.ps[i + 1] : e) - j;
+      return j;
+    };
+    // one chunk of m candidates, lane t's candidate j (run length k, data d)
+    auto chunk = [&](int m, int j, int k, const D& d, bool runs) {
       const bool valid = t < m;
-      int j, k = 1;
-      if (runs) {
-        const int i = c + (valid ? t : 0);
-        j = K.ps[i];
-        k = ((i + 1 < iend) ? K.ps[i + 1] : e) - j;
-      } else {
-        j = c + (valid ? t : 0);
-      }
-      const auto d = ld(j);
       const float4 xj = cand_pos(d);
 #pragma unroll
       for (int p = 0; p < GB; ++p) {
@@ -2016,6 +2022,7 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
           for (int u = 0; u < NT; ++u) a[u] = 0.0f;
           if (in) P(j, d, 1, a);
           const unsigned long long mask = __ballot(in);
+          const int cnt = (int)__builtin_popcountll(mask);
           if (in) {
             const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                            __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
@@ -2023,7 +2030,11 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
             for (int u = 0; u < NT; ++u) tl[(p * NT + u) * RS + pos] = a[u];
             kl[p * RS + pos] = k;
           }
-          if (t == 0) cl[p] = (int)__builtin_popcountll(mask);
+          if (t >= cnt && t < cnt + 4) {  // the rows' tails (RS >= 68 entries)
+#pragma unroll
+            for (int u = 0; u < NT; ++u) tl[(p * NT + u) * RS + t] = -0.0f;
+          }
+          if (t == 0) cl[p] = cnt;
         }
       }
       npx_wave_lds_sync();
@@ -2031,17 +2042,29 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
         const float* row = tl + (ps * NT + us) * RS;
         const int* kr = kl + ps * RS;
         const int n = cl[ps];
-        for (int i = 0; i < n; ++i) add_k(acc, row[i], kr[i]);
+        if (runs) {
+          for (int i = 0; i < n; ++i) add_k(acc, row[i], kr[i]);
+        } else {  // no runs in the range: every count is 1
+          for (int i = 0; i < n; i += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(row + i);
+            acc = acc + v.x;
+            acc = acc + v.y;
+            acc = acc + v.z;
+            acc = acc + v.w;
+          }
+        }
       }
       npx_wave_lds_sync();
     };
+    // a range's chunks in order
     auto range = [&](int b, int e, bool runs) {
       if (b >= e) return;
-      if (runs) {
-        const int iend = K.pi[e - 1] + 1;
-        for (int c = K.pi[b]; c < iend; c += 64) chunk(c, min(64, iend - c), true, iend, e);
-      } else {
-        for (int c = b; c < e; c += 64) chunk(c, min(64, e - c), false, 0, e);
+      const int c0 = runs ? K.pi[b] : b;
+      const int cend = runs ? K.pi[e - 1] + 1 : e;
+      for (int c = c0; c < cend; c += 64) {
+        int k;
+        const int j = cand(c, min(64, cend - c), runs, cend, e, k);
+        chunk(min(64, cend - c), j, k, ld(j), runs);
       }
     };
     for (int r = 0; r < NPX_ROWS; ++r) {
@@ -2191,7 +2214,8 @@ __device__ __forceinline__ void dense_density(int w0, int ws, const SceneC* __re
                                               const int* __restrict__ cstart, const int* __restrict__ cell,
                                               float4* __restrict__ XS, float* __restrict__ RHO,
                                               float* __restrict__ LAM, int iter, const Skin& K) {
-  extern __shared__ float npx_dyn[];
+  extern __shared__ float4 npx_dyn4[];  // 16-byte aligned: the dense rows are read as float4
+  float* npx_dyn = reinterpret_cast<float*>(npx_dyn4);
   for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (iter >= S.iters) continue;
@@ -2274,7 +2298,8 @@ __device__ __forceinline__ void dense_delta(int w0, int ws, const SceneC* __rest
                                             const float4* __restrict__ XS, float4* __restrict__ XSn, int iter,
                                             const float4* __restrict__ X, const float* __restrict__ RHO,
                                             float4* __restrict__ VT, const Skin& K) {
-  extern __shared__ float npx_dyn[];
+  extern __shared__ float4 npx_dyn4[];  // 16-byte aligned: the dense rows are read as float4
+  float* npx_dyn = reinterpret_cast<float*>(npx_dyn4);
   for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (iter >= S.iters) continue;  // the pass-through is the per-lane kernel's
@@ -2336,7 +2361,8 @@ __device__ __forceinline__ void dense_omega(int w0, int ws, const SceneC* __rest
                                             const int* __restrict__ cstart, const int* __restrict__ cell,
                                             const float4* __restrict__ XS, const float4* __restrict__ VT,
                                             float4* __restrict__ OM, float4* __restrict__ VX, const Skin& K) {
-  extern __shared__ float npx_dyn[];
+  extern __shared__ float4 npx_dyn4[];  // 16-byte aligned: the dense rows are read as float4
+  float* npx_dyn = reinterpret_cast<float*>(npx_dyn4);
   for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (!S.xsph_on && !S.vort_on) continue;  // no neighbour sum: the per-lane kernel's
@@ -2389,7 +2415,8 @@ __device__ __forceinline__ void dense_commit(int w0, int ws, const SceneC* __res
                                              const float4* __restrict__ XS, const float4* __restrict__ VT,
                                              const float4* __restrict__ OM, const float4* __restrict__ VX,
                                              float4* __restrict__ X, float4* __restrict__ V, const Skin& K) {
-  extern __shared__ float npx_dyn[];
+  extern __shared__ float4 npx_dyn4[];  // 16-byte aligned: the dense rows are read as float4
+  float* npx_dyn = reinterpret_cast<float*>(npx_dyn4);
   for (int w = w0; w < *K.dn; w += ws) {
     NPX_DENSE_PROLOGUE();
     if (!S.vort_on) continue;  // commit only: the per-lane kernel's
