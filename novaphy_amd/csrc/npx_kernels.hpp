@@ -1963,11 +1963,12 @@ __device__ __forceinline__ void dense_push(const Skin& K, int q, int l) {
 // range without runs it reads four entries per 16-byte LDS read: each row's
 // tail after its last term holds four -0.0f, and x + -0.0f is x for every
 // float x.
+// sel >= 0: only member sel of the batch (another wave takes the others).
 template <int NT, class L, class MK, class FIN>
 __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int q0, bool rs, int l, int wxy,
                                             const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                             const int* __restrict__ cell, float cut, float* __restrict__ lds, L&& ld,
-                                            MK&& make, FIN&& fin) {
+                                            MK&& make, FIN&& fin, int sel = -1) {
   constexpr int GB = NPX_DENSE_BATCH;
   constexpr int RS = NPX_DENSE_RS;
   static_assert(GB * NT <= 64, "one summing lane per row");
@@ -1990,10 +1991,16 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
       if (left) {
         const int b = (int)__builtin_ctzll(left);
         qs[p] = wb + b;
-        bm |= 1ull << b;
+        if (sel < 0 || sel == p) bm |= 1ull << b;
         left &= left - 1ull;
         ++nb;
       }
+    }
+    if (sel >= 0) {  // uniform
+      if (sel >= nb) return;
+#pragma unroll
+      for (int p = 1; p < GB; ++p) qs[0] = (p == sel) ? qs[p] : qs[0];
+      nb = 1;
     }
     float acc = 0.0f;
     using D = decltype(ld(0));
@@ -2111,7 +2118,7 @@ __device__ __forceinline__ void dense_group(const SceneC& S, const Skin& K, int 
 __device__ __forceinline__ void dense_density(int, int, const SceneC* __restrict__, const int* __restrict__,
                                               const int* __restrict__, const int* __restrict__,
                                               const int* __restrict__, float4* __restrict__, float* __restrict__,
-                                              float* __restrict__, int, const Skin&);
+                                              float* __restrict__, int, const Skin&, bool split = false);
 __device__ __forceinline__ void dense_delta(int, int, const SceneC* __restrict__, const int* __restrict__,
                                             const int* __restrict__, const int* __restrict__, const int* __restrict__,
                                             const float4* __restrict__, float4* __restrict__, int,
@@ -2208,29 +2215,35 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 }
 
 // k_density_lambda for the dense list, batches w0, w0 + ws, ... (iterations
-// of the exact path).
+// of the exact path).  split: w counts (batch, member) pairs, one member of
+// a batch per wave.
 __device__ __forceinline__ void dense_density(int w0, int ws, const SceneC* __restrict__ scenes,
                                               const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                                               const int* __restrict__ cstart, const int* __restrict__ cell,
                                               float4* __restrict__ XS, float* __restrict__ RHO,
-                                              float* __restrict__ LAM, int iter, const Skin& K) {
+                                              float* __restrict__ LAM, int iter, const Skin& K, bool split) {
   extern __shared__ float4 npx_dyn4[];  // 16-byte aligned: the dense rows are read as float4
   float* npx_dyn = reinterpret_cast<float*>(npx_dyn4);
-  for (int w = w0; w < *K.dn; w += ws) {
+  const int sh = split ? (NPX_DENSE_BATCH == 2 ? 1 : 0) : 0;  // (batch, member) pairs when splitting
+  for (int wm = w0; wm < (*K.dn << sh); wm += ws) {
+    const int w = wm >> sh;
     NPX_DENSE_PROLOGUE();
     if (iter >= S.iters) continue;
     dense_group<5>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<5>(), [&](int j) { return XS[j]; },
                    [&](int qm) { return DensityPair<false>(S, qm, XS[qm]); },
-                   [&](int qm, const float* a) { density_finish(S, qm, a, XS, RHO, LAM); });
+                   [&](int qm, const float* a) { density_finish(S, qm, a, XS, RHO, LAM); }, sh ? (wm & 1) : -1);
   }
 }
 
-// The build pass's dense list (the first iteration), one wave per block.
+// The build pass's dense list (the first iteration), one wave per block and
+// one member of a batch per wave: the launch runs alone and its end is set
+// by its longest sweeps, which one member per wave shortens.
 __global__ void __launch_bounds__(64)
 k_density_dense(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
                 const int* __restrict__ cstart, const int* __restrict__ cell, float4* __restrict__ XS,
                 float* __restrict__ RHO, float* __restrict__ LAM, int iter, Skin K) {
-  dense_density((int)blockIdx.x, (int)gridDim.x, scenes, blk_scene, clsmap, cstart, cell, XS, RHO, LAM, iter, K);
+  dense_density((int)blockIdx.x, (int)gridDim.x, scenes, blk_scene, clsmap, cstart, cell, XS, RHO, LAM, iter, K,
+                true);
 }
 
 // compute_delta_position (pbf_solver.cpp:181-223) with s_corr, fused with the
