@@ -237,12 +237,15 @@ int npx_batch_step(npx_ctx* ctx, int32_t nsteps);
 /* Launch tunables of a context (any context kind): block size of the
  * neighbour passes (64, 128 or 256; default 128), the XCD-aware block order
  * (1, the default: each of the 8 XCDs sweeps one contiguous run of
- * particles, keeping shared neighbour rows in its own L2; 0: hardware
- * round-robin order) and the skin of the per-step candidate lists as a
- * fraction of the kernel radius ([0.02, 0.5], default 0.1: a larger skin
- * keeps the lists valid through larger per-step motion, a smaller one
- * visits fewer candidates).  Results are bitwise independent of all three;
- * only speed changes.  No reference counterpart (used by the autotuner). */
+ * particles, keeping shared neighbour rows in its own L2, or, in a step
+ * whose computing particles fall unevenly over those runs, chunks of 8192
+ * slots dealt to the XCDs in turn; C in [2, 4096]: always chunks of C
+ * blocks; 0: hardware round-robin order) and the skin of the per-step
+ * candidate lists as a fraction of the kernel radius ([0.02, 0.5], default
+ * 0.1: a larger skin keeps the lists valid through larger per-step motion, a
+ * smaller one visits fewer candidates).  Results are bitwise independent of
+ * all three; only speed changes.  No reference counterpart (used by the
+ * autotuner). */
 int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float skin_fraction);
 
 /* Persistent neighbour passes: each pass launches at most `blocks` per-lane

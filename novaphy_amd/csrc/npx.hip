@@ -293,7 +293,7 @@ void destroy(Ctx* c) {
   npx::DBuf<float>* f1[] = {&c->RHO, &c->RHO2, &c->LAM, &c->LAM2, &c->CDB, &c->PSI, &c->POSE, &c->OUT6};
   for (auto* b : f1) b->release();
   npx::DBuf<int>* i1[] = {&c->ID, &c->ID2, &c->CELL, &c->CELL2, &c->CLS, &c->OFF, &c->TPERM, &c->TKEY,
-                          &c->PERM, &c->MPERM, &c->MKEY, &c->HIDP, &c->HCNT, &c->HUGE, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
+                          &c->PERM, &c->MPERM, &c->MKEY, &c->HIDP, &c->HCNT, &c->HUGE, &c->XORD, &c->SKL, &c->SKN, &c->EXL, &c->EXN, &c->CMAX, &c->clsmap, &c->cstart, &c->bsum, &c->htv, &c->d_blk_scene, &c->d_flags,
                           &c->BODY, &c->BODY_PTR, &c->BODY_IDX, &c->Acstart, &c->ACLS, &c->AOFF, &c->ATPERM,
                           &c->ATKEY, &c->APERM, &c->AMPERM, &c->AMKEY, &c->aux_clsmap, &c->CRF, &c->CRPI,
                           &c->CRPS};
@@ -1142,7 +1142,7 @@ int npx_set_tuning(npx_ctx* ctx, int32_t block_size, int32_t xcd_remap, float sk
     if (!c) return NPX_EINVAL;
     if (block_size != 64 && block_size != 128 && block_size != 256)
       return c->fail(NPX_EINVAL, "block_size must be 64, 128 or 256");
-    if (xcd_remap != 0 && xcd_remap != 1) return c->fail(NPX_EINVAL, "xcd_remap must be 0 or 1");
+    if (xcd_remap < 0 || xcd_remap > 4096) return c->fail(NPX_EINVAL, "xcd_remap must be in [0, 4096]");
     if (!(skin_fraction >= 0.02f && skin_fraction <= 0.5f))
       return c->fail(NPX_EINVAL, "skin_fraction must be in [0.02, 0.5]");
     NPX_HIP_C(c, hipStreamSynchronize(c->stream));

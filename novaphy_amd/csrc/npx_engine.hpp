@@ -260,6 +260,7 @@ struct Ctx {
   // member count (then offset); the step's huge-class count, scenes and slot ranges
   DBuf<uint8_t> HMARK;
   DBuf<int> HIDP, HCNT, HUGE;
+  DBuf<int> XORD;  // the step's XCD order of the neighbour passes (k_pieces)
   DBuf<int> SKL, SKN;  // per-step skin lists (NPX_SKIN_CAP per slot) and counts
   DBuf<int> EXL, EXN;  // in-range lists of the current testing pass and counts
   DBuf<float> CDB;  // summed displacement per slot since the skin lists were built
@@ -825,6 +826,8 @@ struct Ctx {
     NPX_HIP(HIDP.ensure(n));
     NPX_HIP(HCNT.ensure((size_t)NPX_HUGE_MAX * (n / NPX_BLOCK)));
     NPX_HIP(HUGE.ensure(1 + 3 * NPX_HUGE_MAX));
+    NPX_HIP(XORD.ensure(1));
+    NPX_HIP(hipMemsetAsync(XORD.p, 0, sizeof(int), stream));  // k_pieces sets it every step
     NPX_HIP(SKL.ensure(n * NPX_SKIN_CAP));
     NPX_HIP(SKN.ensure(n));
     NPX_HIP(EXL.ensure(n * NPX_SKIN_CAP));
@@ -1146,7 +1149,7 @@ struct Ctx {
                    (unsigned*)CMAX.p + ncls, CMAX.p + 2 * ncls, BOFF.p));
     NPX_TRY(scan(BOFF.p, blocks + 1));
     NPX_TRY(launch("pieces", k_pieces, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p, (const int*)d_blk_scene.p,
-                   (const unsigned char*)RSTART.p, (const int*)BOFF.p, PI.p, PS.p));
+                   (const unsigned char*)RSTART.p, (const int*)BOFF.p, PI.p, PS.p, XORD.p));
     if (carry_rho) {
       NPX_TRY(launch("gather_rho", k_gather_scalar2, blocks, NPX_BLOCK, (const SceneC*)d_scenes.p,
                      (const int*)d_blk_scene.p, (const int*)PERM.p, (const float*)RHO.p, (const float*)LAM.p, RHO2.p,
@@ -1244,8 +1247,8 @@ struct Ctx {
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);
     if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes));
-    const BlockPlan rm{xcd_remap, 0, nvb};
-    const BlockPlan rd{xcd_remap, nd, nvb};
+    const BlockPlan rm{xcd_remap, 0, nvb, (const int*)XORD.p};
+    const BlockPlan rd{xcd_remap, nd, nvb, (const int*)XORD.p};
     int cur = 0;
     const int npass = std::max(imax, 1);
     NPX_TRY(persist ? enqueue_passes<true>(npass, blocks, nd, dense_blocks, lds, R, rm, rd, cur)
@@ -1290,7 +1293,7 @@ struct Ctx {
             (const void*)(intptr_t)(vort * 2 + carry), lo(X.p, X2.p), hi(X.p, X2.p), lo(V.p, V2.p),
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
-            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, HMARK.p, HIDP.p, HCNT.p, HUGE.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
+            d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, HMARK.p, HIDP.p, HCNT.p, HUGE.p, XORD.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
             (const void*)(intptr_t)persist_blocks, (const void*)(intptr_t)huge_levels};
   }
 

@@ -743,11 +743,27 @@ k_gather(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, c
 // The runs of a slot range [b, e) are pieces pi[b] .. pi[e - 1] (b is always
 // a run start: a class start differs from the slot before it in cell), the
 // last ending at e.
+//
+// Block 0 also sets the step's XCD order (order[0]): the run heads, the
+// particles that compute in the neighbour passes, per eighth of the slot
+// range; when the fullest eighth holds more than 5/4 of the mean, one
+// contiguous share per XCD would leave most XCDs idle (a collapsed scene
+// keeps its heads in a few places), and the passes deal chunks to the XCDs
+// in turn instead (1); else contiguous shares (0).
 __global__ void __launch_bounds__(NPX_BLOCK)
 k_pieces(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene,
          const unsigned char* __restrict__ rstart, const int* __restrict__ boff, int* __restrict__ pi,
-         int* __restrict__ ps) {
+         int* __restrict__ ps, int* __restrict__ order) {
   __shared__ unsigned long long smask[NPX_BLOCK / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int nb = (int)gridDim.x;
+    int mx = 0;
+    for (int x = 0; x < 8; ++x) {
+      const int a = (int)((long long)nb * x / 8), b = (int)((long long)nb * (x + 1) / 8);
+      mx = max(mx, boff[b] - boff[a]);
+    }
+    order[0] = (32LL * mx > 5LL * boff[nb]) ? 1 : 0;
+  }
   const int s = blk_scene[blockIdx.x];
   if (s < 0) return;  // uniform per block
   const SceneC& S = scenes[s];
@@ -803,10 +819,14 @@ k_gather_scalar2(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
 // candidate rows its blocks share stay in its L2.  Results do not depend on
 // the order: no pass reads what another block of the same launch writes.
 struct BlockPlan {
-  int remap;  // 1: XCD order; 0: hardware order
+  int remap;  // 1: XCD order (contiguous shares or chunks, per *order); 0: hardware order; C >= 2: chunks of C
   int nd;     // leading blocks that sweep the dense list (a multiple of 8)
   int nvb;    // particle blocks of the pass
+  const int* order;  // this step's XCD order (k_pieces): 0 contiguous shares, 1 chunks
 };
+
+// Slots per chunk of the chunked XCD order.
+#define NPX_XCD_CHUNK_SLOTS 8192
 
 // The particle block of a neighbour-pass launch block (after the nd dense
 // blocks, which keep XCD i mod 8).
@@ -814,7 +834,17 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
   const int i = (int)blockIdx.x - bp.nd;
   if (!bp.remap) return i;
   const int g = (int)gridDim.x - bp.nd, x = i & 7;
-  return x * (g >> 3) + min(x, g & 7) + (i >> 3);
+  int C = bp.remap;
+  if (C == 1) {
+    if (*bp.order == 0) return x * (g >> 3) + min(x, g & 7) + (i >> 3);
+    C = max(NPX_XCD_CHUNK_SLOTS / (int)blockDim.x, 1);
+  }
+  // chunks of C consecutive blocks dealt to the XCDs in turn (the tail past
+  // the last whole round in hardware order)
+  const int full = g - g % (8 * C);
+  if (i >= full) return i;
+  const int j = i >> 3;
+  return ((j / C) * 8 + x) * C + (j % C);
 }
 
 // Per-scene list statistics (Skin::lst): the testing passes count the lanes
@@ -867,7 +897,7 @@ __device__ __forceinline__ void npx_for_blocks(const BlockPlan& bp, F&& f) {
     return;
   }
   const int i = (int)blockIdx.x - bp.nd, g = (int)gridDim.x - bp.nd, V = bp.nvb;
-  if (!bp.remap) {
+  if (bp.remap != 1) {  // (chunked orders: hardware order)
     for (int lb = i; lb < V; lb += g) f(lb);
     return;
   }

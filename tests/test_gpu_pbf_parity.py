@@ -171,18 +171,20 @@ def test_tunables_do_not_change_results(gpu):
     oracle.  The 8000 particles (32 segments of 256 slots) make 128 / 64 / 32
     particle blocks at block sizes 64 / 128 / 256, so persistent counts of
     8-24 workgroups loop over several particle blocks each (in XCD order and
-    in hardware order)."""
+    in hardware order), and chunks of 3 / 2 / 5 blocks dealt to the XCDs
+    cover whole rounds with a tail, exactly, and none."""
     sc = scenes.dam_break_8k()
     prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
     sc = dataclasses.replace(sc, params=prm)
     _say("case: tunables (block size / XCD order / skin / persistent) rollout")
     c = _ctx(sc)
     for k, (b, remap, skin, pers) in enumerate([(64, 0, 0.02, 8), (128, 1, 0.1, 0), (256, 1, 0.5, 16),
-                                                (64, 1, 0.05, 24), (128, 0, 0.3, 8), (256, 0, 0.1, 0)] * 2):
+                                                (64, 1, 0.05, 24), (128, 0, 0.3, 8), (256, 0, 0.1, 0),
+                                                (64, 3, 0.1, 0), (128, 2, 0.2, 0), (256, 5, 0.05, 8)] * 2):
         c.set_tuning(b, remap, skin, pers)
         c.step(sc.dt, sc.gravity, sc.mass, 1)
     gp, gv, gr, gl = c.get_particles()
-    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 12)
+    op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 18)
     np.testing.assert_array_equal(gp, op)
     np.testing.assert_array_equal(gv, ov)
     np.testing.assert_array_equal(gr, orho)
