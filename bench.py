@@ -304,18 +304,18 @@ def run_batch(count, steps, warmup, flags, loop, s_corr="configured"):
     return out
 
 
-def coupled_world(seed=2):
+def coupled_world(seed=2, flags=0):
     """The rigid-fluid coupling workload (scenes.coupled_500k_world)."""
     from novaphy_amd import scenes
-    return scenes.coupled_500k_world(seed)
+    return scenes.coupled_500k_world(seed, flags)
 
 
-def run_coupled(steps, warmup, phase_steps=100):
+def run_coupled(steps, warmup, phase_steps=100, flags=0):
     """FluidWorld.step over the coupled workload: the timed steps run with the
     performance monitor off (the PBF step replays its graph); then
     phase_steps more steps with the monitor on (per-launch HIP events, stream
     launches, a synchronisation after the PBF step) give the phase averages."""
-    world, sc = coupled_world()
+    world, sc = coupled_world(flags=flags)
     log(f"[bench] coupled: {sc.n} fluid particles, {world.num_boundary_particles} boundary particles, 64 bodies")
     dt = np.float32(sc.dt)
     for _ in range(warmup):
@@ -544,7 +544,8 @@ def main():
         print(json.dumps(line), flush=True)
         return
     if args.workload == "coupled":
-        r = run_coupled(args.steps, args.warmup)
+        r = run_coupled(args.steps, args.warmup,
+                        flags=capi.FLAG_NO_GRAPH if args.no_graph else 0)
         line = {**common, "value": r["value"], "steps": args.steps, "warmup": args.warmup,
                 "ms_per_step": r["ms_per_step"],
                 "config": {"workload": f"coupled_500k: {r['particles']} fluid particles + 64 rigid bodies "

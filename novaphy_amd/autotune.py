@@ -1,9 +1,11 @@
 """Autotuner for the launch tunables of a device context.
 
 The tunables are the block size of the neighbour passes (64 / 128 / 256), the
-XCD-aware block order (each of the 8 XCDs sweeps one contiguous run of
-particles so the neighbour rows its blocks share stay in its own L2, or the
-hardware's round-robin order), the skin of the per-step candidate lists
+XCD-aware block order (1: each of the 8 XCDs sweeps one contiguous run of
+particles so the neighbour rows its blocks share stay in its own L2, or, in
+steps whose computing particles fall unevenly over those runs, chunks dealt
+to the XCDs in turn; C >= 2: always chunks of C blocks; 0: the hardware's
+round-robin order), the skin of the per-step candidate lists
 (a fraction of h: larger keeps the lists valid through more motion per step,
 smaller visits fewer candidates) and persistent passes (at most that many
 per-lane workgroups per neighbour pass, each looping over particle blocks;
@@ -27,8 +29,8 @@ import numpy as np
 
 Candidate = Tuple[int, int, float, int]
 DEFAULT_CANDIDATES: Tuple[Candidate, ...] = (
-    (256, 1, 0.1, 0), (128, 1, 0.1, 0), (256, 0, 0.1, 0), (256, 1, 0.05, 0), (256, 1, 0.2, 0),
-    (256, 1, 0.1, 4096), (256, 1, 0.1, 8192))
+    (128, 1, 0.05, 0), (128, 1, 0.1, 0), (256, 1, 0.05, 0), (64, 1, 0.05, 0), (128, 64, 0.05, 0),
+    (128, 0, 0.05, 0), (128, 1, 0.05, 4096))
 
 
 def _norm(c) -> Candidate:

@@ -264,7 +264,7 @@ def served_request(k: int, rng: np.random.Generator, lo: int = 4000, hi: int = 2
     return sc
 
 
-def coupled_500k_world(seed: int = 2):
+def coupled_500k_world(seed: int = 2, flags: int = 0):
     """The rigid-fluid coupling workload as a FluidWorld: the 500k-particle
     block (100x50x100) in the 10x5x5 m box with the dam-break parameters, a
     ground plane sampled with boundary particles over the box, and 64 rigid
@@ -295,7 +295,7 @@ def coupled_500k_world(seed: int = 2):
             i = b.add_body(nv.RigidBody.from_box(float(dens * 8.0 * np.prod(he)), he), t)
             b.add_shape(nv.CollisionShape.make_box(he, i))
     world = nv.FluidWorld(b.build(), [empty_block()], nv.SolverSettings(), pbf_settings(sc.params),
-                          boundary_extent=10.0)
+                          boundary_extent=10.0, flags=flags)
     world.fluid_state.init_arrays(sc.positions, sc.velocities)
     return world, sc
 
