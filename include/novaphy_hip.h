@@ -263,6 +263,15 @@ int npx_set_persistent(npx_ctx* ctx, int32_t blocks);
  * exercise the index-space rank on small scenes). */
 int npx_set_rank_split(npx_ctx* ctx, int32_t huge_levels);
 
+/* Skin lists: 0 (the default) builds none, and every testing pass sweeps the
+ * 27 cells inline; 1 builds per-step skin lists in the steps where at least
+ * a quarter of the testing lanes sampled in the previous step found their
+ * list bound holding, and the testing passes walk them where the bound
+ * holds.  Results are bitwise independent of it; only speed changes (with
+ * the inline sweep of this library, lists measured slower on every workload
+ * tried, moving or at rest).  No reference counterpart (a launch tunable). */
+int npx_set_list_policy(npx_ctx* ctx, int32_t policy);
+
 /* npx_batch_step with a device timestamp after every step: step_ms[k] is the
  * duration of step k between consecutive HIP events on the context's stream
  * (may be NULL).  Blocks until the steps are done. */

@@ -38,7 +38,7 @@ SYMBOLS = (
     "npx_apply_boundary_density", "npx_coupling_forces", "npx_batch_create", "npx_scene_add",
     "npx_scene_remove", "npx_scene_get", "npx_batch_step", "npx_scene_bytes",
     "npx_set_kernel_timing", "npx_kernel_times", "npx_kernel_trace", "npx_batch_step_timed", "npx_scene_status",
-    "npx_set_tuning", "npx_set_persistent", "npx_set_rank_split", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
+    "npx_set_tuning", "npx_set_persistent", "npx_set_rank_split", "npx_set_list_policy", "npx_scene_set_boundary", "npx_batch_num_bodies", "npx_batch_coupling",
     "npx_sample_boundaries", "npx_grid_query_noalloc",
 )
 
@@ -143,6 +143,7 @@ def load() -> C.CDLL:
         "npx_set_tuning": (C.c_int, [vp, i32, i32, f32]),
         "npx_set_persistent": (C.c_int, [vp, i32]),
         "npx_set_rank_split": (C.c_int, [vp, i32]),
+        "npx_set_list_policy": (C.c_int, [vp, i32]),
         "npx_scene_set_boundary": (C.c_int, [vp, i32, vp, vp, vp, i32, i32, vp]),
         "npx_batch_num_bodies": (i32, [vp]),
         "npx_batch_coupling": (C.c_int, [vp, vp, i32, vp]),
@@ -319,6 +320,12 @@ class Context:
         by original index in index space instead of by merge launches (results
         are bitwise independent of it)."""
         self._c(self._L.npx_set_rank_split(self._h, int(huge_levels)))
+
+    def set_list_policy(self, policy: int = 0):
+        """0: no skin lists (every testing pass sweeps inline); 1: skin lists
+        in steps where the previous step's sampled list bounds mostly held
+        (results are bitwise independent of it)."""
+        self._c(self._L.npx_set_list_policy(self._h, int(policy)))
 
 
 def _kernel_times(L, h):
@@ -497,6 +504,12 @@ class Batch:
         by original index in index space instead of by merge launches (results
         are bitwise independent of it)."""
         self._c(self._L.npx_set_rank_split(self._h, int(huge_levels)))
+
+    def set_list_policy(self, policy: int = 0):
+        """0: no skin lists (every testing pass sweeps inline); 1: skin lists
+        in steps where the previous step's sampled list bounds mostly held
+        (results are bitwise independent of it)."""
+        self._c(self._L.npx_set_list_policy(self._h, int(policy)))
 
 
 def scene_bytes(params, n: int) -> int:

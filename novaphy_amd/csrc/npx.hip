@@ -1167,6 +1167,20 @@ int npx_set_persistent(npx_ctx* ctx, int32_t blocks) {
   });
 }
 
+int npx_set_list_policy(npx_ctx* ctx, int32_t policy) {
+  return guard(ctx, __func__, [&]() -> int {
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return NPX_EINVAL;
+    if (policy != 0 && policy != 1) return c->fail(NPX_EINVAL, "list policy must be 0 or 1");
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    c->list_policy = policy;
+    // no step has counted list statistics under the new policy yet
+    NPX_HIP_C(c, hipMemsetAsync(c->LST.p, 0, sizeof(int) * NPX_LST_STRIDE * (size_t)c->max_scenes, c->stream));
+    NPX_HIP_C(c, hipStreamSynchronize(c->stream));
+    return NPX_OK;
+  });
+}
+
 int npx_set_rank_split(npx_ctx* ctx, int32_t huge_levels) {
   return guard(ctx, __func__, [&]() -> int {
     Ctx* c = reinterpret_cast<Ctx*>(ctx);

@@ -244,6 +244,7 @@ struct Ctx {
   int nbr_block = 128;  // block size of the neighbour passes (64..NPX_MAX_BLOCK)
   int persist_blocks = 0;  // > 0: at most this many per-lane blocks per neighbour pass, each looping
   int huge_levels = NPX_HUGE_LEVELS;  // classes spanning more than 2^this blocks rank in index space
+  int list_policy = 0;  // 0: no skin lists (every testing pass sweeps inline); 1: skin lists where they held
   int xcd_remap = 1;    // 1: XCD-aware block order in the neighbour passes
   float skin_frac = 0.1f;  // skin radius of the per-step candidate lists, as a fraction of h
   std::vector<int> h_blk_scene;
@@ -1189,8 +1190,9 @@ struct Ctx {
       auto dens = fast ? (it == 0 ? k_density_lambda<true, true, true, PERSIST> : k_density_lambda<true, false, true, PERSIST>)
                        : (it == 0 ? k_density_lambda<false, true, true, PERSIST> : k_density_lambda<false, false, true, PERSIST>);
       if (it == 0 || fast) {
-        NPX_TRY(launch(it == 0 ? (fast ? "density_lambda_build" : "density_lambda_build_lists") : "density_lambda",
-                       dens, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
+        if (fast || list_policy == 1)  // (without the list policy no scene builds skin lists)
+          NPX_TRY(launch(it == 0 ? (fast ? "density_lambda_build" : "density_lambda_build_lists") : "density_lambda",
+                         dens, blocks, nbr_block, sc, bs, cm, cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
         if (it == 0 && !fast)  // the scenes that build no skin lists this step
           NPX_TRY(launch("density_lambda_build", k_density_lambda<false, true, false, PERSIST>, blocks, nbr_block, sc, bs, cm,
                          cst, cel, xs[cur]->p, RHO.p, LAM.p, it, R, rm));
@@ -1237,7 +1239,8 @@ struct Ctx {
     const int persist = (persist_blocks > 0 && persist_blocks < nvb) ? 1 : 0;
     const int blocks = persist ? persist_blocks : nvb;
     const Skin R{SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, (unsigned*)CMAX.p, RSTART.p, PI.p, PS.p,
-                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1, LST.p};
+                 (const unsigned*)CMAX.p + total_cls(), CMAX.p + 2 * total_cls(), DL.p, DN.p, fast ? 0 : 1, LST.p,
+                 (!fast && list_policy == 1) ? 1 : 0};
     // The build pass's dense list: one 64-lane block per batch, at most this
     // many at a time.  The later passes sweep the list in their first nd
     // launch blocks (a multiple of 8, for the XCD order of the rest), one
@@ -1246,7 +1249,7 @@ struct Ctx {
     const int wpb = nbr_block / 64;
     const int nd = fast ? 0 : cdiv(std::min(NPX_DENSE_WAVES, dense_blocks), wpb * 8) * 8;
     const size_t lds = (size_t)wpb * sizeof(float);
-    if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes));
+    if (!fast) NPX_TRY(launch("dense_reset", k_dense_reset, 1, 256, DN.p, LST.p, max_scenes, list_policy));
     const BlockPlan rm{xcd_remap, 0, nvb, (const int*)XORD.p};
     const BlockPlan rd{xcd_remap, nd, nvb, (const int*)XORD.p};
     int cur = 0;
@@ -1294,7 +1297,8 @@ struct Ctx {
             hi(V.p, V2.p), lo(ID.p, ID2.p), hi(ID.p, ID2.p), lo(CELL.p, CELL2.p), hi(CELL.p, CELL2.p),
             lo(RHO.p, RHO2.p), hi(RHO.p, RHO2.p), lo(LAM.p, LAM2.p), hi(LAM.p, LAM2.p), XS0.p, XS1.p, VT.p, OM.p, VX.p, clsmap.p, cstart.p, bsum.p,
             d_scenes.p, d_blk_scene.p, CLS.p, OFF.p, TPERM.p, TKEY.p, PERM.p, MPERM.p, MKEY.p, BIGB.p, HMARK.p, HIDP.p, HCNT.p, HUGE.p, XORD.p, SKL.p, SKN.p, EXL.p, EXN.p, CDB.p, CMAX.p, RSTART.p, PI.p, PS.p, BOFF.p, DL.p, DN.p, LST.p,
-            (const void*)(intptr_t)persist_blocks, (const void*)(intptr_t)huge_levels};
+            (const void*)(intptr_t)persist_blocks, (const void*)(intptr_t)huge_levels,
+            (const void*)(intptr_t)list_policy};
   }
 
   void drop_graphs() {

@@ -864,7 +864,7 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
 
 // Empties the step's dense list, which the build pass fills after it, and
 // turns the previous step's list statistics into this step's build flags.
-__global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int nscenes) {
+__global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int nscenes, int policy) {
   if (threadIdx.x == 0) *dn = 0;
   for (int s = (int)threadIdx.x; s < nscenes; s += (int)blockDim.x) {
     int* st = lst + (size_t)NPX_LST_STRIDE * s;
@@ -875,7 +875,7 @@ __global__ void k_dense_reset(int* __restrict__ dn, int* __restrict__ lst, int n
       st[16 * k] = 0;
       st[16 * k + 1] = 0;
     }
-    st[NPX_LST_BUILD] = (tested == 0 || 4 * held >= tested) ? 1 : 0;
+    st[NPX_LST_BUILD] = (policy == 1 && (tested == 0 || 4 * held >= tested)) ? 1 : 0;
   }
 }
 
@@ -963,6 +963,7 @@ struct Skin {
   int dense;                               // 1: dense particles go to the per-wave kernels
   int* __restrict__ lst;                   // [scenes][NPX_LST_STRIDE]: sampled counts of testing lanes and of
                                            // lanes whose list bound held, whether this step builds lists
+  int lstat;                               // 1: the testing passes sample those counts (list policy 1)
 };
 
 // Entry k of slot q's skin or in-range list.  Entries go in chunks of four
@@ -1596,7 +1597,7 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool 
   const int n = K.cnt[q];
   const bool held = K.cd[q] + C <= S.skin_allow;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  if (!FAST && K.dense && (wv & 7) == 0) {  // list statistics for the next step, sampled
+  if (!FAST && K.lstat && (wv & 7) == 0) {  // list statistics for the next step, sampled
     const unsigned long long tb = __ballot(1), hb = __ballot(held);
     if ((int)(threadIdx.x & 63) == (int)__builtin_ctzll(tb)) {
       int* c = st + 16 * ((wv >> 3) & (NPX_LST_COPIES - 1));

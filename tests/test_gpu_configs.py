@@ -286,8 +286,8 @@ def _state_hash(pos, vel, rho, lam):
     return h.hexdigest()
 
 
-@pytest.mark.parametrize("huge_levels", [6, 2])
-def test_config0_configured_200_steps_against_oracle_rollout(gpu, huge_levels):
+@pytest.mark.parametrize("huge_levels,lists", [(6, 0), (2, 1)])
+def test_config0_configured_200_steps_against_oracle_rollout(gpu, huge_levels, lists):
     """The 8000-particle dam break with the configured s_corr (k = 0.1,
     dq = 0.2h), the scene the reference's compute_delta_position drives into
     collapse: 200 GPU steps against the oracle's committed rollout
@@ -296,13 +296,15 @@ def test_config0_configured_200_steps_against_oracle_rollout(gpu, huge_levels):
     rollout, and the final state's hash equal (the default kernels are
     bitwise).  huge_levels 2: the corner stack (most of the 8000 particles in
     one class, up to 31 blocks) ranks in index space instead of by merge
-    launches, with the same hash."""
+    launches; with lists 1 the steps build skin lists where the previous
+    step's bounds mostly held; the same hash."""
     with open(os.path.join(HERE, "golden", "config0_configured_200.json")) as f:
         gold = json.load(f)
     sc = scenes.dam_break_8k()
-    _say("case: config 0 configured 200 steps, huge_levels", huge_levels)
+    _say("case: config 0 configured 200 steps, huge_levels", huge_levels, "lists", lists)
     c = capi.Context(sc.params, capacity=sc.n)
     c.set_rank_split(huge_levels)
+    c.set_list_policy(lists)
     c.set_particles(sc.positions, sc.velocities)
     m = float(F32(sc.mass))
     for row in gold["rows"]:

@@ -143,8 +143,9 @@ def test_coupling_with_stacked_fluid_bitexact(gpu, huge_levels):
 
 def test_batch_persistent_passes_equal_alone(gpu):
     """A batch stepped with persistent neighbour passes (16 workgroups per
-    pass, each looping over particle blocks of several scenes) matches each
-    scene alone at the default launch shape, bitwise."""
+    pass, each looping over particle blocks of several scenes) and skin lists
+    (list policy 1) matches each scene alone at the default launch shape,
+    bitwise."""
     dims = [(6, 9, 5), (12, 7, 10), (9, 9, 9), (20, 6, 8)]
     scs = [scenes.batched_scene(nx, ny, nz, 0.4, seed=200 + k, steps=6) for k, (nx, ny, nz) in enumerate(dims)]
     _say("case: persistent batch of", len(scs), "scenes, sizes", [s.n for s in scs])
@@ -152,6 +153,7 @@ def test_batch_persistent_passes_equal_alone(gpu):
     cells = sum(capi.scene_bytes(s.params, 1) for s in scs)  # generous upper bound on cells
     b = capi.Batch(slot_capacity=slots, cell_capacity=cells, max_scenes=8)
     b.set_tuning(128, 1, 0.1, 16)
+    b.set_list_policy(1)
     ids = [b.add(s.params, s.dt, s.gravity, s.mass, s.positions, s.velocities) for s in scs]
     b.step(6)
     for s, sid in zip(scs, ids):

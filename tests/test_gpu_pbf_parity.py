@@ -166,9 +166,10 @@ def test_grid_query_matches_oracle(gpu):
 
 
 def test_tunables_do_not_change_results(gpu):
-    """Block size, XCD block order, skin and persistent passes change speed
-    only: a rollout that switches tunables every step is bitwise equal to the
-    oracle.  The 8000 particles (32 segments of 256 slots) make 128 / 64 / 32
+    """Block size, XCD block order, skin, persistent passes and the skin-list
+    policy change speed only: a rollout that switches tunables every step is
+    bitwise equal to the oracle (steps with list policy 1 build skin lists,
+    their statistics reset by the switch, and walk them where they hold).  The 8000 particles (32 segments of 256 slots) make 128 / 64 / 32
     particle blocks at block sizes 64 / 128 / 256, so persistent counts of
     8-24 workgroups loop over several particle blocks each (in XCD order and
     in hardware order), and chunks of 3 / 2 / 5 blocks dealt to the XCDs
@@ -176,12 +177,13 @@ def test_tunables_do_not_change_results(gpu):
     sc = scenes.dam_break_8k()
     prm = dataclasses.replace(sc.params, corr_k=0.001, corr_delta_q=0.3)
     sc = dataclasses.replace(sc, params=prm)
-    _say("case: tunables (block size / XCD order / skin / persistent) rollout")
+    _say("case: tunables (block size / XCD order / skin / persistent / lists) rollout")
     c = _ctx(sc)
     for k, (b, remap, skin, pers) in enumerate([(64, 0, 0.02, 8), (128, 1, 0.1, 0), (256, 1, 0.5, 16),
                                                 (64, 1, 0.05, 24), (128, 0, 0.3, 8), (256, 0, 0.1, 0),
                                                 (64, 3, 0.1, 0), (128, 2, 0.2, 0), (256, 5, 0.05, 8)] * 2):
         c.set_tuning(b, remap, skin, pers)
+        c.set_list_policy(k % 3 != 0)
         c.step(sc.dt, sc.gravity, sc.mass, 1)
     gp, gv, gr, gl = c.get_particles()
     op, ov, orho, ol = O.pbf_run(sc.params, sc.positions, sc.velocities, sc.dt, sc.gravity, sc.mass, 18)
@@ -345,6 +347,7 @@ def test_crowded_sheet_tunables_bitexact(gpu):
     c.set_particles(pos, vel)
     for b, remap in ((64, 1), (256, 0), (128, 1)):
         c.set_tuning(b, remap, 0.1)
+        c.set_list_policy(1 if b != 256 else 0)
         c.step(dt, g, m, 1)
     gp, gv, gr, gl = c.get_particles()
     c.close()
