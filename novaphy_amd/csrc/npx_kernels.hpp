@@ -1594,7 +1594,7 @@ __device__ __forceinline__ void visit_test(const SceneC& S, const Skin& K, bool 
                                            const float4* __restrict__ XS, int l, int wxy,
                                            const int* __restrict__ cm, const int* __restrict__ cstart, int shift,
                                            bool record, int* __restrict__ st, L&& ld, F&& f) {
-  const int n = K.cnt[q];
+  const int n = (FAST || K.lstat) ? K.cnt[q] : -1;  // (no skin lists without the list policy)
   const bool held = K.cd[q] + C <= S.skin_allow;
   const int wv = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (!FAST && K.lstat && (wv & 7) == 0) {  // list statistics for the next step, sampled
@@ -2234,7 +2234,9 @@ k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_
         K.ecnt[q] = e;
       }
     } else {
-      const float C = candidate_disp(K, l, S.wx, wxy, cm);
+      // the list bound matters only where lists exist or their statistics
+      // are sampled (list policy 1, or the FAST path, which always builds them)
+      const float C = (FAST || K.lstat) ? candidate_disp(K, l, S.wx, wxy, cm) : 0.0f;
       visit_test<FAST, 4, !FAST>(S, K, rs, q, C, xi, cut, XS, l, wxy, cm, cstart, shift, true, K.lst + NPX_LST_STRIDE * s, ld, body);
     }
     // the slot's addresses are formed again here rather than kept live across
@@ -2386,7 +2388,7 @@ k_omega_xsph(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scen
       auto ld = [&](int j) { return XV{XS[j], VT[j]}; };
       auto body = [&](int j, const XV& d, int k) { P(j, d, k, a); };
       if (S.iters > 0) {  // the skin lists exist once a density pass has run
-        const float C = candidate_disp(K, l, S.wx, wxy, cm);
+        const float C = K.lstat ? candidate_disp(K, l, S.wx, wxy, cm) : 0.0f;
         // the vorticity force pass reuses the in-range list
         visit_test<false, 4, true>(S, K, rs, q, C, xi, P.cut, XS, l, wxy, cm, cstart, shift, vort, K.lst + NPX_LST_STRIDE * s, ld,
                                    body);
