@@ -1828,14 +1828,16 @@ __device__ __forceinline__ void velocity_update(const SceneC& S, int q, float px
 
 // Jacobi apply (pbf_solver.cpp:71-77) into the other x* buffer, the velocity
 // update on the last iteration, and the particle's summed displacement since
-// the lists were built (returned).
+// the lists were built (returned; only the skin lists read it, so without
+// them (track = false) it is not kept and 0 is returned).
 __device__ __forceinline__ float delta_finish(const SceneC& S, const Skin& K, int q, float4 xi, const float* a,
                                               float4* __restrict__ XSn, int iter, bool last,
                                               const float4* __restrict__ X, const float* __restrict__ RHO,
-                                              float4* __restrict__ VT) {
+                                              float4* __restrict__ VT, bool track = true) {
   const float px = xi.x + a[0] * S.inv_rho0, py = xi.y + a[1] * S.inv_rho0, pz = xi.z + a[2] * S.inv_rho0;
   XSn[q] = make_float4(px, py, pz, 0.0f);
   if (last) velocity_update(S, q, px, py, pz, X, RHO, VT);
+  if (!track) return 0.0f;
   float moved = sqrtf(sqn3(px - xi.x, py - xi.y, pz - xi.z));
   moved = (iter == 0 ? 0.0f : K.cd[q]) + moved;
   K.cd[q] = moved;
@@ -2318,8 +2320,9 @@ k_delta(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, co
       auto body = [&](int j, float4 xj, int k) { P(j, xj, k, a); };  // k > 1 only without FAST
       // one run / two slots at a time on the inline sweep: within 72 VGPRs
       visit_reuse<FAST, 4, !FAST, 1, 2>(S, K, rs, q, l, wxy, cm, cstart, shift, xi, P.cut, XS, ld, body);
-      moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT);
+      moved = delta_finish(S, K, q, xi, a, XSn, iter, last, X, RHO, VT, FAST || K.lstat);
     }
+    if (!FAST && !K.lstat) return;  // displacements feed only the skin lists' bound
     // Per-class maxima: slots are sorted by class, so the lanes of a class form
     // one run in the wave; a segmented max leaves the run's maximum in its first
     // lane, which alone does the atomic.  (The own class of a particle is the
@@ -2353,7 +2356,7 @@ __device__ __forceinline__ void dense_delta(int w0, int ws, const SceneC* __rest
     dense_group<3>(S, K, q, rs, l, wxy, cm, cstart, shift, cell, S.cut2, npx_dyn + (threadIdx.x >> 6) * npx_dense_lds_floats<3>(), [&](int j) { return XS[j]; },
                    [&](int qm) { return DeltaPair<false>(S, qm, XS[qm]); },
                    [&](int qm, const float* a) {
-                     const float moved = delta_finish(S, K, qm, XS[qm], a, XSn, iter, last, X, RHO, VT);
+                     const float moved = delta_finish(S, K, qm, XS[qm], a, XSn, iter, last, X, RHO, VT, K.lstat != 0);
                      if (moved > 0.0f) atomicMax(K.cmax + cm[l], __float_as_uint(moved));
                    });
   }
