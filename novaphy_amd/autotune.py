@@ -28,9 +28,10 @@ from typing import Dict, Iterable, Optional, Tuple
 import numpy as np
 
 Candidate = Tuple[int, int, float, int]
+# (the skin fraction matters only with skin lists on, npx_set_list_policy 1)
 DEFAULT_CANDIDATES: Tuple[Candidate, ...] = (
-    (128, 1, 0.05, 0), (128, 1, 0.1, 0), (256, 1, 0.05, 0), (64, 1, 0.05, 0), (128, 64, 0.05, 0),
-    (128, 0, 0.05, 0), (128, 1, 0.05, 4096))
+    (128, 1, 0.1, 0), (64, 1, 0.1, 0), (256, 1, 0.1, 0), (128, 64, 0.1, 0), (64, 128, 0.1, 0),
+    (128, 0, 0.1, 0), (128, 1, 0.1, 4096))
 
 
 def _norm(c) -> Candidate:
