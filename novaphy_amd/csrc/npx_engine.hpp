@@ -781,7 +781,8 @@ struct Ctx {
     NPX_HIP(d_scenes.ensure(max_scenes));
     NPX_HIP(d_flags.ensure(1 + max_scenes));
     NPX_HIP(LST.ensure(NPX_LST_STRIDE * (size_t)max_scenes));
-    NPX_HIP(hipMemsetAsync(LST.p, 0, sizeof(int) * NPX_LST_STRIDE * (size_t)max_scenes, stream));  // first step: build lists
+    // no list statistics yet (under list policy 1 the first step builds lists)
+    NPX_HIP(hipMemsetAsync(LST.p, 0, sizeof(int) * NPX_LST_STRIDE * (size_t)max_scenes, stream));
     NPX_HIP(hipHostMalloc((void**)&h_flags, sizeof(int) * (1 + max_scenes), hipHostMallocDefault));
     NPX_HIP(hipMemsetAsync(d_flags.p, 0, sizeof(int) * (1 + max_scenes), stream));
     NPX_TRY(reserve_slots(slots));

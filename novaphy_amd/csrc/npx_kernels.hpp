@@ -847,9 +847,11 @@ __device__ __forceinline__ int npx_lblock(const BlockPlan& bp) {
   return ((j / C) * 8 + x) * C + (j % C);
 }
 
-// Per-scene list statistics (Skin::lst): the testing passes count the lanes
-// that test and those whose list bound (cd + C <= skin_allow) held, whether or
-// not a list was built; the next step builds skin lists only when at least a
+// Per-scene list statistics (Skin::lst), under list policy 1 (the default,
+// 0, builds no skin lists: with the two-phase inline sweep they measured
+// slower on every workload tried): the testing passes count the lanes that
+// test and those whose list bound (cd + C <= skin_allow) held, whether or not
+// a list was built; the next step builds skin lists only when at least a
 // quarter held (or nothing was counted yet).  Where the candidates move
 // farther than the skin per iteration, almost every particle sweeps inline
 // anyway, and building the lists would only cost their writes.  Both ways
@@ -2176,13 +2178,14 @@ __device__ __forceinline__ void dense_commit(int, int, const SceneC* __restrict_
 // (:144-179): rho_i sums m W over all in-range candidates (self and
 // duplicates included); for j != i, sum |grad_j|^2 and grad_i; lambda is 0
 // when C_i < 0, else -C_i / (sum + |grad_i|^2 + eps).  BUILD (the first
-// iteration) builds the step's skin lists, and lists the particles whose
-// lists overflow for the dense kernels; later iterations test the lists.
-// The build pass (BUILD) is two launches: with LISTS it serves the scenes
-// whose flag says this step builds skin lists, without LISTS the others
-// (every block belongs to one scene and leaves at once when its scene is the
-// other launch's).  The list-free variant carries no list-building code and
-// runs at the testing passes' occupancy.
+// iteration) lists the crowded particles for the dense kernels and, under
+// list policy 1, builds the step's skin lists; later iterations test the
+// lists where they exist.  The build pass (BUILD) is then two launches: with
+// LISTS it serves the scenes whose flag says this step builds skin lists,
+// without LISTS the others (every block belongs to one scene and leaves at
+// once when its scene is the other launch's); under policy 0 only the
+// list-free launch is made.  The list-free variant carries no list-building
+// code and runs at the testing passes' occupancy.
 template <bool FAST, bool BUILD, bool LISTS = true, bool PERSIST = false>
 __global__ void __launch_bounds__(NPX_MAX_BLOCK) __attribute__((amdgpu_waves_per_eu((BUILD && LISTS) ? 4 : 6)))
 k_density_lambda(const SceneC* __restrict__ scenes, const int* __restrict__ blk_scene, const int* __restrict__ clsmap,
