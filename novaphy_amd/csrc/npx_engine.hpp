@@ -911,6 +911,9 @@ struct Ctx {
     hw_blocks = 0;
     for (int b = 0; b < slot_cap / NPX_BLOCK; ++b)
       if (h_blk_scene[b] >= 0) hw_blocks = b + 1;
+    hw_cls = 0;
+    for (const auto& H : scenes)
+      if (H.used && H.window_valid) hw_cls = std::max(hw_cls, H.cls_base + H.ncell);
     table_dirty = false;
     return NPX_OK;
   }
@@ -944,6 +947,7 @@ struct Ctx {
     std::copy(wlo, wlo + 3, H.wlo);
     std::copy(wdim, wdim + 3, H.wdim);
     H.window_valid = true;
+    hw_cls = std::max(hw_cls, H.cls_base + H.ncell);
     table_dirty = true;
     NPX_TRY(build_class_table(H.wlo, H.wdim, H.cls_base, clsmap.p + H.cls_base));
     return NPX_OK;
@@ -1091,15 +1095,14 @@ struct Ctx {
   // --------------------------------------------------------- the step
   // Classes the counting sorts and the per-class arrays cover: the single
   // scene's window, or in a batch the end of the highest class range a live
-  // scene holds.  Work over classes (fills, scans, the displacement maxima)
-  // then scales with the classes in use, not with the batch's cell capacity.
+  // scene holds (hw_cls).  Work over classes (fills, scans, the displacement
+  // maxima) then scales with the classes in use, not with the batch's cell
+  // capacity.
   int total_cls() const {
     if (!batch) return scenes[0].used ? scenes[0].ncell : 0;
-    int top = 0;
-    for (const auto& H : scenes)
-      if (H.used && H.window_valid) top = std::max(top, H.cls_base + H.ncell);
-    return top;
+    return hw_cls;
   }
+  int hw_cls = 0;  // set by upload_tables, raised by set_window
 
   // predict + grid: writes the cell-sorted state into the *2 buffers and x*
   // into XS0; the caller swaps the buffers to commit the reorder.
